@@ -1,0 +1,148 @@
+/*
+ * ptg.h -- C ABI of libptg_hip.so, the gfx950 tree-gravity library.
+ *
+ * The reference (pytreegrav) has no C ABI: its operator interface is the Python API in
+ * src/pytreegrav/frontend.py and the Octree jitclass in src/pytreegrav/octree.py.  Each entry point
+ * below replaces the compute behind one of those Python functions; the Python mirror in
+ * pytreegrav_amd/frontend.py binds them with ctypes.
+ *
+ * Conventions
+ *   - Every function returns a ptg_status: 0 on success, a negative code on failure; the message
+ *     is available from ptg_last_error() (thread-local).  The library never aborts or prints.
+ *   - All array arguments are device pointers owned by the caller (the Python side allocates them
+ *     as torch tensors).  Scratch memory is a caller-provided workspace whose size is returned by
+ *     the matching *_workspace_bytes function.
+ *   - Work is enqueued asynchronously on the given HIP stream (NULL = default stream).  The only
+ *     host synchronisation is inside ptg_build_sort, which reads back the node count.
+ *   - Element numbering follows the reference Octree (octree.py:8-28): particles occupy
+ *     [0, n) in Morton order, the root is element n, internal nodes follow.  Links are int32.
+ */
+#ifndef PTG_H
+#define PTG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int ptg_status;
+#define PTG_OK 0
+#define PTG_ERR_ARG (-1)      /* invalid argument */
+#define PTG_ERR_HIP (-2)      /* HIP runtime error */
+#define PTG_ERR_OVERFLOW (-3) /* a walk traversal stack exceeded its spill capacity */
+#define PTG_ERR_WORKSPACE (-4)/* workspace smaller than *_workspace_bytes asked for */
+
+/* field bits, packed in this order in every output row: potential, acceleration(3), tidal(9)
+ * (grouped_treewalk.py:48-52, _field_width :264-266) */
+#define PTG_POT 1
+#define PTG_ACC 2
+#define PTG_TIDAL 4
+
+#define PTG_F32 0
+#define PTG_F64 1
+
+int ptg_version(void);
+const char *ptg_last_error(void);
+
+/* ------------------------------------------------------------------------------------------
+ * Tree build: replaces Octree.__init__ radix path (octree.py:668-723), i.e. BuildTreeRadix
+ * (:817-1091) with _bounding_cube (:539-574), _morton_keys (:69-98), _radix_argsort (:101-132),
+ * _permute_particles (:577-603), RekeySlice (:1093-1126), BuildBucket (:1128-1175), and the
+ * moment passes RollupMoments (:199-231) / RollupQuadrupoles (:252-292).
+ * Two calls, because the node count is data dependent:
+ *   ptg_build_sort   bbox, keys, sort, collision re-keying, node counting (one host sync)
+ *   ptg_build_tree   node emission, links, moments, into caller arrays sized from info.nnodes
+ * ------------------------------------------------------------------------------------------ */
+typedef struct {
+    int64_t n;              /* particles */
+    int64_t nnodes;         /* internal nodes including the root */
+    int32_t has_collisions; /* some particles shared a 63-bit key and were re-keyed */
+    int32_t has_coincident; /* some positions were bit-identical (bucket chains), HasCoincidentPoints */
+    int32_t max_level;      /* deepest node level (root = 0) */
+    int32_t reserved;
+    double center[3];       /* root cube centre */
+    double size;            /* root cube side */
+} ptg_build_info;
+
+typedef struct {
+    double *coordinates;      /* (n+nnodes, 3): particle positions, node COM (geometric centre if mass 0) */
+    double *masses;           /* (n+nnodes) */
+    double *softenings;       /* (n+nnodes): particle h, node max h */
+    double *sizes;            /* (n+nnodes): node side, 0 for particles */
+    double *deltas;           /* (n+nnodes): |COM - geometric centre| */
+    double *quadrupoles;      /* (n+nnodes, 9) traceless quadrupoles, or NULL for a monopole tree */
+    int32_t *first_subnode;   /* (n+nnodes), -1 for particles */
+    int32_t *next_branch;     /* (n+nnodes), -1 terminates the walk */
+    int64_t *treewalk_indices;/* (n): input index of each Morton slot */
+    int32_t *nchild;          /* (n+nnodes): number of children (0 for particles) */
+    int32_t *child_base;      /* (n+nnodes): walk slot of the first child (sibling-contiguous layout) */
+} ptg_tree_arrays;
+
+size_t ptg_build_workspace_bytes(int64_t n);
+ptg_status ptg_build_sort(const double *pos, int64_t n, void *ws, size_t ws_bytes, ptg_build_info *info,
+                          void *stream);
+size_t ptg_node_workspace_bytes(int64_t n, int64_t nnodes, int quadrupole);
+ptg_status ptg_build_tree(const double *pos, const double *m, const double *h, const ptg_build_info *info,
+                          int quadrupole, int compute_moments, void *ws, size_t ws_bytes, void *node_ws,
+                          size_t node_ws_bytes, const ptg_tree_arrays *tree, void *stream);
+
+/* Morton permutation of arbitrary points: replaces grouped_treewalk._morton_order (:269-275).
+ * Uses a ptg_build_workspace_bytes(n) workspace; asynchronous. */
+ptg_status ptg_morton_order(const double *pos, int64_t n, int64_t *order, void *ws, size_t ws_bytes, void *stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Walk tree: the sibling-contiguous packed copy of the tree the walk reads (one slot per
+ * element; the children of a node occupy nchild consecutive slots from child_base).  FP32 packs
+ * store coordinates relative to `origin` (node-relative precision); FP64 packs store them as is.
+ * Per slot: xyzm (4 reals: x, y, z, mass), aux (4 reals: softening, size, delta, 0),
+ * kids (2 int32: first child slot, child count; count 0 marks a particle),
+ * quad (8 reals: xx, xy, xz, yy, yz, zz, 0, 0) when quadrupole != 0.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct {
+    void *xyzm;
+    void *aux;
+    int32_t *kids;
+    void *quad; /* NULL for a monopole pack */
+    int64_t nslots;
+    int32_t prec; /* PTG_F32 / PTG_F64 */
+    int32_t has_quad;
+    double origin[3];
+} ptg_walk_tree;
+
+ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_walk_tree *wt,
+                         void *stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Grouped walk: replaces grouped_treewalk.FieldsTarget_grouped (:350-397) with the core
+ * (:163-241), the per-interaction kernel (:45-155) and acceptance_criterion (treewalk.py:22-25).
+ * Targets are read as pos[perm[t]] for t in Morton order (perm = TreewalkIndices for self
+ * gravity, the _morton_order permutation for external targets) and results are written to
+ * out[perm[t]], i.e. in the caller's original order: out is (nt, W) float64 with
+ * W = 1*POT + 3*ACC + 9*TIDAL, scaled by G.  One wavefront walks each group of group_size
+ * consecutive sorted targets (1..256).  exact != 0 evaluates every pair with the reference's
+ * branches (r2 > 0 self skip, softened iff r < max(h_s, h_t)); exact == 0 (FP32 fast path) skips
+ * those tests where the group geometry proves them false.  counters (nullable, device int64[2])
+ * accumulate: [0] element-target interactions, [1] traversal visits.
+ * ------------------------------------------------------------------------------------------ */
+size_t ptg_walk_workspace_bytes(int64_t nt, int group_size);
+ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const double *soft, const int64_t *perm, int64_t nt,
+                    int fields, int quadrupole, int group_size, double theta, double G, int exact, double *out,
+                    int64_t *counters, void *ws, size_t ws_bytes, void *stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Direct summation, LDS-tiled: replaces bruteforce.py.  self_mode != 0: targets are the sources
+ * (nt == ns), pairs with equal index are excluded (Potential_bruteforce_parallel :86-116,
+ * Accel_bruteforce_parallel :164-206, TidalTensor_bruteforce_parallel :319-350); self_mode == 0:
+ * target form, r == 0 pairs skipped (PotentialTarget_bruteforce :7-42, AccelTarget_bruteforce
+ * :209-248, TidalTensorTarget_bruteforce :353-386).  out is (nt, W) float64, scaled by G.
+ * ------------------------------------------------------------------------------------------ */
+ptg_status ptg_direct(const double *pos_t, const double *soft_t, int64_t nt, const double *pos_s, const double *m_s,
+                      const double *h_s, int64_t ns, int prec, int fields, int self_mode, double G, double *out,
+                      void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PTG_H */

@@ -1,0 +1,115 @@
+"""ctypes binding of libptg_hip.so (declared in include/ptg.h).
+
+The library is built in-tree by ``python -m pytreegrav_amd.build`` (or ``__graft_entry__.build()``).
+There is no fallback: if the library or a GPU is missing, every compute call raises.
+"""
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libptg_hip.so")
+
+PTG_POT, PTG_ACC, PTG_TIDAL = 1, 2, 4
+PTG_F32, PTG_F64 = 0, 1
+PTG_ERR_OVERFLOW = -3
+
+_c = ctypes
+_vp = _c.c_void_p
+_i64 = _c.c_int64
+_i32 = _c.c_int32
+_dbl = _c.c_double
+
+
+class BuildInfo(_c.Structure):
+    _fields_ = [
+        ("n", _i64),
+        ("nnodes", _i64),
+        ("has_collisions", _i32),
+        ("has_coincident", _i32),
+        ("max_level", _i32),
+        ("reserved", _i32),
+        ("center", _dbl * 3),
+        ("size", _dbl),
+    ]
+
+
+class TreeArrays(_c.Structure):
+    _fields_ = [
+        ("coordinates", _vp),
+        ("masses", _vp),
+        ("softenings", _vp),
+        ("sizes", _vp),
+        ("deltas", _vp),
+        ("quadrupoles", _vp),
+        ("first_subnode", _vp),
+        ("next_branch", _vp),
+        ("treewalk_indices", _vp),
+        ("nchild", _vp),
+        ("child_base", _vp),
+    ]
+
+
+class WalkTree(_c.Structure):
+    _fields_ = [
+        ("xyzm", _vp),
+        ("aux", _vp),
+        ("kids", _vp),
+        ("quad", _vp),
+        ("nslots", _i64),
+        ("prec", _i32),
+        ("has_quad", _i32),
+        ("origin", _dbl * 3),
+    ]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "ptg_version": (_c.c_int, []),
+    "ptg_last_error": (_c.c_char_p, []),
+    "ptg_build_workspace_bytes": (_c.c_size_t, [_i64]),
+    "ptg_build_sort": (_c.c_int, [_vp, _i64, _vp, _c.c_size_t, _c.POINTER(BuildInfo), _vp]),
+    "ptg_node_workspace_bytes": (_c.c_size_t, [_i64, _i64, _c.c_int]),
+    "ptg_build_tree": (_c.c_int, [_vp, _vp, _vp, _c.POINTER(BuildInfo), _c.c_int, _c.c_int, _vp, _c.c_size_t, _vp,
+                                  _c.c_size_t, _c.POINTER(TreeArrays), _vp]),
+    "ptg_morton_order": (_c.c_int, [_vp, _i64, _vp, _vp, _c.c_size_t, _vp]),
+    "ptg_pack_walk": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _c.POINTER(WalkTree), _vp]),
+    "ptg_walk_workspace_bytes": (_c.c_size_t, [_i64, _c.c_int]),
+    "ptg_walk": (_c.c_int, [_c.POINTER(WalkTree), _vp, _vp, _vp, _i64, _c.c_int, _c.c_int, _c.c_int, _dbl, _dbl,
+                            _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),
+    "ptg_walk_check": (_c.c_int, [_vp, _vp]),
+    "ptg_direct": (_c.c_int, [_vp, _vp, _i64, _vp, _vp, _vp, _i64, _c.c_int, _c.c_int, _c.c_int, _dbl, _vp, _vp]),
+}
+
+_lib = None
+
+
+class PtgError(RuntimeError):
+    """A libptg_hip call returned a nonzero status."""
+
+
+def load():
+    """Load libptg_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build the HIP library first (python -m pytreegrav_amd.build)")
+        L = _c.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status, what):
+    if status != 0:
+        msg = load().ptg_last_error().decode(errors="replace")
+        raise PtgError(f"{what} failed with status {status}: {msg}")
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (or None -> NULL)."""
+    return None if t is None else _vp(t.data_ptr())

@@ -1,0 +1,1135 @@
+// Tree build for gfx950: bounding cube, Morton keys, LSD radix sort, collision re-keying,
+// Karras-style node emission (octree levels from common key prefixes), links, FP64 moments and
+// the sibling-contiguous walk pack.
+//
+// Compiled with -ffp-contract=off: keys, node geometry and FP64 moments reproduce the reference's
+// IEEE arithmetic bit for bit (octree.py builds these without fastmath).
+#pragma clang fp contract(off)
+
+#include <stdarg.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "ptg_common.hpp"
+
+// ============================================================================================
+// error reporting
+// ============================================================================================
+static thread_local char g_err[1024] = "";
+
+void ptg_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *ptg_last_error(void) { return g_err; }
+extern "C" int ptg_version(void) { return 10000; }
+
+// ============================================================================================
+// generic exclusive scan of int32 values produced by a loader functor
+// ============================================================================================
+#define SC_THREADS 256
+#define SC_ITEMS 8
+#define SC_TILE (SC_THREADS * SC_ITEMS)
+
+template <typename T> __device__ __forceinline__ T block_excl_scan(T v, T *total, T *sh)
+{
+    // sh: at least 4 entries
+    int lane = ptg_lane(), wave = threadIdx.x >> 6;
+    T incl = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        T o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    if (lane == 63) sh[wave] = incl;
+    __syncthreads();
+    T woff = 0, tot = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+        T s = sh[w];
+        if (w < wave) woff += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return woff + incl - v;
+}
+
+template <typename Load>
+__global__ __launch_bounds__(SC_THREADS) void k_scan_reduce(Load ld, int64_t n, int32_t *bsum)
+{
+    __shared__ int32_t sh[8];
+    int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_ITEMS;
+    int32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < SC_ITEMS; q++)
+        if (base + q < n) s += ld(base + q);
+    int32_t tot;
+    block_excl_scan<int32_t>(s, &tot, sh);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// single block: exclusive scan of nb block sums in place; grand total to *total
+__global__ __launch_bounds__(SC_THREADS) void k_scan_top(int32_t *bsum, int64_t nb, int32_t *total)
+{
+    __shared__ int32_t sh[8];
+    int32_t carry = 0;
+    for (int64_t base = 0; base < nb; base += SC_THREADS) {
+        int64_t i = base + threadIdx.x;
+        int32_t v = i < nb ? bsum[i] : 0;
+        int32_t tot;
+        int32_t ex = block_excl_scan<int32_t>(v, &tot, sh);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+template <typename Load>
+__global__ __launch_bounds__(SC_THREADS) void k_scan_down(Load ld, int64_t n, const int32_t *bsum, int32_t *out)
+{
+    __shared__ int32_t sh[8];
+    int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_ITEMS;
+    int32_t v[SC_ITEMS];
+    int32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < SC_ITEMS; q++) {
+        v[q] = base + q < n ? ld(base + q) : 0;
+        s += v[q];
+    }
+    int32_t tot;
+    int32_t ex = block_excl_scan<int32_t>(s, &tot, sh) + bsum[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < SC_ITEMS; q++) {
+        if (base + q < n) out[base + q] = ex;
+        ex += v[q];
+    }
+}
+
+static int64_t scan_blocks(int64_t n) { return std::max<int64_t>(1, ptg_cdiv(n, SC_TILE)); }
+
+template <typename Load>
+static ptg_status excl_scan(Load ld, int64_t n, int32_t *out, int32_t *bsum, int32_t *total, hipStream_t s)
+{
+    int64_t nb = scan_blocks(n);
+    k_scan_reduce<<<nb, SC_THREADS, 0, s>>>(ld, n, bsum);
+    k_scan_top<<<1, SC_THREADS, 0, s>>>(bsum, nb, total);
+    k_scan_down<<<nb, SC_THREADS, 0, s>>>(ld, n, bsum, out);
+    PTG_LAUNCH_CHECK();
+    return PTG_OK;
+}
+
+struct LoadArr {
+    const int32_t *a;
+    __device__ int32_t operator()(int64_t i) const { return a[i]; }
+};
+
+// ============================================================================================
+// LSD radix sort of (u64 key, u32 value) pairs, 8-bit digits.  Each wavefront owns a tile of
+// RS_WTILE consecutive elements; ranks inside a tile come from ballot-based digit matching, so
+// the sort is stable with no block-level synchronisation.
+// ============================================================================================
+#define RS_BITS 8
+#define RS_BINS 256
+#define RS_ITEMS 16
+#define RS_WTILE (64 * RS_ITEMS)
+#define RS_WAVES 4
+
+__global__ __launch_bounds__(64 * RS_WAVES) void k_rs_hist(const uint64_t *keys, int64_t n, int shift, int64_t nwt,
+                                                           int32_t *hist)
+{
+    __shared__ uint32_t cnt[RS_WAVES][RS_BINS];
+    int wave = threadIdx.x >> 6, lane = ptg_lane();
+    int64_t gw = (int64_t)blockIdx.x * RS_WAVES + wave;
+    if (gw >= nwt) return;
+#pragma unroll
+    for (int q = 0; q < RS_BINS / 64; q++) cnt[wave][lane * (RS_BINS / 64) + q] = 0;
+    int64_t base = gw * RS_WTILE;
+#pragma unroll 4
+    for (int r = 0; r < RS_ITEMS; r++) {
+        int64_t i = base + r * 64 + lane;
+        if (i < n) atomicAdd(&cnt[wave][(keys[i] >> shift) & (RS_BINS - 1)], 1u);
+    }
+#pragma unroll
+    for (int q = 0; q < RS_BINS / 64; q++) {
+        int d = lane * (RS_BINS / 64) + q;
+        hist[(int64_t)d * nwt + gw] = (int32_t)cnt[wave][d];
+    }
+}
+
+__global__ __launch_bounds__(64 * RS_WAVES) void k_rs_scatter(const uint64_t *kin, const uint32_t *vin, int64_t n,
+                                                              int shift, int64_t nwt, const int32_t *offs,
+                                                              uint64_t *kout, uint32_t *vout)
+{
+    __shared__ uint32_t run[RS_WAVES][RS_BINS];
+    int wave = threadIdx.x >> 6, lane = ptg_lane();
+    int64_t gw = (int64_t)blockIdx.x * RS_WAVES + wave;
+    if (gw >= nwt) return;
+#pragma unroll
+    for (int q = 0; q < RS_BINS / 64; q++) {
+        int d = lane * (RS_BINS / 64) + q;
+        run[wave][d] = (uint32_t)offs[(int64_t)d * nwt + gw];
+    }
+    uint64_t lt = ptg_lanemask_lt();
+    int64_t base = gw * RS_WTILE;
+    for (int r = 0; r < RS_ITEMS; r++) {
+        int64_t i = base + r * 64 + lane;
+        bool valid = i < n;
+        uint64_t k = valid ? kin[i] : 0;
+        uint32_t v = valid ? vin[i] : 0;
+        uint32_t d = (uint32_t)(k >> shift) & (RS_BINS - 1);
+        uint64_t peers = ptg_ballot(valid);
+#pragma unroll
+        for (int b = 0; b < RS_BITS; b++) {
+            bool bit = (d >> b) & 1;
+            uint64_t bal = ptg_ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & lt);
+        uint32_t dst = run[wave][d] + rank;
+        if (valid) {
+            kout[dst] = k;
+            vout[dst] = v;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (valid && rank == 0) run[wave][d] = dst + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+static int64_t rs_nwt(int64_t n) { return std::max<int64_t>(1, ptg_cdiv(n, RS_WTILE)); }
+
+// sorts (k0,v0) by the low `nbits` bits; the result ends in (k0,v0) or (k1,v1): *in_first tells which
+static ptg_status radix_sort(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,
+                             int32_t *hist, int32_t *bsum, bool *in_first, hipStream_t s)
+{
+    int64_t nwt = rs_nwt(n);
+    int64_t nblk = ptg_cdiv(nwt, RS_WAVES);
+    bool first = true;
+    for (int shift = 0; shift < nbits; shift += RS_BITS) {
+        uint64_t *ki = first ? k0 : k1, *ko = first ? k1 : k0;
+        uint32_t *vi = first ? v0 : v1, *vo = first ? v1 : v0;
+        k_rs_hist<<<nblk, 64 * RS_WAVES, 0, s>>>(ki, n, shift, nwt, hist);
+        ptg_status st = excl_scan(LoadArr{hist}, (int64_t)RS_BINS * nwt, hist, bsum, nullptr, s);
+        if (st) return st;
+        k_rs_scatter<<<nblk, 64 * RS_WAVES, 0, s>>>(ki, vi, n, shift, nwt, hist, ko, vo);
+        PTG_LAUNCH_CHECK();
+        first = !first;
+    }
+    *in_first = first;
+    return PTG_OK;
+}
+
+// ============================================================================================
+// bounding cube (octree.py:539-574) and Morton keys (octree.py:69-98)
+// ============================================================================================
+#define BB_BLOCKS 1024
+struct Geom {
+    double center[3], size, cmin[3], inv;
+};
+
+__global__ __launch_bounds__(256) void k_bbox_partial(const double *pos, int64_t n, double *part)
+{
+    __shared__ double sh[6][4];
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            double x = pos[3 * i + k];
+            lo[k] = fmin(lo[k], x);
+            hi[k] = fmax(hi[k], x);
+        }
+    }
+    int wave = threadIdx.x >> 6, lane = ptg_lane();
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        lo[k] = ptg_wave_min(lo[k]);
+        hi[k] = ptg_wave_max(hi[k]);
+    }
+    if (lane == 0)
+        for (int k = 0; k < 3; k++) {
+            sh[k][wave] = lo[k];
+            sh[3 + k][wave] = hi[k];
+        }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 3; k++) {
+            double a = sh[k][0], b = sh[3 + k][0];
+            for (int w = 1; w < 4; w++) {
+                a = fmin(a, sh[k][w]);
+                b = fmax(b, sh[3 + k][w]);
+            }
+            part[6 * blockIdx.x + k] = a;
+            part[6 * blockIdx.x + 3 + k] = b;
+        }
+    }
+}
+
+__global__ void k_bbox_final(const double *part, int nb, Geom *g)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = 0; b < nb; b++)
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fmin(lo[k], part[6 * b + k]);
+            hi[k] = fmax(hi[k], part[6 * b + 3 + k]);
+        }
+    for (int k = 0; k < 3; k++) g->center[k] = 0.5 * (hi[k] + lo[k]);
+    double s = hi[0] - lo[0];
+    if (hi[1] - lo[1] > s) s = hi[1] - lo[1];
+    if (hi[2] - lo[2] > s) s = hi[2] - lo[2];
+    g->size = s;
+    g->inv = s > 0 ? 1.0 / s : 0.0;
+    for (int k = 0; k < 3; k++) g->cmin[k] = g->center[k] - 0.5 * s;
+}
+
+__device__ __forceinline__ uint64_t morton_key(double x, double y, double z, const double *cmin, double inv)
+{
+    const double SCALE = 2097152.0;
+    const int64_t MAXQ = (1 << 21) - 1;
+    int64_t q[3];
+    double p[3] = {x, y, z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double t = ((p[k] - cmin[k]) * inv) * SCALE;
+        int64_t v = (int64_t)t;
+        v = v < 0 ? 0 : (v > MAXQ ? MAXQ : v);
+        q[k] = v;
+    }
+    return ptg_spread_bits(q[0]) | (ptg_spread_bits(q[1]) << 1) | (ptg_spread_bits(q[2]) << 2);
+}
+
+__global__ __launch_bounds__(256) void k_keys(const double *pos, int64_t n, const Geom *g, uint64_t *keys,
+                                              uint32_t *vals)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = morton_key(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], g->cmin, g->inv);
+    vals[i] = (uint32_t)i;
+}
+
+// ============================================================================================
+// build workspace
+// ============================================================================================
+struct Counters {
+    int32_t ncollide;     // adjacent pairs with equal 63-bit keys
+    int32_t nnodes;       // total internal nodes
+    int32_t nsub;         // particles in collision runs
+    int32_t coincident;   // bit-identical extended keys exist
+    int32_t max_level;
+    int32_t pad[11];
+};
+
+struct BuildWS {
+    double *part;
+    Geom *geom;
+    Counters *ctr;
+    uint64_t *kA, *kB;
+    uint32_t *vA, *vB;
+    int32_t *hist, *bsum;
+    int32_t *lev, *off;
+    // collision re-keying (extended keys K1..K3 per Morton slot, and the compact subset)
+    uint64_t *ext;      // 3 * n
+    int32_t *flag, *subpos;
+    uint64_t *sk0, *sk1, *sk2, *sk3;
+    uint32_t *sslot, *sval;
+    uint64_t *tkA, *tkB;
+    uint32_t *perm, *permB;
+};
+
+static bool carve_build(PtgArena &A, int64_t n, BuildWS &w)
+{
+    int64_t nwt = rs_nwt(n);
+    int64_t nh = (int64_t)RS_BINS * nwt;
+    w.part = A.take<double>(6 * BB_BLOCKS);
+    w.geom = A.take<Geom>(1);
+    w.ctr = A.take<Counters>(1);
+    w.kA = A.take<uint64_t>(n);
+    w.kB = A.take<uint64_t>(n);
+    w.vA = A.take<uint32_t>(n);
+    w.vB = A.take<uint32_t>(n);
+    w.hist = A.take<int32_t>(nh);
+    w.bsum = A.take<int32_t>(scan_blocks(std::max<int64_t>(nh, n)) + 64);
+    w.lev = A.take<int32_t>(n);
+    w.off = A.take<int32_t>(n);
+    w.ext = A.take<uint64_t>(3 * n);
+    w.flag = A.take<int32_t>(n);
+    w.subpos = A.take<int32_t>(n);
+    w.sk0 = A.take<uint64_t>(n);
+    w.sk1 = A.take<uint64_t>(n);
+    w.sk2 = A.take<uint64_t>(n);
+    w.sk3 = A.take<uint64_t>(n);
+    w.sslot = A.take<uint32_t>(n);
+    w.sval = A.take<uint32_t>(n);
+    w.tkA = A.take<uint64_t>(n);
+    w.tkB = A.take<uint64_t>(n);
+    w.perm = A.take<uint32_t>(n);
+    w.permB = A.take<uint32_t>(n);
+    return A.ok();
+}
+
+extern "C" size_t ptg_build_workspace_bytes(int64_t n)
+{
+    PtgArena A(nullptr, 0);
+    BuildWS w;
+    carve_build(A, std::max<int64_t>(n, 1), w);
+    return A.used;
+}
+
+// ============================================================================================
+// pair levels, collision re-keying
+// ============================================================================================
+__global__ void k_zero_counters(Counters *c)
+{
+    int i = threadIdx.x;
+    if (i < (int)(sizeof(Counters) / 4)) ((int32_t *)c)[i] = 0;
+}
+
+// lev[i] = whole octree levels shared by Morton slots i and i+1
+__global__ __launch_bounds__(256) void k_levels(const uint64_t *keys, int64_t n, int32_t *lev, Counters *c)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool coll = false;
+    if (i < n - 1) {
+        int L = ptg_common_levels(keys[i], keys[i + 1]);
+        lev[i] = L;
+        coll = (keys[i] == keys[i + 1]);
+    }
+    uint64_t b = ptg_ballot(coll);
+    if (b && ptg_lane() == 0) atomicAdd(&c->ncollide, (int)__popcll(b));
+}
+
+// slots whose 63-bit key is shared with a neighbour
+__global__ __launch_bounds__(256) void k_collision_flags(const uint64_t *keys, int64_t n, int32_t *flag)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bool f = (i > 0 && keys[i] == keys[i - 1]) || (i < n - 1 && keys[i] == keys[i + 1]);
+    flag[i] = f ? 1 : 0;
+}
+
+// extended keys for one particle: re-keying relative to its level-21, -42, -63 cells
+// (RekeySlice, octree.py:1093-1126, with cell centres from the split's recurrence :974-978)
+__device__ void extended_keys(const double *p, uint64_t k0, const Geom *g, uint64_t *k123)
+{
+    double c[3] = {g->center[0], g->center[1], g->center[2]};
+    double s = g->size;
+    uint64_t key = k0;
+    for (int round = 0; round < 3; round++) {
+        for (int L = 1; L <= 21; L++) {
+            int d = (int)((key >> (63 - 3 * L)) & 7);
+#pragma unroll
+            for (int k = 0; k < 3; k++) c[k] = c[k] + s * ptg_octant_offset(d, k);
+            s = s * 0.5;
+        }
+        double inv = s > 0 ? 1.0 / s : 0.0;
+        double cmin[3];
+        for (int k = 0; k < 3; k++) cmin[k] = c[k] - 0.5 * s;
+        key = morton_key(p[0], p[1], p[2], cmin, inv);
+        k123[round] = key;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_collision_gather(const double *pos, const uint64_t *keys,
+                                                          const uint32_t *order, const int32_t *flag,
+                                                          const int32_t *subpos, int64_t n, const Geom *g,
+                                                          uint64_t *sk0, uint64_t *sk1, uint64_t *sk2,
+                                                          uint64_t *sk3, uint32_t *sslot, uint32_t *sval,
+                                                          uint32_t *perm)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    int64_t j = subpos[i];
+    uint32_t o = order[i];
+    double p[3] = {pos[3 * (int64_t)o], pos[3 * (int64_t)o + 1], pos[3 * (int64_t)o + 2]};
+    uint64_t k3[3];
+    extended_keys(p, keys[i], g, k3);
+    sk0[j] = keys[i];
+    sk1[j] = k3[0];
+    sk2[j] = k3[1];
+    sk3[j] = k3[2];
+    sslot[j] = (uint32_t)i;
+    sval[j] = o;
+    perm[j] = (uint32_t)j;
+}
+
+__global__ __launch_bounds__(256) void k_gather_u64(const uint64_t *src, const uint32_t *perm, int64_t m,
+                                                    uint64_t *dst)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < m) dst[j] = src[perm[j]];
+}
+
+// place the re-sorted collision subset back into its Morton slots
+__global__ __launch_bounds__(256) void k_collision_apply(const uint32_t *perm, int64_t m, const uint32_t *sslot,
+                                                         const uint32_t *sval, const uint64_t *sk1,
+                                                         const uint64_t *sk2, const uint64_t *sk3, uint32_t *order,
+                                                         uint64_t *ext, int64_t n)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    uint32_t src = perm[j];
+    int64_t slot = sslot[j];
+    order[slot] = sval[src];
+    ext[slot] = sk1[src];
+    ext[n + slot] = sk2[src];
+    ext[2 * n + slot] = sk3[src];
+}
+
+__device__ __forceinline__ bool ext_equal(const uint64_t *keys, const uint64_t *ext, int64_t n, int64_t a, int64_t b)
+{
+    return keys[a] == keys[b] && ext[a] == ext[b] && ext[n + a] == ext[n + b] && ext[2 * n + a] == ext[2 * n + b];
+}
+
+__device__ __forceinline__ bool ext_less(const uint64_t *keys, const uint64_t *ext, int64_t n, int64_t a, int64_t b)
+{
+    if (keys[a] != keys[b]) return keys[a] < keys[b];
+    for (int r = 0; r < 3; r++)
+        if (ext[r * n + a] != ext[r * n + b]) return ext[r * n + a] < ext[r * n + b];
+    return false;
+}
+
+// levels of pairs inside collision runs: re-keyed levels 22..84, then bucket chains of 7
+// (BuildBucket, octree.py:1128-1175: 7 particles and one deeper node per chain level)
+__global__ __launch_bounds__(256) void k_levels_ext(const uint64_t *keys, const uint64_t *ext, int64_t n,
+                                                    int32_t *lev, Counters *c)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1 || keys[i] != keys[i + 1]) return;
+    int L;
+    if (ext[i] != ext[i + 1]) L = 21 + ptg_common_levels(ext[i], ext[i + 1]);
+    else if (ext[n + i] != ext[n + i + 1]) L = 42 + ptg_common_levels(ext[n + i], ext[n + i + 1]);
+    else if (ext[2 * n + i] != ext[2 * n + i + 1]) L = 63 + ptg_common_levels(ext[2 * n + i], ext[2 * n + i + 1]);
+    else {
+        // identical to double precision: find where this run of identical extended keys starts
+        int64_t lo = 0, hi = i;  // first index with key == key[i]
+        while (lo < hi) {
+            int64_t mid = (lo + hi) >> 1;
+            if (ext_less(keys, ext, n, mid, i)) lo = mid + 1;
+            else hi = mid;
+        }
+        L = 84 + (int)((i - lo) / 7);
+        c->coincident = 1;
+    }
+    lev[i] = L;
+}
+
+struct LoadNodeCount {
+    const int32_t *lev;
+    __device__ int32_t operator()(int64_t i) const
+    {
+        int32_t p = i == 0 ? -1 : lev[i - 1];
+        int32_t d = lev[i] - p;
+        return d > 0 ? d : 0;
+    }
+};
+
+__global__ __launch_bounds__(256) void k_max_level(const int32_t *lev, int64_t m, Counters *c)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int v = i < m ? lev[i] : 0;
+    v = ptg_wave_max(v);
+    if (ptg_lane() == 0 && v > 0) atomicMax(&c->max_level, v);
+}
+
+__global__ void k_u32_to_i64(const uint32_t *a, int64_t n, int64_t *b)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+static dim3 grid1(int64_t n, int bs = 256) { return dim3((unsigned)std::max<int64_t>(1, ptg_cdiv(n, bs))); }
+
+// bbox + keys + sort; result (sorted keys, order) in (w.kA, w.vA)
+static ptg_status sort_by_keys(const double *pos, int64_t n, BuildWS &w, hipStream_t s)
+{
+    int nb = (int)std::min<int64_t>(BB_BLOCKS, std::max<int64_t>(1, ptg_cdiv(n, 1024)));
+    k_bbox_partial<<<nb, 256, 0, s>>>(pos, n, w.part);
+    k_bbox_final<<<1, 64, 0, s>>>(w.part, nb, w.geom);
+    k_keys<<<grid1(n), 256, 0, s>>>(pos, n, w.geom, w.kA, w.vA);
+    PTG_LAUNCH_CHECK();
+    bool in_first;
+    ptg_status st = radix_sort(w.kA, w.vA, w.kB, w.vB, n, 64, w.hist, w.bsum, &in_first, s);
+    if (st) return st;
+    if (!in_first) {
+        PTG_HIP_CHECK(hipMemcpyAsync(w.kA, w.kB, n * 8, hipMemcpyDeviceToDevice, s));
+        PTG_HIP_CHECK(hipMemcpyAsync(w.vA, w.vB, n * 4, hipMemcpyDeviceToDevice, s));
+    }
+    return PTG_OK;
+}
+
+extern "C" ptg_status ptg_morton_order(const double *pos, int64_t n, int64_t *order, void *ws, size_t ws_bytes,
+                                       void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (n < 0 || (n > 0 && (!pos || !order))) {
+        ptg_set_error("ptg_morton_order: bad arguments");
+        return PTG_ERR_ARG;
+    }
+    if (n == 0) return PTG_OK;
+    if (n > 0x7fffffffLL) {
+        ptg_set_error("ptg_morton_order: n exceeds the int32 index range");
+        return PTG_ERR_ARG;
+    }
+    PtgArena A(ws, ws_bytes);
+    BuildWS w;
+    if (!ws || !carve_build(A, n, w)) {
+        ptg_set_error("ptg_morton_order: workspace too small (%zu < %zu)", ws_bytes, ptg_build_workspace_bytes(n));
+        return PTG_ERR_WORKSPACE;
+    }
+    ptg_status st = sort_by_keys(pos, n, w, s);
+    if (st) return st;
+    k_u32_to_i64<<<grid1(n), 256, 0, s>>>(w.vA, n, order);
+    PTG_LAUNCH_CHECK();
+    return PTG_OK;
+}
+
+extern "C" ptg_status ptg_build_sort(const double *pos, int64_t n, void *ws, size_t ws_bytes, ptg_build_info *info,
+                                     void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (n < 1 || !pos || !info) {
+        ptg_set_error("ptg_build_sort: need n >= 1 particles and non-null pointers");
+        return PTG_ERR_ARG;
+    }
+    if (n > 0x3fffffffLL) {
+        ptg_set_error("ptg_build_sort: n=%lld exceeds the int32 element range", (long long)n);
+        return PTG_ERR_ARG;
+    }
+    PtgArena A(ws, ws_bytes);
+    BuildWS w;
+    if (!ws || !carve_build(A, n, w)) {
+        ptg_set_error("ptg_build_sort: workspace too small (%zu < %zu)", ws_bytes, ptg_build_workspace_bytes(n));
+        return PTG_ERR_WORKSPACE;
+    }
+    k_zero_counters<<<1, 64, 0, s>>>(w.ctr);
+    ptg_status st = sort_by_keys(pos, n, w, s);
+    if (st) return st;
+    Counters hc;
+    if (n >= 2) {
+        k_levels<<<grid1(n - 1), 256, 0, s>>>(w.kA, n, w.lev, w.ctr);
+        PTG_LAUNCH_CHECK();
+        PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+        PTG_HIP_CHECK(hipStreamSynchronize(s));
+        if (hc.ncollide > 0) {
+            // collision runs: compute extended keys and re-sort each run by them (stable)
+            k_collision_flags<<<grid1(n), 256, 0, s>>>(w.kA, n, w.flag);
+            st = excl_scan(LoadArr{w.flag}, n, w.subpos, w.bsum, &w.ctr->nsub, s);
+            if (st) return st;
+            k_collision_gather<<<grid1(n), 256, 0, s>>>(pos, w.kA, w.vA, w.flag, w.subpos, n, w.geom, w.sk0,
+                                                        w.sk1, w.sk2, w.sk3, w.sslot, w.sval, w.perm);
+            PTG_LAUNCH_CHECK();
+            PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+            PTG_HIP_CHECK(hipStreamSynchronize(s));
+            int64_t m = hc.nsub;
+            uint64_t *karr[4] = {w.sk3, w.sk2, w.sk1, w.sk0};
+            for (int r = 0; r < 4; r++) {
+                k_gather_u64<<<grid1(m), 256, 0, s>>>(karr[r], w.perm, m, w.tkA);
+                bool in_first;
+                st = radix_sort(w.tkA, w.perm, w.tkB, w.permB, m, 64, w.hist, w.bsum, &in_first, s);
+                if (st) return st;
+                if (!in_first) PTG_HIP_CHECK(hipMemcpyAsync(w.perm, w.permB, m * 4, hipMemcpyDeviceToDevice, s));
+            }
+            PTG_HIP_CHECK(hipMemsetAsync(w.ext, 0, 3 * n * sizeof(uint64_t), s));
+            k_collision_apply<<<grid1(m), 256, 0, s>>>(w.perm, m, w.sslot, w.sval, w.sk1, w.sk2, w.sk3, w.vA, w.ext,
+                                                       n);
+            k_levels_ext<<<grid1(n - 1), 256, 0, s>>>(w.kA, w.ext, n, w.lev, w.ctr);
+            PTG_LAUNCH_CHECK();
+        }
+        st = excl_scan(LoadNodeCount{w.lev}, n - 1, w.off, w.bsum, &w.ctr->nnodes, s);
+        if (st) return st;
+        k_max_level<<<grid1(n - 1), 256, 0, s>>>(w.lev, n - 1, w.ctr);
+        PTG_LAUNCH_CHECK();
+    }
+    Geom hg;
+    PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    PTG_HIP_CHECK(hipMemcpyAsync(&hg, w.geom, sizeof(Geom), hipMemcpyDeviceToHost, s));
+    PTG_HIP_CHECK(hipStreamSynchronize(s));
+    info->n = n;
+    info->nnodes = n >= 2 ? hc.nnodes : 1;
+    info->has_collisions = hc.ncollide > 0;
+    info->has_coincident = hc.coincident;
+    info->max_level = n >= 2 ? hc.max_level : 0;
+    info->reserved = 0;
+    for (int k = 0; k < 3; k++) info->center[k] = hg.center[k];
+    info->size = hg.size;
+    if (n + info->nnodes > 0x7fffffffLL) {
+        ptg_set_error("ptg_build_sort: %lld elements exceed the int32 link range", (long long)(n + info->nnodes));
+        return PTG_ERR_ARG;
+    }
+    return PTG_OK;
+}
+
+// ============================================================================================
+// node emission: node (i, L) for every level L in (lev[i-1], lev[i]] started by pair i.
+// Node index = n + off[i] + (L - lev[i-1] - 1), i.e. DFS preorder of the octree.
+// ============================================================================================
+struct KeyView {
+    const uint64_t *keys, *ext;  // ext == nullptr when no collisions
+    int64_t n;
+    __device__ int digit(int64_t i, int L) const  // digit selecting the level-L cell (L >= 1)
+    {
+        if (L <= 21) return (int)((keys[i] >> (63 - 3 * L)) & 7);
+        if (L <= 84) {
+            int r = (L - 1) / 21 - 1;  // 0..2 -> K1..K3
+            int l = L - 21 * (r + 1);
+            return (int)((ext[(int64_t)r * n + i] >> (63 - 3 * l)) & 7);
+        }
+        return 7;  // bucket chains continue in octant 7 (octree.py:1146)
+    }
+    // do Morton slots a and b lie in the same level-L cell?
+    __device__ bool same_cell(int64_t a, int64_t b, int L) const
+    {
+        if (L == 0) return true;
+        if (L < 21) return (keys[a] >> (63 - 3 * L)) == (keys[b] >> (63 - 3 * L));
+        if (keys[a] != keys[b]) return false;
+        if (L == 21) return true;
+        for (int r = 0; r < 3; r++) {
+            int l = L - 21 * (r + 1);
+            uint64_t x = ext[(int64_t)r * n + a], y = ext[(int64_t)r * n + b];
+            if (l < 21) return (x >> (63 - 3 * l)) == (y >> (63 - 3 * l));
+            if (x != y) return false;
+            if (l == 21) return true;
+        }
+        return true;  // bucket levels: identical to double precision
+    }
+};
+
+__device__ __forceinline__ int32_t head_of(int64_t q, int64_t n, const int32_t *lev, const int32_t *off)
+{
+    if (q >= n) return -1;
+    if (q <= n - 2 && lev[q] > lev[q - 1]) return (int32_t)(n + off[q]);
+    return (int32_t)q;
+}
+
+__global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *lev, const int32_t *off, int64_t n,
+                                                    const Geom *g, double *coord, double *sizes,
+                                                    int32_t *first, int32_t *next)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int lp = i == 0 ? -1 : lev[i - 1];
+    int li = lev[i];
+    if (li <= lp) return;
+    double c[3] = {g->center[0], g->center[1], g->center[2]};
+    double s = g->size;
+    int64_t e = i;  // running end of the level-L cell containing slot i
+    for (int L = 0; L <= li; L++) {
+        if (L > lp) {
+            int64_t idx = n + off[i] + (L - lp - 1);
+            coord[3 * idx] = c[0];
+            coord[3 * idx + 1] = c[1];
+            coord[3 * idx + 2] = c[2];
+            sizes[idx] = s;
+            first[idx] = (L + 1 <= li) ? (int32_t)(idx + 1) : (int32_t)i;
+            // last slot of this cell: gallop then bisect on "same level-L cell as slot i"
+            int64_t lo = i + 1;  // slot i+1 shares level li >= L with slot i
+            int64_t step = 1;
+            while (lo + step < n && kv.same_cell(i, lo + step, L)) {
+                lo += step;
+                step <<= 1;
+            }
+            int64_t hi = std::min<int64_t>(lo + step, n);  // first candidate known not to match (or n)
+            while (hi - lo > 1) {
+                int64_t mid = (lo + hi) >> 1;
+                if (kv.same_cell(i, mid, L)) lo = mid;
+                else hi = mid;
+            }
+            e = lo;
+            next[idx] = head_of(e + 1, n, lev, off);
+        }
+        if (L < li) {
+            int d = kv.digit(i, L + 1);
+#pragma unroll
+            for (int k = 0; k < 3; k++) c[k] = c[k] + s * ptg_octant_offset(d, k);
+            s = s * 0.5;
+        }
+    }
+}
+
+// particle slots: gather into Morton order (octree.py:577-603) and link to the next element
+__global__ __launch_bounds__(256) void k_particles(const double *pos, const double *m, const double *h,
+                                                   const uint32_t *order, const int32_t *lev, const int32_t *off,
+                                                   int64_t n, const ptg_tree_arrays t)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t o = order[i];
+    t.coordinates[3 * i] = pos[3 * o];
+    t.coordinates[3 * i + 1] = pos[3 * o + 1];
+    t.coordinates[3 * i + 2] = pos[3 * o + 2];
+    t.masses[i] = m[o];
+    t.softenings[i] = h[o];
+    t.sizes[i] = 0.0;
+    t.deltas[i] = 0.0;
+    t.treewalk_indices[i] = o;
+    t.first_subnode[i] = -1;
+    t.next_branch[i] = n == 1 ? -1 : head_of(i + 1, n, lev, off);
+    t.nchild[i] = 0;
+    t.child_base[i] = -1;
+    if (t.quadrupoles)
+        for (int k = 0; k < 9; k++) t.quadrupoles[9 * i + k] = 0.0;
+}
+
+__global__ void k_single_root(const Geom *g, const ptg_tree_arrays t)
+{
+    // N = 1: the root (element 1) holds the lone particle
+    for (int k = 0; k < 3; k++) t.coordinates[3 + k] = g->center[k];
+    t.sizes[1] = g->size;
+    t.first_subnode[1] = 0;
+    t.next_branch[1] = -1;
+}
+
+// ============================================================================================
+// links and moments
+// ============================================================================================
+struct NodeWS {
+    double *acc;        // (nnodes, 3) running sum of m*x, indexed by node - n
+    int32_t *parent;    // (nnodes)
+    int32_t *nodekids;  // (nnodes) number of node children
+    int32_t *arrive;    // (nnodes) arrival counters for the bottom-up pass
+    int32_t *err;       // [0] corrupt-chain flag
+    int32_t *bsum;
+};
+
+static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, NodeWS &w)
+{
+    w.acc = A.take<double>(3 * nnodes);
+    w.parent = A.take<int32_t>(nnodes);
+    w.nodekids = A.take<int32_t>(nnodes);
+    w.arrive = A.take<int32_t>(nnodes);
+    w.err = A.take<int32_t>(64);
+    w.bsum = A.take<int32_t>(scan_blocks(nnodes) + 64);
+    return A.ok();
+}
+
+extern "C" size_t ptg_node_workspace_bytes(int64_t n, int64_t nnodes, int quadrupole)
+{
+    (void)quadrupole;
+    PtgArena A(nullptr, 0);
+    NodeWS w;
+    carve_nodes(A, n, std::max<int64_t>(nnodes, 1), w);
+    return A.used;
+}
+
+// children of a node in link order; -1 if the chain is malformed (more than 8 links or an index
+// outside [0, ne)), so a build bug cannot turn into an out-of-bounds access
+__device__ __forceinline__ int node_children(const int32_t *first, const int32_t *next, int64_t idx, int64_t ne,
+                                             int32_t *kids)
+{
+    int32_t c = first[idx], stop = next[idx];
+    int nk = 0;
+    while (c != stop && c >= 0) {
+        if (nk == 8 || c >= ne) return -1;
+        kids[nk++] = c;
+        c = next[c];
+    }
+    return nk;
+}
+
+__global__ __launch_bounds__(256) void k_node_init(int64_t nnodes, NodeWS w)
+{
+    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnodes) return;
+    w.parent[a] = -1;
+    w.arrive[a] = 0;
+    w.nodekids[a] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_link(int64_t n, int64_t nnodes, const ptg_tree_arrays t, NodeWS w)
+{
+    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnodes) return;
+    int64_t idx = n + a;
+    int32_t kids[8];
+    int nk = node_children(t.first_subnode, t.next_branch, idx, n + nnodes, kids);
+    if (nk < 0) {
+        w.err[0] = 1;
+        nk = 0;
+    }
+    int nn = 0;
+    for (int q = 0; q < nk; q++)
+        if (kids[q] >= n) {
+            w.parent[kids[q] - n] = (int32_t)idx;
+            nn++;
+        }
+    t.nchild[idx] = nk;
+    w.nodekids[a] = nn;
+    t.masses[idx] = 0.0;
+    t.softenings[idx] = 0.0;
+    t.deltas[idx] = 0.0;
+    if (t.quadrupoles)
+        for (int k = 0; k < 9; k++) t.quadrupoles[9 * idx + k] = 0.0;
+}
+
+template <typename T> __device__ __forceinline__ T ld_agent(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One node's moments, in the reference's summation order: particle children in digit order
+// (folded during the split, octree.py:956-966), then node children in descending index order,
+// i.e. descending digit order (RollupMoments' reverse scan, octree.py:212-231).  Zero-mass node
+// children contribute nothing (:215-216).  Quadrupoles follow RollupQuadrupoles (:252-292).
+__device__ void node_moments(int64_t idx, int64_t n, int64_t nnodes, const ptg_tree_arrays &t, NodeWS &w, bool quad)
+{
+    int32_t kids[8];
+    int nk = node_children(t.first_subnode, t.next_branch, idx, n + nnodes, kids);
+    if (nk < 0) {
+        w.err[0] = 1;
+        return;
+    }
+    double m = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, hs = 0.0;
+    for (int q = 0; q < nk; q++) {
+        int32_t c = kids[q];
+        if (c >= n) continue;
+        double mi = t.masses[c];
+        m += mi;
+        a0 += mi * t.coordinates[3 * (int64_t)c];
+        a1 += mi * t.coordinates[3 * (int64_t)c + 1];
+        a2 += mi * t.coordinates[3 * (int64_t)c + 2];
+        double hc = t.softenings[c];
+        if (!(hs >= hc)) hs = hc;
+    }
+    for (int q = nk - 1; q >= 0; q--) {
+        int32_t c = kids[q];
+        if (c < n) continue;
+        double mc = ld_agent(t.masses + c);
+        if (mc == 0.0) continue;
+        int64_t ca = c - n;
+        a0 += ld_agent(w.acc + 3 * ca);
+        a1 += ld_agent(w.acc + 3 * ca + 1);
+        a2 += ld_agent(w.acc + 3 * ca + 2);
+        m += mc;
+        double hc = ld_agent(t.softenings + c);
+        if (!(hs >= hc)) hs = hc;
+    }
+    int64_t ia = idx - n;
+    w.acc[3 * ia] = a0;
+    w.acc[3 * ia + 1] = a1;
+    w.acc[3 * ia + 2] = a2;
+    t.masses[idx] = m;
+    t.softenings[idx] = hs;
+    double com0 = t.coordinates[3 * idx], com1 = t.coordinates[3 * idx + 1], com2 = t.coordinates[3 * idx + 2];
+    if (m != 0.0) {
+        double inv = 1.0 / m;
+        double c0 = a0 * inv, c1 = a1 * inv, c2 = a2 * inv;
+        double dx0 = c0 - com0, dx1 = c1 - com1, dx2 = c2 - com2;
+        double d2 = 0.0;
+        d2 += dx0 * dx0;
+        d2 += dx1 * dx1;
+        d2 += dx2 * dx2;
+        t.deltas[idx] = sqrt(d2);
+        com0 = c0;
+        com1 = c1;
+        com2 = c2;
+        t.coordinates[3 * idx] = c0;
+        t.coordinates[3 * idx + 1] = c1;
+        t.coordinates[3 * idx + 2] = c2;
+    }
+    if (!quad) return;
+    double Q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    auto shift = [&](double mm, double r0, double r1, double r2) {
+        double rsq = r0 * r0 + r1 * r1 + r2 * r2;
+        Q[0] += mm * (3.0 * r0 * r0 - rsq);
+        Q[1] += mm * 3.0 * r0 * r1;
+        Q[2] += mm * 3.0 * r0 * r2;
+        Q[3] += mm * 3.0 * r1 * r0;
+        Q[4] += mm * (3.0 * r1 * r1 - rsq);
+        Q[5] += mm * 3.0 * r1 * r2;
+        Q[6] += mm * 3.0 * r2 * r0;
+        Q[7] += mm * 3.0 * r2 * r1;
+        Q[8] += mm * (3.0 * r2 * r2 - rsq);
+    };
+    for (int q = 0; q < nk; q++) {
+        int32_t c = kids[q];
+        if (c >= n) continue;
+        shift(t.masses[c], t.coordinates[3 * (int64_t)c] - com0, t.coordinates[3 * (int64_t)c + 1] - com1,
+              t.coordinates[3 * (int64_t)c + 2] - com2);
+    }
+    for (int q = nk - 1; q >= 0; q--) {
+        int32_t c = kids[q];
+        if (c < n) continue;
+        double mc = ld_agent(t.masses + c);
+        if (mc == 0.0) continue;
+        for (int k = 0; k < 9; k++) Q[k] += ld_agent(t.quadrupoles + 9 * (int64_t)c + k);
+        shift(mc, ld_agent(t.coordinates + 3 * (int64_t)c) - com0, ld_agent(t.coordinates + 3 * (int64_t)c + 1) - com1,
+              ld_agent(t.coordinates + 3 * (int64_t)c + 2) - com2);
+    }
+    for (int k = 0; k < 9; k++) t.quadrupoles[9 * idx + k] = Q[k];
+}
+
+// bottom-up pass: a thread starts at every node without node children and climbs; at each
+// parent only the last-arriving child continues, so every node is computed exactly once and after
+// all of its children (no FP64 atomics: sums are in a fixed order, hence deterministic).
+__global__ __launch_bounds__(256) void k_moments(int64_t n, int64_t nnodes, const ptg_tree_arrays t, NodeWS w,
+                                                 int quad)
+{
+    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnodes || w.nodekids[a] != 0) return;
+    int64_t idx = n + a;
+    for (int64_t step = 0; step < nnodes; step++) {
+        node_moments(idx, n, nnodes, t, w, quad != 0);
+        int32_t p = w.parent[idx - n];
+        if (p < n || p >= n + nnodes) break;
+        __threadfence();
+        int old = atomicAdd(&w.arrive[p - n], 1);
+        if (old != w.nodekids[p - n] - 1) break;
+        __threadfence();
+        idx = p;
+    }
+}
+
+// compute_moments=False: the split has still folded particle children into their parent's mass
+// and softening (octree.py:956-966); nothing else is rolled up.
+__global__ __launch_bounds__(256) void k_fold_particles(int64_t n, int64_t nnodes, const ptg_tree_arrays t)
+{
+    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnodes) return;
+    int64_t idx = n + a;
+    int32_t kids[8];
+    int nk = node_children(t.first_subnode, t.next_branch, idx, n + nnodes, kids);
+    double m = 0.0, hs = 0.0;
+    for (int q = 0; q < nk; q++) {
+        int32_t c = kids[q];
+        if (c >= n) continue;
+        m += t.masses[c];
+        if (!(hs >= t.softenings[c])) hs = t.softenings[c];
+    }
+    t.masses[idx] = m;
+    t.softenings[idx] = hs;
+}
+
+struct LoadNChild {
+    const int32_t *nchild;
+    int64_t n;
+    __device__ int32_t operator()(int64_t a) const { return nchild[n + a]; }
+};
+
+__global__ __launch_bounds__(256) void k_child_base_fix(int64_t n, int64_t nnodes, int32_t *child_base)
+{
+    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < nnodes) child_base[n + a] += 1;  // slot 0 holds the root
+}
+
+extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const double *h, const ptg_build_info *info,
+                                     int quadrupole, int compute_moments, void *ws, size_t ws_bytes, void *node_ws,
+                                     size_t node_ws_bytes, const ptg_tree_arrays *tree, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!pos || !m || !h || !info || !tree || info->n < 1) {
+        ptg_set_error("ptg_build_tree: bad arguments");
+        return PTG_ERR_ARG;
+    }
+    const ptg_tree_arrays t = *tree;
+    if (!t.coordinates || !t.masses || !t.softenings || !t.sizes || !t.deltas || !t.first_subnode ||
+        !t.next_branch || !t.treewalk_indices || !t.nchild || !t.child_base || (quadrupole && !t.quadrupoles)) {
+        ptg_set_error("ptg_build_tree: a required tree array is NULL");
+        return PTG_ERR_ARG;
+    }
+    int64_t n = info->n, nnodes = info->nnodes;
+    PtgArena A(ws, ws_bytes);
+    BuildWS w;
+    if (!ws || !carve_build(A, n, w)) {
+        ptg_set_error("ptg_build_tree: build workspace too small");
+        return PTG_ERR_WORKSPACE;
+    }
+    PtgArena B(node_ws, node_ws_bytes);
+    NodeWS nw;
+    if (!node_ws || !carve_nodes(B, n, nnodes, nw)) {
+        ptg_set_error("ptg_build_tree: node workspace too small (%zu < %zu)", node_ws_bytes,
+                      ptg_node_workspace_bytes(n, nnodes, quadrupole));
+        return PTG_ERR_WORKSPACE;
+    }
+    KeyView kv{w.kA, info->has_collisions ? w.ext : nullptr, n};
+    k_particles<<<grid1(n), 256, 0, s>>>(pos, m, h, w.vA, w.lev, w.off, n, t);
+    if (n == 1) k_single_root<<<1, 1, 0, s>>>(w.geom, t);
+    else k_emit_nodes<<<grid1(n - 1), 256, 0, s>>>(kv, w.lev, w.off, n, w.geom, t.coordinates, t.sizes,
+                                                   t.first_subnode, t.next_branch);
+    PTG_HIP_CHECK(hipMemsetAsync(nw.err, 0, 64 * sizeof(int32_t), s));
+    k_node_init<<<grid1(nnodes), 256, 0, s>>>(nnodes, nw);
+    k_link<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t, nw);
+    PTG_LAUNCH_CHECK();
+    if (compute_moments) k_moments<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t, nw, quadrupole);
+    else k_fold_particles<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t);
+    PTG_LAUNCH_CHECK();
+    // sibling-contiguous walk slots: exclusive scan of child counts in node order
+    ptg_status st = excl_scan(LoadNChild{t.nchild, n}, nnodes, t.child_base + n, nw.bsum, nullptr, s);
+    if (st) return st;
+    k_child_base_fix<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t.child_base);
+    PTG_LAUNCH_CHECK();
+    return PTG_OK;
+}
+
+// ============================================================================================
+// walk pack
+// ============================================================================================
+template <typename T>
+__device__ void pack_slot(int64_t e, int64_t slot, int64_t n, const ptg_tree_arrays &t, const ptg_walk_tree &wt)
+{
+    T *xyzm = (T *)wt.xyzm + 4 * slot;
+    T *aux = (T *)wt.aux + 4 * slot;
+    xyzm[0] = (T)(t.coordinates[3 * e] - wt.origin[0]);
+    xyzm[1] = (T)(t.coordinates[3 * e + 1] - wt.origin[1]);
+    xyzm[2] = (T)(t.coordinates[3 * e + 2] - wt.origin[2]);
+    xyzm[3] = (T)t.masses[e];
+    aux[0] = (T)t.softenings[e];
+    aux[1] = (T)t.sizes[e];
+    aux[2] = (T)t.deltas[e];
+    aux[3] = (T)0;
+    bool node = e >= n;
+    wt.kids[2 * slot] = node ? t.child_base[e] : (int32_t)e;
+    wt.kids[2 * slot + 1] = node ? t.nchild[e] : 0;
+    if (wt.quad) {
+        T *q = (T *)wt.quad + 8 * slot;
+        if (node && t.quadrupoles) {
+            const double *Q = t.quadrupoles + 9 * e;
+            q[0] = (T)Q[0]; q[1] = (T)Q[1]; q[2] = (T)Q[2];
+            q[3] = (T)Q[4]; q[4] = (T)Q[5]; q[5] = (T)Q[8];
+        } else {
+            for (int k = 0; k < 6; k++) q[k] = (T)0;
+        }
+        q[6] = (T)0;
+        q[7] = (T)0;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_pack(int64_t n, int64_t nnodes, const ptg_tree_arrays t, const ptg_walk_tree wt)
+{
+    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnodes) return;
+    int64_t idx = n + a;
+    if (a == 0) pack_slot<T>(idx, 0, n, t, wt);
+    int32_t kids[8];
+    int nk = node_children(t.first_subnode, t.next_branch, idx, n + nnodes, kids);
+    int64_t base = t.child_base[idx];
+    for (int q = 0; q < nk; q++)
+        if (base + q > 0 && base + q < wt.nslots) pack_slot<T>(kids[q], base + q, n, t, wt);
+}
+
+extern "C" ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_walk_tree *wt,
+                                    void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!tree || !wt || !wt->xyzm || !wt->aux || !wt->kids || n < 1 || nnodes < 1 || wt->nslots != n + nnodes) {
+        ptg_set_error("ptg_pack_walk: bad arguments");
+        return PTG_ERR_ARG;
+    }
+    if (wt->quad && !tree->quadrupoles) {
+        ptg_set_error("ptg_pack_walk: a quadrupole pack needs a tree with quadrupoles");
+        return PTG_ERR_ARG;
+    }
+    if (wt->prec == PTG_F32) k_pack<float><<<grid1(nnodes), 256, 0, s>>>(n, nnodes, *tree, *wt);
+    else if (wt->prec == PTG_F64) k_pack<double><<<grid1(nnodes), 256, 0, s>>>(n, nnodes, *tree, *wt);
+    else {
+        ptg_set_error("ptg_pack_walk: prec must be PTG_F32 or PTG_F64");
+        return PTG_ERR_ARG;
+    }
+    PTG_LAUNCH_CHECK();
+    return PTG_OK;
+}

@@ -1,0 +1,689 @@
+// Grouped Barnes-Hut walk for gfx950: one wavefront walks the tree for one group of Morton-
+// consecutive targets (grouped_treewalk.py:163-241).  Traversal is breadth-ordered over a stack in
+// LDS: each step pops up to 64 elements (one per lane), evaluates the opening test against the
+// group's bounding box in parallel, appends accepted elements to LDS interaction lists with
+// ballot/popcount compaction and pushes the children of opened nodes with a wave prefix sum.
+// When a list holds 64 entries the wave evaluates them against its targets, every lane reading the
+// same LDS entry (broadcast).  Persistent waves take groups from per-XCD counters.
+#include <algorithm>
+
+#include "ptg_common.hpp"
+
+#define WALK_WAVES 4
+#define SCAP 1024      // LDS traversal stack entries per wave
+#define SPILL 512      // entries moved to / from global memory at a time
+#define GCAP 16384     // global spill entries per wave
+#define ICAP 128       // interaction-list entries per list and wave
+#define NXCD 8
+#define CTR_STRIDE 32  // int32 words between per-XCD counters (128 B)
+
+template <typename T> struct alignas(4 * sizeof(T)) V4 {
+    T x, y, z, w;
+};
+
+template <typename T, bool QUAD> struct WaveLds {
+    int stack[SCAP];
+    V4<T> nodes[ICAP];  // accepted nodes: x y z m
+    T nh[ICAP];         // node softening (used by the exact path)
+    T nq[QUAD ? 6 * ICAP : 1];
+    V4<T> parts[ICAP];  // particles: x y z m
+    T ph[ICAP];
+};
+
+struct WalkCtl {
+    int32_t ctr[NXCD * CTR_STRIDE];  // per-XCD group counters
+    int32_t err;
+    int32_t pad[31];
+};
+
+template <typename T> struct WalkArgs {
+    const V4<T> *xyzm;
+    const V4<T> *aux;
+    const int2 *kids;
+    const T *quad;  // 8 per slot
+    const double *pos;
+    const double *soft;
+    const int64_t *perm;
+    int64_t nslots;
+    int64_t nt;
+    int64_t ngroups;
+    int gs;
+    T theta;
+    double G;
+    double ox, oy, oz;
+    double *out;
+    int64_t *counters;
+    WalkCtl *ctl;
+    int *spill;
+};
+
+// ---- softening kernels (kernel.py:4-75), generic precision ----
+template <typename T> __device__ __forceinline__ T force_kernel(T r, T h)
+{
+    if (r > h) return T(1) / (r * r * r);
+    T hinv = T(1) / h;
+    T q = r * hinv;
+    if (q <= T(0.5)) return (T(10.666666666666666666) + q * q * (T(-38.4) + T(32.0) * q)) * hinv * hinv * hinv;
+    return (T(21.333333333333) - T(48.0) * q + T(38.4) * q * q - T(10.666666666667) * q * q * q -
+            T(0.066666666667) / (q * q * q)) *
+           hinv * hinv * hinv;
+}
+
+template <typename T> __device__ __forceinline__ T tidal_kernel(T r, T h)
+{
+    if (r > h) {
+        T r2 = r * r;
+        return T(3) / (r2 * r2 * r);
+    }
+    T hinv = T(1) / h;
+    T hinv2 = hinv * hinv;
+    T hinv5 = hinv2 * hinv2 * hinv;
+    T q = r * hinv;
+    if (q <= T(0.5)) return (T(76.8) - T(96.0) * q) * hinv5;
+    T qinv = T(1) / q;
+    T qinv3 = qinv * qinv * qinv;
+    return (T(48.0) * qinv - T(76.8) + T(32.0) * q - T(0.2) * qinv3 * qinv * qinv) * hinv5;
+}
+
+template <typename T> __device__ __forceinline__ T potential_kernel(T r, T h)
+{
+    if (h == T(0)) return T(-1) / r;
+    T hinv = T(1) / h;
+    T q = r * hinv;
+    if (q <= T(0.5)) return (T(-2.8) + q * q * (T(5.33333333333333333) + q * q * (T(6.4) * q - T(9.6)))) * hinv;
+    if (q <= T(1))
+        return (T(-3.2) + T(0.066666666666666666666) / q +
+                q * q * (T(10.666666666666666666666) + q * (T(-16.0) + q * (T(9.6) - T(2.1333333333333333333333) * q)))) *
+               hinv;
+    return T(-1) / r;
+}
+
+__device__ __forceinline__ float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double fast_rsqrt(double x) { return 1.0 / sqrt(x); }
+
+template <int FIELDS> struct NW {
+    static constexpr int POT = (FIELDS & PTG_POT) ? 1 : 0;
+    static constexpr int ACC = (FIELDS & PTG_ACC) ? 1 : 0;
+    static constexpr int TID = (FIELDS & PTG_TIDAL) ? 1 : 0;
+    static constexpr int W = POT + 3 * ACC + 9 * TID;
+};
+
+// per-target accumulators: phi, g(3), T(6 independent)
+template <typename T> struct Acc {
+    T phi, gx, gy, gz, txx, tyy, tzz, txy, txz, tyz;
+};
+
+// One source element on one target, with the reference's branches (grouped_treewalk.py:75-153):
+// r2 > 0 self skip, softened iff r < max(h_s, h_t), quadrupole terms for nodes only.
+template <typename T, int FIELDS, bool QUADTERM>
+__device__ __forceinline__ void interact_exact(Acc<T> &a, T sx, T sy, T sz, T M, T hs, const T *q, T tx, T ty, T tz,
+                                               T th)
+{
+    constexpr bool P = NW<FIELDS>::POT, G = NW<FIELDS>::ACC, D = NW<FIELDS>::TID;
+    T dx = sx - tx, dy = sy - ty, dz = sz - tz;
+    T r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 > T(0))) return;
+    T r = sqrt(r2);
+    T ht = hs > th ? hs : th;
+    bool softened = r < ht;
+    T phi = 0, K = 0, Q = 0;
+    if (P) phi = softened ? M * potential_kernel(r, ht) : -M / r;
+    if (G || D) K = softened ? M * force_kernel(r, ht) : M / (r * r2);
+    if (D) Q = softened ? M * tidal_kernel(r, ht) : T(3) * K / r2;
+    T gx = 0, gy = 0, gz = 0;
+    if (G) {
+        gx = K * dx;
+        gy = K * dy;
+        gz = K * dz;
+    }
+    T txx = 0, tyy = 0, tzz = 0, txy = 0, txz = 0, tyz = 0;
+    if (D) {
+        txx = Q * dx * dx - K;
+        tyy = Q * dy * dy - K;
+        tzz = Q * dz * dz - K;
+        txy = Q * dx * dy;
+        txz = Q * dx * dz;
+        tyz = Q * dy * dz;
+    }
+    if (QUADTERM) {
+        T qxx = q[0], qxy = q[1], qxz = q[2], qyy = q[3], qyz = q[4], qzz = q[5];
+        T r5inv = T(1) / (r2 * r2 * r);
+        T r7inv = r5inv / r2;
+        T qdx = qxx * dx + qxy * dy + qxz * dz;
+        T qdy = qxy * dx + qyy * dy + qyz * dz;
+        T qdz = qxz * dx + qyz * dy + qzz * dz;
+        T A = dx * qdx + dy * qdy + dz * qdz;
+        if (P) phi += T(-0.5) * A * r5inv;
+        if (G) {
+            T quad_fac = A * r7inv;
+            gx += T(2.5) * quad_fac * dx - qdx * r5inv;
+            gy += T(2.5) * quad_fac * dy - qdy * r5inv;
+            gz += T(2.5) * quad_fac * dz - qdz * r5inv;
+        }
+        if (D) {
+            T diag = T(-2.5) * A * r7inv;
+            T fac = T(17.5) * A * r7inv / r2;
+            txx += qxx * r5inv - T(10.0) * dx * qdx * r7inv + diag + fac * dx * dx;
+            tyy += qyy * r5inv - T(10.0) * dy * qdy * r7inv + diag + fac * dy * dy;
+            tzz += qzz * r5inv - T(10.0) * dz * qdz * r7inv + diag + fac * dz * dz;
+            txy += qxy * r5inv - T(5.0) * (dx * qdy + dy * qdx) * r7inv + fac * dx * dy;
+            txz += qxz * r5inv - T(5.0) * (dx * qdz + dz * qdx) * r7inv + fac * dx * dz;
+            tyz += qyz * r5inv - T(5.0) * (dy * qdz + dz * qdy) * r7inv + fac * dy * dz;
+        }
+    }
+    a.phi += phi;
+    a.gx += gx;
+    a.gy += gy;
+    a.gz += gz;
+    a.txx += txx;
+    a.tyy += tyy;
+    a.tzz += tzz;
+    a.txy += txy;
+    a.txz += txz;
+    a.tyz += tyz;
+}
+
+// Accepted node on one target, fast path: the acceptance test guarantees r > h_t, h_s and r > 0.
+template <typename T, int FIELDS, bool QUAD>
+__device__ __forceinline__ void interact_node_fast(Acc<T> &a, V4<T> s, const T *q, T tx, T ty, T tz)
+{
+    constexpr bool P = NW<FIELDS>::POT, G = NW<FIELDS>::ACC, D = NW<FIELDS>::TID;
+    T dx = s.x - tx, dy = s.y - ty, dz = s.z - tz;
+    T r2 = dx * dx + dy * dy + dz * dz;
+    T rinv = fast_rsqrt(r2);
+    T rinv2 = rinv * rinv;
+    T mr = s.w * rinv;
+    if (P) a.phi -= mr;
+    T K = mr * rinv2;
+    if (G) {
+        a.gx += K * dx;
+        a.gy += K * dy;
+        a.gz += K * dz;
+    }
+    if (D) {
+        T Q = T(3) * K * rinv2;
+        a.txx += Q * dx * dx - K;
+        a.tyy += Q * dy * dy - K;
+        a.tzz += Q * dz * dz - K;
+        a.txy += Q * dx * dy;
+        a.txz += Q * dx * dz;
+        a.tyz += Q * dy * dz;
+    }
+    if (QUAD) {
+        T qxx = q[0], qxy = q[1], qxz = q[2], qyy = q[3], qyz = q[4], qzz = q[5];
+        T r5inv = rinv2 * rinv2 * rinv;
+        T r7inv = r5inv * rinv2;
+        T qdx = qxx * dx + qxy * dy + qxz * dz;
+        T qdy = qxy * dx + qyy * dy + qyz * dz;
+        T qdz = qxz * dx + qyz * dy + qzz * dz;
+        T A = dx * qdx + dy * qdy + dz * qdz;
+        if (P) a.phi += T(-0.5) * A * r5inv;
+        if (G) {
+            T f = T(2.5) * A * r7inv;
+            a.gx += f * dx - qdx * r5inv;
+            a.gy += f * dy - qdy * r5inv;
+            a.gz += f * dz - qdz * r5inv;
+        }
+        if (D) {
+            T diag = T(-2.5) * A * r7inv;
+            T fac = T(17.5) * A * r7inv * rinv2;
+            a.txx += qxx * r5inv - T(10.0) * dx * qdx * r7inv + diag + fac * dx * dx;
+            a.tyy += qyy * r5inv - T(10.0) * dy * qdy * r7inv + diag + fac * dy * dy;
+            a.tzz += qzz * r5inv - T(10.0) * dz * qdz * r7inv + diag + fac * dz * dz;
+            a.txy += qxy * r5inv - T(5.0) * (dx * qdy + dy * qdx) * r7inv + fac * dx * dy;
+            a.txz += qxz * r5inv - T(5.0) * (dx * qdz + dz * qdx) * r7inv + fac * dx * dz;
+            a.tyz += qyz * r5inv - T(5.0) * (dy * qdz + dz * qdy) * r7inv + fac * dy * dz;
+        }
+    }
+}
+
+// Particle on one target, fast path: point mass unless any lane is inside the softening radius,
+// in which case those lanes evaluate the spline (wave-uniform branch); r2 == 0 contributes zero.
+template <typename T, int FIELDS>
+__device__ __forceinline__ void interact_part_fast(Acc<T> &a, V4<T> s, T hs, T tx, T ty, T tz, T th)
+{
+    constexpr bool P = NW<FIELDS>::POT, G = NW<FIELDS>::ACC, D = NW<FIELDS>::TID;
+    T dx = s.x - tx, dy = s.y - ty, dz = s.z - tz;
+    T r2 = dx * dx + dy * dy + dz * dz;
+    T ht = hs > th ? hs : th;
+    bool nonzero = r2 > T(0);
+    bool soft = nonzero && r2 < ht * ht;
+    T rinv = nonzero ? fast_rsqrt(r2) : T(0);
+    T rinv2 = rinv * rinv;
+    T mr = s.w * rinv;
+    T phi = -mr;
+    T K = mr * rinv2;
+    T Q = T(3) * K * rinv2;
+    if (ptg_ballot(soft)) {
+        if (soft) {
+            T r = r2 * rinv;
+            if (P) phi = s.w * potential_kernel(r, ht);
+            K = s.w * force_kernel(r, ht);
+            if (D) Q = s.w * tidal_kernel(r, ht);
+        }
+    }
+    if (P) a.phi += phi;
+    if (G) {
+        a.gx += K * dx;
+        a.gy += K * dy;
+        a.gz += K * dz;
+    }
+    if (D) {
+        a.txx += Q * dx * dx - K;
+        a.tyy += Q * dy * dy - K;
+        a.tzz += Q * dz * dz - K;
+        a.txy += Q * dx * dy;
+        a.txz += Q * dx * dz;
+        a.tyz += Q * dy * dz;
+    }
+}
+
+template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT>
+__device__ __forceinline__ void flush_nodes(const WaveLds<T, QUAD> &L, int cnt, Acc<T> (&acc)[TPL],
+                                            const T (&tx)[TPL], const T (&ty)[TPL], const T (&tz)[TPL],
+                                            const T (&th)[TPL])
+{
+    for (int k = 0; k < cnt; k++) {
+        V4<T> s = L.nodes[k];
+        T q[6];
+        if (QUAD) {
+#pragma unroll
+            for (int c = 0; c < 6; c++) q[c] = L.nq[6 * k + c];
+        }
+#pragma unroll
+        for (int j = 0; j < TPL; j++) {
+            if (EXACT) interact_exact<T, FIELDS, QUAD>(acc[j], s.x, s.y, s.z, s.w, L.nh[k], q, tx[j], ty[j], tz[j], th[j]);
+            else interact_node_fast<T, FIELDS, QUAD>(acc[j], s, q, tx[j], ty[j], tz[j]);
+        }
+    }
+}
+
+template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT>
+__device__ __forceinline__ void flush_parts(const WaveLds<T, QUAD> &L, int cnt, Acc<T> (&acc)[TPL],
+                                            const T (&tx)[TPL], const T (&ty)[TPL], const T (&tz)[TPL],
+                                            const T (&th)[TPL])
+{
+    for (int k = 0; k < cnt; k++) {
+        V4<T> s = L.parts[k];
+        T hs = L.ph[k];
+#pragma unroll
+        for (int j = 0; j < TPL; j++) {
+            if (EXACT) interact_exact<T, FIELDS, false>(acc[j], s.x, s.y, s.z, s.w, hs, nullptr, tx[j], ty[j], tz[j], th[j]);
+            else interact_part_fast<T, FIELDS>(acc[j], s, hs, tx[j], ty[j], tz[j], th[j]);
+        }
+    }
+}
+
+template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT>
+__global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
+{
+    __shared__ WaveLds<T, QUAD> lds[WALK_WAVES];
+    const int wave = threadIdx.x >> 6, lane = ptg_lane();
+    WaveLds<T, QUAD> &L = lds[wave];
+    const uint64_t lt = ptg_lanemask_lt();
+    const int64_t gwave = (int64_t)blockIdx.x * WALK_WAVES + wave;
+    int *spill = A.spill + gwave * GCAP;
+    const int xcd = blockIdx.x % NXCD;
+    int64_t n_int = 0, n_vis = 0;
+
+    for (int attempt = 0; attempt < NXCD; attempt++) {
+        const int x = (xcd + attempt) % NXCD;
+        const int64_t g_lo = A.ngroups * x / NXCD, g_hi = A.ngroups * (x + 1) / NXCD;
+        for (;;) {
+            int gi = 0;
+            if (lane == 0) gi = atomicAdd(&A.ctl->ctr[x * CTR_STRIDE], 1);
+            gi = __builtin_amdgcn_readfirstlane(gi);
+            const int64_t g = g_lo + gi;
+            if (g >= g_hi) break;
+            const int64_t a0 = g * A.gs;
+            const int64_t b0 = std::min<int64_t>(a0 + A.gs, A.nt);
+            const int mt = (int)(b0 - a0);
+
+            // ---- targets: lane holds a0 + lane + 64 j ----
+            T tx[TPL], ty[TPL], tz[TPL], th[TPL];
+            int64_t tidx[TPL];
+            bool tval[TPL];
+            T bmin0 = INFINITY, bmin1 = INFINITY, bmin2 = INFINITY;
+            T bmax0 = -INFINITY, bmax1 = -INFINITY, bmax2 = -INFINITY, hmax = -INFINITY;
+#pragma unroll
+            for (int j = 0; j < TPL; j++) {
+                const int o = lane + 64 * j;
+                tval[j] = o < mt;
+                tidx[j] = tval[j] ? A.perm[a0 + o] : 0;
+                if (tval[j]) {
+                    tx[j] = (T)(A.pos[3 * tidx[j]] - A.ox);
+                    ty[j] = (T)(A.pos[3 * tidx[j] + 1] - A.oy);
+                    tz[j] = (T)(A.pos[3 * tidx[j] + 2] - A.oz);
+                    th[j] = (T)A.soft[tidx[j]];
+                    bmin0 = std::min(bmin0, tx[j]);
+                    bmin1 = std::min(bmin1, ty[j]);
+                    bmin2 = std::min(bmin2, tz[j]);
+                    bmax0 = std::max(bmax0, tx[j]);
+                    bmax1 = std::max(bmax1, ty[j]);
+                    bmax2 = std::max(bmax2, tz[j]);
+                    hmax = std::max(hmax, th[j]);
+                } else {
+                    tx[j] = ty[j] = tz[j] = th[j] = T(0);
+                }
+            }
+            bmin0 = ptg_wave_min(bmin0);
+            bmin1 = ptg_wave_min(bmin1);
+            bmin2 = ptg_wave_min(bmin2);
+            bmax0 = ptg_wave_max(bmax0);
+            bmax1 = ptg_wave_max(bmax1);
+            bmax2 = ptg_wave_max(bmax2);
+            hmax = ptg_wave_max(hmax);
+            // invalid lanes sit at the box corner so they never divide by zero
+#pragma unroll
+            for (int j = 0; j < TPL; j++)
+                if (!tval[j]) {
+                    tx[j] = bmin0 - T(1);
+                    ty[j] = bmin1 - T(1);
+                    tz[j] = bmin2 - T(1);
+                }
+
+            Acc<T> acc[TPL];
+#pragma unroll
+            for (int j = 0; j < TPL; j++) acc[j] = Acc<T>{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+            int sp = 1, gsp = 0, nn = 0, np = 0;
+            bool overflow = false;
+            if (lane == 0) L.stack[0] = 0;  // the root occupies walk slot 0
+            __builtin_amdgcn_wave_barrier();
+
+            for (;;) {
+                if (sp == 0) {
+                    if (gsp == 0) break;
+                    const int k = std::min(gsp, SPILL);
+                    for (int q = lane; q < k; q += 64) L.stack[q] = spill[gsp - k + q];
+                    __builtin_amdgcn_wave_barrier();
+                    gsp -= k;
+                    sp = k;
+                }
+                const int k = std::min(sp, 64);
+                const int base = sp - k;
+                int e = lane < k ? L.stack[base + lane] : 0;
+                const bool ve = lane < k && e >= 0 && e < A.nslots;  // a corrupt link is dropped, not followed
+                if (!ve) e = 0;
+                __builtin_amdgcn_wave_barrier();
+                sp = base;
+                n_vis += k;
+
+                const V4<T> X = A.xyzm[e];
+                const V4<T> X2 = A.aux[e];
+                const int2 K = A.kids[e];
+                const bool ispart = K.y == 0;
+                T dxm = X.x < bmin0 ? bmin0 - X.x : (X.x > bmax0 ? X.x - bmax0 : T(0));
+                T dym = X.y < bmin1 ? bmin1 - X.y : (X.y > bmax1 ? X.y - bmax1 : T(0));
+                T dzm = X.z < bmin2 ? bmin2 - X.z : (X.z > bmax2 ? X.z - bmax2 : T(0));
+                T r_min = sqrt(dxm * dxm + dym * dym + dzm * dzm);
+                T hh = X2.x > hmax ? X2.x : hmax;
+                T c1 = X2.y / A.theta + X2.z;
+                T c2 = hh + X2.y * T(0.6) + X2.z;
+                bool acc_node = ve && !ispart && r_min > (c1 > c2 ? c1 : c2);
+                bool acc_part = ve && ispart;
+                bool open = ve && !ispart && !acc_node;
+
+                const uint64_t bp = ptg_ballot(acc_part), bn = ptg_ballot(acc_node);
+                if (acc_part) {
+                    const int at = np + (int)__popcll(bp & lt);
+                    L.parts[at] = X;
+                    L.ph[at] = X2.x;
+                }
+                if (acc_node) {
+                    const int at = nn + (int)__popcll(bn & lt);
+                    L.nodes[at] = X;
+                    L.nh[at] = X2.x;
+                    if (QUAD) {
+                        const T *q = A.quad + 8 * (int64_t)e;
+#pragma unroll
+                        for (int c = 0; c < 6; c++) L.nq[6 * at + c] = q[c];
+                    }
+                }
+                const int addp = (int)__popcll(bp), addn = (int)__popcll(bn);
+                np += addp;
+                nn += addn;
+                n_int += (int64_t)(addp + addn) * mt;
+
+                // push the children of opened nodes
+                const int cnt = open ? std::min(std::max(K.y, 0), 8) : 0;
+                const int incl = ptg_wave_incl_scan(cnt);
+                const int tot = __shfl(incl, 63, 64);
+                if (sp + tot > SCAP) {
+                    // move the bottom SPILL entries of the LDS stack to this wave's global region
+                    if (gsp + SPILL > GCAP) {
+                        overflow = true;
+                        break;
+                    }
+                    for (int q = lane; q < SPILL; q += 64) spill[gsp + q] = L.stack[q];
+                    __builtin_amdgcn_wave_barrier();
+                    for (int q = 0; q < sp - SPILL; q += 64) {
+                        int v = (q + lane < sp - SPILL) ? L.stack[SPILL + q + lane] : 0;
+                        __builtin_amdgcn_wave_barrier();
+                        if (q + lane < sp - SPILL) L.stack[q + lane] = v;
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                    gsp += SPILL;
+                    sp -= SPILL;
+                }
+                {
+                    const int at = sp + incl - cnt;
+                    for (int c = 0; c < cnt; c++) L.stack[at + c] = K.x + c;
+                }
+                sp += tot;
+                __builtin_amdgcn_wave_barrier();
+
+                if (nn >= 64) {
+                    flush_nodes<T, TPL, FIELDS, QUAD, EXACT>(L, 64, acc, tx, ty, tz, th);
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane < nn - 64) {
+                        V4<T> v = L.nodes[64 + lane];
+                        T hv = L.nh[64 + lane];
+                        T qv[6];
+                        if (QUAD)
+                            for (int c = 0; c < 6; c++) qv[c] = L.nq[6 * (64 + lane) + c];
+                        __builtin_amdgcn_wave_barrier();
+                        L.nodes[lane] = v;
+                        L.nh[lane] = hv;
+                        if (QUAD)
+                            for (int c = 0; c < 6; c++) L.nq[6 * lane + c] = qv[c];
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    nn -= 64;
+                }
+                if (np >= 64) {
+                    flush_parts<T, TPL, FIELDS, QUAD, EXACT>(L, 64, acc, tx, ty, tz, th);
+                    __builtin_amdgcn_wave_barrier();
+                    if (lane < np - 64) {
+                        V4<T> v = L.parts[64 + lane];
+                        T hv = L.ph[64 + lane];
+                        __builtin_amdgcn_wave_barrier();
+                        L.parts[lane] = v;
+                        L.ph[lane] = hv;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    np -= 64;
+                }
+            }
+            if (overflow) {
+                if (lane == 0) atomicOr(&A.ctl->err, 1);
+                continue;
+            }
+            flush_nodes<T, TPL, FIELDS, QUAD, EXACT>(L, nn, acc, tx, ty, tz, th);
+            flush_parts<T, TPL, FIELDS, QUAD, EXACT>(L, np, acc, tx, ty, tz, th);
+            __builtin_amdgcn_wave_barrier();
+
+            // ---- write G * fields in the caller's order ----
+            constexpr int W = NW<FIELDS>::W;
+#pragma unroll
+            for (int j = 0; j < TPL; j++) {
+                if (!tval[j]) continue;
+                double *o = A.out + tidx[j] * W;
+                int c = 0;
+                if (NW<FIELDS>::POT) o[c++] = A.G * (double)acc[j].phi;
+                if (NW<FIELDS>::ACC) {
+                    o[c++] = A.G * (double)acc[j].gx;
+                    o[c++] = A.G * (double)acc[j].gy;
+                    o[c++] = A.G * (double)acc[j].gz;
+                }
+                if (NW<FIELDS>::TID) {
+                    double xx = A.G * (double)acc[j].txx, yy = A.G * (double)acc[j].tyy, zz = A.G * (double)acc[j].tzz;
+                    double xy = A.G * (double)acc[j].txy, xz = A.G * (double)acc[j].txz, yz = A.G * (double)acc[j].tyz;
+                    o[c++] = xx; o[c++] = xy; o[c++] = xz;
+                    o[c++] = xy; o[c++] = yy; o[c++] = yz;
+                    o[c++] = xz; o[c++] = yz; o[c++] = zz;
+                }
+            }
+        }
+    }
+    if (lane == 0 && A.counters) {
+        atomicAdd((unsigned long long *)&A.counters[0], (unsigned long long)n_int);
+        atomicAdd((unsigned long long *)&A.counters[1], (unsigned long long)n_vis);
+    }
+}
+
+__global__ void k_walk_init(WalkCtl *ctl)
+{
+    for (int i = threadIdx.x; i < (int)(sizeof(WalkCtl) / 4); i += blockDim.x) ((int32_t *)ctl)[i] = 0;
+}
+
+// ============================================================================================
+// host side
+// ============================================================================================
+static int g_num_cus = 0;
+
+static int num_cus()
+{
+    if (g_num_cus == 0) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess) g_num_cus = p.multiProcessorCount;
+        if (g_num_cus <= 0) g_num_cus = 256;
+    }
+    return g_num_cus;
+}
+
+#define WALK_MAX_BLOCKS_PER_CU 4
+
+// persistent grid: resident blocks per CU from the occupancy API, capped by the spill allocation
+static int64_t walk_grid_blocks() { return (int64_t)num_cus() * WALK_MAX_BLOCKS_PER_CU; }
+
+extern "C" size_t ptg_walk_workspace_bytes(int64_t nt, int group_size)
+{
+    (void)nt;
+    (void)group_size;
+    size_t waves = (size_t)walk_grid_blocks() * WALK_WAVES;
+    return ptg_align256(sizeof(WalkCtl)) + ptg_align256(waves * GCAP * sizeof(int));
+}
+
+template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT>
+static ptg_status launch_t(const WalkArgs<T> &a, int64_t max_blocks, hipStream_t s)
+{
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_walk<T, TPL, FIELDS, QUAD, EXACT>, 64 * WALK_WAVES, 0) !=
+                hipSuccess || nb < 1)
+            nb = 1;
+        per_cu = std::min(nb, WALK_MAX_BLOCKS_PER_CU);
+    }
+    int64_t blocks = std::min<int64_t>(max_blocks, (int64_t)num_cus() * per_cu);
+    k_walk<T, TPL, FIELDS, QUAD, EXACT><<<(unsigned)blocks, 64 * WALK_WAVES, 0, s>>>(a);
+    PTG_LAUNCH_CHECK();
+    return PTG_OK;
+}
+
+template <typename T, int TPL, bool EXACT>
+static ptg_status launch_fq(const WalkArgs<T> &a, int fields, bool quad, int64_t blocks, hipStream_t s)
+{
+#define PTG_F(F)                                                              \
+    case F:                                                                   \
+        return quad ? launch_t<T, TPL, F, true, EXACT>(a, blocks, s)          \
+                    : launch_t<T, TPL, F, false, EXACT>(a, blocks, s);
+    switch (fields) {
+        PTG_F(1) PTG_F(2) PTG_F(3) PTG_F(4) PTG_F(5) PTG_F(6) PTG_F(7)
+    }
+#undef PTG_F
+    ptg_set_error("ptg_walk: fields must be a nonzero combination of PTG_POT|PTG_ACC|PTG_TIDAL");
+    return PTG_ERR_ARG;
+}
+
+template <typename T, bool EXACT>
+static ptg_status launch_walk(const WalkArgs<T> &a, int fields, bool quad, int64_t blocks, hipStream_t s)
+{
+    if (a.gs <= 64) return launch_fq<T, 1, EXACT>(a, fields, quad, blocks, s);
+    if (a.gs <= 128) return launch_fq<T, 2, EXACT>(a, fields, quad, blocks, s);
+    return launch_fq<T, 4, EXACT>(a, fields, quad, blocks, s);
+}
+
+extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const double *soft, const int64_t *perm,
+                               int64_t nt, int fields, int quadrupole, int group_size, double theta, double G,
+                               int exact, double *out, int64_t *counters, void *ws, size_t ws_bytes, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (!wt || nt < 0 || (nt > 0 && (!pos || !soft || !perm || !out))) {
+        ptg_set_error("ptg_walk: bad arguments");
+        return PTG_ERR_ARG;
+    }
+    if (group_size < 1 || group_size > 256) {
+        ptg_set_error("ptg_walk: group_size must be in [1, 256], got %d", group_size);
+        return PTG_ERR_ARG;
+    }
+    if (!(theta > 0)) {
+        ptg_set_error("ptg_walk: theta must be positive");
+        return PTG_ERR_ARG;
+    }
+    if (quadrupole && (!wt->has_quad || !wt->quad)) {
+        ptg_set_error("ptg_walk: quadrupole=True requires a walk tree packed with quadrupoles");
+        return PTG_ERR_ARG;
+    }
+    if (fields < 1 || fields > 7) {
+        ptg_set_error("ptg_walk: fields must be a nonzero combination of PTG_POT|PTG_ACC|PTG_TIDAL");
+        return PTG_ERR_ARG;
+    }
+    if (nt == 0) return PTG_OK;
+    if (ws_bytes < ptg_walk_workspace_bytes(nt, group_size) || !ws) {
+        ptg_set_error("ptg_walk: workspace too small (%zu < %zu)", ws_bytes, ptg_walk_workspace_bytes(nt, group_size));
+        return PTG_ERR_WORKSPACE;
+    }
+    WalkCtl *ctl = (WalkCtl *)ws;
+    int *spill = (int *)((char *)ws + ptg_align256(sizeof(WalkCtl)));
+    int64_t blocks = walk_grid_blocks();
+    k_walk_init<<<1, 64, 0, s>>>(ctl);
+    PTG_LAUNCH_CHECK();
+    ptg_status st;
+    if (wt->prec == PTG_F32) {
+        WalkArgs<float> a{(const V4<float> *)wt->xyzm, (const V4<float> *)wt->aux, (const int2 *)wt->kids,
+                          (const float *)wt->quad, pos, soft, perm, wt->nslots, nt, ptg_cdiv(nt, group_size), group_size,
+                          (float)theta, G, wt->origin[0], wt->origin[1], wt->origin[2], out, counters, ctl, spill};
+        if (exact) {
+            ptg_set_error("ptg_walk: the exact branch set is evaluated in FP64; use prec=PTG_F64");
+            return PTG_ERR_ARG;
+        }
+        st = launch_walk<float, false>(a, fields, quadrupole != 0, blocks, s);
+    } else if (wt->prec == PTG_F64) {
+        WalkArgs<double> a{(const V4<double> *)wt->xyzm, (const V4<double> *)wt->aux, (const int2 *)wt->kids,
+                           (const double *)wt->quad, pos, soft, perm, wt->nslots, nt, ptg_cdiv(nt, group_size), group_size,
+                           theta, G, wt->origin[0], wt->origin[1], wt->origin[2], out, counters, ctl, spill};
+        st = launch_walk<double, true>(a, fields, quadrupole != 0, blocks, s);
+    } else {
+        ptg_set_error("ptg_walk: prec must be PTG_F32 or PTG_F64");
+        return PTG_ERR_ARG;
+    }
+    return st;
+}
+
+// overflow flag of the last walk on this workspace (0 = fine); synchronises the stream
+extern "C" ptg_status ptg_walk_check(void *ws, void *stream)
+{
+    int32_t err = 0;
+    hipStream_t s = (hipStream_t)stream;
+    PTG_HIP_CHECK(hipMemcpyAsync(&err, &((WalkCtl *)ws)->err, 4, hipMemcpyDeviceToHost, s));
+    PTG_HIP_CHECK(hipStreamSynchronize(s));
+    if (err) {
+        ptg_set_error("ptg_walk: a traversal stack exceeded its spill capacity");
+        return PTG_ERR_OVERFLOW;
+    }
+    return PTG_OK;
+}

@@ -1,0 +1,203 @@
+"""Device-side engine: build, pack, walk and direct sum on the GPU through libptg_hip.
+
+torch is used only to hold device memory and to name the current HIP stream; every computation
+is a libptg_hip kernel.  Inputs may be numpy arrays (copied to the device) or torch tensors already
+on the GPU (used in place).
+"""
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import PTG_ACC, PTG_F32, PTG_F64, PTG_POT, PTG_TIDAL, BuildInfo, TreeArrays, WalkTree, check, ptr
+
+
+def device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("pytreegrav_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream():
+    return _lib._vp(torch.cuda.current_stream().cuda_stream)
+
+
+def to_dev(a, shape=None):
+    """float64 contiguous device tensor from a numpy array or torch tensor."""
+    if isinstance(a, torch.Tensor):
+        t = a.to(device=device(), dtype=torch.float64)
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(np.asarray(a, dtype=np.float64))).to(device())
+    t = t.contiguous()
+    if shape is not None:
+        t = t.reshape(shape)
+    return t
+
+
+def workspace(nbytes):
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device())
+
+
+def fields_mask(potential=False, accel=False, tidal=False):
+    return (PTG_POT if potential else 0) | (PTG_ACC if accel else 0) | (PTG_TIDAL if tidal else 0)
+
+
+def field_width(mask):
+    return (1 if mask & PTG_POT else 0) + (3 if mask & PTG_ACC else 0) + (9 if mask & PTG_TIDAL else 0)
+
+
+def unpack_fields(out, mask):
+    """Split an (N, W) buffer into the reference's dict of views (grouped_treewalk.py:387-397)."""
+    res = {}
+    o = 0
+    if mask & PTG_POT:
+        res["potential"] = out[:, 0]
+        o = 1
+    if mask & PTG_ACC:
+        res["accel"] = out[:, o:o + 3]
+        o += 3
+    if mask & PTG_TIDAL:
+        res["tidal"] = out[:, o:o + 9].reshape(-1, 3, 3)
+    return res
+
+
+# ---------------------------------------------------------------------------------------------
+# tree build
+# ---------------------------------------------------------------------------------------------
+class DeviceTree:
+    """Device arrays of one octree in the reference's element layout (octree.py:8-28)."""
+
+    def __init__(self, pos, m, h, quadrupole=False, compute_moments=True):
+        L = _lib.load()
+        dev = device()
+        pos = to_dev(pos, (-1, 3))
+        n = pos.shape[0]
+        m = to_dev(m, (n,))
+        h = to_dev(h, (n,))
+        s = stream()
+        ws = workspace(L.ptg_build_workspace_bytes(n))
+        info = BuildInfo()
+        check(L.ptg_build_sort(ptr(pos), n, ptr(ws), ws.numel(), _lib.ctypes.byref(info), s), "ptg_build_sort")
+        nn = int(info.nnodes)
+        ne = n + nn
+        f64 = dict(dtype=torch.float64, device=dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        self.n, self.nnodes, self.ne = n, nn, ne
+        self.quadrupole = bool(quadrupole)
+        self.info = info
+        self.coordinates = torch.empty((ne, 3), **f64)
+        self.masses = torch.empty(ne, **f64)
+        self.softenings = torch.empty(ne, **f64)
+        self.sizes = torch.empty(ne, **f64)
+        self.deltas = torch.empty(ne, **f64)
+        self.quadrupoles = torch.empty((ne, 9), **f64) if quadrupole else None
+        self.first_subnode = torch.empty(ne, **i32)
+        self.next_branch = torch.empty(ne, **i32)
+        self.order = torch.empty(n, dtype=torch.int64, device=dev)
+        self.nchild = torch.empty(ne, **i32)
+        self.child_base = torch.empty(ne, **i32)
+        self.arrays = TreeArrays(
+            ptr(self.coordinates).value, ptr(self.masses).value, ptr(self.softenings).value, ptr(self.sizes).value,
+            ptr(self.deltas).value, ptr(self.quadrupoles).value if quadrupole else None,
+            ptr(self.first_subnode).value, ptr(self.next_branch).value, ptr(self.order).value,
+            ptr(self.nchild).value, ptr(self.child_base).value)
+        nws = workspace(L.ptg_node_workspace_bytes(n, nn, int(quadrupole)))
+        check(L.ptg_build_tree(ptr(pos), ptr(m), ptr(h), _lib.ctypes.byref(info), int(quadrupole),
+                               int(compute_moments), ptr(ws), ws.numel(), ptr(nws), nws.numel(),
+                               _lib.ctypes.byref(self.arrays), s), "ptg_build_tree")
+        # ws / nws go back to torch's stream-ordered caching allocator here; later allocations on
+        # this stream are ordered after the build kernels that use them
+        self._packs = {}
+
+    @property
+    def center(self):
+        return np.array(self.info.center[:])
+
+    @property
+    def size(self):
+        return float(self.info.size)
+
+    @property
+    def has_coincident(self):
+        return bool(self.info.has_coincident)
+
+    def pack(self, prec, quadrupole):
+        """Sibling-contiguous walk copy (cached per precision and moment order)."""
+        key = (prec, bool(quadrupole))
+        if key in self._packs:
+            return self._packs[key]
+        dev = device()
+        dt = torch.float32 if prec == PTG_F32 else torch.float64
+        ne = self.ne
+        xyzm = torch.empty((ne, 4), dtype=dt, device=dev)
+        aux = torch.empty((ne, 4), dtype=dt, device=dev)
+        kids = torch.empty((ne, 2), dtype=torch.int32, device=dev)
+        quad = torch.empty((ne, 8), dtype=dt, device=dev) if quadrupole else None
+        origin = self.center if prec == PTG_F32 else np.zeros(3)
+        wt = WalkTree(ptr(xyzm).value, ptr(aux).value, ptr(kids).value, ptr(quad).value if quad is not None else None,
+                      ne, prec, int(bool(quadrupole)), (_lib._dbl * 3)(*origin))
+        check(_lib.load().ptg_pack_walk(_lib.ctypes.byref(self.arrays), self.n, self.nnodes, _lib.ctypes.byref(wt),
+                                        stream()), "ptg_pack_walk")
+        p = {"wt": wt, "tensors": (xyzm, aux, kids, quad)}
+        self._packs[key] = p
+        return p
+
+
+def morton_order(pos):
+    """Device int64 permutation sorting points into Morton order (grouped_treewalk.py:269-275)."""
+    L = _lib.load()
+    pos = to_dev(pos, (-1, 3))
+    n = pos.shape[0]
+    order = torch.empty(n, dtype=torch.int64, device=device())
+    if n:
+        ws = workspace(L.ptg_build_workspace_bytes(n))
+        check(L.ptg_morton_order(ptr(pos), n, ptr(order), ptr(ws), ws.numel(), stream()), "ptg_morton_order")
+    return order
+
+
+_walk_ws = {}
+
+
+def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7, G=1.0, precision="float64",
+         counters=None):
+    """Grouped walk of `tree` at targets pos[perm[t]]; returns an (nt, W) float64 device tensor in the
+    original target order (FieldsTarget_grouped, grouped_treewalk.py:350-397)."""
+    L = _lib.load()
+    pos = to_dev(pos, (-1, 3))
+    nt = pos.shape[0]
+    soft = to_dev(soft, (nt,))
+    W = field_width(mask)
+    out = torch.empty((nt, W), dtype=torch.float64, device=device())
+    if nt == 0:
+        return out
+    prec = PTG_F32 if precision in ("float32", "f32", np.float32, torch.float32) else PTG_F64
+    p = tree.pack(prec, quadrupole)
+    wsb = L.ptg_walk_workspace_bytes(nt, int(group_size))
+    key = (torch.cuda.current_device(), wsb)
+    ws = _walk_ws.get(key)
+    if ws is None:
+        ws = workspace(wsb)
+        _walk_ws.clear()
+        _walk_ws[key] = ws
+    check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), ptr(pos), ptr(soft), ptr(perm), nt, int(mask), int(bool(quadrupole)),
+                     int(group_size), float(theta), float(G), int(prec == PTG_F64), ptr(out), ptr(counters), ptr(ws),
+                     ws.numel(), stream()), "ptg_walk")
+    check(L.ptg_walk_check(ptr(ws), stream()), "ptg_walk")
+    return out
+
+
+def direct(pos_t, soft_t, pos_s, m_s, h_s, mask, G=1.0, self_mode=False, precision="float64"):
+    """LDS-tiled direct sum (bruteforce.py); (nt, W) float64 device tensor."""
+    L = _lib.load()
+    pos_t = to_dev(pos_t, (-1, 3))
+    nt = pos_t.shape[0]
+    soft_t = to_dev(soft_t, (nt,))
+    pos_s = to_dev(pos_s, (-1, 3))
+    ns = pos_s.shape[0]
+    m_s = to_dev(m_s, (ns,))
+    h_s = to_dev(h_s, (ns,))
+    out = torch.empty((nt, field_width(mask)), dtype=torch.float64, device=device())
+    prec = PTG_F32 if precision in ("float32", "f32", np.float32, torch.float32) else PTG_F64
+    check(L.ptg_direct(ptr(pos_t), ptr(soft_t), nt, ptr(pos_s), ptr(m_s), ptr(h_s), ns, prec, int(mask),
+                       int(bool(self_mode)), float(G), ptr(out), stream()), "ptg_direct")
+    return out

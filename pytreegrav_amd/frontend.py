@@ -1,0 +1,456 @@
+"""Drop-in gravity API: the names, signatures, defaults, return shapes and error behaviour of
+pytreegrav's frontend (src/pytreegrav/frontend.py:20-1044), computed on an MI355X by libptg_hip.
+
+Numerics.  ``precision="float64"`` reproduces the reference's CPU path (FP64, the reference's
+branch set per pair, reference group_size default 8).  ``precision="float32"`` is the fast walk:
+FP32 node-relative coordinates with FP32 accumulation, FP64 keys and moments.  The self-gravity
+functions keep the reference's ``device`` argument: ``device="cpu"`` (default) selects float64,
+``device="cuda"`` selects float32 -- the precisions of the reference's two backends -- and both run
+on the GPU.  ``parallel`` is accepted and ignored: every path is parallel.
+
+Inputs may be array-likes (results come back as float64 numpy arrays in input order, like the
+reference) or torch tensors on the GPU (results stay on the GPU as float64 tensors).
+"""
+
+import warnings
+
+import numpy as np
+import torch
+
+from . import engine
+from .engine import PTG_ACC, PTG_POT, PTG_TIDAL
+
+# group size of the FP32 walk when the caller does not choose one (one target per lane)
+FP32_DEFAULT_GROUP = 64
+
+
+def _f64(a):
+    """Cast to float64 without collapsing a length-1 array to a scalar (frontend.py:20-25)."""
+    if isinstance(a, torch.Tensor):
+        return a.to(torch.float64)
+    return np.asarray(a, dtype=np.float64)
+
+
+def _atleast_2d(a):
+    if isinstance(a, torch.Tensor):
+        return a.reshape(1, -1) if a.dim() == 1 else a
+    return np.atleast_2d(a)
+
+
+def _atleast_1d(a):
+    if isinstance(a, torch.Tensor):
+        return a.reshape(1) if a.dim() == 0 else a
+    return np.atleast_1d(a)
+
+
+def _is_dev(a):
+    return isinstance(a, torch.Tensor) and a.is_cuda
+
+
+def _finish(t, like_device):
+    """Device tensor -> numpy (host callers) or the tensor itself (GPU callers)."""
+    return t if like_device else t.cpu().numpy()
+
+
+def _zeros_like_len(a):
+    if isinstance(a, torch.Tensor):
+        return torch.zeros(a.shape[0], dtype=torch.float64, device=a.device)
+    return np.zeros(len(a))
+
+
+def _finite(*arrays):
+    for a in arrays:
+        if isinstance(a, torch.Tensor):
+            if not bool(torch.isfinite(a).all()):
+                return False
+        elif not np.all(np.isfinite(a)):
+            return False
+    return True
+
+
+def _precision(precision, device=None):
+    if precision is None:
+        return "float32" if device == "cuda" else "float64"
+    if precision in ("float32", "f32", np.float32, torch.float32):
+        return "float32"
+    if precision in ("float64", "f64", np.float64, torch.float64):
+        return "float64"
+    raise ValueError(f"precision must be 'float64' or 'float32', got {precision!r}")
+
+
+def _group(group_size, precision):
+    if group_size is None:
+        return FP32_DEFAULT_GROUP if precision == "float32" else 8
+    group_size = int(group_size)
+    if not 1 <= group_size <= 256:
+        raise ValueError(f"group_size must be between 1 and 256, got {group_size}")
+    return group_size
+
+
+def checkTreeQuadrupoles(tree, quadrupole):
+    """Raise ValueError if a quadrupole walk is requested on a tree without them (frontend.py:28-37)."""
+    if quadrupole and not tree.HasQuads:
+        raise ValueError(
+            "quadrupole=True requires a tree built with quadrupole moments: pass a tree from "
+            "ConstructTree(..., quadrupole=True), or evaluate with quadrupole=False."
+        )
+
+
+def valueTestMethod(method):
+    """Raise TypeError/ValueError unless method is 'adaptive', 'bruteforce' or 'tree' (frontend.py:40-50)."""
+    methods = ["adaptive", "bruteforce", "tree"]
+    if type(method) != str:
+        raise TypeError("Invalid method type %s, must be str" % type(method))
+    if method not in methods:
+        raise ValueError("Invalid method %s. Must be one of: %s" % (method, str(methods)))
+
+
+def warn_if_coincident_positions(tree, softening=None):
+    """Warn when the build found bit-identical positions (frontend.py:53-70)."""
+    if not tree.HasCoincidentPoints:
+        return
+    if softening is not None and bool((softening > 0).any()):
+        warnings.warn(
+            "Warning: Particle positions are non-unique. Softening will determine the answer for overlapping particles."
+        )
+        return
+    warnings.warn(
+        "Warning: Particle positions are non-unique. The answer will be singular or garbage for overlapping particles."
+    )
+
+
+def _check_device(device):
+    if device not in ("cpu", "cuda"):
+        raise ValueError(f"device must be 'cpu' or 'cuda', got {device!r}")
+
+
+class Octree:
+    """GPU-resident octree with the reference Octree's attributes (octree.py:8-28).
+
+    Particles occupy [0, NumParticles) in Morton order, the root is NumParticles, nodes follow in
+    depth-first preorder.  Array attributes are host numpy views materialised on first access;
+    the device arrays stay in ``self.dev`` for the walk.  ``NumNodes`` is the exact element count
+    (the reference over-allocates node slots; every slot here is a built node).
+    """
+
+    def __init__(self, points, masses, softening, quadrupole=False, compute_moments=True):
+        self.dev = engine.DeviceTree(points, masses, softening, quadrupole=quadrupole,
+                                     compute_moments=compute_moments)
+        self.NumParticles = self.dev.n
+        self.NumNodes = self.dev.ne
+        self.HasQuads = bool(quadrupole)
+        self.HasCoincidentPoints = self.dev.has_coincident
+        self._host = {}
+
+    def _get(self, name, fn):
+        if name not in self._host:
+            self._host[name] = fn()
+        return self._host[name]
+
+    @property
+    def Coordinates(self):
+        return self._get("Coordinates", lambda: self.dev.coordinates.cpu().numpy())
+
+    @property
+    def Masses(self):
+        return self._get("Masses", lambda: self.dev.masses.cpu().numpy())
+
+    @property
+    def Softenings(self):
+        return self._get("Softenings", lambda: self.dev.softenings.cpu().numpy())
+
+    @property
+    def Sizes(self):
+        return self._get("Sizes", lambda: self.dev.sizes.cpu().numpy())
+
+    @property
+    def Deltas(self):
+        return self._get("Deltas", lambda: self.dev.deltas.cpu().numpy())
+
+    @property
+    def Quadrupoles(self):
+        if not self.HasQuads:
+            return np.zeros((0, 3, 3))
+        return self._get("Quadrupoles", lambda: self.dev.quadrupoles.cpu().numpy().reshape(-1, 3, 3))
+
+    @property
+    def FirstSubnode(self):
+        return self._get("FirstSubnode", lambda: self.dev.first_subnode.cpu().numpy().astype(np.int64))
+
+    @property
+    def NextBranch(self):
+        return self._get("NextBranch", lambda: self.dev.next_branch.cpu().numpy().astype(np.int64))
+
+    @property
+    def TreewalkIndices(self):
+        return self._get("TreewalkIndices", lambda: self.dev.order.cpu().numpy())
+
+
+def ConstructTree(pos, m=None, softening=None, quadrupole=False, vel=None, compute_moments=True, morton_order=True,
+                  radix=True):
+    """Build an octree on the GPU (frontend.py:73-142).
+
+    Always the radix build: ``morton_order``/``radix`` are accepted for compatibility (the
+    reference's insertion build is not part of this library), and ``vel`` is rejected because the
+    velocity-carrying DynamicOctree serves the correlation functions, which are not provided.
+    """
+    if vel is not None:
+        raise NotImplementedError("ConstructTree(vel=...) builds the reference's DynamicOctree, which serves the "
+                                  "correlation functions; it is not provided by pytreegrav_amd")
+    pos = _atleast_2d(_f64(pos))
+    if m is None:
+        m = _zeros_like_len(pos)
+        compute_moments = False
+    m = _atleast_1d(_f64(m))
+    if softening is None:
+        softening = m * 0 if isinstance(m, torch.Tensor) else np.zeros_like(m)
+    softening = _atleast_1d(_f64(softening))
+    if not _finite(pos, m, softening):
+        print("Invalid input detected - aborting treebuild to avoid going into an infinite loop!")
+        raise RuntimeError("Invalid input detected - aborting treebuild to avoid going into an infinite loop!")
+    if len(pos) == 0:
+        raise ValueError("ConstructTree needs at least one particle")
+    tree = Octree(pos, m, softening, quadrupole=quadrupole, compute_moments=compute_moments)
+    warn_if_coincident_positions(tree, softening)
+    return tree
+
+
+def _morton_order(points):
+    """Morton permutation of arbitrary points (grouped_treewalk.py:269-275); device int64 tensor."""
+    return engine.morton_order(points)
+
+
+def FieldsTarget_grouped(pos, soft, tree, potential=False, accel=False, tidal=False, group_size=8, G=1.0, theta=0.7,
+                         quadrupole=False, parallel=True, precision="float64"):
+    """Any subset of potential/accel/tidal at ``pos`` (already in a spatially compact order) from
+    ``tree`` in one traversal (grouped_treewalk.py:350-397).  Returns a dict of arrays."""
+    if not (potential or accel or tidal):
+        raise ValueError("request at least one of potential=True, accel=True, tidal=True")
+    checkTreeQuadrupoles(tree, quadrupole)
+    on_dev = _is_dev(pos)
+    pos = _atleast_2d(_f64(pos))
+    mask = engine.fields_mask(potential, accel, tidal)
+    perm = torch.arange(len(pos), dtype=torch.int64, device=engine.device())
+    out = engine.walk(tree.dev, pos, _atleast_1d(_f64(soft)), perm, mask, quadrupole=quadrupole,
+                      group_size=_group(group_size, _precision(precision)), theta=theta, G=G,
+                      precision=_precision(precision))
+    return {k: _finish(v, on_dev) for k, v in engine.unpack_fields(out, mask).items()}
+
+
+def AccelTarget_grouped(pos, soft, tree, group_size=8, G=1.0, theta=0.7, quadrupole=False, parallel=True,
+                        precision="float64"):
+    """grouped_treewalk.py:285-311"""
+    return FieldsTarget_grouped(pos, soft, tree, accel=True, group_size=group_size, G=G, theta=theta,
+                                quadrupole=quadrupole, precision=precision)["accel"]
+
+
+def PotentialTarget_grouped(pos, soft, tree, group_size=8, G=1.0, theta=0.7, quadrupole=False, parallel=True,
+                            precision="float64"):
+    """grouped_treewalk.py:314-329"""
+    return FieldsTarget_grouped(pos, soft, tree, potential=True, group_size=group_size, G=G, theta=theta,
+                                quadrupole=quadrupole, precision=precision)["potential"]
+
+
+def TidalTensorTarget_grouped(pos, soft, tree, group_size=8, G=1.0, theta=0.7, quadrupole=False, parallel=True,
+                              precision="float64"):
+    """grouped_treewalk.py:332-347"""
+    return FieldsTarget_grouped(pos, soft, tree, tidal=True, group_size=group_size, G=G, theta=theta,
+                                quadrupole=quadrupole, precision=precision)["tidal"]
+
+
+# ---------------------------------------------------------------------------------------------
+# shared drivers
+# ---------------------------------------------------------------------------------------------
+def _self_fields(pos, m, softening, mask, G, theta, tree, return_tree, parallel, method, quadrupole, group_size,
+                 precision, adaptive_n):
+    """Potential/Accel/TidalTensor (frontend.py:145-293, 421-560, 688-789)."""
+    valueTestMethod(method)
+    on_dev = _is_dev(pos)
+    pos = _atleast_2d(_f64(pos))
+    m = _atleast_1d(_f64(m))
+    if softening is None:
+        softening = m * 0 if isinstance(m, torch.Tensor) else np.zeros_like(m)
+    softening = _atleast_1d(_f64(softening))
+    if method == "adaptive":
+        if tree is not None or len(pos) > (4000 if parallel else 1000):
+            method = "tree"
+        else:
+            method = "bruteforce"
+    if method == "bruteforce":
+        out = engine.direct(pos, softening, pos, m, softening, mask, G=G, self_mode=True, precision=precision)
+        if return_tree:
+            tree = None
+    else:
+        if tree is None:
+            tree = ConstructTree(pos, m, softening, quadrupole=quadrupole)
+            idx = tree.dev.order  # built from pos, so its walk order already indexes pos
+        else:
+            idx = _morton_order(pos)  # sort, do not reuse the stored permutation (frontend.py:253-258)
+        checkTreeQuadrupoles(tree, quadrupole)
+        out = engine.walk(tree.dev, pos, softening, idx, mask, quadrupole=quadrupole, group_size=group_size,
+                          theta=theta, G=G, precision=precision)
+    return out, tree, on_dev
+
+
+def _target_fields(pos_target, pos_source, m_source, softening_target, softening_source, mask, G, theta, tree,
+                   return_tree, method, quadrupole, group_size, precision):
+    """PotentialTarget/AccelTarget/TidalTensorTarget (frontend.py:296-418, 563-685, 792-901)."""
+    valueTestMethod(method)
+    if tree is None and (pos_source is None or m_source is None):
+        raise ValueError("Must pass either pos_source & m_source or source tree.")
+    on_dev = _is_dev(pos_target)
+    pos_target = _atleast_2d(_f64(pos_target))
+    if softening_target is None:
+        softening_target = _zeros_like_len(pos_target)
+    softening_target = _atleast_1d(_f64(softening_target))
+    if pos_source is not None:
+        pos_source = _atleast_2d(_f64(pos_source))
+        m_source = _atleast_1d(_f64(m_source))
+        if softening_source is None:
+            softening_source = _zeros_like_len(pos_source)
+        softening_source = _atleast_1d(_f64(softening_source))
+    if method == "adaptive":
+        if pos_source is None or len(pos_target) * len(pos_source) > 10 ** 6:
+            method = "tree"
+        else:
+            method = "bruteforce"
+    if method == "bruteforce":
+        out = engine.direct(pos_target, softening_target, pos_source, m_source, softening_source, mask, G=G,
+                            self_mode=False, precision=precision)
+        if return_tree:
+            tree = None
+    else:
+        if tree is None:
+            tree = ConstructTree(pos_source, m_source, softening_source, quadrupole=quadrupole)
+        checkTreeQuadrupoles(tree, quadrupole)
+        tsort = _morton_order(pos_target)
+        out = engine.walk(tree.dev, pos_target, softening_target, tsort, mask, quadrupole=quadrupole,
+                          group_size=group_size, theta=theta, G=G, precision=precision)
+    return out, tree, on_dev
+
+
+def _ret(val, tree, return_tree):
+    return (val, tree) if return_tree else val
+
+
+# ---------------------------------------------------------------------------------------------
+# public API (frontend.py:145-901)
+# ---------------------------------------------------------------------------------------------
+def Potential(pos, m, softening=None, G=1.0, theta=0.7, tree=None, return_tree=False, parallel=False,
+              method="adaptive", quadrupole=False, group_size=None, device="cpu", precision=None):
+    """Gravitational potential at the particle positions, shape (N,) (frontend.py:145-293)."""
+    valueTestMethod(method)
+    _check_device(device)
+    prec = _precision(precision, device)
+    out, tree, on_dev = _self_fields(pos, m, softening, PTG_POT, G, theta, tree, return_tree, parallel, method,
+                                     quadrupole, _group(group_size, prec), prec, None)
+    return _ret(_finish(out[:, 0], on_dev), tree, return_tree)
+
+
+def Accel(pos, m, softening=None, G=1.0, theta=0.7, tree=None, return_tree=False, parallel=False,
+          method="adaptive", quadrupole=False, group_size=None, device="cpu", precision=None):
+    """Gravitational acceleration at the particle positions, shape (N,3) (frontend.py:421-560)."""
+    valueTestMethod(method)
+    _check_device(device)
+    prec = _precision(precision, device)
+    out, tree, on_dev = _self_fields(pos, m, softening, PTG_ACC, G, theta, tree, return_tree, parallel, method,
+                                     quadrupole, _group(group_size, prec), prec, None)
+    return _ret(_finish(out, on_dev), tree, return_tree)
+
+
+def TidalTensor(pos, m, softening=None, G=1.0, theta=0.7, tree=None, return_tree=False, parallel=False,
+                method="adaptive", quadrupole=False, group_size=None, precision="float64"):
+    """Tidal tensor T_ij = dg_i/dx_j at the particle positions, shape (N,3,3) (frontend.py:688-789)."""
+    prec = _precision(precision)
+    out, tree, on_dev = _self_fields(pos, m, softening, PTG_TIDAL, G, theta, tree, return_tree, parallel, method,
+                                     quadrupole, _group(group_size, prec), prec, None)
+    return _ret(_finish(out.reshape(-1, 3, 3), on_dev), tree, return_tree)
+
+
+def PotentialTarget(pos_target, pos_source, m_source, softening_target=None, softening_source=None, G=1.0, theta=0.7,
+                    tree=None, return_tree=False, parallel=False, method="adaptive", quadrupole=False,
+                    group_size=None, precision="float64"):
+    """Potential at arbitrary targets from sources, shape (N,) (frontend.py:296-418)."""
+    prec = _precision(precision)
+    out, tree, on_dev = _target_fields(pos_target, pos_source, m_source, softening_target, softening_source, PTG_POT,
+                                       G, theta, tree, return_tree, method, quadrupole, _group(group_size, prec), prec)
+    return _ret(_finish(out[:, 0], on_dev), tree, return_tree)
+
+
+def AccelTarget(pos_target, pos_source, m_source, softening_target=None, softening_source=None, G=1.0, theta=0.7,
+                tree=None, return_tree=False, parallel=False, method="adaptive", quadrupole=False, group_size=None,
+                precision="float64"):
+    """Acceleration at arbitrary targets from sources, shape (N,3) (frontend.py:563-685)."""
+    prec = _precision(precision)
+    out, tree, on_dev = _target_fields(pos_target, pos_source, m_source, softening_target, softening_source, PTG_ACC,
+                                       G, theta, tree, return_tree, method, quadrupole, _group(group_size, prec), prec)
+    return _ret(_finish(out, on_dev), tree, return_tree)
+
+
+def TidalTensorTarget(pos_target, pos_source, m_source, softening_target=None, softening_source=None, G=1.0,
+                      theta=0.7, tree=None, return_tree=False, parallel=False, method="adaptive", quadrupole=False,
+                      group_size=None, precision="float64"):
+    """Tidal tensor at arbitrary targets, shape (N,3,3) (frontend.py:792-901)."""
+    prec = _precision(precision)
+    out, tree, on_dev = _target_fields(pos_target, pos_source, m_source, softening_target, softening_source,
+                                       PTG_TIDAL, G, theta, tree, return_tree, method, quadrupole,
+                                       _group(group_size, prec), prec)
+    return _ret(_finish(out.reshape(-1, 3, 3), on_dev), tree, return_tree)
+
+
+class Field:
+    """A reusable gravitational field: build once, evaluate any field subset in one traversal
+    (frontend.py:904-1044).  The tree stays resident on the GPU between calls."""
+
+    def __init__(self, pos, m, softening=None, G=1.0, theta=0.7, quadrupole=False, group_size=None, parallel=False,
+                 precision="float64"):
+        pos = _atleast_2d(_f64(pos))
+        m = _atleast_1d(_f64(m))
+        if softening is None:
+            softening = m * 0 if isinstance(m, torch.Tensor) else np.zeros_like(m)
+        softening = _atleast_1d(_f64(softening))
+        self.G = G
+        self.theta = theta
+        self.quadrupole = quadrupole
+        self.precision = _precision(precision)
+        self.group_size = _group(group_size, self.precision)
+        self.parallel = parallel
+        self.tree = ConstructTree(pos, m, softening, quadrupole=quadrupole)
+        self._pos = engine.to_dev(pos, (-1, 3))
+        self._soft = engine.to_dev(softening, (-1,))
+
+    def __repr__(self):
+        return (f"<pytreegrav_amd.Field: {self.tree.NumParticles} particles, "
+                f"{'quadrupole' if self.quadrupole else 'monopole'}, theta={self.theta}, "
+                f"G={self.G}, group_size={self.group_size}, precision={self.precision}>")
+
+    def evaluate(self, potential=False, accel=False, tidal=False, pos_target=None, softening_target=None, theta=None,
+                 parallel=None, group_size=None):
+        """Any combination of fields from one traversal; dict in the order of pos_target (or the sources)."""
+        if not (potential or accel or tidal):
+            raise ValueError("request at least one of potential=True, accel=True, tidal=True")
+        theta = self.theta if theta is None else theta
+        group_size = self.group_size if group_size is None else _group(group_size, self.precision)
+        mask = engine.fields_mask(potential, accel, tidal)
+        if pos_target is None:
+            on_dev = False
+            pos, soft, perm = self._pos, self._soft, self.tree.dev.order
+        else:
+            on_dev = _is_dev(pos_target)
+            pos = _atleast_2d(_f64(pos_target))
+            if softening_target is None:
+                softening_target = _zeros_like_len(pos)
+            soft = _atleast_1d(_f64(softening_target))
+            perm = _morton_order(pos)
+        out = engine.walk(self.tree.dev, pos, soft, perm, mask, quadrupole=self.quadrupole, group_size=group_size,
+                          theta=theta, G=self.G, precision=self.precision)
+        return {k: _finish(v, on_dev) for k, v in engine.unpack_fields(out, mask).items()}
+
+    def potential(self, pos_target=None, **kwargs):
+        return self.evaluate(potential=True, pos_target=pos_target, **kwargs)["potential"]
+
+    def accel(self, pos_target=None, **kwargs):
+        return self.evaluate(accel=True, pos_target=pos_target, **kwargs)["accel"]
+
+    def tidal(self, pos_target=None, **kwargs):
+        return self.evaluate(tidal=True, pos_target=pos_target, **kwargs)["tidal"]

@@ -1,0 +1,86 @@
+"""GPU tree build against the oracle restatement of octree.py: bit-exact Morton order, index-free
+topology and FP64 moments (monopole and quadrupole), on the reference's own test inputs, plus
+invariants at larger N."""
+
+import numpy as np
+import pytest
+
+import oracle
+from gpu_cases import BUILD_CASES, case
+from treeprint import assert_same_tree
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_tree(x, m, h, quadrupole):
+    import warnings
+
+    from pytreegrav_amd import ConstructTree
+
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        return ConstructTree(x, m, h, quadrupole=quadrupole)
+
+
+@pytest.mark.parametrize("name", BUILD_CASES)
+@pytest.mark.parametrize("quadrupole", [False, True])
+def test_build_bit_exact_against_oracle(name, quadrupole):
+    x, m, h = case(name)
+    print(f"[build] case={name} n={len(x)} quad={quadrupole}", flush=True)
+    g = _gpu_tree(x, m, h, quadrupole)
+    o = oracle.build_tree(x, m, h, quadrupole=quadrupole)
+    assert g.NumParticles == o.NumParticles
+    assert np.array_equal(g.TreewalkIndices, o.TreewalkIndices)
+    assert g.HasCoincidentPoints == o.HasCoincidentPoints
+    assert g.NumNodes == o.node_hi
+    assert_same_tree(g, o, with_quads=quadrupole)
+
+
+def test_build_plummer_1e6_bit_exact():
+    x, m, h = case("plummer_1e6")
+    print("[build] case=plummer_1e6", flush=True)
+    g = _gpu_tree(x, m, h, True)
+    o = oracle.build_tree(x, m, h, quadrupole=True)
+    assert np.array_equal(g.TreewalkIndices, o.TreewalkIndices)
+    assert_same_tree(g, o, with_quads=True)
+
+
+def test_build_is_deterministic():
+    x, m, h = case("plummer_1e5_h")
+    a = _gpu_tree(x, m, h, True)
+    b = _gpu_tree(x, m, h, True)
+    for k in ("Coordinates", "Masses", "Softenings", "Deltas", "Quadrupoles", "FirstSubnode", "NextBranch"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
+
+
+@pytest.mark.parametrize("n", [3_000_000])
+def test_large_build_invariants(n):
+    """No oracle at this size: mass conservation, the Morton order is a permutation sorted by key,
+    every element is the child of exactly one node, and the link walk covers slots 0..n-1 in order."""
+    from pytreegrav_amd.sampling import plummer_truncated
+
+    x, m = plummer_truncated(n, seed=11)
+    h = np.full(n, 0.01)
+    print(f"[build] invariants n={n}", flush=True)
+    t = _gpu_tree(x, m, h, False)
+    assert np.array_equal(np.sort(t.TreewalkIndices), np.arange(n))
+    keys = oracle.morton_keys(x, *oracle.bounding_cube(x))
+    assert np.all(np.diff(keys[t.TreewalkIndices]) >= 0)
+    assert np.isclose(t.Masses[n], m.sum(), rtol=1e-12)
+    nc = t.dev.nchild.cpu().numpy()
+    assert nc[:n].sum() == 0 and nc[n:].sum() == t.NumNodes - 1
+    # links: following FirstSubnode/NextBranch visits particle slots in increasing order
+    First, Next = t.FirstSubnode, t.NextBranch
+    nxt_particle = Next[:n]
+    # a particle's NextBranch is either the next slot or a node whose leftmost particle is that slot
+    ok = (nxt_particle[:-1] == np.arange(1, n)) | (nxt_particle[:-1] >= n)
+    assert ok.all() and nxt_particle[-1] == -1
+    nodes = np.nonzero(nxt_particle[:-1] >= n)[0]
+    # descend FirstSubnode from those nodes to a particle: must be the next slot
+    cur = nxt_particle[nodes]
+    for _ in range(100):
+        isn = cur >= n
+        if not isn.any():
+            break
+        cur[isn] = First[cur[isn]]
+    assert np.array_equal(cur, nodes + 1)
