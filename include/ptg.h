@@ -83,7 +83,7 @@ typedef struct {
 size_t ptg_build_workspace_bytes(int64_t n);
 ptg_status ptg_build_sort(const double *pos, int64_t n, void *ws, size_t ws_bytes, ptg_build_info *info,
                           void *stream);
-size_t ptg_node_workspace_bytes(int64_t n, int64_t nnodes, int quadrupole);
+size_t ptg_node_workspace_bytes(int64_t n, int64_t nnodes, int max_level);
 ptg_status ptg_build_tree(const double *pos, const double *m, const double *h, const ptg_build_info *info,
                           int quadrupole, int compute_moments, void *ws, size_t ws_bytes, void *node_ws,
                           size_t node_ws_bytes, const ptg_tree_arrays *tree, void *stream);
@@ -91,6 +91,15 @@ ptg_status ptg_build_tree(const double *pos, const double *m, const double *h, c
 /* Morton permutation of arbitrary points: replaces grouped_treewalk._morton_order (:269-275).
  * Uses a ptg_build_workspace_bytes(n) workspace; asynchronous. */
 ptg_status ptg_morton_order(const double *pos, int64_t n, int64_t *order, void *ws, size_t ws_bytes, void *stream);
+
+/* Cell-aligned target groups for the FP32 walk (no reference counterpart: the reference forms
+ * fixed runs of group_size targets, grouped_treewalk.py:179-182).  Targets pos[perm[t]] must be
+ * in Morton order.  Groups never cross the boundary of a cell holding more than 4*group_size
+ * targets and hold at most group_size targets.  Writes offsets[0..G] (capacity n+1) and the group
+ * count G to the device scalar *ngroups; asynchronous. */
+size_t ptg_groups_workspace_bytes(int64_t n);
+ptg_status ptg_target_groups(const double *pos, const int64_t *perm, int64_t n, int group_size, int32_t *offsets,
+                             int32_t *ngroups, void *ws, size_t ws_bytes, void *stream);
 
 /* ------------------------------------------------------------------------------------------
  * Walk tree: the sibling-contiguous packed copy of the tree the walk reads (one slot per
@@ -121,15 +130,20 @@ ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
  * gravity, the _morton_order permutation for external targets) and results are written to
  * out[perm[t]], i.e. in the caller's original order: out is (nt, W) float64 with
  * W = 1*POT + 3*ACC + 9*TIDAL, scaled by G.  One wavefront walks each group of group_size
- * consecutive sorted targets (1..256).  exact != 0 evaluates every pair with the reference's
+ * consecutive sorted targets (1..256), or, when group_offsets/group_count (device arrays from
+ * ptg_target_groups) are given, each group [offsets[g], offsets[g+1]).  exact != 0 evaluates every pair with the reference's
  * branches (r2 > 0 self skip, softened iff r < max(h_s, h_t)); exact == 0 (FP32 fast path) skips
  * those tests where the group geometry proves them false.  counters (nullable, device int64[2])
  * accumulate: [0] element-target interactions, [1] traversal visits.
  * ------------------------------------------------------------------------------------------ */
 size_t ptg_walk_workspace_bytes(int64_t nt, int group_size);
 ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const double *soft, const int64_t *perm, int64_t nt,
-                    int fields, int quadrupole, int group_size, double theta, double G, int exact, double *out,
-                    int64_t *counters, void *ws, size_t ws_bytes, void *stream);
+                    int fields, int quadrupole, int group_size, const int32_t *group_offsets,
+                    const int32_t *group_count, double theta, double G, int exact, double *out, int64_t *counters,
+                    void *ws, size_t ws_bytes, void *stream);
+/* Reads the overflow flag of the last walk on this workspace (synchronises the stream):
+ * PTG_ERR_OVERFLOW if a traversal stack exceeded its spill capacity. */
+ptg_status ptg_walk_check(void *ws, void *stream);
 
 /* ------------------------------------------------------------------------------------------
  * Direct summation, LDS-tiled: replaces bruteforce.py.  self_mode != 0: targets are the sources

@@ -70,13 +70,15 @@ SIGNATURES = {
     "ptg_build_workspace_bytes": (_c.c_size_t, [_i64]),
     "ptg_build_sort": (_c.c_int, [_vp, _i64, _vp, _c.c_size_t, _c.POINTER(BuildInfo), _vp]),
     "ptg_node_workspace_bytes": (_c.c_size_t, [_i64, _i64, _c.c_int]),
+    "ptg_groups_workspace_bytes": (_c.c_size_t, [_i64]),
+    "ptg_target_groups": (_c.c_int, [_vp, _vp, _i64, _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),
     "ptg_build_tree": (_c.c_int, [_vp, _vp, _vp, _c.POINTER(BuildInfo), _c.c_int, _c.c_int, _vp, _c.c_size_t, _vp,
                                   _c.c_size_t, _c.POINTER(TreeArrays), _vp]),
     "ptg_morton_order": (_c.c_int, [_vp, _i64, _vp, _vp, _c.c_size_t, _vp]),
     "ptg_pack_walk": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _c.POINTER(WalkTree), _vp]),
     "ptg_walk_workspace_bytes": (_c.c_size_t, [_i64, _c.c_int]),
-    "ptg_walk": (_c.c_int, [_c.POINTER(WalkTree), _vp, _vp, _vp, _i64, _c.c_int, _c.c_int, _c.c_int, _dbl, _dbl,
-                            _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),
+    "ptg_walk": (_c.c_int, [_c.POINTER(WalkTree), _vp, _vp, _vp, _i64, _c.c_int, _c.c_int, _c.c_int, _vp, _vp,
+                            _dbl, _dbl, _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),
     "ptg_walk_check": (_c.c_int, [_vp, _vp]),
     "ptg_direct": (_c.c_int, [_vp, _vp, _i64, _vp, _vp, _vp, _i64, _c.c_int, _c.c_int, _c.c_int, _dbl, _vp, _vp]),
 }

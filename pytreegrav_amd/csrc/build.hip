@@ -530,10 +530,17 @@ struct LoadNodeCount {
 
 __global__ __launch_bounds__(256) void k_max_level(const int32_t *lev, int64_t m, Counters *c)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int v = i < m ? lev[i] : 0;
+    int v = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        v = max(v, lev[i]);
     v = ptg_wave_max(v);
     if (ptg_lane() == 0 && v > 0) atomicMax(&c->max_level, v);
+}
+
+__global__ void k_iota_u32(uint32_t *a, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = (uint32_t)i;
 }
 
 __global__ void k_u32_to_i64(const uint32_t *a, int64_t n, int64_t *b)
@@ -642,7 +649,7 @@ extern "C" ptg_status ptg_build_sort(const double *pos, int64_t n, void *ws, siz
         }
         st = excl_scan(LoadNodeCount{w.lev}, n - 1, w.off, w.bsum, &w.ctr->nnodes, s);
         if (st) return st;
-        k_max_level<<<grid1(n - 1), 256, 0, s>>>(w.lev, n - 1, w.ctr);
+        k_max_level<<<256, 256, 0, s>>>(w.lev, n - 1, w.ctr);
         PTG_LAUNCH_CHECK();
     }
     Geom hg;
@@ -708,7 +715,7 @@ __device__ __forceinline__ int32_t head_of(int64_t q, int64_t n, const int32_t *
 
 __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *lev, const int32_t *off, int64_t n,
                                                     const Geom *g, double *coord, double *sizes,
-                                                    int32_t *first, int32_t *next)
+                                                    int32_t *first, int32_t *next, uint64_t *nlevel)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
@@ -725,6 +732,7 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
             coord[3 * idx + 1] = c[1];
             coord[3 * idx + 2] = c[2];
             sizes[idx] = s;
+            nlevel[idx - n] = (uint64_t)L;
             first[idx] = (L + 1 <= li) ? (int32_t)(idx + 1) : (int32_t)i;
             // last slot of this cell: gallop then bisect on "same level-L cell as slot i"
             int64_t lo = i + 1;  // slot i+1 shares level li >= L with slot i
@@ -791,28 +799,38 @@ struct NodeWS {
     double *acc;        // (nnodes, 3) running sum of m*x, indexed by node - n
     int32_t *parent;    // (nnodes)
     int32_t *nodekids;  // (nnodes) number of node children
-    int32_t *arrive;    // (nnodes) arrival counters for the bottom-up pass
+    int32_t *arrive;    // (nnodes) unused by the level-synchronous pass; kept zero
     int32_t *err;       // [0] corrupt-chain flag
     int32_t *bsum;
+    uint64_t *lvA, *lvB;   // node levels (sort keys)
+    uint32_t *idA, *idB;   // node offsets (sort values): level-sorted node list
+    int32_t *hist;
+    int32_t *lvstart;      // (max_level + 2) first position of each level in the sorted list
 };
 
-static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, NodeWS &w)
+static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, int64_t max_level, NodeWS &w)
 {
+    int64_t nh = (int64_t)RS_BINS * rs_nwt(nnodes);
     w.acc = A.take<double>(3 * nnodes);
     w.parent = A.take<int32_t>(nnodes);
     w.nodekids = A.take<int32_t>(nnodes);
     w.arrive = A.take<int32_t>(nnodes);
     w.err = A.take<int32_t>(64);
-    w.bsum = A.take<int32_t>(scan_blocks(nnodes) + 64);
+    w.bsum = A.take<int32_t>(scan_blocks(std::max<int64_t>(nnodes, nh)) + 64);
+    w.lvA = A.take<uint64_t>(nnodes);
+    w.lvB = A.take<uint64_t>(nnodes);
+    w.idA = A.take<uint32_t>(nnodes);
+    w.idB = A.take<uint32_t>(nnodes);
+    w.hist = A.take<int32_t>(nh);
+    w.lvstart = A.take<int32_t>(max_level + 2);
     return A.ok();
 }
 
-extern "C" size_t ptg_node_workspace_bytes(int64_t n, int64_t nnodes, int quadrupole)
+extern "C" size_t ptg_node_workspace_bytes(int64_t n, int64_t nnodes, int max_level)
 {
-    (void)quadrupole;
     PtgArena A(nullptr, 0);
     NodeWS w;
-    carve_nodes(A, n, std::max<int64_t>(nnodes, 1), w);
+    carve_nodes(A, n, std::max<int64_t>(nnodes, 1), std::max(max_level, 0), w);
     return A.used;
 }
 
@@ -866,10 +884,7 @@ __global__ __launch_bounds__(256) void k_link(int64_t n, int64_t nnodes, const p
         for (int k = 0; k < 9; k++) t.quadrupoles[9 * idx + k] = 0.0;
 }
 
-template <typename T> __device__ __forceinline__ T ld_agent(const T *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+template <typename T> __device__ __forceinline__ T ld_agent(const T *p) { return *p; }
 
 // One node's moments, in the reference's summation order: particle children in digit order
 // (folded during the split, octree.py:956-966), then node children in descending index order,
@@ -963,25 +978,27 @@ __device__ void node_moments(int64_t idx, int64_t n, int64_t nnodes, const ptg_t
     for (int k = 0; k < 9; k++) t.quadrupoles[9 * idx + k] = Q[k];
 }
 
-// bottom-up pass: a thread starts at every node without node children and climbs; at each
-// parent only the last-arriving child continues, so every node is computed exactly once and after
-// all of its children (no FP64 atomics: sums are in a fixed order, hence deterministic).
-__global__ __launch_bounds__(256) void k_moments(int64_t n, int64_t nnodes, const ptg_tree_arrays t, NodeWS w,
-                                                 int quad)
+// Bottom-up moments, one launch per level from the deepest up: every node of a level is computed
+// after all of its children (a kernel boundary orders the levels), sums run in a fixed order, so
+// the result is deterministic and needs no device-scope fences or FP64 atomics.
+__global__ __launch_bounds__(256) void k_level_starts(const uint64_t *lv, int64_t nnodes, int max_level,
+                                                      int32_t *lvstart)
 {
-    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nnodes || w.nodekids[a] != 0) return;
-    int64_t idx = n + a;
-    for (int64_t step = 0; step < nnodes; step++) {
-        node_moments(idx, n, nnodes, t, w, quad != 0);
-        int32_t p = w.parent[idx - n];
-        if (p < n || p >= n + nnodes) break;
-        __threadfence();
-        int old = atomicAdd(&w.arrive[p - n], 1);
-        if (old != w.nodekids[p - n] - 1) break;
-        __threadfence();
-        idx = p;
-    }
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nnodes) return;
+    // lvstart[L] = first sorted position with level >= L
+    int64_t a = i == 0 ? -1 : (int64_t)lv[i - 1];
+    int64_t b = i == nnodes ? (int64_t)max_level + 1 : (int64_t)lv[i];
+    for (int64_t L = a + 1; L <= b && L <= max_level + 1; L++) lvstart[L] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_moments_level(int64_t n, int64_t nnodes, int L, const int32_t *lvstart,
+                                                       const uint32_t *byid, const ptg_tree_arrays t, NodeWS w,
+                                                       int quad)
+{
+    int64_t lo = lvstart[L], hi = lvstart[L + 1];
+    for (int64_t q = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < hi; q += (int64_t)gridDim.x * blockDim.x)
+        node_moments(n + byid[q], n, nnodes, t, w, quad != 0);
 }
 
 // compute_moments=False: the split has still folded particle children into their parent's mass
@@ -1040,22 +1057,43 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     }
     PtgArena B(node_ws, node_ws_bytes);
     NodeWS nw;
-    if (!node_ws || !carve_nodes(B, n, nnodes, nw)) {
+    const int max_level = std::max(info->max_level, 0);
+    if (!node_ws || !carve_nodes(B, n, nnodes, max_level, nw)) {
         ptg_set_error("ptg_build_tree: node workspace too small (%zu < %zu)", node_ws_bytes,
-                      ptg_node_workspace_bytes(n, nnodes, quadrupole));
+                      ptg_node_workspace_bytes(n, nnodes, max_level));
         return PTG_ERR_WORKSPACE;
     }
     KeyView kv{w.kA, info->has_collisions ? w.ext : nullptr, n};
     k_particles<<<grid1(n), 256, 0, s>>>(pos, m, h, w.vA, w.lev, w.off, n, t);
-    if (n == 1) k_single_root<<<1, 1, 0, s>>>(w.geom, t);
-    else k_emit_nodes<<<grid1(n - 1), 256, 0, s>>>(kv, w.lev, w.off, n, w.geom, t.coordinates, t.sizes,
-                                                   t.first_subnode, t.next_branch);
+    if (n == 1) {
+        k_single_root<<<1, 1, 0, s>>>(w.geom, t);
+        PTG_HIP_CHECK(hipMemsetAsync(nw.lvA, 0, sizeof(uint64_t), s));
+    } else {
+        k_emit_nodes<<<grid1(n - 1), 256, 0, s>>>(kv, w.lev, w.off, n, w.geom, t.coordinates, t.sizes,
+                                                  t.first_subnode, t.next_branch, nw.lvA);
+    }
     PTG_HIP_CHECK(hipMemsetAsync(nw.err, 0, 64 * sizeof(int32_t), s));
     k_node_init<<<grid1(nnodes), 256, 0, s>>>(nnodes, nw);
     k_link<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t, nw);
     PTG_LAUNCH_CHECK();
-    if (compute_moments) k_moments<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t, nw, quadrupole);
-    else k_fold_particles<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t);
+    if (compute_moments) {
+        // counting sort of node offsets by level (stable LSD radix sort on the level)
+        k_iota_u32<<<grid1(nnodes), 256, 0, s>>>(nw.idA, nnodes);
+        int nbits = 1;
+        while ((1ll << nbits) <= (int64_t)max_level) nbits++;
+        nbits = ((nbits + RS_BITS - 1) / RS_BITS) * RS_BITS;
+        bool in_first;
+        ptg_status st0 = radix_sort(nw.lvA, nw.idA, nw.lvB, nw.idB, nnodes, nbits, nw.hist, nw.bsum, &in_first, s);
+        if (st0) return st0;
+        const uint64_t *lv = in_first ? nw.lvA : nw.lvB;
+        const uint32_t *byid = in_first ? nw.idA : nw.idB;
+        k_level_starts<<<grid1(nnodes + 1), 256, 0, s>>>(lv, nnodes, max_level, nw.lvstart);
+        PTG_LAUNCH_CHECK();
+        for (int L = max_level; L >= 0; L--)
+            k_moments_level<<<512, 256, 0, s>>>(n, nnodes, L, nw.lvstart, byid, t, nw, quadrupole);
+    } else {
+        k_fold_particles<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t);
+    }
     PTG_LAUNCH_CHECK();
     // sibling-contiguous walk slots: exclusive scan of child counts in node order
     ptg_status st = excl_scan(LoadNChild{t.nchild, n}, nnodes, t.child_base + n, nw.bsum, nullptr, s);
@@ -1130,6 +1168,161 @@ extern "C" ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int6
         ptg_set_error("ptg_pack_walk: prec must be PTG_F32 or PTG_F64");
         return PTG_ERR_ARG;
     }
+    PTG_LAUNCH_CHECK();
+    return PTG_OK;
+}
+
+// ============================================================================================
+// cell-aligned target groups for the FP32 walk
+//
+// Fixed runs of group_size Morton-consecutive targets can straddle the boundary of a large cell:
+// such a group's bounding box spans the cell and its walk opens most of the tree.  Here a group
+// never crosses the boundary of an octree cell that holds more than `cell_cap` targets: targets
+// are first split into maximal cells of at most cell_cap (= 4 x group_size) targets, and each such
+// cell is cut into ceil(len / group_size) near-equal consecutive chunks.
+// ============================================================================================
+__global__ __launch_bounds__(256) void k_sorted_keys(const double *pos, const int64_t *perm, int64_t n, const Geom *g,
+                                                     uint64_t *keys)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int64_t o = perm[t];
+    keys[t] = morton_key(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], g->cmin, g->inv);
+}
+
+// number of consecutive positions on one side of t (direction dir) whose key shares the top 3L
+// bits with keys[t], capped at cap
+__device__ int64_t same_cell_extent(const uint64_t *keys, int64_t n, int64_t t, int L, int dir, int64_t cap)
+{
+    auto same = [&](int64_t j) {
+        if (L == 0) return true;
+        if (L >= 21) return keys[j] == keys[t];
+        return (keys[j] >> (63 - 3 * L)) == (keys[t] >> (63 - 3 * L));
+    };
+    int64_t good = 0, step = 1;
+    while (good < cap) {
+        int64_t probe = std::min(good + step, cap);
+        int64_t j = t + dir * probe;
+        if (j < 0 || j >= n || !same(j)) {
+            // bisect between good (matching) and probe (not matching or outside)
+            int64_t lo = good, hi = probe;
+            while (hi - lo > 1) {
+                int64_t mid = (lo + hi) >> 1;
+                int64_t jm = t + dir * mid;
+                if (jm >= 0 && jm < n && same(jm)) lo = mid;
+                else hi = mid;
+            }
+            return lo;
+        }
+        good = probe;
+        step <<= 1;
+    }
+    return cap;
+}
+
+// pflag[t] = 1 where a new capped cell starts (t == 0, or the smallest cell holding t-1 and t has
+// more than cap targets)
+__global__ __launch_bounds__(256) void k_cell_cuts(const uint64_t *keys, int64_t n, int64_t cap, int32_t *pflag)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (t == 0) {
+        pflag[0] = 1;
+        return;
+    }
+    int L = ptg_common_levels(keys[t - 1], keys[t]);
+    int64_t left = same_cell_extent(keys, n, t - 1, L, -1, cap);
+    int64_t right = left >= cap ? 0 : same_cell_extent(keys, n, t, L, +1, cap);
+    pflag[t] = (left + right + 2 > cap) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_scatter_starts(const int32_t *flag, const int32_t *ex, int64_t n,
+                                                        int32_t *starts, int32_t *count)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (flag[t]) starts[ex[t]] = (int32_t)t;
+    if (t == n - 1) {
+        int32_t total = ex[t] + flag[t];
+        starts[total] = (int32_t)n;
+        if (count) *count = total;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_chunk_flags(const int32_t *pflag, const int32_t *pex, const int32_t *pstart,
+                                                     int64_t n, int gs, int32_t *gflag)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int32_t c = pex[t] + pflag[t] - 1;
+    int64_t ps = pstart[c], pe = pstart[c + 1];
+    int64_t len = pe - ps;
+    int64_t nch = (len + gs - 1) / gs;
+    int64_t cs = (len + nch - 1) / nch;
+    gflag[t] = ((t - ps) % cs == 0) ? 1 : 0;
+}
+
+struct GroupWS {
+    double *part;
+    Geom *geom;
+    uint64_t *keys;
+    int32_t *pflag, *pex, *pstart, *gflag, *gex, *bsum;
+};
+
+static bool carve_groups(PtgArena &A, int64_t n, GroupWS &w)
+{
+    w.part = A.take<double>(6 * BB_BLOCKS);
+    w.geom = A.take<Geom>(1);
+    w.keys = A.take<uint64_t>(n);
+    w.pflag = A.take<int32_t>(n);
+    w.pex = A.take<int32_t>(n);
+    w.pstart = A.take<int32_t>(n + 1);
+    w.gflag = A.take<int32_t>(n);
+    w.gex = A.take<int32_t>(n);
+    w.bsum = A.take<int32_t>(scan_blocks(n) + 64);
+    return A.ok();
+}
+
+extern "C" size_t ptg_groups_workspace_bytes(int64_t n)
+{
+    PtgArena A(nullptr, 0);
+    GroupWS w;
+    carve_groups(A, std::max<int64_t>(n, 1), w);
+    return A.used;
+}
+
+extern "C" ptg_status ptg_target_groups(const double *pos, const int64_t *perm, int64_t n, int group_size,
+                                        int32_t *offsets, int32_t *ngroups, void *ws, size_t ws_bytes, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (n < 1 || !pos || !perm || !offsets || !ngroups || group_size < 1 || group_size > 256) {
+        ptg_set_error("ptg_target_groups: bad arguments");
+        return PTG_ERR_ARG;
+    }
+    if (n > 0x7fffffffLL) {
+        ptg_set_error("ptg_target_groups: n exceeds the int32 range");
+        return PTG_ERR_ARG;
+    }
+    PtgArena A(ws, ws_bytes);
+    GroupWS w;
+    if (!ws || !carve_groups(A, n, w)) {
+        ptg_set_error("ptg_target_groups: workspace too small (%zu < %zu)", ws_bytes, ptg_groups_workspace_bytes(n));
+        return PTG_ERR_WORKSPACE;
+    }
+    int nb = (int)std::min<int64_t>(BB_BLOCKS, std::max<int64_t>(1, ptg_cdiv(n, 1024)));
+    k_bbox_partial<<<nb, 256, 0, s>>>(pos, n, w.part);
+    k_bbox_final<<<1, 64, 0, s>>>(w.part, nb, w.geom);
+    k_sorted_keys<<<grid1(n), 256, 0, s>>>(pos, perm, n, w.geom, w.keys);
+    k_cell_cuts<<<grid1(n), 256, 0, s>>>(w.keys, n, 4 * (int64_t)group_size, w.pflag);
+    PTG_LAUNCH_CHECK();
+    ptg_status st = excl_scan(LoadArr{w.pflag}, n, w.pex, w.bsum, nullptr, s);
+    if (st) return st;
+    k_scatter_starts<<<grid1(n), 256, 0, s>>>(w.pflag, w.pex, n, w.pstart, nullptr);
+    k_chunk_flags<<<grid1(n), 256, 0, s>>>(w.pflag, w.pex, w.pstart, n, group_size, w.gflag);
+    PTG_LAUNCH_CHECK();
+    st = excl_scan(LoadArr{w.gflag}, n, w.gex, w.bsum, nullptr, s);
+    if (st) return st;
+    k_scatter_starts<<<grid1(n), 256, 0, s>>>(w.gflag, w.gex, n, offsets, ngroups);
     PTG_LAUNCH_CHECK();
     return PTG_OK;
 }

@@ -47,6 +47,8 @@ template <typename T> struct WalkArgs {
     int64_t nslots;
     int64_t nt;
     int64_t ngroups;
+    const int32_t *goff;    // optional group offsets (ngroups + 1), else fixed groups of gs
+    const int32_t *gcount;  // device group count when goff is given
     int gs;
     T theta;
     double G;
@@ -325,18 +327,20 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
     int *spill = A.spill + gwave * GCAP;
     const int xcd = blockIdx.x % NXCD;
     int64_t n_int = 0, n_vis = 0;
+    const int64_t ngroups = A.goff ? (int64_t)__builtin_amdgcn_readfirstlane(*A.gcount) : A.ngroups;
 
     for (int attempt = 0; attempt < NXCD; attempt++) {
         const int x = (xcd + attempt) % NXCD;
-        const int64_t g_lo = A.ngroups * x / NXCD, g_hi = A.ngroups * (x + 1) / NXCD;
+        const int64_t g_lo = ngroups * x / NXCD, g_hi = ngroups * (x + 1) / NXCD;
         for (;;) {
             int gi = 0;
             if (lane == 0) gi = atomicAdd(&A.ctl->ctr[x * CTR_STRIDE], 1);
             gi = __builtin_amdgcn_readfirstlane(gi);
             const int64_t g = g_lo + gi;
             if (g >= g_hi) break;
-            const int64_t a0 = g * A.gs;
-            const int64_t b0 = std::min<int64_t>(a0 + A.gs, A.nt);
+            const int64_t a0 = A.goff ? (int64_t)A.goff[g] : g * A.gs;
+            const int64_t b0 = A.goff ? std::min<int64_t>((int64_t)A.goff[g + 1], a0 + A.gs)
+                                      : std::min<int64_t>(a0 + A.gs, A.nt);
             const int mt = (int)(b0 - a0);
 
             // ---- targets: lane holds a0 + lane + 64 j ----
@@ -618,8 +622,9 @@ static ptg_status launch_walk(const WalkArgs<T> &a, int fields, bool quad, int64
 }
 
 extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const double *soft, const int64_t *perm,
-                               int64_t nt, int fields, int quadrupole, int group_size, double theta, double G,
-                               int exact, double *out, int64_t *counters, void *ws, size_t ws_bytes, void *stream)
+                               int64_t nt, int fields, int quadrupole, int group_size, const int32_t *group_offsets,
+                               const int32_t *group_count, double theta, double G, int exact, double *out,
+                               int64_t *counters, void *ws, size_t ws_bytes, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!wt || nt < 0 || (nt > 0 && (!pos || !soft || !perm || !out))) {
@@ -642,6 +647,10 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const
         ptg_set_error("ptg_walk: fields must be a nonzero combination of PTG_POT|PTG_ACC|PTG_TIDAL");
         return PTG_ERR_ARG;
     }
+    if ((group_offsets == nullptr) != (group_count == nullptr)) {
+        ptg_set_error("ptg_walk: group_offsets and group_count must be given together");
+        return PTG_ERR_ARG;
+    }
     if (nt == 0) return PTG_OK;
     if (ws_bytes < ptg_walk_workspace_bytes(nt, group_size) || !ws) {
         ptg_set_error("ptg_walk: workspace too small (%zu < %zu)", ws_bytes, ptg_walk_workspace_bytes(nt, group_size));
@@ -655,8 +664,8 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const
     ptg_status st;
     if (wt->prec == PTG_F32) {
         WalkArgs<float> a{(const V4<float> *)wt->xyzm, (const V4<float> *)wt->aux, (const int2 *)wt->kids,
-                          (const float *)wt->quad, pos, soft, perm, wt->nslots, nt, ptg_cdiv(nt, group_size), group_size,
-                          (float)theta, G, wt->origin[0], wt->origin[1], wt->origin[2], out, counters, ctl, spill};
+                          (const float *)wt->quad, pos, soft, perm, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
+                          group_size, (float)theta, G, wt->origin[0], wt->origin[1], wt->origin[2], out, counters, ctl, spill};
         if (exact) {
             ptg_set_error("ptg_walk: the exact branch set is evaluated in FP64; use prec=PTG_F64");
             return PTG_ERR_ARG;
@@ -664,8 +673,8 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const
         st = launch_walk<float, false>(a, fields, quadrupole != 0, blocks, s);
     } else if (wt->prec == PTG_F64) {
         WalkArgs<double> a{(const V4<double> *)wt->xyzm, (const V4<double> *)wt->aux, (const int2 *)wt->kids,
-                           (const double *)wt->quad, pos, soft, perm, wt->nslots, nt, ptg_cdiv(nt, group_size), group_size,
-                           theta, G, wt->origin[0], wt->origin[1], wt->origin[2], out, counters, ctl, spill};
+                           (const double *)wt->quad, pos, soft, perm, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
+                           group_size, theta, G, wt->origin[0], wt->origin[1], wt->origin[2], out, counters, ctl, spill};
         st = launch_walk<double, true>(a, fields, quadrupole != 0, blocks, s);
     } else {
         ptg_set_error("ptg_walk: prec must be PTG_F32 or PTG_F64");

@@ -101,7 +101,7 @@ class DeviceTree:
             ptr(self.deltas).value, ptr(self.quadrupoles).value if quadrupole else None,
             ptr(self.first_subnode).value, ptr(self.next_branch).value, ptr(self.order).value,
             ptr(self.nchild).value, ptr(self.child_base).value)
-        nws = workspace(L.ptg_node_workspace_bytes(n, nn, int(quadrupole)))
+        nws = workspace(L.ptg_node_workspace_bytes(n, nn, int(info.max_level)))
         check(L.ptg_build_tree(ptr(pos), ptr(m), ptr(h), _lib.ctypes.byref(info), int(quadrupole),
                                int(compute_moments), ptr(ws), ws.numel(), ptr(nws), nws.numel(),
                                _lib.ctypes.byref(self.arrays), s), "ptg_build_tree")
@@ -158,10 +158,28 @@ def morton_order(pos):
 _walk_ws = {}
 
 
+def target_groups(pos, perm, group_size):
+    """Cell-aligned groups of the Morton-sorted targets pos[perm[t]]: (offsets, count) device int32
+    tensors for ptg_walk (ptg_target_groups)."""
+    L = _lib.load()
+    pos = to_dev(pos, (-1, 3))
+    n = pos.shape[0]
+    offsets = torch.empty(n + 1, dtype=torch.int32, device=device())
+    count = torch.empty(1, dtype=torch.int32, device=device())
+    ws = workspace(L.ptg_groups_workspace_bytes(n))
+    check(L.ptg_target_groups(ptr(pos), ptr(perm), n, int(group_size), ptr(offsets), ptr(count), ptr(ws), ws.numel(),
+                              stream()), "ptg_target_groups")
+    return offsets, count
+
+
 def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7, G=1.0, precision="float64",
-         counters=None):
+         counters=None, grouping=None, groups=None):
     """Grouped walk of `tree` at targets pos[perm[t]]; returns an (nt, W) float64 device tensor in the
-    original target order (FieldsTarget_grouped, grouped_treewalk.py:350-397)."""
+    original target order (FieldsTarget_grouped, grouped_treewalk.py:350-397).
+
+    grouping: "fixed" = runs of group_size sorted targets (the reference's groups), "cells" =
+    cell-aligned groups (ptg_target_groups); default "fixed" for float64, "cells" for float32.
+    groups: precomputed (offsets, count) from target_groups, overriding grouping."""
     L = _lib.load()
     pos = to_dev(pos, (-1, 3))
     nt = pos.shape[0]
@@ -171,6 +189,13 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     if nt == 0:
         return out
     prec = PTG_F32 if precision in ("float32", "f32", np.float32, torch.float32) else PTG_F64
+    if grouping is None:
+        grouping = "cells" if prec == PTG_F32 else "fixed"
+    if grouping not in ("fixed", "cells"):
+        raise ValueError(f"grouping must be 'fixed' or 'cells', got {grouping!r}")
+    if groups is None and grouping == "cells":
+        groups = target_groups(pos, perm, group_size)
+    goff, gcnt = groups if groups is not None else (None, None)
     p = tree.pack(prec, quadrupole)
     wsb = L.ptg_walk_workspace_bytes(nt, int(group_size))
     key = (torch.cuda.current_device(), wsb)
@@ -180,8 +205,8 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
         _walk_ws.clear()
         _walk_ws[key] = ws
     check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), ptr(pos), ptr(soft), ptr(perm), nt, int(mask), int(bool(quadrupole)),
-                     int(group_size), float(theta), float(G), int(prec == PTG_F64), ptr(out), ptr(counters), ptr(ws),
-                     ws.numel(), stream()), "ptg_walk")
+                     int(group_size), ptr(goff), ptr(gcnt), float(theta), float(G), int(prec == PTG_F64), ptr(out),
+                     ptr(counters), ptr(ws), ws.numel(), stream()), "ptg_walk")
     check(L.ptg_walk_check(ptr(ws), stream()), "ptg_walk")
     return out
 

@@ -15,7 +15,7 @@ from gpu_cases import case
 pytestmark = pytest.mark.gpu
 
 
-def _walk(x, m, h, mask, quadrupole, gs, theta, precision, G=1.0):
+def _walk(x, m, h, mask, quadrupole, gs, theta, precision, G=1.0, grouping="fixed"):
     import torch
 
     from pytreegrav_amd import ConstructTree, engine
@@ -23,7 +23,7 @@ def _walk(x, m, h, mask, quadrupole, gs, theta, precision, G=1.0):
     t = ConstructTree(x, m, h, quadrupole=quadrupole)
     ctr = torch.zeros(2, dtype=torch.int64, device=engine.device())
     out = engine.walk(t.dev, x, h, t.dev.order, mask, quadrupole=quadrupole, group_size=gs, theta=theta, G=G,
-                      precision=precision, counters=ctr)
+                      precision=precision, counters=ctr, grouping=grouping)
     res = {k: v.cpu().numpy() for k, v in engine.unpack_fields(out, mask).items()}
     return res, ctr.cpu().numpy()
 
@@ -90,13 +90,14 @@ def test_fp32_self_term_excluded():
     assert np.abs(g32["potential"] - g64["potential"]).max() < 1e-3 * 2.8 * m[0] / h[0]
 
 
-def test_fp32_accuracy_against_bruteforce():
+@pytest.mark.parametrize("grouping", ["fixed", "cells"])
+def test_fp32_accuracy_against_bruteforce(grouping):
     from pytreegrav_amd.sampling import plummer
 
     x, m = plummer(100000, seed=42)
     h = np.zeros_like(m)
     exact = oracle.fields_self_bruteforce(x, m, h, potential=True, accel=True)
-    got, _ = _walk(x, m, h, 3, False, 64, 0.7, "float32")
+    got, _ = _walk(x, m, h, 3, False, 64, 0.7, "float32", grouping=grouping)
     scale = np.sqrt(np.mean(np.sum(exact["accel"] ** 2, 1)))
     ea = np.sqrt(np.mean(np.sum((got["accel"] - exact["accel"]) ** 2, 1))) / scale
     # group_size 64 opens a superset of what group_size 8 opens: at least as accurate as the
@@ -124,3 +125,31 @@ def test_target_form_and_permutation_invariance():
         ref = np.empty_like(a)
         ref[oracle.morton_order(xt)] = o["potential" if fn is PotentialTarget else "accel"]
         assert np.abs(a - ref).max() / np.abs(ref).max() < 1e-10
+
+
+@pytest.mark.parametrize("name,gs", [("plummer_1e5_h", 64), ("plummer_1e5_h", 256), ("clustered", 32), ("dups", 64),
+                                     ("n3", 64)])
+def test_cell_groups_partition_targets(name, gs):
+    """Cell-aligned groups: contiguous, cover every sorted target once, at most gs each, and never
+    straddle a cell holding more than 4*gs targets (checked against the oracle's keys)."""
+    from pytreegrav_amd import ConstructTree, engine
+
+    x, m, h = case(name)
+    t = ConstructTree(x, m, h)
+    off, cnt = engine.target_groups(x, t.dev.order, gs)
+    G = int(cnt.item())
+    off = off.cpu().numpy()[:G + 1]
+    n = len(x)
+    assert off[0] == 0 and off[G] == n and np.all(np.diff(off) >= 1) and np.all(np.diff(off) <= gs)
+    keys = oracle.morton_keys(x[t.TreewalkIndices], *oracle.bounding_cube(x))
+    for g in range(min(G, 2000)):
+        a, b = off[g], off[g + 1] - 1
+        if a == b:
+            continue
+        # the smallest cell holding the whole group has at most 4*gs targets
+        L = 0
+        while L < 21 and (keys[a] >> (63 - 3 * (L + 1))) == (keys[b] >> (63 - 3 * (L + 1))):
+            L += 1
+        pref = keys[a] >> (63 - 3 * L) if L > 0 else 0
+        inside = (keys >> (63 - 3 * L)) == pref if L > 0 else np.ones(n, bool)
+        assert inside.sum() <= 4 * gs or L == 21
