@@ -38,6 +38,10 @@ FLOPS_PER_INTERACTION = 20  # monopole potential + acceleration
 # tree Potential + Accel at N=1e7 Plummer, theta=0.7, 32 threads, two calls:
 # 1.005e-6 + 1.124e-6 s/particle (examples/benchmark_scaling.json:51,131)
 REF_PUBLISHED = 1.0 / (1.005e-6 + 1.124e-6)
+# Calibrated HBM bytes per walk launch from two rocprofv3 --pmc passes (FETCH_SIZE x2.000,
+# WRITE_SIZE x1.000, factors from a 2048 MiB device copy); see
+# profiles/k_walk_plummer1e7_hbm_traffic.md.  Keyed by (N, group_size, theta, quadrupole).
+WALK_TRAFFIC = {(10_000_000, 64, 0.7, False): 8.329e9}
 
 
 def parse():
@@ -54,6 +58,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--cpu-sample-frac", type=float, default=0.125)
+    ap.add_argument("--quadrupole", action="store_true", help="walk with quadrupole moments")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
+                    help="only run a plain device copy of this many MiB (HBM counter calibration) and exit")
     return ap.parse_args()
 
 
@@ -116,6 +123,15 @@ def main():
 
     import torch
 
+    if args.copy_probe:
+        a = torch.empty(args.copy_probe * 2 ** 20, dtype=torch.uint8, device="cuda").fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel()}))
+        return
+
     from pytreegrav_amd import ConstructTree, engine
     from pytreegrav_amd.sampling import plummer_truncated
 
@@ -131,15 +147,17 @@ def main():
 
     walk_times = []
 
+    quad = bool(args.quadrupole)
+
     def step(record):
-        tree = ConstructTree(pos, mass, soft)
-        tree.dev.pack(engine.PTG_F32, False)
+        tree = ConstructTree(pos, mass, soft, quadrupole=quad)
+        tree.dev.pack(engine.PTG_F32, quad)
         if record:
             ctr.zero_()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
-        out = engine.walk(tree.dev, pos, soft, tree.dev.order, mask, group_size=args.group_size, theta=args.theta,
-                          precision="float32", counters=ctr if record else None)
+        out = engine.walk(tree.dev, pos, soft, tree.dev.order, mask, quadrupole=quad, group_size=args.group_size,
+                          theta=args.theta, precision="float32", counters=ctr if record else None)
         if record:
             e1.record()
             walk_times.append((e0, e1))
@@ -161,8 +179,8 @@ def main():
     torch.cuda.synchronize()
     tb0 = time.perf_counter()
     for _ in range(args.steps):
-        tr = ConstructTree(pos, mass, soft)
-        tr.dev.pack(engine.PTG_F32, False)
+        tr = ConstructTree(pos, mass, soft, quadrupole=quad)
+        tr.dev.pack(engine.PTG_F32, quad)
     torch.cuda.synchronize()
     build_ms = (time.perf_counter() - tb0) / args.steps * 1e3
 
@@ -218,7 +236,7 @@ def main():
             "group_size": args.group_size,
             "walk_precision": "fp32 node-relative, fp32 accumulation",
             "build_precision": "fp64 keys and moments",
-            "moments": "monopole",
+            "moments": "quadrupole" if quad else "monopole",
             "parallelism": "single GPU",
         },
         "breakdown": {
@@ -235,7 +253,7 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": WALK_TRAFFIC.get((N, args.group_size, args.theta, bool(args.quadrupole))),
             "flops_per_launch": flops,
             "convention": "20 flops per target-element interaction (monopole pot+acc)",
         },
