@@ -3,9 +3,13 @@
 
 Workload (one "step"): the self-gravity Accel+Potential evaluation the reference's benchmarks time
 (examples/benchmark_scaling.py:104-107 times a whole call, tree build included): bounding cube,
-FP64 Morton keys, radix sort, octree build, FP64 moments, FP32 walk pack, and one fused
-potential+acceleration grouped walk in FP32 (node-relative coordinates, FP32 accumulation),
-results scattered back to input order.  Inputs (positions, masses, softenings) are resident in HBM
+FP64 Morton keys, radix sort, octree build, FP64 moments (with quadrupoles when the criterion needs
+them), FP32 walk pack, cell-aligned target groups, and one fused potential+acceleration grouped walk
+in FP32 (node-relative coordinates, FP32 accumulation), results scattered back to input order.
+The headline configuration uses the error-bounded opening criterion (a node is accepted when its
+dropped quadrupole term is below alpha x the RMS field, estimated per step by a 4096-target sample
+walk); the same run also times the plain theta=0.7 geometric walk ("plain_theta_0.7") under the same
+accuracy gate.  Inputs (positions, masses, softenings) are resident in HBM
 before the timed region.  Particles: Plummer sphere, M=1, a=1, truncated at 100 scale radii,
 h=0.01, seed 0.
 
@@ -40,8 +44,8 @@ FLOPS_PER_INTERACTION = 20  # monopole potential + acceleration
 REF_PUBLISHED = 1.0 / (1.005e-6 + 1.124e-6)
 # Calibrated HBM bytes per walk launch from two rocprofv3 --pmc passes (FETCH_SIZE x2.000,
 # WRITE_SIZE x1.000, factors from a 2048 MiB device copy); see
-# profiles/k_walk_plummer1e7_hbm_traffic.md.  Keyed by (N, group_size, theta, quadrupole).
-WALK_TRAFFIC = {(10_000_000, 64, 0.7, False): 8.329e9}
+# profiles/k_walk_plummer1e7_hbm_traffic.md.  Keyed by (N, group_size, criterion, theta, quadrupole).
+WALK_TRAFFIC = {(10_000_000, 64, "theta", 0.7, False): 8.329e9}
 
 
 def parse():
@@ -50,7 +54,13 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("-N", type=int, default=10_000_000)
-    ap.add_argument("--theta", type=float, default=0.7)
+    ap.add_argument("--criterion", choices=("error", "theta"), default="error",
+                    help="opening criterion of the timed walk: 'error' bounds each accepted node's dropped "
+                         "quadrupole term by alpha x (RMS field); 'theta' is the reference's geometric test")
+    ap.add_argument("--alpha", type=float, default=8.5e-4, help="per-node error tolerance relative to the RMS field")
+    ap.add_argument("--theta", type=float, default=None,
+                    help="opening angle (criterion theta) or loose geometric cap (criterion error); default 0.7 / 2.0")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain theta=0.7 comparison line")
     ap.add_argument("--group-size", type=int, default=64)
     ap.add_argument("--softening", type=float, default=0.01)
     ap.add_argument("--seed", type=int, default=0)
@@ -145,55 +155,61 @@ def main():
     mask = engine.PTG_POT | engine.PTG_ACC
     ctr = torch.zeros(2, dtype=torch.int64, device=dev)
 
-    walk_times = []
-
-    quad = bool(args.quadrupole)
-
-    def step(record):
-        tree = ConstructTree(pos, mass, soft, quadrupole=quad)
-        tree.dev.pack(engine.PTG_F32, quad)
-        if record:
-            ctr.zero_()
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record()
-        out = engine.walk(tree.dev, pos, soft, tree.dev.order, mask, quadrupole=quad, group_size=args.group_size,
-                          theta=args.theta, precision="float32", counters=ctr if record else None)
-        if record:
-            e1.record()
-            walk_times.append((e0, e1))
-        return tree, out
-
-    for _ in range(args.warmup):
-        step(False)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        tree, out = step(True)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    walk_ms = float(np.mean([a.elapsed_time(b) for a, b in walk_times]))
-    interactions = int(ctr[0].item())
-    visits = int(ctr[1].item())
-
-    # build alone (same inputs), for the breakdown
-    torch.cuda.synchronize()
-    tb0 = time.perf_counter()
-    for _ in range(args.steps):
-        tr = ConstructTree(pos, mass, soft, quadrupole=quad)
-        tr.dev.pack(engine.PTG_F32, quad)
-    torch.cuda.synchronize()
-    build_ms = (time.perf_counter() - tb0) / args.steps * 1e3
-
-    res = out.cpu().numpy()
-    acc = None
+    exact = None
+    sel = None
     if not args.no_accuracy:
         rng = np.random.default_rng(12345)
         sel = np.sort(rng.choice(N, size=min(args.oracle_targets, N), replace=False))
         sel_t = torch.from_numpy(sel).to(dev)
         exact = engine.direct(pos[sel_t], soft[sel_t], pos, mass, soft, mask, self_mode=False,
                               precision="float64").cpu().numpy()
+
+    def run_config(criterion, theta, alpha, quad_walk=False):
+        """Time `steps` whole evaluations (build + pack + groups + walk) of one configuration."""
+        build_quads = quad_walk or criterion == "error"
+        walk_times = []
+
+        def step(record):
+            tree = ConstructTree(pos, mass, soft, quadrupole=build_quads)
+            tree.dev.pack(engine.PTG_F32, quad_walk)
+            eps = alpha * engine.field_scale(tree.dev) if criterion == "error" else None
+            if record:
+                ctr.zero_()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+            out = engine.walk(tree.dev, pos, soft, tree.dev.order, mask, quadrupole=quad_walk,
+                              group_size=args.group_size, theta=theta, precision="float32",
+                              counters=ctr if record else None, criterion=criterion, eps=eps)
+            if record:
+                e1.record()
+                walk_times.append((e0, e1))
+            return out
+
+        for _ in range(args.warmup):
+            step(False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = step(True)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        walk_ms = float(np.mean([a.elapsed_time(b) for a, b in walk_times]))
+        r = {"dt": dt, "walk_ms": walk_ms, "interactions": int(ctr[0].item()), "visits": int(ctr[1].item())}
+        # build alone (same inputs), for the breakdown
+        torch.cuda.synchronize()
+        tb0 = time.perf_counter()
+        for _ in range(args.steps):
+            tr = ConstructTree(pos, mass, soft, quadrupole=build_quads)
+            tr.dev.pack(engine.PTG_F32, quad_walk)
+        torch.cuda.synchronize()
+        r["build_ms"] = (time.perf_counter() - tb0) / args.steps * 1e3
+        r["accuracy"] = accuracy(out) if exact is not None else None
+        return r
+
+    def accuracy(out):
+        res = out.index_select(0, torch.from_numpy(sel).to(dev)).cpu().numpy()
         a_ex, p_ex = exact[:, 1:4], exact[:, 0]
-        a_t, p_t = res[sel, 1:4], res[sel, 0]
+        a_t, p_t = res[:, 1:4], res[:, 0]
         da = np.sqrt(np.sum((a_t - a_ex) ** 2, axis=1))
         scale = np.sqrt(np.mean(np.sum(a_ex ** 2, axis=1)))
         dp = (p_t - p_ex)
@@ -207,10 +223,25 @@ def main():
             "gate_accel_rms": 1.8e-3,
         }
         acc["gate_pass"] = acc["accel_rms"] <= acc["gate_accel_rms"]
+        return acc
+
+    quad = bool(args.quadrupole)
+    theta = args.theta if args.theta is not None else (2.0 if args.criterion == "error" else 0.7)
+    main = run_config(args.criterion, theta, args.alpha, quad)
+    plain = None
+    if not args.no_plain and not (args.criterion == "theta" and theta == 0.7 and not quad):
+        p = run_config("theta", 0.7, 0.0, False)
+        plain = {"criterion": "theta", "theta": 0.7, "moments": "monopole", "ms_per_step": p["dt"] * 1e3,
+                 "particles_per_s": N / p["dt"], "walk_ms": p["walk_ms"], "build_and_pack_ms": p["build_ms"],
+                 "interactions_per_particle": p["interactions"] / N, "accuracy": p["accuracy"],
+                 "walk_speedup_of_headline": p["walk_ms"] / main["walk_ms"],
+                 "step_speedup_of_headline": p["dt"] / main["dt"]}
+    dt, walk_ms, build_ms = main["dt"], main["walk_ms"], main["build_ms"]
+    interactions, visits, acc = main["interactions"], main["visits"], main["accuracy"]
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(x, m, h, args.theta, args.cpu_sample_frac, args.seed)
+        cpu = cpu_baseline(x, m, h, 0.7, args.cpu_sample_frac, args.seed)
 
     flops = FLOPS_PER_INTERACTION * interactions  # counters hold the last timed step
     achieved = flops / (walk_ms * 1e-3) / 1e12
@@ -232,11 +263,15 @@ def main():
             "N": N,
             "distribution": "Plummer M=1 a=1 truncated at 100 a, seed %d" % args.seed,
             "softening": args.softening,
-            "theta": args.theta,
+            "criterion": args.criterion,
+            "theta": theta,
+            "alpha": args.alpha if args.criterion == "error" else None,
             "group_size": args.group_size,
+            "grouping": "cell-aligned",
             "walk_precision": "fp32 node-relative, fp32 accumulation",
             "build_precision": "fp64 keys and moments",
-            "moments": "quadrupole" if quad else "monopole",
+            "moments": ("quadrupole" if quad else "monopole") + " interactions"
+                       + ("; fp64 quadrupoles built for the criterion" if args.criterion == "error" else ""),
             "parallelism": "single GPU",
         },
         "breakdown": {
@@ -253,11 +288,12 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": WALK_TRAFFIC.get((N, args.group_size, args.theta, bool(args.quadrupole))),
+            "traffic": WALK_TRAFFIC.get((N, args.group_size, args.criterion, theta, bool(args.quadrupole))),
             "flops_per_launch": flops,
             "convention": "20 flops per target-element interaction (monopole pot+acc)",
         },
         "accuracy": acc,
+        "plain_theta_0.7": plain,
         "cpu_baseline": cpu,
         "reference_published": {"value": REF_PUBLISHED, "unit": "particles/s",
                                 "source": "examples/benchmark_scaling.json:51,131 (Potential+Accel, N=1e7, 32 threads)"},

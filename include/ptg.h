@@ -78,6 +78,7 @@ typedef struct {
     int64_t *treewalk_indices;/* (n): input index of each Morton slot */
     int32_t *nchild;          /* (n+nnodes): number of children (0 for particles) */
     int32_t *child_base;      /* (n+nnodes): walk slot of the first child (sibling-contiguous layout) */
+    int32_t *slot;            /* (n+nnodes): walk slot of each element (the root is slot 0) */
 } ptg_tree_arrays;
 
 size_t ptg_build_workspace_bytes(int64_t n);
@@ -136,11 +137,24 @@ ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
  * those tests where the group geometry proves them false.  counters (nullable, device int64[2])
  * accumulate: [0] element-target interactions, [1] traversal visits.
  * ------------------------------------------------------------------------------------------ */
+/* Opening criterion.  PTG_CRIT_THETA: the reference's geometric test (treewalk.py:22-25).
+ * PTG_CRIT_QERR (no reference counterpart): a node is accepted only if it also passes the
+ * geometric test at `theta` (used as a loose cap) and its dropped quadrupole term, estimated as
+ * 3.5 |Q|_F / r_min^4 with r_min the distance to the group's bounding box, is below `eps`
+ * (an absolute acceleration tolerance, in units of G=1); needs a tree built with quadrupoles. */
+#define PTG_CRIT_THETA 0
+#define PTG_CRIT_QERR 1
+typedef struct {
+    int32_t criterion;
+    int32_t reserved;
+    double eps;
+} ptg_criterion;
+
 size_t ptg_walk_workspace_bytes(int64_t nt, int group_size);
 ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const double *soft, const int64_t *perm, int64_t nt,
                     int fields, int quadrupole, int group_size, const int32_t *group_offsets,
-                    const int32_t *group_count, double theta, double G, int exact, double *out, int64_t *counters,
-                    void *ws, size_t ws_bytes, void *stream);
+                    const int32_t *group_count, double theta, const ptg_criterion *criterion, double G,
+                    int exact, double *out, int64_t *counters, void *ws, size_t ws_bytes, void *stream);
 /* Reads the overflow flag of the last walk on this workspace (synchronises the stream):
  * PTG_ERR_OVERFLOW if a traversal stack exceeded its spill capacity. */
 ptg_status ptg_walk_check(void *ws, void *stream);

@@ -47,6 +47,7 @@ class TreeArrays(_c.Structure):
         ("treewalk_indices", _vp),
         ("nchild", _vp),
         ("child_base", _vp),
+        ("slot", _vp),
     ]
 
 
@@ -63,6 +64,12 @@ class WalkTree(_c.Structure):
     ]
 
 
+class Criterion(_c.Structure):
+    _fields_ = [("criterion", _i32), ("reserved", _i32), ("eps", _dbl)]
+
+
+PTG_CRIT_THETA, PTG_CRIT_QERR = 0, 1
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "ptg_version": (_c.c_int, []),
@@ -78,7 +85,7 @@ SIGNATURES = {
     "ptg_pack_walk": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _c.POINTER(WalkTree), _vp]),
     "ptg_walk_workspace_bytes": (_c.c_size_t, [_i64, _c.c_int]),
     "ptg_walk": (_c.c_int, [_c.POINTER(WalkTree), _vp, _vp, _vp, _i64, _c.c_int, _c.c_int, _c.c_int, _vp, _vp,
-                            _dbl, _dbl, _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),
+                            _dbl, _c.POINTER(Criterion), _dbl, _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),
     "ptg_walk_check": (_c.c_int, [_vp, _vp]),
     "ptg_direct": (_c.c_int, [_vp, _vp, _i64, _vp, _vp, _vp, _i64, _c.c_int, _c.c_int, _c.c_int, _dbl, _vp, _vp]),
 }

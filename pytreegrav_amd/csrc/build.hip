@@ -269,22 +269,42 @@ __global__ __launch_bounds__(256) void k_bbox_partial(const double *pos, int64_t
     }
 }
 
-__global__ void k_bbox_final(const double *part, int nb, Geom *g)
+__global__ __launch_bounds__(256) void k_bbox_final(const double *part, int nb, Geom *g)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double sh[6][4];
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int b = 0; b < nb; b++)
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
         for (int k = 0; k < 3; k++) {
             lo[k] = fmin(lo[k], part[6 * b + k]);
             hi[k] = fmax(hi[k], part[6 * b + 3 + k]);
         }
+    int wave = threadIdx.x >> 6, lane = ptg_lane();
+    for (int k = 0; k < 3; k++) {
+        lo[k] = ptg_wave_min(lo[k]);
+        hi[k] = ptg_wave_max(hi[k]);
+    }
+    if (lane == 0)
+        for (int k = 0; k < 3; k++) {
+            sh[k][wave] = lo[k];
+            sh[3 + k][wave] = hi[k];
+        }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 3; k++) {
+        lo[k] = sh[k][0];
+        hi[k] = sh[3 + k][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            lo[k] = fmin(lo[k], sh[k][w]);
+            hi[k] = fmax(hi[k], sh[3 + k][w]);
+        }
+    }
     for (int k = 0; k < 3; k++) g->center[k] = 0.5 * (hi[k] + lo[k]);
-    double s = hi[0] - lo[0];
-    if (hi[1] - lo[1] > s) s = hi[1] - lo[1];
-    if (hi[2] - lo[2] > s) s = hi[2] - lo[2];
-    g->size = s;
-    g->inv = s > 0 ? 1.0 / s : 0.0;
-    for (int k = 0; k < 3; k++) g->cmin[k] = g->center[k] - 0.5 * s;
+    double sz = hi[0] - lo[0];
+    if (hi[1] - lo[1] > sz) sz = hi[1] - lo[1];
+    if (hi[2] - lo[2] > sz) sz = hi[2] - lo[2];
+    g->size = sz;
+    g->inv = sz > 0 ? 1.0 / sz : 0.0;
+    for (int k = 0; k < 3; k++) g->cmin[k] = g->center[k] - 0.5 * sz;
 }
 
 __device__ __forceinline__ uint64_t morton_key(double x, double y, double z, const double *cmin, double inv)
@@ -457,6 +477,23 @@ __global__ __launch_bounds__(256) void k_collision_gather(const double *pos, con
     perm[j] = (uint32_t)j;
 }
 
+// small collision subsets: stable sort by (K0, K1, K2, K3) through rank counting, rank of j = number
+// of subset elements ordered before it (index breaks ties, so the order is stable)
+__global__ __launch_bounds__(256) void k_rank_sort4(const uint64_t *k0, const uint64_t *k1, const uint64_t *k2,
+                                                    const uint64_t *k3, int64_t m, uint32_t *perm)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const uint64_t a0 = k0[j], a1 = k1[j], a2 = k2[j], a3 = k3[j];
+    int64_t rank = 0;
+    for (int64_t i = 0; i < m; i++) {
+        uint64_t b0 = k0[i], b1 = k1[i], b2 = k2[i], b3 = k3[i];
+        bool before = b0 != a0 ? b0 < a0 : b1 != a1 ? b1 < a1 : b2 != a2 ? b2 < a2 : b3 != a3 ? b3 < a3 : i < j;
+        rank += before;
+    }
+    perm[rank] = (uint32_t)j;
+}
+
 __global__ __launch_bounds__(256) void k_gather_u64(const uint64_t *src, const uint32_t *perm, int64_t m,
                                                     uint64_t *dst)
 {
@@ -556,7 +593,7 @@ static ptg_status sort_by_keys(const double *pos, int64_t n, BuildWS &w, hipStre
 {
     int nb = (int)std::min<int64_t>(BB_BLOCKS, std::max<int64_t>(1, ptg_cdiv(n, 1024)));
     k_bbox_partial<<<nb, 256, 0, s>>>(pos, n, w.part);
-    k_bbox_final<<<1, 64, 0, s>>>(w.part, nb, w.geom);
+    k_bbox_final<<<1, 256, 0, s>>>(w.part, nb, w.geom);
     k_keys<<<grid1(n), 256, 0, s>>>(pos, n, w.geom, w.kA, w.vA);
     PTG_LAUNCH_CHECK();
     bool in_first;
@@ -633,13 +670,19 @@ extern "C" ptg_status ptg_build_sort(const double *pos, int64_t n, void *ws, siz
             PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
             PTG_HIP_CHECK(hipStreamSynchronize(s));
             int64_t m = hc.nsub;
-            uint64_t *karr[4] = {w.sk3, w.sk2, w.sk1, w.sk0};
-            for (int r = 0; r < 4; r++) {
-                k_gather_u64<<<grid1(m), 256, 0, s>>>(karr[r], w.perm, m, w.tkA);
-                bool in_first;
-                st = radix_sort(w.tkA, w.perm, w.tkB, w.permB, m, 64, w.hist, w.bsum, &in_first, s);
-                if (st) return st;
-                if (!in_first) PTG_HIP_CHECK(hipMemcpyAsync(w.perm, w.permB, m * 4, hipMemcpyDeviceToDevice, s));
+            if (m <= 8192) {
+                k_rank_sort4<<<grid1(m), 256, 0, s>>>(w.sk0, w.sk1, w.sk2, w.sk3, m, w.perm);
+                PTG_LAUNCH_CHECK();
+            } else {
+                uint64_t *karr[4] = {w.sk3, w.sk2, w.sk1, w.sk0};
+                for (int r = 0; r < 4; r++) {
+                    k_gather_u64<<<grid1(m), 256, 0, s>>>(karr[r], w.perm, m, w.tkA);
+                    bool in_first;
+                    st = radix_sort(w.tkA, w.perm, w.tkB, w.permB, m, 64, w.hist, w.bsum, &in_first, s);
+                    if (st) return st;
+                    if (!in_first)
+                        PTG_HIP_CHECK(hipMemcpyAsync(w.perm, w.permB, m * 4, hipMemcpyDeviceToDevice, s));
+                }
             }
             PTG_HIP_CHECK(hipMemsetAsync(w.ext, 0, 3 * n * sizeof(uint64_t), s));
             k_collision_apply<<<grid1(m), 256, 0, s>>>(w.perm, m, w.sslot, w.sval, w.sk1, w.sk2, w.sk3, w.vA, w.ext,
@@ -806,6 +849,7 @@ struct NodeWS {
     uint32_t *idA, *idB;   // node offsets (sort values): level-sorted node list
     int32_t *hist;
     int32_t *lvstart;      // (max_level + 2) first position of each level in the sorted list
+    int32_t *elem;         // (n + nnodes) element stored at each walk slot
 };
 
 static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, int64_t max_level, NodeWS &w)
@@ -823,6 +867,7 @@ static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, int64_t max_leve
     w.idB = A.take<uint32_t>(nnodes);
     w.hist = A.take<int32_t>(nh);
     w.lvstart = A.take<int32_t>(max_level + 2);
+    w.elem = A.take<int32_t>(n + nnodes);
     return A.ok();
 }
 
@@ -884,6 +929,35 @@ __global__ __launch_bounds__(256) void k_link(int64_t n, int64_t nnodes, const p
         for (int k = 0; k < 9; k++) t.quadrupoles[9 * idx + k] = 0.0;
 }
 
+// walk slots: child q of node idx sits at slot child_base[idx] + q (the root at slot 0); record the
+// element -> slot map (for the pack) and its inverse (for the moments pass)
+__global__ __launch_bounds__(256) void k_slots(int64_t n, int64_t nnodes, const ptg_tree_arrays t, NodeWS w)
+{
+    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnodes) return;
+    int64_t idx = n + a;
+    if (a == 0) {
+        t.slot[n] = 0;
+        w.elem[0] = (int32_t)n;
+    }
+    int32_t kids[8];
+    int nk = node_children(t.first_subnode, t.next_branch, idx, n + nnodes, kids);
+    if (nk < 0) {
+        w.err[0] = 1;
+        return;
+    }
+    int64_t base = t.child_base[idx];
+    for (int q = 0; q < nk; q++) {
+        int64_t sl = base + q;
+        if (sl <= 0 || sl >= n + nnodes) {
+            w.err[0] = 1;
+            continue;
+        }
+        t.slot[kids[q]] = (int32_t)sl;
+        w.elem[sl] = kids[q];
+    }
+}
+
 template <typename T> __device__ __forceinline__ T ld_agent(const T *p) { return *p; }
 
 // One node's moments, in the reference's summation order: particle children in digit order
@@ -893,11 +967,13 @@ template <typename T> __device__ __forceinline__ T ld_agent(const T *p) { return
 __device__ void node_moments(int64_t idx, int64_t n, int64_t nnodes, const ptg_tree_arrays &t, NodeWS &w, bool quad)
 {
     int32_t kids[8];
-    int nk = node_children(t.first_subnode, t.next_branch, idx, n + nnodes, kids);
-    if (nk < 0) {
+    int nk = t.nchild[idx];
+    int64_t base = t.child_base[idx];
+    if (nk < 0 || nk > 8 || base < 1 || base + nk > n + nnodes) {
         w.err[0] = 1;
         return;
     }
+    for (int q = 0; q < nk; q++) kids[q] = w.elem[base + q];
     double m = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, hs = 0.0;
     for (int q = 0; q < nk; q++) {
         int32_t c = kids[q];
@@ -1044,7 +1120,8 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     }
     const ptg_tree_arrays t = *tree;
     if (!t.coordinates || !t.masses || !t.softenings || !t.sizes || !t.deltas || !t.first_subnode ||
-        !t.next_branch || !t.treewalk_indices || !t.nchild || !t.child_base || (quadrupole && !t.quadrupoles)) {
+        !t.next_branch || !t.treewalk_indices || !t.nchild || !t.child_base || !t.slot ||
+        (quadrupole && !t.quadrupoles)) {
         ptg_set_error("ptg_build_tree: a required tree array is NULL");
         return PTG_ERR_ARG;
     }
@@ -1076,6 +1153,12 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     k_node_init<<<grid1(nnodes), 256, 0, s>>>(nnodes, nw);
     k_link<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t, nw);
     PTG_LAUNCH_CHECK();
+    // sibling-contiguous walk slots: exclusive scan of child counts in node order
+    ptg_status st = excl_scan(LoadNChild{t.nchild, n}, nnodes, t.child_base + n, nw.bsum, nullptr, s);
+    if (st) return st;
+    k_child_base_fix<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t.child_base);
+    k_slots<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t, nw);
+    PTG_LAUNCH_CHECK();
     if (compute_moments) {
         // counting sort of node offsets by level (stable LSD radix sort on the level)
         k_iota_u32<<<grid1(nnodes), 256, 0, s>>>(nw.idA, nnodes);
@@ -1095,11 +1178,6 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
         k_fold_particles<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t);
     }
     PTG_LAUNCH_CHECK();
-    // sibling-contiguous walk slots: exclusive scan of child counts in node order
-    ptg_status st = excl_scan(LoadNChild{t.nchild, n}, nnodes, t.child_base + n, nw.bsum, nullptr, s);
-    if (st) return st;
-    k_child_base_fix<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t.child_base);
-    PTG_LAUNCH_CHECK();
     return PTG_OK;
 }
 
@@ -1118,7 +1196,18 @@ __device__ void pack_slot(int64_t e, int64_t slot, int64_t n, const ptg_tree_arr
     aux[0] = (T)t.softenings[e];
     aux[1] = (T)t.sizes[e];
     aux[2] = (T)t.deltas[e];
-    aux[3] = (T)0;
+    // Frobenius norm of the node's traceless quadrupole (the monopole truncation error scale used by
+    // the error-bounded criterion); -1 when the tree carries no quadrupoles
+    double qn = -1.0;
+    if (e >= n && t.quadrupoles) {
+        const double *Q = t.quadrupoles + 9 * e;
+        double s2 = 0.0;
+        for (int k = 0; k < 9; k++) s2 += Q[k] * Q[k];
+        qn = sqrt(s2);
+    } else if (e < n) {
+        qn = 0.0;
+    }
+    aux[3] = (T)qn;
     bool node = e >= n;
     wt.kids[2 * slot] = node ? t.child_base[e] : (int32_t)e;
     wt.kids[2 * slot + 1] = node ? t.nchild[e] : 0;
@@ -1139,15 +1228,10 @@ __device__ void pack_slot(int64_t e, int64_t slot, int64_t n, const ptg_tree_arr
 template <typename T>
 __global__ __launch_bounds__(256) void k_pack(int64_t n, int64_t nnodes, const ptg_tree_arrays t, const ptg_walk_tree wt)
 {
-    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nnodes) return;
-    int64_t idx = n + a;
-    if (a == 0) pack_slot<T>(idx, 0, n, t, wt);
-    int32_t kids[8];
-    int nk = node_children(t.first_subnode, t.next_branch, idx, n + nnodes, kids);
-    int64_t base = t.child_base[idx];
-    for (int q = 0; q < nk; q++)
-        if (base + q > 0 && base + q < wt.nslots) pack_slot<T>(kids[q], base + q, n, t, wt);
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n + nnodes) return;
+    int64_t sl = t.slot[e];
+    if (sl >= 0 && sl < wt.nslots) pack_slot<T>(e, sl, n, t, wt);
 }
 
 extern "C" ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_walk_tree *wt,
@@ -1162,8 +1246,8 @@ extern "C" ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int6
         ptg_set_error("ptg_pack_walk: a quadrupole pack needs a tree with quadrupoles");
         return PTG_ERR_ARG;
     }
-    if (wt->prec == PTG_F32) k_pack<float><<<grid1(nnodes), 256, 0, s>>>(n, nnodes, *tree, *wt);
-    else if (wt->prec == PTG_F64) k_pack<double><<<grid1(nnodes), 256, 0, s>>>(n, nnodes, *tree, *wt);
+    if (wt->prec == PTG_F32) k_pack<float><<<grid1(n + nnodes), 256, 0, s>>>(n, nnodes, *tree, *wt);
+    else if (wt->prec == PTG_F64) k_pack<double><<<grid1(n + nnodes), 256, 0, s>>>(n, nnodes, *tree, *wt);
     else {
         ptg_set_error("ptg_pack_walk: prec must be PTG_F32 or PTG_F64");
         return PTG_ERR_ARG;
@@ -1311,7 +1395,7 @@ extern "C" ptg_status ptg_target_groups(const double *pos, const int64_t *perm, 
     }
     int nb = (int)std::min<int64_t>(BB_BLOCKS, std::max<int64_t>(1, ptg_cdiv(n, 1024)));
     k_bbox_partial<<<nb, 256, 0, s>>>(pos, n, w.part);
-    k_bbox_final<<<1, 64, 0, s>>>(w.part, nb, w.geom);
+    k_bbox_final<<<1, 256, 0, s>>>(w.part, nb, w.geom);
     k_sorted_keys<<<grid1(n), 256, 0, s>>>(pos, perm, n, w.geom, w.keys);
     k_cell_cuts<<<grid1(n), 256, 0, s>>>(w.keys, n, 4 * (int64_t)group_size, w.pflag);
     PTG_LAUNCH_CHECK();

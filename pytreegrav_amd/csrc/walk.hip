@@ -51,6 +51,8 @@ template <typename T> struct WalkArgs {
     const int32_t *gcount;  // device group count when goff is given
     int gs;
     T theta;
+    int crit;  // 0: geometric (treewalk.py:22-25); 1: quadrupole error bound; 2: learned
+    T eps;     // criterion 1/2: per-node acceleration error tolerance (already divided by 3.5)
     double G;
     double ox, oy, oz;
     double *out;
@@ -425,6 +427,12 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                 T c1 = X2.y / A.theta + X2.z;
                 T c2 = hh + X2.y * T(0.6) + X2.z;
                 bool acc_node = ve && !ispart && r_min > (c1 > c2 ? c1 : c2);
+                if (A.crit == 1) {
+                    // error-bounded: also require the quadrupole term the monopole drops,
+                    // 3.5 |Q| / r^4, to stay below eps (theta is then only a loose cap)
+                    T r2m = r_min * r_min;
+                    acc_node = acc_node && X2.w < A.eps * r2m * r2m;
+                }
                 bool acc_part = ve && ispart;
                 bool open = ve && !ispart && !acc_node;
 
@@ -623,8 +631,8 @@ static ptg_status launch_walk(const WalkArgs<T> &a, int fields, bool quad, int64
 
 extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const double *soft, const int64_t *perm,
                                int64_t nt, int fields, int quadrupole, int group_size, const int32_t *group_offsets,
-                               const int32_t *group_count, double theta, double G, int exact, double *out,
-                               int64_t *counters, void *ws, size_t ws_bytes, void *stream)
+                               const int32_t *group_count, double theta, const ptg_criterion *criterion, double G,
+                               int exact, double *out, int64_t *counters, void *ws, size_t ws_bytes, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!wt || nt < 0 || (nt > 0 && (!pos || !soft || !perm || !out))) {
@@ -651,6 +659,16 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const
         ptg_set_error("ptg_walk: group_offsets and group_count must be given together");
         return PTG_ERR_ARG;
     }
+    const int crit = criterion ? criterion->criterion : PTG_CRIT_THETA;
+    if (crit != PTG_CRIT_THETA && crit != PTG_CRIT_QERR) {
+        ptg_set_error("ptg_walk: unknown criterion %d", crit);
+        return PTG_ERR_ARG;
+    }
+    if (crit == PTG_CRIT_QERR && (exact || !(criterion->eps > 0))) {
+        ptg_set_error("ptg_walk: the error-bounded criterion needs eps > 0 and the FP32 walk");
+        return PTG_ERR_ARG;
+    }
+    const double eps = crit == PTG_CRIT_QERR ? criterion->eps / 3.5 : 0.0;
     if (nt == 0) return PTG_OK;
     if (ws_bytes < ptg_walk_workspace_bytes(nt, group_size) || !ws) {
         ptg_set_error("ptg_walk: workspace too small (%zu < %zu)", ws_bytes, ptg_walk_workspace_bytes(nt, group_size));
@@ -665,7 +683,7 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const
     if (wt->prec == PTG_F32) {
         WalkArgs<float> a{(const V4<float> *)wt->xyzm, (const V4<float> *)wt->aux, (const int2 *)wt->kids,
                           (const float *)wt->quad, pos, soft, perm, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
-                          group_size, (float)theta, G, wt->origin[0], wt->origin[1], wt->origin[2], out, counters, ctl, spill};
+                          group_size, (float)theta, crit, (float)eps, G, wt->origin[0], wt->origin[1], wt->origin[2], out, counters, ctl, spill};
         if (exact) {
             ptg_set_error("ptg_walk: the exact branch set is evaluated in FP64; use prec=PTG_F64");
             return PTG_ERR_ARG;
@@ -674,7 +692,7 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const
     } else if (wt->prec == PTG_F64) {
         WalkArgs<double> a{(const V4<double> *)wt->xyzm, (const V4<double> *)wt->aux, (const int2 *)wt->kids,
                            (const double *)wt->quad, pos, soft, perm, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
-                           group_size, theta, G, wt->origin[0], wt->origin[1], wt->origin[2], out, counters, ctl, spill};
+                           group_size, theta, crit, eps, G, wt->origin[0], wt->origin[1], wt->origin[2], out, counters, ctl, spill};
         st = launch_walk<double, true>(a, fields, quadrupole != 0, blocks, s);
     } else {
         ptg_set_error("ptg_walk: prec must be PTG_F32 or PTG_F64");

@@ -96,11 +96,12 @@ class DeviceTree:
         self.order = torch.empty(n, dtype=torch.int64, device=dev)
         self.nchild = torch.empty(ne, **i32)
         self.child_base = torch.empty(ne, **i32)
+        self.slot = torch.empty(ne, **i32)
         self.arrays = TreeArrays(
             ptr(self.coordinates).value, ptr(self.masses).value, ptr(self.softenings).value, ptr(self.sizes).value,
             ptr(self.deltas).value, ptr(self.quadrupoles).value if quadrupole else None,
             ptr(self.first_subnode).value, ptr(self.next_branch).value, ptr(self.order).value,
-            ptr(self.nchild).value, ptr(self.child_base).value)
+            ptr(self.nchild).value, ptr(self.child_base).value, ptr(self.slot).value)
         nws = workspace(L.ptg_node_workspace_bytes(n, nn, int(info.max_level)))
         check(L.ptg_build_tree(ptr(pos), ptr(m), ptr(h), _lib.ctypes.byref(info), int(quadrupole),
                                int(compute_moments), ptr(ws), ws.numel(), ptr(nws), nws.numel(),
@@ -173,13 +174,16 @@ def target_groups(pos, perm, group_size):
 
 
 def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7, G=1.0, precision="float64",
-         counters=None, grouping=None, groups=None):
+         counters=None, grouping=None, groups=None, criterion="theta", eps=None):
     """Grouped walk of `tree` at targets pos[perm[t]]; returns an (nt, W) float64 device tensor in the
     original target order (FieldsTarget_grouped, grouped_treewalk.py:350-397).
 
     grouping: "fixed" = runs of group_size sorted targets (the reference's groups), "cells" =
     cell-aligned groups (ptg_target_groups); default "fixed" for float64, "cells" for float32.
-    groups: precomputed (offsets, count) from target_groups, overriding grouping."""
+    groups: precomputed (offsets, count) from target_groups, overriding grouping.
+    criterion: "theta" (the reference's geometric test) or "error" (FP32 only: also bound the
+    dropped quadrupole term 3.5|Q|/r^4 by eps, an absolute acceleration tolerance for G=1; theta is
+    then a loose cap; the tree must carry quadrupoles)."""
     L = _lib.load()
     pos = to_dev(pos, (-1, 3))
     nt = pos.shape[0]
@@ -196,6 +200,17 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     if groups is None and grouping == "cells":
         groups = target_groups(pos, perm, group_size)
     goff, gcnt = groups if groups is not None else (None, None)
+    crit = _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
+    if criterion == "error":
+        if prec != PTG_F32:
+            raise ValueError("criterion='error' is available for the float32 walk")
+        if not tree.quadrupole:
+            raise ValueError("criterion='error' needs a tree built with quadrupole=True (it bounds the quadrupole term)")
+        if eps is None or not eps > 0:
+            raise ValueError("criterion='error' needs eps > 0")
+        crit = _lib.Criterion(_lib.PTG_CRIT_QERR, 0, float(eps))
+    elif criterion != "theta":
+        raise ValueError(f"criterion must be 'theta' or 'error', got {criterion!r}")
     p = tree.pack(prec, quadrupole)
     wsb = L.ptg_walk_workspace_bytes(nt, int(group_size))
     key = (torch.cuda.current_device(), wsb)
@@ -205,7 +220,8 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
         _walk_ws.clear()
         _walk_ws[key] = ws
     check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), ptr(pos), ptr(soft), ptr(perm), nt, int(mask), int(bool(quadrupole)),
-                     int(group_size), ptr(goff), ptr(gcnt), float(theta), float(G), int(prec == PTG_F64), ptr(out),
+                     int(group_size), ptr(goff), ptr(gcnt), float(theta), _lib.ctypes.byref(crit), float(G),
+                     int(prec == PTG_F64), ptr(out),
                      ptr(counters), ptr(ws), ws.numel(), stream()), "ptg_walk")
     check(L.ptg_walk_check(ptr(ws), stream()), "ptg_walk")
     return out
@@ -226,3 +242,19 @@ def direct(pos_t, soft_t, pos_s, m_s, h_s, mask, G=1.0, self_mode=False, precisi
     check(L.ptg_direct(ptr(pos_t), ptr(soft_t), nt, ptr(pos_s), ptr(m_s), ptr(h_s), ns, prec, int(mask),
                        int(bool(self_mode)), float(G), ptr(out), stream()), "ptg_direct")
     return out
+
+
+def field_scale(tree, groups=64, group_size=64):
+    """RMS |acceleration| (G=1) of the tree's own particles, estimated by a theta=1 FP32 walk of
+    `groups` runs of `group_size` Morton-consecutive particles spread evenly over the Morton order
+    (compact runs, so each sample group's walk costs what an ordinary group's does); turns a relative
+    accuracy target into the absolute tolerance of the error-bounded criterion."""
+    n = tree.n
+    k = min(groups, max(1, n // group_size))
+    starts = (torch.arange(k, device=device(), dtype=torch.int64) * (n // k))
+    idx = (starts[:, None] + torch.arange(min(group_size, n), device=device(), dtype=torch.int64)[None, :]).reshape(-1)
+    pos = tree.coordinates.index_select(0, idx)  # particle slots [0, n) are in Morton order
+    h = tree.softenings.index_select(0, idx)
+    a = walk(tree, pos, h, torch.arange(idx.shape[0], device=device()), PTG_ACC, group_size=min(group_size, n),
+             theta=1.0, precision="float32", grouping="fixed")
+    return float(torch.sqrt(torch.mean(torch.sum(a * a, dim=1))).item())
