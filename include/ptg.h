@@ -12,8 +12,8 @@
  *   - All array arguments are device pointers owned by the caller (the Python side allocates them
  *     as torch tensors).  Scratch memory is a caller-provided workspace whose size is returned by
  *     the matching *_workspace_bytes function.
- *   - Work is enqueued asynchronously on the given HIP stream (NULL = default stream).  The only
- *     host synchronisation is inside ptg_build_sort, which reads back the node count.
+ *   - Work is enqueued asynchronously on the given HIP stream (NULL = default stream).  Host
+ *     synchronisations: ptg_build_sort (finiteness flag, node count) and ptg_walk_check.
  *   - Element numbering follows the reference Octree (octree.py:8-28): particles occupy
  *     [0, n) in Morton order, the root is element n, internal nodes follow.  Links are int32.
  */
@@ -61,7 +61,7 @@ typedef struct {
     int32_t has_collisions; /* some particles shared a 63-bit key and were re-keyed */
     int32_t has_coincident; /* some positions were bit-identical (bucket chains), HasCoincidentPoints */
     int32_t max_level;      /* deepest node level (root = 0) */
-    int32_t reserved;
+    int32_t nonfinite;      /* set when the input held a NaN or infinity (the call then fails) */
     double center[3];       /* root cube centre */
     double size;            /* root cube side */
 } ptg_build_info;
@@ -82,8 +82,10 @@ typedef struct {
 } ptg_tree_arrays;
 
 size_t ptg_build_workspace_bytes(int64_t n);
-ptg_status ptg_build_sort(const double *pos, int64_t n, void *ws, size_t ws_bytes, ptg_build_info *info,
-                          void *stream);
+/* m and h may be NULL (then only positions are checked for finiteness); a non-finite input
+ * fails with PTG_ERR_ARG and info->nonfinite = 1 (ConstructTree's check, frontend.py:123-125). */
+ptg_status ptg_build_sort(const double *pos, const double *m, const double *h, int64_t n, void *ws, size_t ws_bytes,
+                          ptg_build_info *info, void *stream);
 size_t ptg_node_workspace_bytes(int64_t n, int64_t nnodes, int max_level);
 ptg_status ptg_build_tree(const double *pos, const double *m, const double *h, const ptg_build_info *info,
                           int quadrupole, int compute_moments, void *ws, size_t ws_bytes, void *node_ws,

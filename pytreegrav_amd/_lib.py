@@ -28,7 +28,7 @@ class BuildInfo(_c.Structure):
         ("has_collisions", _i32),
         ("has_coincident", _i32),
         ("max_level", _i32),
-        ("reserved", _i32),
+        ("nonfinite", _i32),
         ("center", _dbl * 3),
         ("size", _dbl),
     ]
@@ -75,7 +75,7 @@ SIGNATURES = {
     "ptg_version": (_c.c_int, []),
     "ptg_last_error": (_c.c_char_p, []),
     "ptg_build_workspace_bytes": (_c.c_size_t, [_i64]),
-    "ptg_build_sort": (_c.c_int, [_vp, _i64, _vp, _c.c_size_t, _c.POINTER(BuildInfo), _vp]),
+    "ptg_build_sort": (_c.c_int, [_vp, _vp, _vp, _i64, _vp, _c.c_size_t, _c.POINTER(BuildInfo), _vp]),
     "ptg_node_workspace_bytes": (_c.c_size_t, [_i64, _i64, _c.c_int]),
     "ptg_groups_workspace_bytes": (_c.c_size_t, [_i64]),
     "ptg_target_groups": (_c.c_int, [_vp, _vp, _i64, _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),

@@ -129,94 +129,129 @@ struct LoadArr {
 };
 
 // ============================================================================================
-// LSD radix sort of (u64 key, u32 value) pairs, 8-bit digits.  Each wavefront owns a tile of
-// RS_WTILE consecutive elements; ranks inside a tile come from ballot-based digit matching, so
-// the sort is stable with no block-level synchronisation.
+// LSD radix sort of (u64 key, u32 value) pairs, 8-bit digits.  Each 256-thread block owns a tile of
+// RS_TILE consecutive elements.  Per pass: a tile histogram, a digit-major exclusive scan of all tile
+// histograms (the global offset of every (digit, tile)), and a scatter that ranks the tile's keys
+// stably (ballot-based digit matching inside each wave, per-digit prefix over the waves), reorders
+// the tile in LDS and writes each digit's run contiguously, so global writes come in runs rather
+// than one scattered element at a time.
 // ============================================================================================
 #define RS_BITS 8
 #define RS_BINS 256
 #define RS_ITEMS 16
-#define RS_WTILE (64 * RS_ITEMS)
 #define RS_WAVES 4
+#define RS_THREADS (64 * RS_WAVES)
+#define RS_TILE (RS_THREADS * RS_ITEMS)
 
-__global__ __launch_bounds__(64 * RS_WAVES) void k_rs_hist(const uint64_t *keys, int64_t n, int shift, int64_t nwt,
-                                                           int32_t *hist)
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *keys, int64_t n, int shift, int64_t ntiles,
+                                                        int32_t *hist)
 {
-    __shared__ uint32_t cnt[RS_WAVES][RS_BINS];
-    int wave = threadIdx.x >> 6, lane = ptg_lane();
-    int64_t gw = (int64_t)blockIdx.x * RS_WAVES + wave;
-    if (gw >= nwt) return;
-#pragma unroll
-    for (int q = 0; q < RS_BINS / 64; q++) cnt[wave][lane * (RS_BINS / 64) + q] = 0;
-    int64_t base = gw * RS_WTILE;
+    __shared__ uint32_t cnt[RS_BINS];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
 #pragma unroll 4
     for (int r = 0; r < RS_ITEMS; r++) {
-        int64_t i = base + r * 64 + lane;
-        if (i < n) atomicAdd(&cnt[wave][(keys[i] >> shift) & (RS_BINS - 1)], 1u);
+        int64_t i = base + r * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & (RS_BINS - 1)], 1u);
     }
-#pragma unroll
-    for (int q = 0; q < RS_BINS / 64; q++) {
-        int d = lane * (RS_BINS / 64) + q;
-        hist[(int64_t)d * nwt + gw] = (int32_t)cnt[wave][d];
-    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = (int32_t)cnt[threadIdx.x];
 }
 
-__global__ __launch_bounds__(64 * RS_WAVES) void k_rs_scatter(const uint64_t *kin, const uint32_t *vin, int64_t n,
-                                                              int shift, int64_t nwt, const int32_t *offs,
-                                                              uint64_t *kout, uint32_t *vout)
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *kin, const uint32_t *vin, int64_t n,
+                                                           int shift, int64_t ntiles, const int32_t *offs,
+                                                           uint64_t *kout, uint32_t *vout)
 {
+    __shared__ uint64_t sk[RS_TILE];
+    __shared__ uint32_t sv[RS_TILE];
     __shared__ uint32_t run[RS_WAVES][RS_BINS];
-    int wave = threadIdx.x >> 6, lane = ptg_lane();
-    int64_t gw = (int64_t)blockIdx.x * RS_WAVES + wave;
-    if (gw >= nwt) return;
+    __shared__ int64_t gbase[RS_BINS];
+    __shared__ int32_t sh[8];
+    const int wave = threadIdx.x >> 6, lane = ptg_lane();
+    const uint64_t lt = ptg_lanemask_lt();
+    const int64_t tile0 = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t base = tile0 + (int64_t)wave * (64 * RS_ITEMS);
 #pragma unroll
-    for (int q = 0; q < RS_BINS / 64; q++) {
-        int d = lane * (RS_BINS / 64) + q;
-        run[wave][d] = (uint32_t)offs[(int64_t)d * nwt + gw];
-    }
-    uint64_t lt = ptg_lanemask_lt();
-    int64_t base = gw * RS_WTILE;
+    for (int q = 0; q < RS_BINS / 64; q++) run[wave][lane * (RS_BINS / 64) + q] = 0;
+    __builtin_amdgcn_wave_barrier();
+    uint64_t K[RS_ITEMS];
+    uint32_t V[RS_ITEMS], DR[RS_ITEMS];  // digit << 16 | rank inside the wave's quarter
+#pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
-        int64_t i = base + r * 64 + lane;
-        bool valid = i < n;
-        uint64_t k = valid ? kin[i] : 0;
-        uint32_t v = valid ? vin[i] : 0;
-        uint32_t d = (uint32_t)(k >> shift) & (RS_BINS - 1);
+        const int64_t i = base + r * 64 + lane;
+        const bool valid = i < n;
+        const uint64_t k = valid ? kin[i] : 0;
+        const uint32_t d = (uint32_t)(k >> shift) & (RS_BINS - 1);
         uint64_t peers = ptg_ballot(valid);
 #pragma unroll
         for (int b = 0; b < RS_BITS; b++) {
-            bool bit = (d >> b) & 1;
-            uint64_t bal = ptg_ballot(bit);
+            const bool bit = (d >> b) & 1;
+            const uint64_t bal = ptg_ballot(bit);
             peers &= bit ? bal : ~bal;
         }
-        uint32_t rank = (uint32_t)__popcll(peers & lt);
-        uint32_t dst = run[wave][d] + rank;
-        if (valid) {
-            kout[dst] = k;
-            vout[dst] = v;
+        const uint32_t rank = run[wave][d] + (uint32_t)__popcll(peers & lt);
+        __builtin_amdgcn_wave_barrier();
+        if (valid && (peers & lt) == 0) run[wave][d] = rank + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        K[r] = k;
+        V[r] = valid ? vin[i] : 0;
+        DR[r] = valid ? ((d << 16) | rank) : 0xffffffffu;
+    }
+    __syncthreads();
+    // per digit: tile-local start, offset of each wave's quarter, global base of the digit's run
+    {
+        const int d = threadIdx.x;
+        uint32_t c[RS_WAVES], tot = 0;
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; w++) {
+            c[w] = run[w][d];
+            tot += c[w];
         }
-        __builtin_amdgcn_wave_barrier();
-        if (valid && rank == 0) run[wave][d] = dst + (uint32_t)__popcll(peers);
-        __builtin_amdgcn_wave_barrier();
+        int32_t all;
+        const int32_t start = block_excl_scan<int32_t>((int32_t)tot, &all, sh);
+        gbase[d] = (int64_t)offs[(int64_t)d * ntiles + blockIdx.x] - start;
+        uint32_t o = (uint32_t)start;
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; w++) {
+            run[w][d] = o;
+            o += c[w];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        if (DR[r] == 0xffffffffu) continue;
+        const uint32_t p = run[wave][DR[r] >> 16] + (DR[r] & 0xffffu);
+        sk[p] = K[r];
+        sv[p] = V[r];
+    }
+    __syncthreads();
+    const int64_t cnt = n - tile0 < RS_TILE ? n - tile0 : RS_TILE;
+    for (int64_t p = threadIdx.x; p < cnt; p += RS_THREADS) {
+        const uint64_t k = sk[p];
+        const int64_t g = gbase[(k >> shift) & (RS_BINS - 1)] + p;
+        kout[g] = k;
+        vout[g] = sv[p];
     }
 }
 
-static int64_t rs_nwt(int64_t n) { return std::max<int64_t>(1, ptg_cdiv(n, RS_WTILE)); }
+// number of sort tiles (sizes the histogram array: RS_BINS entries per tile)
+static int64_t rs_nwt(int64_t n) { return std::max<int64_t>(1, ptg_cdiv(n, RS_TILE)); }
 
 // sorts (k0,v0) by the low `nbits` bits; the result ends in (k0,v0) or (k1,v1): *in_first tells which
 static ptg_status radix_sort(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,
                              int32_t *hist, int32_t *bsum, bool *in_first, hipStream_t s)
 {
-    int64_t nwt = rs_nwt(n);
-    int64_t nblk = ptg_cdiv(nwt, RS_WAVES);
+    int64_t ntiles = rs_nwt(n);
     bool first = true;
     for (int shift = 0; shift < nbits; shift += RS_BITS) {
         uint64_t *ki = first ? k0 : k1, *ko = first ? k1 : k0;
         uint32_t *vi = first ? v0 : v1, *vo = first ? v1 : v0;
-        k_rs_hist<<<nblk, 64 * RS_WAVES, 0, s>>>(ki, n, shift, nwt, hist);
-        ptg_status st = excl_scan(LoadArr{hist}, (int64_t)RS_BINS * nwt, hist, bsum, nullptr, s);
+        k_rs_hist<<<ntiles, RS_THREADS, 0, s>>>(ki, n, shift, ntiles, hist);
+        ptg_status st = excl_scan(LoadArr{hist}, (int64_t)RS_BINS * ntiles, hist, bsum, nullptr, s);
         if (st) return st;
-        k_rs_scatter<<<nblk, 64 * RS_WAVES, 0, s>>>(ki, vi, n, shift, nwt, hist, ko, vo);
+        k_rs_scatter<<<ntiles, RS_THREADS, 0, s>>>(ki, vi, n, shift, ntiles, hist, ko, vo);
         PTG_LAUNCH_CHECK();
         first = !first;
     }
@@ -341,8 +376,21 @@ struct Counters {
     int32_t nsub;         // particles in collision runs
     int32_t coincident;   // bit-identical extended keys exist
     int32_t max_level;
-    int32_t pad[11];
+    int32_t nonfinite;    // some position, mass or softening is NaN or infinite
+    int32_t pad[10];
 };
+
+__global__ __launch_bounds__(256) void k_check_finite(const double *pos, const double *m, const double *h, int64_t n,
+                                                      Counters *c)
+{
+    bool bad = false;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        bad |= !isfinite(pos[3 * i]) || !isfinite(pos[3 * i + 1]) || !isfinite(pos[3 * i + 2]);
+        if (m) bad |= !isfinite(m[i]);
+        if (h) bad |= !isfinite(h[i]);
+    }
+    if (ptg_ballot(bad) && ptg_lane() == 0) c->nonfinite = 1;
+}
 
 struct BuildWS {
     double *part;
@@ -632,8 +680,8 @@ extern "C" ptg_status ptg_morton_order(const double *pos, int64_t n, int64_t *or
     return PTG_OK;
 }
 
-extern "C" ptg_status ptg_build_sort(const double *pos, int64_t n, void *ws, size_t ws_bytes, ptg_build_info *info,
-                                     void *stream)
+extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const double *h, int64_t n, void *ws,
+                                     size_t ws_bytes, ptg_build_info *info, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (n < 1 || !pos || !info) {
@@ -651,9 +699,19 @@ extern "C" ptg_status ptg_build_sort(const double *pos, int64_t n, void *ws, siz
         return PTG_ERR_WORKSPACE;
     }
     k_zero_counters<<<1, 64, 0, s>>>(w.ctr);
+    k_check_finite<<<(unsigned)std::min<int64_t>(1024, ptg_cdiv(n, 256)), 256, 0, s>>>(pos, m, h, n, w.ctr);
+    Counters hc;
+    PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    PTG_HIP_CHECK(hipStreamSynchronize(s));
+    if (hc.nonfinite) {
+        info->n = n;
+        info->nnodes = 0;
+        info->nonfinite = 1;
+        ptg_set_error("ptg_build_sort: non-finite position, mass or softening");
+        return PTG_ERR_ARG;
+    }
     ptg_status st = sort_by_keys(pos, n, w, s);
     if (st) return st;
-    Counters hc;
     if (n >= 2) {
         k_levels<<<grid1(n - 1), 256, 0, s>>>(w.kA, n, w.lev, w.ctr);
         PTG_LAUNCH_CHECK();
@@ -704,7 +762,7 @@ extern "C" ptg_status ptg_build_sort(const double *pos, int64_t n, void *ws, siz
     info->has_collisions = hc.ncollide > 0;
     info->has_coincident = hc.coincident;
     info->max_level = n >= 2 ? hc.max_level : 0;
-    info->reserved = 0;
+    info->nonfinite = 0;
     for (int k = 0; k < 3; k++) info->center[k] = hg.center[k];
     info->size = hg.size;
     if (n + info->nnodes > 0x7fffffffLL) {

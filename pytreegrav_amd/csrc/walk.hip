@@ -241,6 +241,40 @@ __device__ __forceinline__ void interact_node_fast(Acc<T> &a, V4<T> s, const T *
     }
 }
 
+// The M4 spline terms for r < h (kernel.py:4-75) with reciprocals instead of IEEE divisions, for
+// the FP32 fast path: hinv = 1/h, qinv = h/r = h * rinv.
+template <typename T> struct SplineFast {
+    T K, phi, Q;
+};
+template <typename T, bool P, bool D>
+__device__ __forceinline__ SplineFast<T> spline_fast(T r, T rinv, T h)
+{
+    T hinv = __builtin_amdgcn_rcpf(h);
+    T q = r * hinv;
+    T hinv2 = hinv * hinv;
+    T hinv3 = hinv2 * hinv;
+    SplineFast<T> o;
+    o.phi = 0;
+    o.Q = 0;
+    if (q <= T(0.5)) {
+        T q2 = q * q;
+        o.K = (T(10.666666666666666666) + q2 * (T(-38.4) + T(32.0) * q)) * hinv3;
+        if (P) o.phi = (T(-2.8) + q2 * (T(5.33333333333333333) + q2 * (T(6.4) * q - T(9.6)))) * hinv;
+        if (D) o.Q = (T(76.8) - T(96.0) * q) * hinv3 * hinv2;
+    } else {
+        T qinv = h * rinv;
+        T qinv3 = qinv * qinv * qinv;
+        o.K = (T(21.333333333333) - T(48.0) * q + T(38.4) * q * q - T(10.666666666667) * q * q * q -
+               T(0.066666666667) * qinv3) * hinv3;
+        if (P)
+            o.phi = (T(-3.2) + T(0.066666666666666666666) * qinv +
+                     q * q * (T(10.666666666666666666666) + q * (T(-16.0) + q * (T(9.6) - T(2.1333333333333333333333) * q)))) *
+                    hinv;
+        if (D) o.Q = (T(48.0) * qinv - T(76.8) + T(32.0) * q - T(0.2) * qinv3 * qinv * qinv) * hinv3 * hinv2;
+    }
+    return o;
+}
+
 // Particle on one target, fast path: point mass unless any lane is inside the softening radius,
 // in which case those lanes evaluate the spline (wave-uniform branch); r2 == 0 contributes zero.
 template <typename T, int FIELDS>
@@ -249,7 +283,7 @@ __device__ __forceinline__ void interact_part_fast(Acc<T> &a, V4<T> s, T hs, T t
     constexpr bool P = NW<FIELDS>::POT, G = NW<FIELDS>::ACC, D = NW<FIELDS>::TID;
     T dx = s.x - tx, dy = s.y - ty, dz = s.z - tz;
     T r2 = dx * dx + dy * dy + dz * dz;
-    T ht = hs > th ? hs : th;
+    T ht = fmaxf(hs, th);
     bool nonzero = r2 > T(0);
     bool soft = nonzero && r2 < ht * ht;
     T rinv = nonzero ? fast_rsqrt(r2) : T(0);
@@ -260,10 +294,10 @@ __device__ __forceinline__ void interact_part_fast(Acc<T> &a, V4<T> s, T hs, T t
     T Q = T(3) * K * rinv2;
     if (ptg_ballot(soft)) {
         if (soft) {
-            T r = r2 * rinv;
-            if (P) phi = s.w * potential_kernel(r, ht);
-            K = s.w * force_kernel(r, ht);
-            if (D) Q = s.w * tidal_kernel(r, ht);
+            SplineFast<T> sp = spline_fast<T, P, D>(r2 * rinv, rinv, ht);
+            if (P) phi = s.w * sp.phi;
+            K = s.w * sp.K;
+            if (D) Q = s.w * sp.Q;
         }
     }
     if (P) a.phi += phi;
@@ -296,8 +330,10 @@ __device__ __forceinline__ void flush_nodes(const WaveLds<T, QUAD> &L, int cnt, 
         }
 #pragma unroll
         for (int j = 0; j < TPL; j++) {
-            if (EXACT) interact_exact<T, FIELDS, QUAD>(acc[j], s.x, s.y, s.z, s.w, L.nh[k], q, tx[j], ty[j], tz[j], th[j]);
-            else interact_node_fast<T, FIELDS, QUAD>(acc[j], s, q, tx[j], ty[j], tz[j]);
+            if constexpr (EXACT)
+                interact_exact<T, FIELDS, QUAD>(acc[j], s.x, s.y, s.z, s.w, L.nh[k], q, tx[j], ty[j], tz[j], th[j]);
+            else
+                interact_node_fast<T, FIELDS, QUAD>(acc[j], s, q, tx[j], ty[j], tz[j]);
         }
     }
 }
@@ -312,8 +348,10 @@ __device__ __forceinline__ void flush_parts(const WaveLds<T, QUAD> &L, int cnt, 
         T hs = L.ph[k];
 #pragma unroll
         for (int j = 0; j < TPL; j++) {
-            if (EXACT) interact_exact<T, FIELDS, false>(acc[j], s.x, s.y, s.z, s.w, hs, nullptr, tx[j], ty[j], tz[j], th[j]);
-            else interact_part_fast<T, FIELDS>(acc[j], s, hs, tx[j], ty[j], tz[j], th[j]);
+            if constexpr (EXACT)
+                interact_exact<T, FIELDS, false>(acc[j], s.x, s.y, s.z, s.w, hs, nullptr, tx[j], ty[j], tz[j], th[j]);
+            else
+                interact_part_fast<T, FIELDS>(acc[j], s, hs, tx[j], ty[j], tz[j], th[j]);
         }
     }
 }
@@ -341,8 +379,8 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
             const int64_t g = g_lo + gi;
             if (g >= g_hi) break;
             const int64_t a0 = A.goff ? (int64_t)A.goff[g] : g * A.gs;
-            const int64_t b0 = A.goff ? std::min<int64_t>((int64_t)A.goff[g + 1], a0 + A.gs)
-                                      : std::min<int64_t>(a0 + A.gs, A.nt);
+            const int64_t e0 = A.goff ? (int64_t)A.goff[g + 1] : A.nt;
+            const int64_t b0 = e0 < a0 + A.gs ? e0 : a0 + A.gs;
             const int mt = (int)(b0 - a0);
 
             // ---- targets: lane holds a0 + lane + 64 j ----
@@ -427,13 +465,17 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                 T c1 = X2.y / A.theta + X2.z;
                 T c2 = hh + X2.y * T(0.6) + X2.z;
                 bool acc_node = ve && !ispart && r_min > (c1 > c2 ? c1 : c2);
+                // fast path: a particle farther from the group's box than every softening radius
+                // (and not inside it) needs neither the softening nor the self test -> node list
+                const bool far_part = !EXACT && ve && ispart && r_min > (X2.x > hmax ? X2.x : hmax);
                 if (A.crit == 1) {
                     // error-bounded: also require the quadrupole term the monopole drops,
                     // 3.5 |Q| / r^4, to stay below eps (theta is then only a loose cap)
                     T r2m = r_min * r_min;
                     acc_node = acc_node && X2.w < A.eps * r2m * r2m;
                 }
-                bool acc_part = ve && ispart;
+                bool acc_part = ve && ispart && !far_part;
+                acc_node = acc_node || far_part;
                 bool open = ve && !ispart && !acc_node;
 
                 const uint64_t bp = ptg_ballot(acc_part), bn = ptg_ballot(acc_node);
@@ -488,17 +530,11 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                 if (nn >= 64) {
                     flush_nodes<T, TPL, FIELDS, QUAD, EXACT>(L, 64, acc, tx, ty, tz, th);
                     __builtin_amdgcn_wave_barrier();
-                    if (lane < nn - 64) {
-                        V4<T> v = L.nodes[64 + lane];
-                        T hv = L.nh[64 + lane];
-                        T qv[6];
+                    if (lane < nn - 64) {  // reads [64, nn) and writes [0, nn-64): disjoint
+                        L.nodes[lane] = L.nodes[64 + lane];
+                        L.nh[lane] = L.nh[64 + lane];
                         if (QUAD)
-                            for (int c = 0; c < 6; c++) qv[c] = L.nq[6 * (64 + lane) + c];
-                        __builtin_amdgcn_wave_barrier();
-                        L.nodes[lane] = v;
-                        L.nh[lane] = hv;
-                        if (QUAD)
-                            for (int c = 0; c < 6; c++) L.nq[6 * lane + c] = qv[c];
+                            for (int c = 0; c < 6; c++) L.nq[6 * lane + c] = L.nq[6 * (64 + lane) + c];
                     }
                     __builtin_amdgcn_wave_barrier();
                     nn -= 64;
@@ -506,12 +542,9 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                 if (np >= 64) {
                     flush_parts<T, TPL, FIELDS, QUAD, EXACT>(L, 64, acc, tx, ty, tz, th);
                     __builtin_amdgcn_wave_barrier();
-                    if (lane < np - 64) {
-                        V4<T> v = L.parts[64 + lane];
-                        T hv = L.ph[64 + lane];
-                        __builtin_amdgcn_wave_barrier();
-                        L.parts[lane] = v;
-                        L.ph[lane] = hv;
+                    if (lane < np - 64) {  // disjoint ranges, as above
+                        L.parts[lane] = L.parts[64 + lane];
+                        L.ph[lane] = L.ph[64 + lane];
                     }
                     __builtin_amdgcn_wave_barrier();
                     np -= 64;

@@ -64,6 +64,10 @@ def unpack_fields(out, mask):
 # ---------------------------------------------------------------------------------------------
 # tree build
 # ---------------------------------------------------------------------------------------------
+class NonFiniteInput(ValueError):
+    """A position, mass or softening passed to the build was NaN or infinite."""
+
+
 class DeviceTree:
     """Device arrays of one octree in the reference's element layout (octree.py:8-28)."""
 
@@ -77,7 +81,10 @@ class DeviceTree:
         s = stream()
         ws = workspace(L.ptg_build_workspace_bytes(n))
         info = BuildInfo()
-        check(L.ptg_build_sort(ptr(pos), n, ptr(ws), ws.numel(), _lib.ctypes.byref(info), s), "ptg_build_sort")
+        st = L.ptg_build_sort(ptr(pos), ptr(m), ptr(h), n, ptr(ws), ws.numel(), _lib.ctypes.byref(info), s)
+        if st != 0 and info.nonfinite:
+            raise NonFiniteInput("non-finite position, mass or softening")
+        check(st, "ptg_build_sort")
         nn = int(info.nnodes)
         ne = n + nn
         f64 = dict(dtype=torch.float64, device=dev)
@@ -244,7 +251,7 @@ def direct(pos_t, soft_t, pos_s, m_s, h_s, mask, G=1.0, self_mode=False, precisi
     return out
 
 
-def field_scale(tree, groups=64, group_size=64):
+def field_scale(tree, groups=64, group_size=64, theta=1.0):
     """RMS |acceleration| (G=1) of the tree's own particles, estimated by a theta=1 FP32 walk of
     `groups` runs of `group_size` Morton-consecutive particles spread evenly over the Morton order
     (compact runs, so each sample group's walk costs what an ordinary group's does); turns a relative
@@ -256,5 +263,5 @@ def field_scale(tree, groups=64, group_size=64):
     pos = tree.coordinates.index_select(0, idx)  # particle slots [0, n) are in Morton order
     h = tree.softenings.index_select(0, idx)
     a = walk(tree, pos, h, torch.arange(idx.shape[0], device=device()), PTG_ACC, group_size=min(group_size, n),
-             theta=1.0, precision="float32", grouping="fixed")
+             theta=theta, precision="float32", grouping="fixed")
     return float(torch.sqrt(torch.mean(torch.sum(a * a, dim=1))).item())

@@ -205,12 +205,18 @@ def ConstructTree(pos, m=None, softening=None, quadrupole=False, vel=None, compu
     if softening is None:
         softening = m * 0 if isinstance(m, torch.Tensor) else np.zeros_like(m)
     softening = _atleast_1d(_f64(softening))
-    if not _finite(pos, m, softening):
-        print("Invalid input detected - aborting treebuild to avoid going into an infinite loop!")
-        raise RuntimeError("Invalid input detected - aborting treebuild to avoid going into an infinite loop!")
+    msg = "Invalid input detected - aborting treebuild to avoid going into an infinite loop!"
+    # host arrays are checked here; device tensors by the build itself (no extra passes)
+    if not all(isinstance(a, torch.Tensor) for a in (pos, m, softening)) and not _finite(pos, m, softening):
+        print(msg)
+        raise RuntimeError(msg)
     if len(pos) == 0:
         raise ValueError("ConstructTree needs at least one particle")
-    tree = Octree(pos, m, softening, quadrupole=quadrupole, compute_moments=compute_moments)
+    try:
+        tree = Octree(pos, m, softening, quadrupole=quadrupole, compute_moments=compute_moments)
+    except engine.NonFiniteInput:
+        print(msg)
+        raise RuntimeError(msg) from None
     warn_if_coincident_positions(tree, softening)
     return tree
 

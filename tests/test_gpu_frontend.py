@@ -238,3 +238,17 @@ def test_adaptive_dispatch_thresholds():
     xt = x[:100]
     assert np.array_equal(PotentialTarget(xt, x, m, h[:100], h), PotentialTarget(xt, x, m, h[:100], h,
                                                                                   method="bruteforce"))
+
+
+def test_non_finite_device_input_aborts_the_build(capsys):
+    import torch
+
+    from pytreegrav_amd import ConstructTree
+
+    x, m, h = cloud(1000)
+    for bad in ("pos", "m", "h"):
+        X, M, H = (torch.from_numpy(a.copy()).cuda() for a in (x, m, h))
+        {"pos": X, "m": M, "h": H}[bad].view(-1)[7] = float("inf") if bad == "m" else float("nan")
+        with pytest.raises(RuntimeError):
+            ConstructTree(X, M, H)
+        assert "Invalid input detected" in capsys.readouterr().out
