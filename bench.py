@@ -54,9 +54,14 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("-N", type=int, default=10_000_000)
-    ap.add_argument("--criterion", choices=("error", "theta"), default="error",
+    ap.add_argument("--workload", choices=("plummer", "batched"), default="plummer",
+                    help="plummer: one N-particle Plummer sphere (the headline); batched: 512 independent "
+                         "Plummer clusters (N log-uniform in [1e3, 1e5], scale radii 0.1-10) in one forest")
+    ap.add_argument("--criterion", choices=("error", "theta", "learned"), default="error",
                     help="opening criterion of the timed walk: 'error' bounds each accepted node's dropped "
-                         "quadrupole term by alpha x (RMS field); 'theta' is the reference's geometric test")
+                         "quadrupole term by alpha x (RMS field); 'learned' accepts a node beyond the opening "
+                         "radius the trained MLP predicts for that tolerance; 'theta' is the reference's "
+                         "geometric test")
     ap.add_argument("--alpha", type=float, default=8.5e-4, help="per-node error tolerance relative to the RMS field")
     ap.add_argument("--theta", type=float, default=None,
                     help="opening angle (criterion theta) or loose geometric cap (criterion error); default 0.7 / 2.0")
@@ -123,6 +128,96 @@ def _cpu_model():
     return "unknown"
 
 
+def run_batched(args):
+    """512 independent Plummer clusters in one forest: one build, one grouping and one walk launch for
+    all systems; the walk stage is also captured into a HIP graph and replayed."""
+    import torch
+
+    from pytreegrav_amd import Forest, engine
+    from pytreegrav_amd.sampling import plummer_clusters
+
+    dev = engine.device()
+    systems = plummer_clusters(512, seed=args.seed)
+    x = np.vstack([s[0] for s in systems])
+    m = np.concatenate([s[1] for s in systems])
+    h = np.concatenate([s[2] for s in systems])
+    off = np.concatenate([[0], np.cumsum([len(s[0]) for s in systems])])
+    N = len(x)
+    pos, mass, soft = (torch.from_numpy(a).to(dev) for a in (x, m, h))
+    theta = args.theta if args.theta is not None else 0.7
+    gs = args.group_size
+    ctr = torch.zeros(2, dtype=torch.int64, device=dev)
+
+    def step():
+        F = Forest(pos, mass, soft, off, precision="float32", group_size=gs)
+        return F, F.evaluate(potential=True, accel=True, theta=theta)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        F, r = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # walk stage alone: eager, and as a replayed HIP graph
+    ctr.zero_()
+    F.evaluate(potential=True, accel=True, theta=theta, counters=ctr)
+    interactions = int(ctr[0].item())
+    replay, out = F.capture(potential=True, accel=True, theta=theta)
+    replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        replay()
+    e1.record()
+    torch.cuda.synchronize()
+    graph_ms = e0.elapsed_time(e1) / args.steps
+    graph_equal = bool(torch.equal(out[:, 0], r["potential"]) and torch.equal(out[:, 1:4], r["accel"]))
+    # accuracy: 16 systems, up to 1000 targets each, FP64 direct sum within the system
+    rng = np.random.default_rng(7)
+    errs = []
+    acc_np = r["accel"].cpu().numpy()
+    for s in rng.choice(len(systems), 16, replace=False):
+        a, b = off[s], off[s + 1]
+        idx = np.sort(rng.choice(b - a, min(1000, b - a), replace=False))
+        xs, ms, hs = systems[s]
+        d = engine.direct(xs[idx], hs[idx], xs, ms, hs, engine.PTG_ACC).cpu().numpy()
+        # r == 0 pairs are skipped by the target form, so the target's own mass is excluded as in the walk
+        e = np.sqrt(np.mean(np.sum((acc_np[a + idx] - d) ** 2, 1)) / np.mean(np.sum(d * d, 1)))
+        errs.append(float(e))
+    line = {
+        "metric": "batched Accel+Potential particles/sec, 512 independent Plummer clusters in one forest",
+        "value": N / dt,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched_plummer_clusters", "systems": len(systems), "N_total": N,
+                   "N_range": [int(np.diff(off).min()), int(np.diff(off).max())], "criterion": "theta",
+                   "theta": theta, "group_size": gs, "grouping": "cell-aligned per system",
+                   "softening": "0.01 x scale radius", "scale_radius": "log-uniform 0.1-10"},
+        "breakdown": {"walk_stage_graph_ms": graph_ms, "graph_replay_equals_eager": graph_equal,
+                      "interactions_per_particle": interactions / N},
+        "roofline": {"kernel": "k_walk (FP32 grouped walk)", "bound": "valu",
+                     "achieved": FLOPS_PER_INTERACTION * interactions / (graph_ms * 1e-3) / 1e12,
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": FLOPS_PER_INTERACTION * interactions / (graph_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                     "traffic": None, "convention": "walk stage (groups + walk) per replay"},
+        "accuracy": {"accel_rms_per_system_max": max(errs), "accel_rms_per_system_median": float(np.median(errs)),
+                     "systems_checked": 16},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -145,6 +240,8 @@ def main():
     from pytreegrav_amd import ConstructTree, engine
     from pytreegrav_amd.sampling import plummer_truncated
 
+    if args.workload == "batched":
+        return run_batched(args)
     dev = engine.device()
     N = args.N
     x, m = plummer_truncated(N, seed=args.seed, rmax=100.0)
@@ -164,22 +261,34 @@ def main():
         exact = engine.direct(pos[sel_t], soft[sel_t], pos, mass, soft, mask, self_mode=False,
                               precision="float64").cpu().numpy()
 
+    model = None
+    if args.criterion == "learned":
+        from pytreegrav_amd.learned import OpeningMLP
+
+        model = OpeningMLP.load()
+
     def run_config(criterion, theta, alpha, quad_walk=False):
-        """Time `steps` whole evaluations (build + pack + groups + walk) of one configuration."""
-        build_quads = quad_walk or criterion == "error"
+        """Time `steps` whole evaluations (build + pack + groups + walk) of one configuration; the
+        HIP events bracket the walk launches alone (k_walk_init + k_walk)."""
+        build_quads = quad_walk or criterion in ("error", "learned")
         walk_times = []
 
         def step(record):
             tree = ConstructTree(pos, mass, soft, quadrupole=build_quads)
-            tree.dev.pack(engine.PTG_F32, quad_walk)
-            eps = alpha * engine.field_scale(tree.dev) if criterion == "error" else None
+            eps = alpha * engine.field_scale(tree.dev) if criterion in ("error", "learned") else None
+            if criterion == "learned":
+                tree.dev.pack_learned(model, eps, quad_walk)
+            else:
+                tree.dev.pack(engine.PTG_F32, quad_walk)
+            groups = engine.target_groups(pos, tree.dev.order, args.group_size)
             if record:
                 ctr.zero_()
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record()
             out = engine.walk(tree.dev, pos, soft, tree.dev.order, mask, quadrupole=quad_walk,
                               group_size=args.group_size, theta=theta, precision="float32",
-                              counters=ctr if record else None, criterion=criterion, eps=eps)
+                              counters=ctr if record else None, criterion=criterion, eps=eps, model=model,
+                              groups=groups)
             if record:
                 e1.record()
                 walk_times.append((e0, e1))
@@ -200,7 +309,10 @@ def main():
         tb0 = time.perf_counter()
         for _ in range(args.steps):
             tr = ConstructTree(pos, mass, soft, quadrupole=build_quads)
-            tr.dev.pack(engine.PTG_F32, quad_walk)
+            if criterion == "learned":
+                tr.dev.pack_learned(model, alpha * engine.field_scale(tr.dev), quad_walk)
+            else:
+                tr.dev.pack(engine.PTG_F32, quad_walk)
         torch.cuda.synchronize()
         r["build_ms"] = (time.perf_counter() - tb0) / args.steps * 1e3
         r["accuracy"] = accuracy(out) if exact is not None else None
@@ -265,17 +377,19 @@ def main():
             "softening": args.softening,
             "criterion": args.criterion,
             "theta": theta,
-            "alpha": args.alpha if args.criterion == "error" else None,
+            "alpha": args.alpha if args.criterion in ("error", "learned") else None,
+            "model": (model.meta.get("data", "") + f"; margin {model.margin:.3f}") if model is not None else None,
             "group_size": args.group_size,
             "grouping": "cell-aligned",
             "walk_precision": "fp32 node-relative, fp32 accumulation",
             "build_precision": "fp64 keys and moments",
             "moments": ("quadrupole" if quad else "monopole") + " interactions"
-                       + ("; fp64 quadrupoles built for the criterion" if args.criterion == "error" else ""),
+                       + ("; fp64 quadrupoles built for the criterion" if args.criterion != "theta" else ""),
             "parallelism": "single GPU",
         },
         "breakdown": {
             "build_and_pack_ms": build_ms,
+            "other_ms": dt * 1e3 - build_ms - walk_ms,
             "walk_ms": walk_ms,
             "walk_only_particles_per_s": N / (walk_ms * 1e-3),
             "interactions_per_particle": interactions / N,

@@ -81,33 +81,56 @@ typedef struct {
     int32_t *slot;            /* (n+nnodes): walk slot of each element (the root is slot 0) */
 } ptg_tree_arrays;
 
-size_t ptg_build_workspace_bytes(int64_t n);
+/* Forest: several independent systems (no reference counterpart; the reference builds one Octree
+ * per call).  Each system gets its own bounding cube, Morton keys and tree, exactly as if it were
+ * built alone, but all systems are built, packed, grouped and walked by the same launches.
+ * System s owns input particles [offsets[s], offsets[s+1]), which after the build are also its
+ * Morton slots (the tree stays system-major), and its root is element roots[s], at walk slot s.
+ * Every system needs at least two particles.  A NULL forest means one tree; a forest with
+ * nsys == 1 is the same tree and also fills cubes/roots. */
+typedef struct {
+    int64_t nsys;           /* number of systems */
+    const int64_t *offsets; /* device (nsys + 1): 0 = offsets[0] < ... < offsets[nsys] = n */
+    int32_t *roots;         /* device (nsys): element index of each root; written by ptg_build_tree */
+    double *cubes;          /* device (nsys, 4): root cube centre x, y, z and side; written by ptg_build_sort */
+} ptg_forest;
+
+size_t ptg_build_workspace_bytes(int64_t n, int64_t nsys);
 /* m and h may be NULL (then only positions are checked for finiteness); a non-finite input
- * fails with PTG_ERR_ARG and info->nonfinite = 1 (ConstructTree's check, frontend.py:123-125). */
-ptg_status ptg_build_sort(const double *pos, const double *m, const double *h, int64_t n, void *ws, size_t ws_bytes,
-                          ptg_build_info *info, void *stream);
+ * fails with PTG_ERR_ARG and info->nonfinite = 1 (ConstructTree's check, frontend.py:123-125).
+ * For a forest, info->center/size describe system 0 (all cubes are in forest->cubes). */
+ptg_status ptg_build_sort(const double *pos, const double *m, const double *h, int64_t n, const ptg_forest *forest,
+                          void *ws, size_t ws_bytes, ptg_build_info *info, void *stream);
 size_t ptg_node_workspace_bytes(int64_t n, int64_t nnodes, int max_level);
 ptg_status ptg_build_tree(const double *pos, const double *m, const double *h, const ptg_build_info *info,
-                          int quadrupole, int compute_moments, void *ws, size_t ws_bytes, void *node_ws,
-                          size_t node_ws_bytes, const ptg_tree_arrays *tree, void *stream);
+                          const ptg_forest *forest, int quadrupole, int compute_moments, void *ws, size_t ws_bytes,
+                          void *node_ws, size_t node_ws_bytes, const ptg_tree_arrays *tree, void *stream);
 
 /* Morton permutation of arbitrary points: replaces grouped_treewalk._morton_order (:269-275).
- * Uses a ptg_build_workspace_bytes(n) workspace; asynchronous. */
+ * Uses a ptg_build_workspace_bytes(n, 1) workspace; asynchronous. */
 ptg_status ptg_morton_order(const double *pos, int64_t n, int64_t *order, void *ws, size_t ws_bytes, void *stream);
 
-/* Cell-aligned target groups for the FP32 walk (no reference counterpart: the reference forms
- * fixed runs of group_size targets, grouped_treewalk.py:179-182).  Targets pos[perm[t]] must be
- * in Morton order.  Groups never cross the boundary of a cell holding more than 4*group_size
- * targets and hold at most group_size targets.  Writes offsets[0..G] (capacity n+1) and the group
- * count G to the device scalar *ngroups; asynchronous. */
-size_t ptg_groups_workspace_bytes(int64_t n);
-ptg_status ptg_target_groups(const double *pos, const int64_t *perm, int64_t n, int group_size, int32_t *offsets,
-                             int32_t *ngroups, void *ws, size_t ws_bytes, void *stream);
+/* Target groups for the walk.  PTG_GROUPS_RUNS: runs of group_size consecutive sorted targets,
+ * the reference's grouping (grouped_treewalk.py:179-182).  PTG_GROUPS_CELLS (no reference
+ * counterpart, the FP32 default): groups never cross the boundary of a cell holding more than
+ * 4*group_size targets and hold at most group_size targets.  Targets pos[perm[t]] must be in
+ * Morton order.  Writes offsets[0..G] (capacity n+1) and the group count G to the device scalar
+ * *ngroups; asynchronous.  With a forest the targets are the forest's own particles (perm = its
+ * treewalk_indices): runs restart and cells are cut at every system start, and each system's
+ * cube defines its cells, so every system is grouped exactly as if it were alone. */
+#define PTG_GROUPS_RUNS 0
+#define PTG_GROUPS_CELLS 1
+size_t ptg_groups_workspace_bytes(int64_t n, int64_t nsys);
+ptg_status ptg_target_groups(const double *pos, const int64_t *perm, int64_t n, const ptg_forest *forest,
+                             int group_size, int mode, int32_t *offsets, int32_t *ngroups, void *ws,
+                             size_t ws_bytes, void *stream);
 
 /* ------------------------------------------------------------------------------------------
  * Walk tree: the sibling-contiguous packed copy of the tree the walk reads (one slot per
- * element; the children of a node occupy nchild consecutive slots from child_base).  FP32 packs
- * store coordinates relative to `origin` (node-relative precision); FP64 packs store them as is.
+ * element; the children of a node occupy nchild consecutive slots from child_base).  Coordinates
+ * are stored relative to `origin` (node-relative precision for FP32 packs; pass 0 for FP64).  For
+ * a forest, `origin` is ignored: FP32 packs are relative to each system's cube centre and FP64
+ * packs absolute.
  * Per slot: xyzm (4 reals: x, y, z, mass), aux (4 reals: softening, size, delta, 0),
  * kids (2 int32: first child slot, child count; count 0 marks a particle),
  * quad (8 reals: xx, xy, xz, yy, yz, zz, 0, 0) when quadrupole != 0.
@@ -123,8 +146,8 @@ typedef struct {
     double origin[3];
 } ptg_walk_tree;
 
-ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_walk_tree *wt,
-                         void *stream);
+ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_forest *forest,
+                         const ptg_walk_tree *wt, void *stream);
 
 /* ------------------------------------------------------------------------------------------
  * Grouped walk: replaces grouped_treewalk.FieldsTarget_grouped (:350-397) with the core
@@ -137,7 +160,11 @@ ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
  * ptg_target_groups) are given, each group [offsets[g], offsets[g+1]).  exact != 0 evaluates every pair with the reference's
  * branches (r2 > 0 self skip, softened iff r < max(h_s, h_t)); exact == 0 (FP32 fast path) skips
  * those tests where the group geometry proves them false.  counters (nullable, device int64[2])
- * accumulate: [0] element-target interactions, [1] traversal visits.
+ * accumulate: [0] element-target interactions, [1] traversal visits.  With a forest (the same
+ * descriptor the tree was built and packed with) the targets are the forest's own particles
+ * (perm = its treewalk_indices, nt = n, groups that never cross a system, e.g. from
+ * ptg_target_groups with the forest or group_size-runs when every system size is a multiple of
+ * group_size): each group walks the tree of the system holding its first target.
  * ------------------------------------------------------------------------------------------ */
 /* Opening criterion.  PTG_CRIT_THETA: the reference's geometric test (treewalk.py:22-25).
  * PTG_CRIT_QERR (no reference counterpart): a node is accepted only if it also passes the
@@ -146,6 +173,10 @@ ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
  * (an absolute acceleration tolerance, in units of G=1); needs a tree built with quadrupoles. */
 #define PTG_CRIT_THETA 0
 #define PTG_CRIT_QERR 1
+/* PTG_CRIT_RADIUS (no reference counterpart): a node is accepted only if it also passes the
+ * geometric test at `theta` (the guard) and r_min exceeds the opening radius stored in aux[3] of
+ * its slot by ptg_open_radii (the learned criterion); eps is unused (it is baked into the radii). */
+#define PTG_CRIT_RADIUS 2
 typedef struct {
     int32_t criterion;
     int32_t reserved;
@@ -153,13 +184,56 @@ typedef struct {
 } ptg_criterion;
 
 size_t ptg_walk_workspace_bytes(int64_t nt, int group_size);
-ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const double *soft, const int64_t *perm, int64_t nt,
-                    int fields, int quadrupole, int group_size, const int32_t *group_offsets,
+ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest, const double *pos, const double *soft,
+                    const int64_t *perm, int64_t nt, int fields, int quadrupole, int group_size,
+                    const int32_t *group_offsets,
                     const int32_t *group_count, double theta, const ptg_criterion *criterion, double G,
                     int exact, double *out, int64_t *counters, void *ws, size_t ws_bytes, void *stream);
 /* Reads the overflow flag of the last walk on this workspace (synchronises the stream):
  * PTG_ERR_OVERFLOW if a traversal stack exceeded its spill capacity. */
 ptg_status ptg_walk_check(void *ws, void *stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Learned opening criterion (no reference counterpart; it replaces the geometric test of
+ * treewalk.py:22-25 by a per-node opening radius, keeping that test as the guard).  A 10-32-32-1
+ * MLP with tanh hidden units maps scale-free node features and the tolerance eps (an absolute
+ * acceleration, G = 1) to y = log10(r_open / size).  Parameters: PTG_MLP_NPARAM float32 values,
+ * W1[10][32], b1[32], W2[32][32] (row = input unit), b2[32], W3[32], b3.
+ * ------------------------------------------------------------------------------------------ */
+#define PTG_MLP_NPARAM 1441
+/* features[count][10] of the given node elements (indices >= n) at tolerance eps */
+ptg_status ptg_node_features(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const int32_t *nodes,
+                             int64_t count, double eps, float *features, void *stream);
+/* Labels by direct summation: err[count][nradii] = worst monopole error over 32 directions at
+ * distance rho_j * size from the node's COM (rho log-spaced from rho_lo to rho_hi), in units of
+ * mass / size^2; rows of nodes with more than cap particles (or massless) are set to -1.  nradii
+ * <= 64.  forest (nullable) gives the particle range ends of forest trees. */
+ptg_status ptg_node_errcurve(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_forest *forest,
+                             const int32_t *nodes, int64_t count, int64_t cap, int nradii, double rho_lo,
+                             double rho_hi, double *err, void *stream);
+/* Training samples: sample i = (curve row rows[i] of node nodes[rows[i]], tolerance eps[i]) ->
+ * x[i][10], label y[i] = log10 of the smallest distance (in sizes) beyond which the curve stays
+ * within eps, valid[i] (0 skipped, 1 ok, 2 label clipped at rho_hi). */
+ptg_status ptg_mlp_samples(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const int32_t *nodes,
+                           const int32_t *rows, const double *eps, int64_t count, const double *err, int nradii,
+                           double rho_lo, double rho_hi, float *x, float *y, int32_t *valid, void *stream);
+/* Gradient of the loss (1/B) sum_b w_b (yhat_b - y_b)^2, w_b = under_weight where yhat_b < y_b
+ * else 1, over a minibatch of B = batch samples: sample b is index[b] if index is given, else a
+ * splitmix64 hash of (seed, b) modulo nsamples.  grad[PTG_MLP_NPARAM] receives the gradient and
+ * grad[PTG_MLP_NPARAM] the loss (so grad holds PTG_MLP_NPARAM + 1 floats).  Deterministic. */
+size_t ptg_mlp_grad_workspace_bytes(int64_t batch);
+ptg_status ptg_mlp_grad(const float *params, const float *x, const float *y, int64_t nsamples, const int32_t *index,
+                        uint64_t seed, int64_t batch, float under_weight, float *grad, void *ws, size_t ws_bytes,
+                        void *stream);
+/* One Adam step (step counts from 1) on params with moment buffers m, v. */
+ptg_status ptg_mlp_adam(float *params, float *m, float *v, const float *grad, int64_t step, float lr, float beta1,
+                        float beta2, float eps, void *stream);
+/* y[i] = MLP(x[i]); bf16 != 0: weights and activations rounded to bfloat16, FP32 sums. */
+ptg_status ptg_mlp_forward(const float *params, const float *x, int64_t count, int bf16, float *y, void *stream);
+/* Fused inference: writes size * 10^(MLP(features(node, eps)) + margin) into aux[3] of every node
+ * slot of an FP32 pack (0 for particles and massless nodes), for PTG_CRIT_RADIUS walks. */
+ptg_status ptg_open_radii(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_walk_tree *wt,
+                          const float *params, float margin, double eps, int bf16, void *stream);
 
 /* ------------------------------------------------------------------------------------------
  * Direct summation, LDS-tiled: replaces bruteforce.py.  self_mode != 0: targets are the sources

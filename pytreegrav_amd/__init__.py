@@ -25,5 +25,6 @@ from .frontend import (  # noqa: F401
     valueTestMethod,
     warn_if_coincident_positions,
 )
+from .forest import BatchedAccelPotential, Forest  # noqa: F401
 
 __version__ = "0.1.0"

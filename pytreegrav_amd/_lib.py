@@ -64,29 +64,48 @@ class WalkTree(_c.Structure):
     ]
 
 
+class Forest(_c.Structure):
+    _fields_ = [("nsys", _i64), ("offsets", _vp), ("roots", _vp), ("cubes", _vp)]
+
+
 class Criterion(_c.Structure):
     _fields_ = [("criterion", _i32), ("reserved", _i32), ("eps", _dbl)]
 
 
-PTG_CRIT_THETA, PTG_CRIT_QERR = 0, 1
+PTG_CRIT_THETA, PTG_CRIT_QERR, PTG_CRIT_RADIUS = 0, 1, 2
+PTG_MLP_NPARAM = 1441
+PTG_GROUPS_RUNS, PTG_GROUPS_CELLS = 0, 1
+_F = _c.POINTER(Forest)
 
 # name -> (restype, argtypes)
 SIGNATURES = {
     "ptg_version": (_c.c_int, []),
     "ptg_last_error": (_c.c_char_p, []),
-    "ptg_build_workspace_bytes": (_c.c_size_t, [_i64]),
-    "ptg_build_sort": (_c.c_int, [_vp, _vp, _vp, _i64, _vp, _c.c_size_t, _c.POINTER(BuildInfo), _vp]),
+    "ptg_build_workspace_bytes": (_c.c_size_t, [_i64, _i64]),
+    "ptg_build_sort": (_c.c_int, [_vp, _vp, _vp, _i64, _F, _vp, _c.c_size_t, _c.POINTER(BuildInfo), _vp]),
     "ptg_node_workspace_bytes": (_c.c_size_t, [_i64, _i64, _c.c_int]),
-    "ptg_groups_workspace_bytes": (_c.c_size_t, [_i64]),
-    "ptg_target_groups": (_c.c_int, [_vp, _vp, _i64, _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),
-    "ptg_build_tree": (_c.c_int, [_vp, _vp, _vp, _c.POINTER(BuildInfo), _c.c_int, _c.c_int, _vp, _c.c_size_t, _vp,
-                                  _c.c_size_t, _c.POINTER(TreeArrays), _vp]),
+    "ptg_groups_workspace_bytes": (_c.c_size_t, [_i64, _i64]),
+    "ptg_target_groups": (_c.c_int, [_vp, _vp, _i64, _F, _c.c_int, _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),
+    "ptg_build_tree": (_c.c_int, [_vp, _vp, _vp, _c.POINTER(BuildInfo), _F, _c.c_int, _c.c_int, _vp, _c.c_size_t,
+                                  _vp, _c.c_size_t, _c.POINTER(TreeArrays), _vp]),
     "ptg_morton_order": (_c.c_int, [_vp, _i64, _vp, _vp, _c.c_size_t, _vp]),
-    "ptg_pack_walk": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _c.POINTER(WalkTree), _vp]),
+    "ptg_pack_walk": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _F, _c.POINTER(WalkTree), _vp]),
     "ptg_walk_workspace_bytes": (_c.c_size_t, [_i64, _c.c_int]),
-    "ptg_walk": (_c.c_int, [_c.POINTER(WalkTree), _vp, _vp, _vp, _i64, _c.c_int, _c.c_int, _c.c_int, _vp, _vp,
+    "ptg_walk": (_c.c_int, [_c.POINTER(WalkTree), _F, _vp, _vp, _vp, _i64, _c.c_int, _c.c_int, _c.c_int, _vp, _vp,
                             _dbl, _c.POINTER(Criterion), _dbl, _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),
     "ptg_walk_check": (_c.c_int, [_vp, _vp]),
+    "ptg_node_features": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _vp, _i64, _dbl, _vp, _vp]),
+    "ptg_node_errcurve": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _F, _vp, _i64, _i64, _c.c_int, _dbl, _dbl,
+                                     _vp, _vp]),
+    "ptg_mlp_samples": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _vp, _vp, _vp, _i64, _vp, _c.c_int, _dbl,
+                                   _dbl, _vp, _vp, _vp, _vp]),
+    "ptg_mlp_grad_workspace_bytes": (_c.c_size_t, [_i64]),
+    "ptg_mlp_grad": (_c.c_int, [_vp, _vp, _vp, _i64, _vp, _c.c_uint64, _i64, _c.c_float, _vp, _vp, _c.c_size_t,
+                                _vp]),
+    "ptg_mlp_adam": (_c.c_int, [_vp, _vp, _vp, _vp, _i64, _c.c_float, _c.c_float, _c.c_float, _c.c_float, _vp]),
+    "ptg_mlp_forward": (_c.c_int, [_vp, _vp, _i64, _c.c_int, _vp, _vp]),
+    "ptg_open_radii": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _c.POINTER(WalkTree), _vp, _c.c_float, _dbl,
+                                  _c.c_int, _vp]),
     "ptg_direct": (_c.c_int, [_vp, _vp, _i64, _vp, _vp, _vp, _i64, _c.c_int, _c.c_int, _c.c_int, _dbl, _vp, _vp]),
 }
 

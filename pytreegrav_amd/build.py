@@ -34,6 +34,7 @@ UNITS = {
     "build.hip": ["-ffp-contract=off"],
     "walk.hip": [],
     "direct.hip": [],
+    "learned.hip": [],
 }
 
 

@@ -367,6 +367,97 @@ __global__ __launch_bounds__(256) void k_keys(const double *pos, int64_t n, cons
     vals[i] = (uint32_t)i;
 }
 
+// ---- forest: several independent systems built in one pass ----
+__global__ __launch_bounds__(256) void k_sys_of(const int64_t *offsets, int64_t nsys, int64_t n, uint32_t *sys_in)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t lo = 0, hi = nsys;  // last s with offsets[s] <= i
+    while (hi - lo > 1) {
+        int64_t mid = (lo + hi) >> 1;
+        if (offsets[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    sys_in[i] = (uint32_t)lo;
+}
+
+// one block per system: its bounding cube (octree.py:539-574, per system)
+__global__ __launch_bounds__(256) void k_bbox_sys(const double *pos, const int64_t *offsets, Geom *geoms)
+{
+    __shared__ double sh[6][4];
+    const int64_t a = offsets[blockIdx.x], b = offsets[blockIdx.x + 1];
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = a + threadIdx.x; i < b; i += blockDim.x)
+        for (int k = 0; k < 3; k++) {
+            double x = pos[3 * i + k];
+            lo[k] = fmin(lo[k], x);
+            hi[k] = fmax(hi[k], x);
+        }
+    int wave = threadIdx.x >> 6, lane = ptg_lane();
+    for (int k = 0; k < 3; k++) {
+        lo[k] = ptg_wave_min(lo[k]);
+        hi[k] = ptg_wave_max(hi[k]);
+    }
+    if (lane == 0)
+        for (int k = 0; k < 3; k++) {
+            sh[k][wave] = lo[k];
+            sh[3 + k][wave] = hi[k];
+        }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 3; k++) {
+        lo[k] = sh[k][0];
+        hi[k] = sh[3 + k][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+            lo[k] = fmin(lo[k], sh[k][w]);
+            hi[k] = fmax(hi[k], sh[3 + k][w]);
+        }
+    }
+    Geom *g = geoms + blockIdx.x;
+    for (int k = 0; k < 3; k++) g->center[k] = 0.5 * (hi[k] + lo[k]);
+    double sz = hi[0] - lo[0];
+    if (hi[1] - lo[1] > sz) sz = hi[1] - lo[1];
+    if (hi[2] - lo[2] > sz) sz = hi[2] - lo[2];
+    g->size = sz;
+    g->inv = sz > 0 ? 1.0 / sz : 0.0;
+    for (int k = 0; k < 3; k++) g->cmin[k] = g->center[k] - 0.5 * sz;
+}
+
+__global__ __launch_bounds__(256) void k_keys_sys(const double *pos, int64_t n, const Geom *geoms,
+                                                  const uint32_t *sys, uint64_t *keys, uint64_t *kU, uint32_t *vals)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Geom *g = geoms + sys[i];
+    uint64_t k = morton_key(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], g->cmin, g->inv);
+    keys[i] = k;
+    kU[i] = k;
+    vals[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_gather_sys_key(const uint32_t *sys, const uint32_t *vals, int64_t n,
+                                                        uint64_t *out)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) out[j] = sys[vals[j]];
+}
+
+__global__ __launch_bounds__(256) void k_gather_keys(const uint64_t *kU, const uint32_t *vals, int64_t n,
+                                                     uint64_t *keys)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) keys[j] = kU[vals[j]];
+}
+
+// root cube of every system, as (centre x, y, z, side)
+__global__ __launch_bounds__(256) void k_forest_cubes(const Geom *geoms, int64_t nsys, double *cubes)
+{
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    for (int k = 0; k < 3; k++) cubes[4 * s + k] = geoms[s].center[k];
+    cubes[4 * s + 3] = geoms[s].size;
+}
+
 // ============================================================================================
 // build workspace
 // ============================================================================================
@@ -377,7 +468,8 @@ struct Counters {
     int32_t coincident;   // bit-identical extended keys exist
     int32_t max_level;
     int32_t nonfinite;    // some position, mass or softening is NaN or infinite
-    int32_t pad[10];
+    int32_t badforest;    // forest offsets do not describe systems of >= 2 particles covering [0, n)
+    int32_t pad[9];
 };
 
 __global__ __launch_bounds__(256) void k_check_finite(const double *pos, const double *m, const double *h, int64_t n,
@@ -407,9 +499,17 @@ struct BuildWS {
     uint32_t *sslot, *sval;
     uint64_t *tkA, *tkB;
     uint32_t *perm, *permB;
+    // forest of several independent systems: per-system cubes, the system of each particle (system s
+    // owns input indices [offsets[s], offsets[s+1]) and, after the per-system sort, the same range of
+    // Morton slots, so one array serves both), keys before sorting
+    Geom *geoms;
+    uint32_t *sys;
+    uint64_t *kU;
+    const int64_t *soff;
+    int64_t nsys;
 };
 
-static bool carve_build(PtgArena &A, int64_t n, BuildWS &w)
+static bool carve_build(PtgArena &A, int64_t n, BuildWS &w, int64_t nsys = 1)
 {
     int64_t nwt = rs_nwt(n);
     int64_t nh = (int64_t)RS_BINS * nwt;
@@ -437,14 +537,24 @@ static bool carve_build(PtgArena &A, int64_t n, BuildWS &w)
     w.tkB = A.take<uint64_t>(n);
     w.perm = A.take<uint32_t>(n);
     w.permB = A.take<uint32_t>(n);
+    w.nsys = nsys;
+    w.soff = nullptr;
+    w.geoms = A.take<Geom>(nsys);
+    if (nsys > 1) {
+        w.sys = A.take<uint32_t>(n);
+        w.kU = A.take<uint64_t>(n);
+    } else {
+        w.sys = nullptr;
+        w.kU = nullptr;
+    }
     return A.ok();
 }
 
-extern "C" size_t ptg_build_workspace_bytes(int64_t n)
+extern "C" size_t ptg_build_workspace_bytes(int64_t n, int64_t nsys)
 {
     PtgArena A(nullptr, 0);
     BuildWS w;
-    carve_build(A, std::max<int64_t>(n, 1), w);
+    carve_build(A, std::max<int64_t>(n, 1), w, std::max<int64_t>(nsys, 1));
     return A.used;
 }
 
@@ -457,26 +567,34 @@ __global__ void k_zero_counters(Counters *c)
     if (i < (int)(sizeof(Counters) / 4)) ((int32_t *)c)[i] = 0;
 }
 
-// lev[i] = whole octree levels shared by Morton slots i and i+1
-__global__ __launch_bounds__(256) void k_levels(const uint64_t *keys, int64_t n, int32_t *lev, Counters *c)
+// lev[i] = whole octree levels shared by Morton slots i and i+1; -1 where slots i and i+1 belong to
+// different systems of a forest (the pair then starts no node and the next pair starts a root)
+__global__ __launch_bounds__(256) void k_levels(const uint64_t *keys, const uint32_t *sys, int64_t n, int32_t *lev,
+                                                Counters *c)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool coll = false;
     if (i < n - 1) {
-        int L = ptg_common_levels(keys[i], keys[i + 1]);
-        lev[i] = L;
-        coll = (keys[i] == keys[i + 1]);
+        if (sys && sys[i] != sys[i + 1]) {
+            lev[i] = -1;
+        } else {
+            int L = ptg_common_levels(keys[i], keys[i + 1]);
+            lev[i] = L;
+            coll = (keys[i] == keys[i + 1]);
+        }
     }
     uint64_t b = ptg_ballot(coll);
     if (b && ptg_lane() == 0) atomicAdd(&c->ncollide, (int)__popcll(b));
 }
 
-// slots whose 63-bit key is shared with a neighbour
-__global__ __launch_bounds__(256) void k_collision_flags(const uint64_t *keys, int64_t n, int32_t *flag)
+// slots whose 63-bit key is shared with a neighbour of the same system
+__global__ __launch_bounds__(256) void k_collision_flags(const uint64_t *keys, const uint32_t *sys, int64_t n,
+                                                         int32_t *flag)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    bool f = (i > 0 && keys[i] == keys[i - 1]) || (i < n - 1 && keys[i] == keys[i + 1]);
+    bool f = (i > 0 && keys[i] == keys[i - 1] && (!sys || sys[i] == sys[i - 1])) ||
+             (i < n - 1 && keys[i] == keys[i + 1] && (!sys || sys[i] == sys[i + 1]));
     flag[i] = f ? 1 : 0;
 }
 
@@ -504,10 +622,10 @@ __device__ void extended_keys(const double *p, uint64_t k0, const Geom *g, uint6
 
 __global__ __launch_bounds__(256) void k_collision_gather(const double *pos, const uint64_t *keys,
                                                           const uint32_t *order, const int32_t *flag,
-                                                          const int32_t *subpos, int64_t n, const Geom *g,
-                                                          uint64_t *sk0, uint64_t *sk1, uint64_t *sk2,
-                                                          uint64_t *sk3, uint32_t *sslot, uint32_t *sval,
-                                                          uint32_t *perm)
+                                                          const int32_t *subpos, int64_t n, const Geom *geoms,
+                                                          const uint32_t *sys, uint64_t *sk0, uint64_t *sk1,
+                                                          uint64_t *sk2, uint64_t *sk3, uint32_t *sslot,
+                                                          uint32_t *sval, uint32_t *perm)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !flag[i]) return;
@@ -515,7 +633,7 @@ __global__ __launch_bounds__(256) void k_collision_gather(const double *pos, con
     uint32_t o = order[i];
     double p[3] = {pos[3 * (int64_t)o], pos[3 * (int64_t)o + 1], pos[3 * (int64_t)o + 2]};
     uint64_t k3[3];
-    extended_keys(p, keys[i], g, k3);
+    extended_keys(p, keys[i], geoms + (sys ? sys[i] : 0), k3);
     sk0[j] = keys[i];
     sk1[j] = k3[0];
     sk2[j] = k3[1];
@@ -525,21 +643,38 @@ __global__ __launch_bounds__(256) void k_collision_gather(const double *pos, con
     perm[j] = (uint32_t)j;
 }
 
-// small collision subsets: stable sort by (K0, K1, K2, K3) through rank counting, rank of j = number
-// of subset elements ordered before it (index breaks ties, so the order is stable)
+// small collision subsets: stable sort by (system, K0, K1, K2, K3) through rank counting, rank of j =
+// number of subset elements ordered before it (index breaks ties, so the order is stable).  The
+// subset is gathered in slot order, so its system ids are already non-decreasing: only elements of
+// j's own system are compared.
 __global__ __launch_bounds__(256) void k_rank_sort4(const uint64_t *k0, const uint64_t *k1, const uint64_t *k2,
-                                                    const uint64_t *k3, int64_t m, uint32_t *perm)
+                                                    const uint64_t *k3, const uint32_t *sslot, const uint32_t *sys,
+                                                    int64_t m, uint32_t *perm)
 {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m) return;
     const uint64_t a0 = k0[j], a1 = k1[j], a2 = k2[j], a3 = k3[j];
-    int64_t rank = 0;
-    for (int64_t i = 0; i < m; i++) {
+    int64_t lo = 0, hi = m;
+    if (sys) {
+        const uint32_t sj = sys[sslot[j]];
+        while (lo < j && sys[sslot[lo]] != sj) lo++;  // subsets here are small (<= 8192)
+        hi = j + 1;
+        while (hi < m && sys[sslot[hi]] == sj) hi++;
+    }
+    int64_t rank = lo;
+    for (int64_t i = lo; i < hi; i++) {
         uint64_t b0 = k0[i], b1 = k1[i], b2 = k2[i], b3 = k3[i];
         bool before = b0 != a0 ? b0 < a0 : b1 != a1 ? b1 < a1 : b2 != a2 ? b2 < a2 : b3 != a3 ? b3 < a3 : i < j;
         rank += before;
     }
     perm[rank] = (uint32_t)j;
+}
+
+__global__ __launch_bounds__(256) void k_gather_sys_sub(const uint32_t *sys, const uint32_t *sslot,
+                                                        const uint32_t *perm, int64_t m, uint64_t *dst)
+{
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < m) dst[j] = sys[sslot[perm[j]]];
 }
 
 __global__ __launch_bounds__(256) void k_gather_u64(const uint64_t *src, const uint32_t *perm, int64_t m,
@@ -580,18 +715,19 @@ __device__ __forceinline__ bool ext_less(const uint64_t *keys, const uint64_t *e
 
 // levels of pairs inside collision runs: re-keyed levels 22..84, then bucket chains of 7
 // (BuildBucket, octree.py:1128-1175: 7 particles and one deeper node per chain level)
-__global__ __launch_bounds__(256) void k_levels_ext(const uint64_t *keys, const uint64_t *ext, int64_t n,
-                                                    int32_t *lev, Counters *c)
+__global__ __launch_bounds__(256) void k_levels_ext(const uint64_t *keys, const uint64_t *ext, const uint32_t *sys,
+                                                    const int64_t *soff, int64_t n, int32_t *lev, Counters *c)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1 || keys[i] != keys[i + 1]) return;
+    if (sys && sys[i] != sys[i + 1]) return;
     int L;
     if (ext[i] != ext[i + 1]) L = 21 + ptg_common_levels(ext[i], ext[i + 1]);
     else if (ext[n + i] != ext[n + i + 1]) L = 42 + ptg_common_levels(ext[n + i], ext[n + i + 1]);
     else if (ext[2 * n + i] != ext[2 * n + i + 1]) L = 63 + ptg_common_levels(ext[2 * n + i], ext[2 * n + i + 1]);
     else {
         // identical to double precision: find where this run of identical extended keys starts
-        int64_t lo = 0, hi = i;  // first index with key == key[i]
+        int64_t lo = sys ? soff[sys[i]] : 0, hi = i;  // first index of the system with key == key[i]
         while (lo < hi) {
             int64_t mid = (lo + hi) >> 1;
             if (ext_less(keys, ext, n, mid, i)) lo = mid + 1;
@@ -620,6 +756,12 @@ __global__ __launch_bounds__(256) void k_max_level(const int32_t *lev, int64_t m
         v = max(v, lev[i]);
     v = ptg_wave_max(v);
     if (ptg_lane() == 0 && v > 0) atomicMax(&c->max_level, v);
+}
+
+__global__ void k_fill_i32(int32_t *a, int64_t n, int32_t v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
 }
 
 __global__ void k_iota_u32(uint32_t *a, int64_t n)
@@ -654,6 +796,49 @@ static ptg_status sort_by_keys(const double *pos, int64_t n, BuildWS &w, hipStre
     return PTG_OK;
 }
 
+static int sys_bits(int64_t nsys)
+{
+    int nbits = 1;
+    while ((1ll << nbits) < nsys) nbits++;
+    return ((nbits + RS_BITS - 1) / RS_BITS) * RS_BITS;
+}
+
+// forest: per-system cubes and keys, stable sort by key, then a stable sort by system, which leaves
+// every system's particles in its own input range sorted by its own keys; result in (w.kA, w.vA)
+static ptg_status sort_by_keys_forest(const double *pos, int64_t n, BuildWS &w, hipStream_t s)
+{
+    k_sys_of<<<grid1(n), 256, 0, s>>>(w.soff, w.nsys, n, w.sys);
+    k_bbox_sys<<<(unsigned)w.nsys, 256, 0, s>>>(pos, w.soff, w.geoms);
+    k_keys_sys<<<grid1(n), 256, 0, s>>>(pos, n, w.geoms, w.sys, w.kA, w.kU, w.vA);
+    PTG_LAUNCH_CHECK();
+    bool in_first;
+    ptg_status st = radix_sort(w.kA, w.vA, w.kB, w.vB, n, 64, w.hist, w.bsum, &in_first, s);
+    if (st) return st;
+    uint32_t *v = in_first ? w.vA : w.vB, *vo = in_first ? w.vB : w.vA;
+    k_gather_sys_key<<<grid1(n), 256, 0, s>>>(w.sys, v, n, w.tkA);
+    PTG_LAUNCH_CHECK();
+    st = radix_sort(w.tkA, v, w.tkB, vo, n, sys_bits(w.nsys), w.hist, w.bsum, &in_first, s);
+    if (st) return st;
+    const uint32_t *vs = in_first ? v : vo;
+    if (vs != w.vA) PTG_HIP_CHECK(hipMemcpyAsync(w.vA, vs, n * 4, hipMemcpyDeviceToDevice, s));
+    k_gather_keys<<<grid1(n), 256, 0, s>>>(w.kU, w.vA, n, w.kA);
+    PTG_LAUNCH_CHECK();
+    return PTG_OK;
+}
+
+// validates a forest descriptor against n; nsys_out = 1 for a plain tree
+static ptg_status forest_check(const ptg_forest *f, int64_t n, const char *who, int64_t *nsys_out)
+{
+    *nsys_out = 1;
+    if (!f) return PTG_OK;
+    if (f->nsys < 1 || f->nsys > n / 2 || !f->offsets || !f->cubes || !f->roots) {
+        ptg_set_error("%s: a forest needs 1 <= nsys <= n/2 and non-null offsets, cubes and roots", who);
+        return PTG_ERR_ARG;
+    }
+    *nsys_out = f->nsys;
+    return PTG_OK;
+}
+
 extern "C" ptg_status ptg_morton_order(const double *pos, int64_t n, int64_t *order, void *ws, size_t ws_bytes,
                                        void *stream)
 {
@@ -670,7 +855,7 @@ extern "C" ptg_status ptg_morton_order(const double *pos, int64_t n, int64_t *or
     PtgArena A(ws, ws_bytes);
     BuildWS w;
     if (!ws || !carve_build(A, n, w)) {
-        ptg_set_error("ptg_morton_order: workspace too small (%zu < %zu)", ws_bytes, ptg_build_workspace_bytes(n));
+        ptg_set_error("ptg_morton_order: workspace too small (%zu < %zu)", ws_bytes, ptg_build_workspace_bytes(n, 1));
         return PTG_ERR_WORKSPACE;
     }
     ptg_status st = sort_by_keys(pos, n, w, s);
@@ -680,8 +865,22 @@ extern "C" ptg_status ptg_morton_order(const double *pos, int64_t n, int64_t *or
     return PTG_OK;
 }
 
-extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const double *h, int64_t n, void *ws,
-                                     size_t ws_bytes, ptg_build_info *info, void *stream)
+// forest offsets: 0 = offsets[0] < offsets[1] - 1 < ... with offsets[nsys] = n (every system holds at
+// least two particles)
+__global__ __launch_bounds__(256) void k_check_offsets(const int64_t *off, int64_t nsys, int64_t n, Counters *c)
+{
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > nsys) return;
+    bool bad = false;
+    if (s == 0) bad = off[0] != 0;
+    else bad = off[s] - off[s - 1] < 2;
+    if (s == nsys) bad |= off[s] != n;
+    if (bad) c->badforest = 1;
+}
+
+extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const double *h, int64_t n,
+                                     const ptg_forest *forest, void *ws, size_t ws_bytes, ptg_build_info *info,
+                                     void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (n < 1 || !pos || !info) {
@@ -692,14 +891,20 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
         ptg_set_error("ptg_build_sort: n=%lld exceeds the int32 element range", (long long)n);
         return PTG_ERR_ARG;
     }
+    int64_t nsys;
+    ptg_status st = forest_check(forest, n, "ptg_build_sort", &nsys);
+    if (st) return st;
     PtgArena A(ws, ws_bytes);
     BuildWS w;
-    if (!ws || !carve_build(A, n, w)) {
-        ptg_set_error("ptg_build_sort: workspace too small (%zu < %zu)", ws_bytes, ptg_build_workspace_bytes(n));
+    if (!ws || !carve_build(A, n, w, nsys)) {
+        ptg_set_error("ptg_build_sort: workspace too small (%zu < %zu)", ws_bytes, ptg_build_workspace_bytes(n, nsys));
         return PTG_ERR_WORKSPACE;
     }
+    const bool F = nsys > 1;
+    if (F) w.soff = forest->offsets;
     k_zero_counters<<<1, 64, 0, s>>>(w.ctr);
     k_check_finite<<<(unsigned)std::min<int64_t>(1024, ptg_cdiv(n, 256)), 256, 0, s>>>(pos, m, h, n, w.ctr);
+    if (forest) k_check_offsets<<<grid1(nsys + 1), 256, 0, s>>>(forest->offsets, nsys, n, w.ctr);
     Counters hc;
     PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
     PTG_HIP_CHECK(hipStreamSynchronize(s));
@@ -710,26 +915,35 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
         ptg_set_error("ptg_build_sort: non-finite position, mass or softening");
         return PTG_ERR_ARG;
     }
-    ptg_status st = sort_by_keys(pos, n, w, s);
+    if (hc.badforest) {
+        ptg_set_error("ptg_build_sort: forest offsets must start at 0, end at n and give every system >= 2 particles");
+        return PTG_ERR_ARG;
+    }
+    st = F ? sort_by_keys_forest(pos, n, w, s) : sort_by_keys(pos, n, w, s);
     if (st) return st;
+    if (!F) PTG_HIP_CHECK(hipMemcpyAsync(w.geoms, w.geom, sizeof(Geom), hipMemcpyDeviceToDevice, s));
+    if (forest) {
+        k_forest_cubes<<<grid1(nsys), 256, 0, s>>>(w.geoms, nsys, forest->cubes);
+        PTG_LAUNCH_CHECK();
+    }
     if (n >= 2) {
-        k_levels<<<grid1(n - 1), 256, 0, s>>>(w.kA, n, w.lev, w.ctr);
+        k_levels<<<grid1(n - 1), 256, 0, s>>>(w.kA, w.sys, n, w.lev, w.ctr);
         PTG_LAUNCH_CHECK();
         PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
         PTG_HIP_CHECK(hipStreamSynchronize(s));
         if (hc.ncollide > 0) {
             // collision runs: compute extended keys and re-sort each run by them (stable)
-            k_collision_flags<<<grid1(n), 256, 0, s>>>(w.kA, n, w.flag);
+            k_collision_flags<<<grid1(n), 256, 0, s>>>(w.kA, w.sys, n, w.flag);
             st = excl_scan(LoadArr{w.flag}, n, w.subpos, w.bsum, &w.ctr->nsub, s);
             if (st) return st;
-            k_collision_gather<<<grid1(n), 256, 0, s>>>(pos, w.kA, w.vA, w.flag, w.subpos, n, w.geom, w.sk0,
-                                                        w.sk1, w.sk2, w.sk3, w.sslot, w.sval, w.perm);
+            k_collision_gather<<<grid1(n), 256, 0, s>>>(pos, w.kA, w.vA, w.flag, w.subpos, n, w.geoms, w.sys,
+                                                        w.sk0, w.sk1, w.sk2, w.sk3, w.sslot, w.sval, w.perm);
             PTG_LAUNCH_CHECK();
             PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
             PTG_HIP_CHECK(hipStreamSynchronize(s));
             int64_t m = hc.nsub;
             if (m <= 8192) {
-                k_rank_sort4<<<grid1(m), 256, 0, s>>>(w.sk0, w.sk1, w.sk2, w.sk3, m, w.perm);
+                k_rank_sort4<<<grid1(m), 256, 0, s>>>(w.sk0, w.sk1, w.sk2, w.sk3, w.sslot, w.sys, m, w.perm);
                 PTG_LAUNCH_CHECK();
             } else {
                 uint64_t *karr[4] = {w.sk3, w.sk2, w.sk1, w.sk0};
@@ -741,11 +955,19 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
                     if (!in_first)
                         PTG_HIP_CHECK(hipMemcpyAsync(w.perm, w.permB, m * 4, hipMemcpyDeviceToDevice, s));
                 }
+                if (F) {  // most significant: the system
+                    k_gather_sys_sub<<<grid1(m), 256, 0, s>>>(w.sys, w.sslot, w.perm, m, w.tkA);
+                    bool in_first;
+                    st = radix_sort(w.tkA, w.perm, w.tkB, w.permB, m, sys_bits(nsys), w.hist, w.bsum, &in_first, s);
+                    if (st) return st;
+                    if (!in_first)
+                        PTG_HIP_CHECK(hipMemcpyAsync(w.perm, w.permB, m * 4, hipMemcpyDeviceToDevice, s));
+                }
             }
             PTG_HIP_CHECK(hipMemsetAsync(w.ext, 0, 3 * n * sizeof(uint64_t), s));
             k_collision_apply<<<grid1(m), 256, 0, s>>>(w.perm, m, w.sslot, w.sval, w.sk1, w.sk2, w.sk3, w.vA, w.ext,
                                                        n);
-            k_levels_ext<<<grid1(n - 1), 256, 0, s>>>(w.kA, w.ext, n, w.lev, w.ctr);
+            k_levels_ext<<<grid1(n - 1), 256, 0, s>>>(w.kA, w.ext, w.sys, w.soff, n, w.lev, w.ctr);
             PTG_LAUNCH_CHECK();
         }
         st = excl_scan(LoadNodeCount{w.lev}, n - 1, w.off, w.bsum, &w.ctr->nnodes, s);
@@ -755,7 +977,7 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
     }
     Geom hg;
     PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
-    PTG_HIP_CHECK(hipMemcpyAsync(&hg, w.geom, sizeof(Geom), hipMemcpyDeviceToHost, s));
+    PTG_HIP_CHECK(hipMemcpyAsync(&hg, w.geoms, sizeof(Geom), hipMemcpyDeviceToHost, s));
     PTG_HIP_CHECK(hipStreamSynchronize(s));
     info->n = n;
     info->nnodes = n >= 2 ? hc.nnodes : 1;
@@ -778,6 +1000,7 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
 // ============================================================================================
 struct KeyView {
     const uint64_t *keys, *ext;  // ext == nullptr when no collisions
+    const uint32_t *sys;         // system of each slot (forest), nullptr for one tree
     int64_t n;
     __device__ int digit(int64_t i, int L) const  // digit selecting the level-L cell (L >= 1)
     {
@@ -792,6 +1015,7 @@ struct KeyView {
     // do Morton slots a and b lie in the same level-L cell?
     __device__ bool same_cell(int64_t a, int64_t b, int L) const
     {
+        if (sys && sys[a] != sys[b]) return false;
         if (L == 0) return true;
         if (L < 21) return (keys[a] >> (63 - 3 * L)) == (keys[b] >> (63 - 3 * L));
         if (keys[a] != keys[b]) return false;
@@ -807,15 +1031,19 @@ struct KeyView {
     }
 };
 
-__device__ __forceinline__ int32_t head_of(int64_t q, int64_t n, const int32_t *lev, const int32_t *off)
+// element that starts at Morton slot q (q >= 1): the outermost node opened by pair q, else the
+// particle; -1 past the end of the tree (the end of the particles, or of the system in a forest)
+__device__ __forceinline__ int32_t head_of(int64_t q, int64_t n, const int32_t *lev, const int32_t *off,
+                                           const uint32_t *sys)
 {
     if (q >= n) return -1;
+    if (sys && sys[q] != sys[q - 1]) return -1;
     if (q <= n - 2 && lev[q] > lev[q - 1]) return (int32_t)(n + off[q]);
     return (int32_t)q;
 }
 
 __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *lev, const int32_t *off, int64_t n,
-                                                    const Geom *g, double *coord, double *sizes,
+                                                    const Geom *geoms, double *coord, double *sizes,
                                                     int32_t *first, int32_t *next, uint64_t *nlevel)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -823,6 +1051,7 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
     int lp = i == 0 ? -1 : lev[i - 1];
     int li = lev[i];
     if (li <= lp) return;
+    const Geom *g = geoms + (kv.sys ? kv.sys[i] : 0);
     double c[3] = {g->center[0], g->center[1], g->center[2]};
     double s = g->size;
     int64_t e = i;  // running end of the level-L cell containing slot i
@@ -849,7 +1078,7 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
                 else hi = mid;
             }
             e = lo;
-            next[idx] = head_of(e + 1, n, lev, off);
+            next[idx] = head_of(e + 1, n, lev, off, kv.sys);
         }
         if (L < li) {
             int d = kv.digit(i, L + 1);
@@ -863,7 +1092,7 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
 // particle slots: gather into Morton order (octree.py:577-603) and link to the next element
 __global__ __launch_bounds__(256) void k_particles(const double *pos, const double *m, const double *h,
                                                    const uint32_t *order, const int32_t *lev, const int32_t *off,
-                                                   int64_t n, const ptg_tree_arrays t)
+                                                   const uint32_t *sys, int64_t n, const ptg_tree_arrays t)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -877,7 +1106,7 @@ __global__ __launch_bounds__(256) void k_particles(const double *pos, const doub
     t.deltas[i] = 0.0;
     t.treewalk_indices[i] = o;
     t.first_subnode[i] = -1;
-    t.next_branch[i] = n == 1 ? -1 : head_of(i + 1, n, lev, off);
+    t.next_branch[i] = n == 1 ? -1 : head_of(i + 1, n, lev, off, sys);
     t.nchild[i] = 0;
     t.child_base[i] = -1;
     if (t.quadrupoles)
@@ -955,6 +1184,7 @@ __device__ __forceinline__ int node_children(const int32_t *first, const int32_t
 __global__ __launch_bounds__(256) void k_node_init(int64_t nnodes, NodeWS w)
 {
     int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < 64) w.err[a] = 0;
     if (a >= nnodes) return;
     w.parent[a] = -1;
     w.arrive[a] = 0;
@@ -987,16 +1217,18 @@ __global__ __launch_bounds__(256) void k_link(int64_t n, int64_t nnodes, const p
         for (int k = 0; k < 9; k++) t.quadrupoles[9 * idx + k] = 0.0;
 }
 
-// walk slots: child q of node idx sits at slot child_base[idx] + q (the root at slot 0); record the
-// element -> slot map (for the pack) and its inverse (for the moments pass)
-__global__ __launch_bounds__(256) void k_slots(int64_t n, int64_t nnodes, const ptg_tree_arrays t, NodeWS w)
+// walk slots: the root of system s sits at slot s, child q of node idx at slot child_base[idx] + q;
+// record the element -> slot map (for the pack) and its inverse (for the moments pass)
+__global__ __launch_bounds__(256) void k_slots(int64_t n, int64_t nnodes, int64_t nsys, const int32_t *roots,
+                                               const ptg_tree_arrays t, NodeWS w)
 {
     int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nnodes) return;
     int64_t idx = n + a;
-    if (a == 0) {
-        t.slot[n] = 0;
-        w.elem[0] = (int32_t)n;
+    if (a < nsys) {
+        const int32_t r = roots ? roots[a] : (int32_t)n;
+        t.slot[r] = (int32_t)a;
+        w.elem[a] = r;
     }
     int32_t kids[8];
     int nk = node_children(t.first_subnode, t.next_branch, idx, n + nnodes, kids);
@@ -1007,7 +1239,7 @@ __global__ __launch_bounds__(256) void k_slots(int64_t n, int64_t nnodes, const 
     int64_t base = t.child_base[idx];
     for (int q = 0; q < nk; q++) {
         int64_t sl = base + q;
-        if (sl <= 0 || sl >= n + nnodes) {
+        if (sl < nsys || sl >= n + nnodes) {
             w.err[0] = 1;
             continue;
         }
@@ -1161,21 +1393,33 @@ struct LoadNChild {
     __device__ int32_t operator()(int64_t a) const { return nchild[n + a]; }
 };
 
-__global__ __launch_bounds__(256) void k_child_base_fix(int64_t n, int64_t nnodes, int32_t *child_base)
+__global__ __launch_bounds__(256) void k_child_base_fix(int64_t n, int64_t nnodes, int64_t nsys, int32_t *child_base)
 {
     int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < nnodes) child_base[n + a] += 1;  // slot 0 holds the root
+    if (a < nnodes) child_base[n + a] += (int32_t)nsys;  // slots [0, nsys) hold the roots
+}
+
+// forest roots: pair offsets[s] (the first pair of system s) opens its root
+__global__ __launch_bounds__(256) void k_forest_roots(const int64_t *soff, int64_t nsys, const int32_t *off, int64_t n,
+                                                      int32_t *roots)
+{
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsys) roots[s] = (int32_t)(n + off[soff[s]]);
 }
 
 extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const double *h, const ptg_build_info *info,
-                                     int quadrupole, int compute_moments, void *ws, size_t ws_bytes, void *node_ws,
-                                     size_t node_ws_bytes, const ptg_tree_arrays *tree, void *stream)
+                                     const ptg_forest *forest, int quadrupole, int compute_moments, void *ws,
+                                     size_t ws_bytes, void *node_ws, size_t node_ws_bytes,
+                                     const ptg_tree_arrays *tree, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!pos || !m || !h || !info || !tree || info->n < 1) {
         ptg_set_error("ptg_build_tree: bad arguments");
         return PTG_ERR_ARG;
     }
+    int64_t nsys;
+    ptg_status st = forest_check(forest, info->n, "ptg_build_tree", &nsys);
+    if (st) return st;
     const ptg_tree_arrays t = *tree;
     if (!t.coordinates || !t.masses || !t.softenings || !t.sizes || !t.deltas || !t.first_subnode ||
         !t.next_branch || !t.treewalk_indices || !t.nchild || !t.child_base || !t.slot ||
@@ -1186,7 +1430,7 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     int64_t n = info->n, nnodes = info->nnodes;
     PtgArena A(ws, ws_bytes);
     BuildWS w;
-    if (!ws || !carve_build(A, n, w)) {
+    if (!ws || !carve_build(A, n, w, nsys)) {
         ptg_set_error("ptg_build_tree: build workspace too small");
         return PTG_ERR_WORKSPACE;
     }
@@ -1198,24 +1442,27 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
                       ptg_node_workspace_bytes(n, nnodes, max_level));
         return PTG_ERR_WORKSPACE;
     }
-    KeyView kv{w.kA, info->has_collisions ? w.ext : nullptr, n};
-    k_particles<<<grid1(n), 256, 0, s>>>(pos, m, h, w.vA, w.lev, w.off, n, t);
+    const bool F = nsys > 1;
+    if (F) w.soff = forest->offsets;
+    KeyView kv{w.kA, info->has_collisions ? w.ext : nullptr, w.sys, n};
+    k_particles<<<grid1(n), 256, 0, s>>>(pos, m, h, w.vA, w.lev, w.off, w.sys, n, t);
     if (n == 1) {
-        k_single_root<<<1, 1, 0, s>>>(w.geom, t);
+        k_single_root<<<1, 1, 0, s>>>(w.geoms, t);
         PTG_HIP_CHECK(hipMemsetAsync(nw.lvA, 0, sizeof(uint64_t), s));
     } else {
-        k_emit_nodes<<<grid1(n - 1), 256, 0, s>>>(kv, w.lev, w.off, n, w.geom, t.coordinates, t.sizes,
+        k_emit_nodes<<<grid1(n - 1), 256, 0, s>>>(kv, w.lev, w.off, n, w.geoms, t.coordinates, t.sizes,
                                                   t.first_subnode, t.next_branch, nw.lvA);
     }
-    PTG_HIP_CHECK(hipMemsetAsync(nw.err, 0, 64 * sizeof(int32_t), s));
-    k_node_init<<<grid1(nnodes), 256, 0, s>>>(nnodes, nw);
+    if (F) k_forest_roots<<<grid1(nsys), 256, 0, s>>>(w.soff, nsys, w.off, n, forest->roots);
+    else if (forest) k_fill_i32<<<1, 64, 0, s>>>(forest->roots, 1, (int32_t)n);
+    k_node_init<<<grid1(std::max<int64_t>(nnodes, 64)), 256, 0, s>>>(nnodes, nw);
     k_link<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t, nw);
     PTG_LAUNCH_CHECK();
     // sibling-contiguous walk slots: exclusive scan of child counts in node order
-    ptg_status st = excl_scan(LoadNChild{t.nchild, n}, nnodes, t.child_base + n, nw.bsum, nullptr, s);
+    st = excl_scan(LoadNChild{t.nchild, n}, nnodes, t.child_base + n, nw.bsum, nullptr, s);
     if (st) return st;
-    k_child_base_fix<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t.child_base);
-    k_slots<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t, nw);
+    k_child_base_fix<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, t.child_base);
+    k_slots<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, F ? forest->roots : nullptr, t, nw);
     PTG_LAUNCH_CHECK();
     if (compute_moments) {
         // counting sort of node offsets by level (stable LSD radix sort on the level)
@@ -1243,13 +1490,14 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
 // walk pack
 // ============================================================================================
 template <typename T>
-__device__ void pack_slot(int64_t e, int64_t slot, int64_t n, const ptg_tree_arrays &t, const ptg_walk_tree &wt)
+__device__ void pack_slot(int64_t e, int64_t slot, int64_t n, const ptg_tree_arrays &t, const ptg_walk_tree &wt,
+                          const double *origin)
 {
     T *xyzm = (T *)wt.xyzm + 4 * slot;
     T *aux = (T *)wt.aux + 4 * slot;
-    xyzm[0] = (T)(t.coordinates[3 * e] - wt.origin[0]);
-    xyzm[1] = (T)(t.coordinates[3 * e + 1] - wt.origin[1]);
-    xyzm[2] = (T)(t.coordinates[3 * e + 2] - wt.origin[2]);
+    xyzm[0] = (T)(t.coordinates[3 * e] - origin[0]);
+    xyzm[1] = (T)(t.coordinates[3 * e + 1] - origin[1]);
+    xyzm[2] = (T)(t.coordinates[3 * e + 2] - origin[2]);
     xyzm[3] = (T)t.masses[e];
     aux[0] = (T)t.softenings[e];
     aux[1] = (T)t.sizes[e];
@@ -1283,29 +1531,63 @@ __device__ void pack_slot(int64_t e, int64_t slot, int64_t n, const ptg_tree_arr
     }
 }
 
+// last index s in [0, cnt) with a[s] <= v (a ascending, a[0] <= v)
+template <typename I> __device__ __forceinline__ int64_t upper_index(const I *a, int64_t cnt, int64_t v)
+{
+    int64_t lo = 0, hi = cnt;
+    while (hi - lo > 1) {
+        int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)a[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// forest view for the pack: system s owns slots [soff[s], soff[s+1]) and nodes [roots[s], roots[s+1]);
+// FP32 packs store coordinates relative to the system's cube centre, FP64 packs absolute ones
+struct PackForest {
+    int64_t nsys;
+    const int64_t *soff;
+    const int32_t *roots;
+    const double *cubes;
+};
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_pack(int64_t n, int64_t nnodes, const ptg_tree_arrays t, const ptg_walk_tree wt)
+__global__ __launch_bounds__(256) void k_pack(int64_t n, int64_t nnodes, const ptg_tree_arrays t, const ptg_walk_tree wt,
+                                              PackForest f)
 {
     int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n + nnodes) return;
     int64_t sl = t.slot[e];
-    if (sl >= 0 && sl < wt.nslots) pack_slot<T>(e, sl, n, t, wt);
+    if (sl < 0 || sl >= wt.nslots) return;
+    double origin[3] = {wt.origin[0], wt.origin[1], wt.origin[2]};
+    if (f.nsys > 0) {
+        const int64_t sys = f.nsys == 1 ? 0 : e < n ? upper_index(f.soff, f.nsys, e) : upper_index(f.roots, f.nsys, e);
+        const bool rel = sizeof(T) == 4;
+        for (int k = 0; k < 3; k++) origin[k] = rel ? f.cubes[4 * sys + k] : 0.0;
+    }
+    pack_slot<T>(e, sl, n, t, wt, origin);
 }
 
-extern "C" ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_walk_tree *wt,
-                                    void *stream)
+extern "C" ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_forest *forest,
+                                    const ptg_walk_tree *wt, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!tree || !wt || !wt->xyzm || !wt->aux || !wt->kids || n < 1 || nnodes < 1 || wt->nslots != n + nnodes) {
         ptg_set_error("ptg_pack_walk: bad arguments");
         return PTG_ERR_ARG;
     }
+    int64_t nsys;
+    ptg_status st = forest_check(forest, n, "ptg_pack_walk", &nsys);
+    if (st) return st;
+    PackForest f{forest ? nsys : 0, forest ? forest->offsets : nullptr, forest ? forest->roots : nullptr,
+                 forest ? forest->cubes : nullptr};
     if (wt->quad && !tree->quadrupoles) {
         ptg_set_error("ptg_pack_walk: a quadrupole pack needs a tree with quadrupoles");
         return PTG_ERR_ARG;
     }
-    if (wt->prec == PTG_F32) k_pack<float><<<grid1(n + nnodes), 256, 0, s>>>(n, nnodes, *tree, *wt);
-    else if (wt->prec == PTG_F64) k_pack<double><<<grid1(n + nnodes), 256, 0, s>>>(n, nnodes, *tree, *wt);
+    if (wt->prec == PTG_F32) k_pack<float><<<grid1(n + nnodes), 256, 0, s>>>(n, nnodes, *tree, *wt, f);
+    else if (wt->prec == PTG_F64) k_pack<double><<<grid1(n + nnodes), 256, 0, s>>>(n, nnodes, *tree, *wt, f);
     else {
         ptg_set_error("ptg_pack_walk: prec must be PTG_F32 or PTG_F64");
         return PTG_ERR_ARG;
@@ -1323,18 +1605,21 @@ extern "C" ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int6
 // are first split into maximal cells of at most cell_cap (= 4 x group_size) targets, and each such
 // cell is cut into ceil(len / group_size) near-equal consecutive chunks.
 // ============================================================================================
-__global__ __launch_bounds__(256) void k_sorted_keys(const double *pos, const int64_t *perm, int64_t n, const Geom *g,
+__global__ __launch_bounds__(256) void k_sorted_keys(const double *pos, const int64_t *perm, int64_t n,
+                                                     const Geom *geoms, const int64_t *soff, int64_t nsys,
                                                      uint64_t *keys)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int64_t o = perm[t];
+    const Geom *g = geoms + (nsys > 1 ? upper_index(soff, nsys, t) : 0);
     keys[t] = morton_key(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], g->cmin, g->inv);
 }
 
-// number of consecutive positions on one side of t (direction dir) whose key shares the top 3L
-// bits with keys[t], capped at cap
-__device__ int64_t same_cell_extent(const uint64_t *keys, int64_t n, int64_t t, int L, int dir, int64_t cap)
+// number of consecutive positions on one side of t (direction dir), inside [lo, hi), whose key
+// shares the top 3L bits with keys[t], capped at cap
+__device__ int64_t same_cell_extent(const uint64_t *keys, int64_t lo_b, int64_t hi_b, int64_t t, int L, int dir,
+                                    int64_t cap)
 {
     auto same = [&](int64_t j) {
         if (L == 0) return true;
@@ -1345,13 +1630,13 @@ __device__ int64_t same_cell_extent(const uint64_t *keys, int64_t n, int64_t t, 
     while (good < cap) {
         int64_t probe = std::min(good + step, cap);
         int64_t j = t + dir * probe;
-        if (j < 0 || j >= n || !same(j)) {
+        if (j < lo_b || j >= hi_b || !same(j)) {
             // bisect between good (matching) and probe (not matching or outside)
             int64_t lo = good, hi = probe;
             while (hi - lo > 1) {
                 int64_t mid = (lo + hi) >> 1;
                 int64_t jm = t + dir * mid;
-                if (jm >= 0 && jm < n && same(jm)) lo = mid;
+                if (jm >= lo_b && jm < hi_b && same(jm)) lo = mid;
                 else hi = mid;
             }
             return lo;
@@ -1362,19 +1647,26 @@ __device__ int64_t same_cell_extent(const uint64_t *keys, int64_t n, int64_t t, 
     return cap;
 }
 
-// pflag[t] = 1 where a new capped cell starts (t == 0, or the smallest cell holding t-1 and t has
-// more than cap targets)
-__global__ __launch_bounds__(256) void k_cell_cuts(const uint64_t *keys, int64_t n, int64_t cap, int32_t *pflag)
+// pflag[t] = 1 where a new capped cell starts (t == 0 or the start of a system, or the smallest cell
+// holding t-1 and t has more than cap targets)
+__global__ __launch_bounds__(256) void k_cell_cuts(const uint64_t *keys, int64_t n, const int64_t *soff, int64_t nsys,
+                                                   int64_t cap, int32_t *pflag)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    if (t == 0) {
-        pflag[0] = 1;
+    int64_t lo_b = 0, hi_b = n;
+    if (nsys > 1) {
+        const int64_t sy = upper_index(soff, nsys, t);
+        lo_b = soff[sy];
+        hi_b = soff[sy + 1];
+    }
+    if (t == lo_b) {
+        pflag[t] = 1;
         return;
     }
     int L = ptg_common_levels(keys[t - 1], keys[t]);
-    int64_t left = same_cell_extent(keys, n, t - 1, L, -1, cap);
-    int64_t right = left >= cap ? 0 : same_cell_extent(keys, n, t, L, +1, cap);
+    int64_t left = same_cell_extent(keys, lo_b, hi_b, t - 1, L, -1, cap);
+    int64_t right = left >= cap ? 0 : same_cell_extent(keys, lo_b, hi_b, t, L, +1, cap);
     pflag[t] = (left + right + 2 > cap) ? 1 : 0;
 }
 
@@ -1404,6 +1696,16 @@ __global__ __launch_bounds__(256) void k_chunk_flags(const int32_t *pflag, const
     gflag[t] = ((t - ps) % cs == 0) ? 1 : 0;
 }
 
+// fixed runs of gs consecutive targets, restarted at every system start (the reference's grouping,
+// grouped_treewalk.py:179-182, applied to each system of a forest)
+__global__ __launch_bounds__(256) void k_run_flags(int64_t n, const int64_t *soff, int64_t nsys, int gs, int32_t *gflag)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t start = nsys > 1 ? soff[upper_index(soff, nsys, t)] : 0;
+    gflag[t] = ((t - start) % gs == 0) ? 1 : 0;
+}
+
 struct GroupWS {
     double *part;
     Geom *geom;
@@ -1411,10 +1713,10 @@ struct GroupWS {
     int32_t *pflag, *pex, *pstart, *gflag, *gex, *bsum;
 };
 
-static bool carve_groups(PtgArena &A, int64_t n, GroupWS &w)
+static bool carve_groups(PtgArena &A, int64_t n, int64_t nsys, GroupWS &w)
 {
     w.part = A.take<double>(6 * BB_BLOCKS);
-    w.geom = A.take<Geom>(1);
+    w.geom = A.take<Geom>(nsys);
     w.keys = A.take<uint64_t>(n);
     w.pflag = A.take<int32_t>(n);
     w.pex = A.take<int32_t>(n);
@@ -1425,16 +1727,17 @@ static bool carve_groups(PtgArena &A, int64_t n, GroupWS &w)
     return A.ok();
 }
 
-extern "C" size_t ptg_groups_workspace_bytes(int64_t n)
+extern "C" size_t ptg_groups_workspace_bytes(int64_t n, int64_t nsys)
 {
     PtgArena A(nullptr, 0);
     GroupWS w;
-    carve_groups(A, std::max<int64_t>(n, 1), w);
+    carve_groups(A, std::max<int64_t>(n, 1), std::max<int64_t>(nsys, 1), w);
     return A.used;
 }
 
-extern "C" ptg_status ptg_target_groups(const double *pos, const int64_t *perm, int64_t n, int group_size,
-                                        int32_t *offsets, int32_t *ngroups, void *ws, size_t ws_bytes, void *stream)
+extern "C" ptg_status ptg_target_groups(const double *pos, const int64_t *perm, int64_t n, const ptg_forest *forest,
+                                        int group_size, int mode, int32_t *offsets, int32_t *ngroups, void *ws,
+                                        size_t ws_bytes, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (n < 1 || !pos || !perm || !offsets || !ngroups || group_size < 1 || group_size > 256) {
@@ -1445,19 +1748,43 @@ extern "C" ptg_status ptg_target_groups(const double *pos, const int64_t *perm, 
         ptg_set_error("ptg_target_groups: n exceeds the int32 range");
         return PTG_ERR_ARG;
     }
+    int64_t nsys;
+    ptg_status st = forest_check(forest, n, "ptg_target_groups", &nsys);
+    if (st) return st;
     PtgArena A(ws, ws_bytes);
     GroupWS w;
-    if (!ws || !carve_groups(A, n, w)) {
-        ptg_set_error("ptg_target_groups: workspace too small (%zu < %zu)", ws_bytes, ptg_groups_workspace_bytes(n));
+    if (!ws || !carve_groups(A, n, nsys, w)) {
+        ptg_set_error("ptg_target_groups: workspace too small (%zu < %zu)", ws_bytes,
+                      ptg_groups_workspace_bytes(n, nsys));
         return PTG_ERR_WORKSPACE;
     }
-    int nb = (int)std::min<int64_t>(BB_BLOCKS, std::max<int64_t>(1, ptg_cdiv(n, 1024)));
-    k_bbox_partial<<<nb, 256, 0, s>>>(pos, n, w.part);
-    k_bbox_final<<<1, 256, 0, s>>>(w.part, nb, w.geom);
-    k_sorted_keys<<<grid1(n), 256, 0, s>>>(pos, perm, n, w.geom, w.keys);
-    k_cell_cuts<<<grid1(n), 256, 0, s>>>(w.keys, n, 4 * (int64_t)group_size, w.pflag);
+    const int64_t *soff = nsys > 1 ? forest->offsets : nullptr;
+    if (mode != PTG_GROUPS_CELLS && mode != PTG_GROUPS_RUNS) {
+        ptg_set_error("ptg_target_groups: mode must be PTG_GROUPS_RUNS or PTG_GROUPS_CELLS");
+        return PTG_ERR_ARG;
+    }
+    if (mode == PTG_GROUPS_RUNS) {
+        k_run_flags<<<grid1(n), 256, 0, s>>>(n, soff, nsys, group_size, w.gflag);
+        PTG_LAUNCH_CHECK();
+        st = excl_scan(LoadArr{w.gflag}, n, w.gex, w.bsum, nullptr, s);
+        if (st) return st;
+        k_scatter_starts<<<grid1(n), 256, 0, s>>>(w.gflag, w.gex, n, offsets, ngroups);
+        PTG_LAUNCH_CHECK();
+        return PTG_OK;
+    }
+    if (nsys > 1) {
+        // targets are the forest's own particles: system s owns input indices and sorted positions
+        // [offsets[s], offsets[s+1])
+        k_bbox_sys<<<(unsigned)nsys, 256, 0, s>>>(pos, soff, w.geom);
+    } else {
+        int nb = (int)std::min<int64_t>(BB_BLOCKS, std::max<int64_t>(1, ptg_cdiv(n, 1024)));
+        k_bbox_partial<<<nb, 256, 0, s>>>(pos, n, w.part);
+        k_bbox_final<<<1, 256, 0, s>>>(w.part, nb, w.geom);
+    }
+    k_sorted_keys<<<grid1(n), 256, 0, s>>>(pos, perm, n, w.geom, soff, nsys, w.keys);
+    k_cell_cuts<<<grid1(n), 256, 0, s>>>(w.keys, n, soff, nsys, 4 * (int64_t)group_size, w.pflag);
     PTG_LAUNCH_CHECK();
-    ptg_status st = excl_scan(LoadArr{w.pflag}, n, w.pex, w.bsum, nullptr, s);
+    st = excl_scan(LoadArr{w.pflag}, n, w.pex, w.bsum, nullptr, s);
     if (st) return st;
     k_scatter_starts<<<grid1(n), 256, 0, s>>>(w.pflag, w.pex, n, w.pstart, nullptr);
     k_chunk_flags<<<grid1(n), 256, 0, s>>>(w.pflag, w.pex, w.pstart, n, group_size, w.gflag);

@@ -55,6 +55,11 @@ template <typename T> struct WalkArgs {
     T eps;     // criterion 1/2: per-node acceleration error tolerance (already divided by 3.5)
     double G;
     double ox, oy, oz;
+    // forest: group g walks the tree of the system s holding its first target (root at slot s),
+    // with coordinates relative to that system's cube centre (FP32) or absolute (FP64)
+    int64_t nsys;          // 0: one tree
+    const int64_t *soff;   // (nsys + 1) system target ranges
+    const double *cubes;   // (nsys, 4)
     double *out;
     int64_t *counters;
     WalkCtl *ctl;
@@ -382,6 +387,24 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
             const int64_t e0 = A.goff ? (int64_t)A.goff[g + 1] : A.nt;
             const int64_t b0 = e0 < a0 + A.gs ? e0 : a0 + A.gs;
             const int mt = (int)(b0 - a0);
+            int root = 0;
+            double ox = A.ox, oy = A.oy, oz = A.oz;
+            if (A.nsys > 0) {
+                int64_t lo = 0, hi = A.nsys;  // last system with soff[s] <= a0
+                while (hi - lo > 1) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (A.soff[mid] <= a0) lo = mid;
+                    else hi = mid;
+                }
+                root = (int)lo;
+                if (sizeof(T) == 4) {
+                    ox = A.cubes[4 * lo];
+                    oy = A.cubes[4 * lo + 1];
+                    oz = A.cubes[4 * lo + 2];
+                } else {
+                    ox = oy = oz = 0.0;
+                }
+            }
 
             // ---- targets: lane holds a0 + lane + 64 j ----
             T tx[TPL], ty[TPL], tz[TPL], th[TPL];
@@ -395,9 +418,9 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                 tval[j] = o < mt;
                 tidx[j] = tval[j] ? A.perm[a0 + o] : 0;
                 if (tval[j]) {
-                    tx[j] = (T)(A.pos[3 * tidx[j]] - A.ox);
-                    ty[j] = (T)(A.pos[3 * tidx[j] + 1] - A.oy);
-                    tz[j] = (T)(A.pos[3 * tidx[j] + 2] - A.oz);
+                    tx[j] = (T)(A.pos[3 * tidx[j]] - ox);
+                    ty[j] = (T)(A.pos[3 * tidx[j] + 1] - oy);
+                    tz[j] = (T)(A.pos[3 * tidx[j] + 2] - oz);
                     th[j] = (T)A.soft[tidx[j]];
                     bmin0 = std::min(bmin0, tx[j]);
                     bmin1 = std::min(bmin1, ty[j]);
@@ -432,7 +455,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
 
             int sp = 1, gsp = 0, nn = 0, np = 0;
             bool overflow = false;
-            if (lane == 0) L.stack[0] = 0;  // the root occupies walk slot 0
+            if (lane == 0) L.stack[0] = root;  // the root of system s occupies walk slot s
             __builtin_amdgcn_wave_barrier();
 
             for (;;) {
@@ -473,6 +496,9 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                     // 3.5 |Q| / r^4, to stay below eps (theta is then only a loose cap)
                     T r2m = r_min * r_min;
                     acc_node = acc_node && X2.w < A.eps * r2m * r2m;
+                } else if (A.crit == 2) {
+                    // learned: r_min beyond the node's predicted opening radius
+                    acc_node = acc_node && r_min > X2.w;
                 }
                 bool acc_part = ve && ispart && !far_part;
                 acc_node = acc_node || far_part;
@@ -662,14 +688,24 @@ static ptg_status launch_walk(const WalkArgs<T> &a, int fields, bool quad, int64
     return launch_fq<T, 4, EXACT>(a, fields, quad, blocks, s);
 }
 
-extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const double *soft, const int64_t *perm,
-                               int64_t nt, int fields, int quadrupole, int group_size, const int32_t *group_offsets,
-                               const int32_t *group_count, double theta, const ptg_criterion *criterion, double G,
-                               int exact, double *out, int64_t *counters, void *ws, size_t ws_bytes, void *stream)
+extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest, const double *pos,
+                               const double *soft, const int64_t *perm, int64_t nt, int fields, int quadrupole,
+                               int group_size, const int32_t *group_offsets, const int32_t *group_count, double theta,
+                               const ptg_criterion *criterion, double G, int exact, double *out, int64_t *counters,
+                               void *ws, size_t ws_bytes, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!wt || nt < 0 || (nt > 0 && (!pos || !soft || !perm || !out))) {
         ptg_set_error("ptg_walk: bad arguments");
+        return PTG_ERR_ARG;
+    }
+    if (forest && (forest->nsys < 1 || !forest->offsets || !forest->cubes)) {
+        ptg_set_error("ptg_walk: a forest needs nsys >= 1, offsets and cubes");
+        return PTG_ERR_ARG;
+    }
+    if (forest && forest->nsys > 1 && !group_offsets) {
+        ptg_set_error("ptg_walk: a forest walk needs group offsets that respect system boundaries "
+                      "(ptg_target_groups with the forest)");
         return PTG_ERR_ARG;
     }
     if (group_size < 1 || group_size > 256) {
@@ -693,12 +729,16 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const
         return PTG_ERR_ARG;
     }
     const int crit = criterion ? criterion->criterion : PTG_CRIT_THETA;
-    if (crit != PTG_CRIT_THETA && crit != PTG_CRIT_QERR) {
+    if (crit != PTG_CRIT_THETA && crit != PTG_CRIT_QERR && crit != PTG_CRIT_RADIUS) {
         ptg_set_error("ptg_walk: unknown criterion %d", crit);
         return PTG_ERR_ARG;
     }
     if (crit == PTG_CRIT_QERR && (exact || !(criterion->eps > 0))) {
         ptg_set_error("ptg_walk: the error-bounded criterion needs eps > 0 and the FP32 walk");
+        return PTG_ERR_ARG;
+    }
+    if (crit == PTG_CRIT_RADIUS && (exact || wt->prec != PTG_F32)) {
+        ptg_set_error("ptg_walk: the radius criterion runs on the FP32 walk with radii from ptg_open_radii");
         return PTG_ERR_ARG;
     }
     const double eps = crit == PTG_CRIT_QERR ? criterion->eps / 3.5 : 0.0;
@@ -710,13 +750,17 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const
     WalkCtl *ctl = (WalkCtl *)ws;
     int *spill = (int *)((char *)ws + ptg_align256(sizeof(WalkCtl)));
     int64_t blocks = walk_grid_blocks();
+    const int64_t fn = forest ? forest->nsys : 0;
+    const int64_t *fo = forest ? forest->offsets : nullptr;
+    const double *fc = forest ? forest->cubes : nullptr;
     k_walk_init<<<1, 64, 0, s>>>(ctl);
     PTG_LAUNCH_CHECK();
     ptg_status st;
     if (wt->prec == PTG_F32) {
         WalkArgs<float> a{(const V4<float> *)wt->xyzm, (const V4<float> *)wt->aux, (const int2 *)wt->kids,
                           (const float *)wt->quad, pos, soft, perm, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
-                          group_size, (float)theta, crit, (float)eps, G, wt->origin[0], wt->origin[1], wt->origin[2], out, counters, ctl, spill};
+                          group_size, (float)theta, crit, (float)eps, G, wt->origin[0], wt->origin[1], wt->origin[2],
+                          fn, fo, fc, out, counters, ctl, spill};
         if (exact) {
             ptg_set_error("ptg_walk: the exact branch set is evaluated in FP64; use prec=PTG_F64");
             return PTG_ERR_ARG;
@@ -725,7 +769,8 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const double *pos, const
     } else if (wt->prec == PTG_F64) {
         WalkArgs<double> a{(const V4<double> *)wt->xyzm, (const V4<double> *)wt->aux, (const int2 *)wt->kids,
                            (const double *)wt->quad, pos, soft, perm, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
-                           group_size, theta, crit, eps, G, wt->origin[0], wt->origin[1], wt->origin[2], out, counters, ctl, spill};
+                           group_size, theta, crit, eps, G, wt->origin[0], wt->origin[1], wt->origin[2],
+                           fn, fo, fc, out, counters, ctl, spill};
         st = launch_walk<double, true>(a, fields, quadrupole != 0, blocks, s);
     } else {
         ptg_set_error("ptg_walk: prec must be PTG_F32 or PTG_F64");

@@ -69,9 +69,14 @@ class NonFiniteInput(ValueError):
 
 
 class DeviceTree:
-    """Device arrays of one octree in the reference's element layout (octree.py:8-28)."""
+    """Device arrays of one octree in the reference's element layout (octree.py:8-28).
 
-    def __init__(self, pos, m, h, quadrupole=False, compute_moments=True):
+    offsets (optional, nsys + 1 increasing integers from 0 to n): build a forest of nsys independent
+    systems, system s being particles [offsets[s], offsets[s+1]) (each needs >= 2 particles).  Every
+    system gets its own cube, keys and tree, exactly as if it were built alone; particles stay
+    system-major in Morton order and the root of system s sits at walk slot s."""
+
+    def __init__(self, pos, m, h, quadrupole=False, compute_moments=True, offsets=None):
         L = _lib.load()
         dev = device()
         pos = to_dev(pos, (-1, 3))
@@ -79,9 +84,25 @@ class DeviceTree:
         m = to_dev(m, (n,))
         h = to_dev(h, (n,))
         s = stream()
-        ws = workspace(L.ptg_build_workspace_bytes(n))
+        self.forest = None
+        self.nsys = 1
+        if offsets is not None:
+            if isinstance(offsets, torch.Tensor):
+                off = offsets.to(device=dev, dtype=torch.int64).contiguous()
+            else:
+                off = torch.from_numpy(np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))).to(dev)
+            nsys = off.numel() - 1
+            if nsys < 1:
+                raise ValueError("offsets must hold nsys + 1 >= 2 entries")
+            self.nsys = nsys
+            self.sys_offsets = off
+            self.roots = torch.empty(nsys, dtype=torch.int32, device=dev)
+            self.cubes = torch.empty((nsys, 4), dtype=torch.float64, device=dev)
+            self.forest = _lib.Forest(nsys, ptr(off).value, ptr(self.roots).value, ptr(self.cubes).value)
+        fref = _lib.ctypes.byref(self.forest) if self.forest is not None else None
+        ws = workspace(L.ptg_build_workspace_bytes(n, self.nsys))
         info = BuildInfo()
-        st = L.ptg_build_sort(ptr(pos), ptr(m), ptr(h), n, ptr(ws), ws.numel(), _lib.ctypes.byref(info), s)
+        st = L.ptg_build_sort(ptr(pos), ptr(m), ptr(h), n, fref, ptr(ws), ws.numel(), _lib.ctypes.byref(info), s)
         if st != 0 and info.nonfinite:
             raise NonFiniteInput("non-finite position, mass or softening")
         check(st, "ptg_build_sort")
@@ -110,12 +131,17 @@ class DeviceTree:
             ptr(self.first_subnode).value, ptr(self.next_branch).value, ptr(self.order).value,
             ptr(self.nchild).value, ptr(self.child_base).value, ptr(self.slot).value)
         nws = workspace(L.ptg_node_workspace_bytes(n, nn, int(info.max_level)))
-        check(L.ptg_build_tree(ptr(pos), ptr(m), ptr(h), _lib.ctypes.byref(info), int(quadrupole),
+        check(L.ptg_build_tree(ptr(pos), ptr(m), ptr(h), _lib.ctypes.byref(info), fref, int(quadrupole),
                                int(compute_moments), ptr(ws), ws.numel(), ptr(nws), nws.numel(),
                                _lib.ctypes.byref(self.arrays), s), "ptg_build_tree")
         # ws / nws go back to torch's stream-ordered caching allocator here; later allocations on
         # this stream are ordered after the build kernels that use them
         self._packs = {}
+
+    @property
+    def forest_ref(self):
+        """ctypes reference to the forest descriptor (None for a single tree)."""
+        return _lib.ctypes.byref(self.forest) if self.forest is not None else None
 
     @property
     def center(self):
@@ -129,10 +155,22 @@ class DeviceTree:
     def has_coincident(self):
         return bool(self.info.has_coincident)
 
-    def pack(self, prec, quadrupole):
+    def pack_learned(self, model, eps, quadrupole=False, bf16=False):
+        """FP32 walk copy whose node slots carry the learned opening radii at tolerance eps
+        (ptg_open_radii); cached per (model, eps, moment order, bf16)."""
+        from .learned import open_radii
+
+        key = ("learned", model.key(), float(eps), bool(quadrupole), bool(bf16))
+        if key not in self._packs:
+            p = self.pack(PTG_F32, quadrupole, cache=False)
+            open_radii(self, p, model, eps, bf16=bf16)
+            self._packs[key] = p
+        return self._packs[key]
+
+    def pack(self, prec, quadrupole, cache=True):
         """Sibling-contiguous walk copy (cached per precision and moment order)."""
         key = (prec, bool(quadrupole))
-        if key in self._packs:
+        if cache and key in self._packs:
             return self._packs[key]
         dev = device()
         dt = torch.float32 if prec == PTG_F32 else torch.float64
@@ -144,10 +182,11 @@ class DeviceTree:
         origin = self.center if prec == PTG_F32 else np.zeros(3)
         wt = WalkTree(ptr(xyzm).value, ptr(aux).value, ptr(kids).value, ptr(quad).value if quad is not None else None,
                       ne, prec, int(bool(quadrupole)), (_lib._dbl * 3)(*origin))
-        check(_lib.load().ptg_pack_walk(_lib.ctypes.byref(self.arrays), self.n, self.nnodes, _lib.ctypes.byref(wt),
-                                        stream()), "ptg_pack_walk")
+        check(_lib.load().ptg_pack_walk(_lib.ctypes.byref(self.arrays), self.n, self.nnodes, self.forest_ref,
+                                        _lib.ctypes.byref(wt), stream()), "ptg_pack_walk")
         p = {"wt": wt, "tensors": (xyzm, aux, kids, quad)}
-        self._packs[key] = p
+        if cache:
+            self._packs[key] = p
         return p
 
 
@@ -158,7 +197,7 @@ def morton_order(pos):
     n = pos.shape[0]
     order = torch.empty(n, dtype=torch.int64, device=device())
     if n:
-        ws = workspace(L.ptg_build_workspace_bytes(n))
+        ws = workspace(L.ptg_build_workspace_bytes(n, 1))
         check(L.ptg_morton_order(ptr(pos), n, ptr(order), ptr(ws), ws.numel(), stream()), "ptg_morton_order")
     return order
 
@@ -166,22 +205,28 @@ def morton_order(pos):
 _walk_ws = {}
 
 
-def target_groups(pos, perm, group_size):
-    """Cell-aligned groups of the Morton-sorted targets pos[perm[t]]: (offsets, count) device int32
-    tensors for ptg_walk (ptg_target_groups)."""
+def target_groups(pos, perm, group_size, tree=None, mode="cells"):
+    """Groups of the Morton-sorted targets pos[perm[t]]: (offsets, count) device int32 tensors for
+    ptg_walk (ptg_target_groups).  mode "cells": cell-aligned groups; "fixed": runs of group_size.
+    tree: the forest whose own particles are the targets (groups then respect its systems)."""
     L = _lib.load()
     pos = to_dev(pos, (-1, 3))
     n = pos.shape[0]
     offsets = torch.empty(n + 1, dtype=torch.int32, device=device())
     count = torch.empty(1, dtype=torch.int32, device=device())
-    ws = workspace(L.ptg_groups_workspace_bytes(n))
-    check(L.ptg_target_groups(ptr(pos), ptr(perm), n, int(group_size), ptr(offsets), ptr(count), ptr(ws), ws.numel(),
-                              stream()), "ptg_target_groups")
+    fref = tree.forest_ref if tree is not None else None
+    nsys = tree.nsys if tree is not None else 1
+    if mode not in ("cells", "fixed"):
+        raise ValueError(f"mode must be 'cells' or 'fixed', got {mode!r}")
+    m = _lib.PTG_GROUPS_CELLS if mode == "cells" else _lib.PTG_GROUPS_RUNS
+    ws = workspace(L.ptg_groups_workspace_bytes(n, nsys))
+    check(L.ptg_target_groups(ptr(pos), ptr(perm), n, fref, int(group_size), m, ptr(offsets), ptr(count), ptr(ws),
+                              ws.numel(), stream()), "ptg_target_groups")
     return offsets, count
 
 
 def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7, G=1.0, precision="float64",
-         counters=None, grouping=None, groups=None, criterion="theta", eps=None):
+         counters=None, grouping=None, groups=None, criterion="theta", eps=None, model=None, bf16=False):
     """Grouped walk of `tree` at targets pos[perm[t]]; returns an (nt, W) float64 device tensor in the
     original target order (FieldsTarget_grouped, grouped_treewalk.py:350-397).
 
@@ -190,7 +235,9 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     groups: precomputed (offsets, count) from target_groups, overriding grouping.
     criterion: "theta" (the reference's geometric test) or "error" (FP32 only: also bound the
     dropped quadrupole term 3.5|Q|/r^4 by eps, an absolute acceleration tolerance for G=1; theta is
-    then a loose cap; the tree must carry quadrupoles)."""
+    then a loose cap; the tree must carry quadrupoles), or "learned" (FP32 only: accept a node only
+    beyond the opening radius the MLP `model` predicts for tolerance eps; theta is the guard; bf16
+    runs the MLP with bfloat16 weights)."""
     L = _lib.load()
     pos = to_dev(pos, (-1, 3))
     nt = pos.shape[0]
@@ -200,12 +247,19 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     if nt == 0:
         return out
     prec = PTG_F32 if precision in ("float32", "f32", np.float32, torch.float32) else PTG_F64
+    if group_size == "auto":
+        group_size = autotune_group_size(tree, pos, soft, perm, mask, quadrupole=quadrupole, theta=theta,
+                                         precision=precision, grouping=grouping, criterion=criterion, eps=eps,
+                                         model=model, bf16=bf16)["choice"]
     if grouping is None:
         grouping = "cells" if prec == PTG_F32 else "fixed"
     if grouping not in ("fixed", "cells"):
         raise ValueError(f"grouping must be 'fixed' or 'cells', got {grouping!r}")
-    if groups is None and grouping == "cells":
-        groups = target_groups(pos, perm, group_size)
+    forest = tree.nsys > 1 or tree.forest is not None
+    if tree.nsys > 1 and (nt != tree.n or perm.data_ptr() != tree.order.data_ptr()):
+        raise ValueError("a forest is walked at its own particles (perm = tree.order)")
+    if groups is None and (grouping == "cells" or tree.nsys > 1):
+        groups = target_groups(pos, perm, group_size, tree=tree if forest else None, mode=grouping)
     goff, gcnt = groups if groups is not None else (None, None)
     crit = _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
     if criterion == "error":
@@ -216,9 +270,19 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
         if eps is None or not eps > 0:
             raise ValueError("criterion='error' needs eps > 0")
         crit = _lib.Criterion(_lib.PTG_CRIT_QERR, 0, float(eps))
+    elif criterion == "learned":
+        if prec != PTG_F32:
+            raise ValueError("criterion='learned' is available for the float32 walk")
+        if not tree.quadrupole:
+            raise ValueError("criterion='learned' needs a tree built with quadrupole=True (a feature is |Q|)")
+        if model is None:
+            raise ValueError("criterion='learned' needs model= (pytreegrav_amd.learned.OpeningMLP)")
+        if eps is None or not eps > 0:
+            raise ValueError("criterion='learned' needs eps > 0")
+        crit = _lib.Criterion(_lib.PTG_CRIT_RADIUS, 0, float(eps))
     elif criterion != "theta":
-        raise ValueError(f"criterion must be 'theta' or 'error', got {criterion!r}")
-    p = tree.pack(prec, quadrupole)
+        raise ValueError(f"criterion must be 'theta', 'error' or 'learned', got {criterion!r}")
+    p = tree.pack_learned(model, eps, quadrupole, bf16) if criterion == "learned" else tree.pack(prec, quadrupole)
     wsb = L.ptg_walk_workspace_bytes(nt, int(group_size))
     key = (torch.cuda.current_device(), wsb)
     ws = _walk_ws.get(key)
@@ -226,7 +290,8 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
         ws = workspace(wsb)
         _walk_ws.clear()
         _walk_ws[key] = ws
-    check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), ptr(pos), ptr(soft), ptr(perm), nt, int(mask), int(bool(quadrupole)),
+    check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), tree.forest_ref if forest else None, ptr(pos), ptr(soft), ptr(perm),
+                     nt, int(mask), int(bool(quadrupole)),
                      int(group_size), ptr(goff), ptr(gcnt), float(theta), _lib.ctypes.byref(crit), float(G),
                      int(prec == PTG_F64), ptr(out),
                      ptr(counters), ptr(ws), ws.numel(), stream()), "ptg_walk")
@@ -265,3 +330,57 @@ def field_scale(tree, groups=64, group_size=64, theta=1.0):
     a = walk(tree, pos, h, torch.arange(idx.shape[0], device=device()), PTG_ACC, group_size=min(group_size, n),
              theta=theta, precision="float32", grouping="fixed")
     return float(torch.sqrt(torch.mean(torch.sum(a * a, dim=1))).item())
+
+
+# ---------------------------------------------------------------------------------------------
+# group-size autotuner
+# ---------------------------------------------------------------------------------------------
+TUNE_SIZES = (32, 64, 128, 256)
+_tuned = {}
+
+
+def _bucket(n):
+    return int(round(np.log2(max(n, 1)) * 2))  # half-octave buckets of the problem size
+
+
+def tuning_log():
+    """Every autotuning decision taken in this process: {key: {"choice", "ms", "interactions"}}."""
+    return dict(_tuned)
+
+
+def autotune_group_size(tree, pos, soft, perm, mask, quadrupole=False, theta=0.7, precision="float32", grouping=None,
+                        criterion="theta", eps=None, model=None, bf16=False, sizes=TUNE_SIZES, reps=2, force=False):
+    """Pick the walk's target-group size by timing the walk (groups included) at each candidate.
+
+    The choice depends on the target count, the tree and the opening criterion, not on the values
+    the walk returns: every candidate evaluates the same sums to the same truncation order (a
+    bigger group opens nodes for its whole bounding box, so it trades more interactions per target
+    for fewer, longer traversals).  Decisions are cached per (tree size, target count, fields,
+    moment order, theta, precision, grouping, criterion) and listed by tuning_log()."""
+    pos = to_dev(pos, (-1, 3))
+    nt = pos.shape[0]
+    key = (_bucket(tree.n), _bucket(nt), int(mask), bool(quadrupole), float(theta), str(precision), grouping,
+           criterion, None if eps is None else float(eps), None if model is None else model.key(), bool(bf16))
+    if key in _tuned and not force:
+        return _tuned[key]
+    times, inter = {}, {}
+    for gs in sizes:
+        ctr = torch.zeros(2, dtype=torch.int64, device=device())
+        kw = dict(quadrupole=quadrupole, group_size=int(gs), theta=theta, precision=precision, grouping=grouping,
+                  criterion=criterion, eps=eps, model=model, bf16=bf16)
+        walk(tree, pos, soft, perm, mask, counters=ctr, **kw)  # warm-up (and interaction count)
+        torch.cuda.synchronize()
+        inter[int(gs)] = int(ctr[0].item())
+        best = float("inf")
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            walk(tree, pos, soft, perm, mask, **kw)
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1))
+        times[int(gs)] = best
+    choice = min(times, key=times.get)
+    rec = {"choice": choice, "ms": times, "interactions": inter}
+    _tuned[key] = rec
+    return rec

@@ -79,6 +79,10 @@ def _precision(precision, device=None):
 
 
 def _group(group_size, precision):
+    if isinstance(group_size, str):
+        if group_size != "auto":
+            raise ValueError(f"group_size must be an integer, None or 'auto', got {group_size!r}")
+        return group_size  # timed per problem by engine.autotune_group_size
     if group_size is None:
         return FP32_DEFAULT_GROUP if precision == "float32" else 8
     group_size = int(group_size)

@@ -1,0 +1,150 @@
+"""Forest (batched independent systems): every system of a batched build and walk must equal the
+single-system GPU path run on that system alone, bit for bit (tree arrays, FP32 and FP64 fields),
+and its fields must match FP64 direct summation to the walk's accuracy.  The HIP-graph replay of the
+batched walk stage must reproduce the eager result on every replay."""
+
+import numpy as np
+import pytest
+import torch
+
+from gpu_cases import case
+from pytreegrav_amd import Forest, engine
+from pytreegrav_amd._lib import PTG_ACC, PTG_POT, PtgError
+from pytreegrav_amd.sampling import plummer_truncated
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems():
+    """Uneven systems, including the reference tests' collision cases (re-keying, bucket chains)."""
+    out = []
+    rng = np.random.default_rng(21)
+    for k, n in enumerate([2, 3, 700, 4096, 257, 12000, 64, 1500]):
+        x, m = plummer_truncated(n, seed=100 + k)
+        a = 10 ** rng.uniform(-1, 1)
+        out.append((x * a + rng.normal(size=3) * 5, m, np.full(n, 0.01 * a)))
+    for name in ("dups", "rekey", "bucket_chain", "all_identical", "dups_origin", "clustered"):
+        out.append(case(name))
+    return out
+
+
+def _concat(systems):
+    x = np.vstack([s[0] for s in systems])
+    m = np.concatenate([s[1] for s in systems])
+    h = np.concatenate([s[2] for s in systems])
+    off = np.concatenate([[0], np.cumsum([len(s[0]) for s in systems])])
+    return x, m, h, off
+
+
+def _node_ranges(t):
+    roots = t.roots.cpu().numpy().astype(np.int64)
+    ends = np.append(roots[1:], t.n + t.nnodes)
+    return roots, ends
+
+
+def _sub(t, off, s, name):
+    """System s's rows of a forest array, particles then nodes, with links renumbered locally."""
+    roots, ends = _node_ranges(t)
+    a, b = off[s], off[s + 1]
+    arr = getattr(t, name).cpu().numpy()
+    rows = np.concatenate([arr[a:b], arr[roots[s]:ends[s]]])
+    if name in ("first_subnode", "next_branch"):
+        r = rows.astype(np.int64)
+        part = (r >= 0) & (r < t.n)
+        node = r >= t.n
+        r[part] -= a
+        r[node] = r[node] - roots[s] + (b - a)
+        rows = r
+    return rows
+
+
+ARRAYS = ("coordinates", "masses", "softenings", "sizes", "deltas", "first_subnode", "next_branch", "nchild")
+
+
+@pytest.mark.parametrize("quadrupole", [False, True])
+def test_forest_tree_equals_single_builds(quadrupole):
+    systems = _systems()
+    x, m, h, off = _concat(systems)
+    print(f"[forest] build nsys={len(systems)} n={len(x)} quad={quadrupole}", flush=True)
+    f = engine.DeviceTree(x, m, h, quadrupole=quadrupole, offsets=off)
+    cubes = f.cubes.cpu().numpy()
+    order = f.order.cpu().numpy()
+    nn_total = 0
+    for s, (xs, ms, hs) in enumerate(systems):
+        u = engine.DeviceTree(xs, ms, hs, quadrupole=quadrupole)
+        nn_total += u.nnodes
+        assert np.array_equal(cubes[s, :3], u.center) and cubes[s, 3] == u.size, s
+        assert np.array_equal(order[off[s]:off[s + 1]] - off[s], u.order.cpu().numpy()), s
+        for name in ARRAYS + (("quadrupoles",) if quadrupole else ()):
+            single = getattr(u, name).cpu().numpy()
+            if name in ("first_subnode", "next_branch"):
+                single = single.astype(np.int64)
+            assert np.array_equal(_sub(f, off, s, name), single), (s, name)
+    assert f.nnodes == nn_total
+
+
+@pytest.mark.parametrize("precision,quadrupole", [("float32", False), ("float32", True), ("float64", False),
+                                                  ("float64", True)])
+def test_forest_fields_equal_single_walks(precision, quadrupole):
+    systems = _systems()
+    x, m, h, off = _concat(systems)
+    gs = 64 if precision == "float32" else 8
+    print(f"[forest] walk prec={precision} quad={quadrupole}", flush=True)
+    F = Forest(x, m, h, off, quadrupole=quadrupole, precision=precision, group_size=gs)
+    r = F.evaluate(potential=True, accel=True)
+    fa, fp = r["accel"].cpu().numpy(), r["potential"].cpu().numpy()
+    for s, (xs, ms, hs) in enumerate(systems):
+        u = engine.DeviceTree(xs, ms, hs, quadrupole=quadrupole)
+        o = engine.walk(u, xs, hs, u.order, PTG_POT | PTG_ACC, quadrupole=quadrupole, group_size=gs, theta=0.7,
+                        precision=precision).cpu().numpy()
+        assert np.array_equal(fp[off[s]:off[s + 1]], o[:, 0]), s
+        assert np.array_equal(fa[off[s]:off[s + 1]], o[:, 1:4]), s
+
+
+def test_forest_accuracy_against_direct_sum():
+    systems = _systems()[:8]
+    x, m, h, off = _concat(systems)
+    F = Forest(x, m, h, off, quadrupole=True, precision="float32")
+    a = F.evaluate(potential=False, accel=True, theta=0.5)["accel"].cpu().numpy()
+    for s, (xs, ms, hs) in enumerate(systems):
+        d = engine.direct(xs, hs, xs, ms, hs, PTG_ACC, self_mode=True).cpu().numpy()
+        err = np.sqrt(np.mean(np.sum((a[off[s]:off[s + 1]] - d) ** 2, 1)) / np.mean(np.sum(d * d, 1)))
+        assert err < 2e-3, (s, err)
+
+
+def test_forest_graph_replays_match_eager():
+    systems = _systems()[:8]
+    x, m, h, off = _concat(systems)
+    F = Forest(x, m, h, off, quadrupole=False, precision="float32")
+    ref = F.evaluate(potential=True, accel=True)
+    ref = torch.cat([ref["potential"][:, None], ref["accel"]], 1).clone()
+    replay, out = F.capture(potential=True, accel=True)
+    for k in range(4):
+        out.zero_()
+        torch.cuda.synchronize()
+        replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), k
+        # eager work between replays
+        e = F.evaluate(potential=True, accel=True)
+        assert torch.equal(torch.cat([e["potential"][:, None], e["accel"]], 1), ref)
+
+
+def test_forest_single_system_equals_plain_tree():
+    x, m, h = case("plummer_1e5_h")
+    F = Forest(x, m, h, [0, len(x)], precision="float32")
+    r = F.evaluate(potential=True, accel=True)
+    u = engine.DeviceTree(x, m, h)
+    o = engine.walk(u, x, h, u.order, PTG_POT | PTG_ACC, group_size=64, theta=0.7, precision="float32")
+    assert torch.equal(r["potential"], o[:, 0]) and torch.equal(r["accel"], o[:, 1:4])
+
+
+def test_forest_rejects_bad_offsets():
+    x, m, h = case("uniform_1e4")
+    with pytest.raises(ValueError):
+        Forest(x, m, h, [0, 1, len(x)])  # a one-particle system
+    with pytest.raises(ValueError):
+        Forest(x, m, h, [0, 5000, len(x) - 1])  # does not cover all particles
+    # the library checks the device offsets itself too
+    with pytest.raises(PtgError):
+        engine.DeviceTree(x, m, h, offsets=np.array([0, 5000, 5001, len(x)]))

@@ -1,0 +1,227 @@
+"""Learned opening criterion on the GPU: the training kernels against the numpy restatement (whose
+gradient is pinned by finite differences in test_mlp_cpu.py), features against a host
+recomputation, labels against an independent float64 direct sum and against the quadrupole far
+field, determinism of training, the geometric guard, and the gain of a trained network over the
+geometric criterion at equal or lower error (FP64 direct summation as the judge)."""
+
+import numpy as np
+import pytest
+import torch
+
+import mlp_ref
+from gpu_cases import case
+from pytreegrav_amd import engine, learned
+from pytreegrav_amd._lib import PTG_ACC, PTG_POT
+from pytreegrav_amd.sampling import gaussian_mixture, plummer_truncated
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a, dt=torch.float32):
+    return torch.as_tensor(np.asarray(a), dtype=dt).cuda().contiguous()
+
+
+def test_forward_fp32_and_bf16_against_numpy():
+    rng = np.random.default_rng(1)
+    p = rng.normal(scale=0.3, size=mlp_ref.NPARAM).astype(np.float32)
+    x = rng.normal(size=(5000, 10)).astype(np.float32)
+    ref = mlp_ref.forward(p.astype(np.float64), x.astype(np.float64))
+    m = learned.OpeningMLP(p)
+    y32 = m.forward(_dev(x)).cpu().numpy()
+    assert np.max(np.abs(y32 - ref)) < 2e-5 * (1 + np.max(np.abs(ref)))
+    y16 = m.forward(_dev(x), bf16=True).cpu().numpy()
+    err16 = np.max(np.abs(y16 - ref)) / np.max(np.abs(ref))
+    print(f"[learned] bf16 max rel err {err16:.3g}", flush=True)
+    assert err16 < 0.05
+
+
+@pytest.mark.parametrize("under", [1.0, 4.0])
+def test_gradient_against_numpy_backprop(under):
+    rng = np.random.default_rng(2)
+    p = rng.normal(scale=0.3, size=mlp_ref.NPARAM).astype(np.float32)
+    x = rng.normal(size=(3000, 10)).astype(np.float32)
+    y = rng.normal(size=3000).astype(np.float32)
+    index = rng.integers(0, 3000, size=1000).astype(np.int32)
+    g, loss = learned.grad(_dev(p), _dev(x), _dev(y), index=_dev(index, torch.int32), under=under)
+    xi, yi = x[index].astype(np.float64), y[index].astype(np.float64)
+    gref = mlp_ref.grad(p.astype(np.float64), xi, yi, under)
+    lref = mlp_ref.loss(p.astype(np.float64), xi, yi, under)
+    g = g.cpu().numpy()
+    assert np.linalg.norm(g - gref) <= 1e-4 * np.linalg.norm(gref)
+    assert np.max(np.abs(g - gref)) <= 1e-4 * np.max(np.abs(gref))
+    assert abs(float(loss) - lref) <= 1e-5 * lref
+
+
+def test_features_against_host_recomputation():
+    x, m, h = case("plummer_1e5_h")
+    t = engine.DeviceTree(x, m, h, quadrupole=True)
+    rng = np.random.default_rng(4)
+    nodes = (rng.choice(t.nnodes, 500, replace=False) + t.n).astype(np.int32)
+    eps = 3e-3
+    f = learned.node_features(t, _dev(nodes, torch.int32), eps).cpu().numpy()
+    C, M, S = t.coordinates.cpu().numpy(), t.masses.cpu().numpy(), t.sizes.cpu().numpy()
+    D, H, Q = t.deltas.cpu().numpy(), t.softenings.cpu().numpy(), t.quadrupoles.cpu().numpy()
+    F, N = t.first_subnode.cpu().numpy(), t.next_branch.cpu().numpy()
+    for i, e in enumerate(nodes):
+        s, mm = S[e], M[e]
+        kids = []
+        c = F[e]
+        while c != N[e] and c >= 0:
+            kids.append(c)
+            c = N[c]
+        d = np.linalg.norm(C[kids] - C[e], axis=1)
+        qn = np.linalg.norm(Q[e])
+        rq = (3.5 * qn / eps) ** 0.25
+        want = [np.clip(np.log10(max(rq / s, 1e-3)), -2, 3) / 2, min(D[e] / s, 2), min(qn / (mm * s * s), 2),
+                min(H[e] / s, 4), len(kids) / 8, M[kids].max() / mm, min(np.sum(M[kids] * d ** 2) / (mm * s * s), 2),
+                min(np.sum(M[kids] * d ** 3) / (mm * s ** 3), 2), min(d.max() / s, 2),
+                np.clip(np.log10(mm / (eps * s * s)), -4, 12) / 4]
+        assert np.allclose(f[i], want, rtol=2e-5, atol=2e-6), (e, f[i], want)
+
+
+def _leaves(t, e):
+    """Particle slots below node e (host walk of the reference links)."""
+    F, N, n = t.first_subnode.cpu().numpy(), t.next_branch.cpu().numpy(), t.n
+    out, stack = [], [e]
+    while stack:
+        u = stack.pop()
+        c = F[u]
+        while c != N[u] and c >= 0:
+            if c < n:
+                out.append(c)
+            else:
+                stack.append(c)
+            c = N[c]
+    return np.array(sorted(out))
+
+
+def test_error_curves_against_direct_sum_and_far_field():
+    x, m, h = case("plummer_3e4")
+    t = engine.DeviceTree(x, m, np.full(len(m), 0.01), quadrupole=True)
+    C, M, S, Q = (t.coordinates.cpu().numpy(), t.masses.cpu().numpy(), t.sizes.cpu().numpy(),
+                  t.quadrupoles.cpu().numpy().reshape(-1, 3, 3))
+    rng = np.random.default_rng(5)
+    nodes = (rng.choice(t.nnodes, 40, replace=False) + t.n).astype(np.int32)
+    err = learned.error_curves(t, _dev(nodes, torch.int32), cap=100000).cpu().numpy()
+    rho = learned.radii_ladder()
+    k = np.arange(32)
+    z = 1 - (2 * k + 1) / 32
+    u = np.stack([np.sqrt(1 - z * z) * np.cos(2.399963229728653 * k), np.sqrt(1 - z * z) *
+                  np.sin(2.399963229728653 * k), z], 1)
+    for row, e in enumerate(nodes):
+        leaves = _leaves(t, e)
+        s, mm, c = S[e], M[e], C[e]
+        tgt = c[None, None, :] + (rho[:, None, None] * s) * u[None, :, :]  # (R, 32, 3)
+        d = C[leaves][None, None, :, :] - tgt[:, :, None, :]
+        r = np.linalg.norm(d, axis=-1)
+        a = np.sum(M[leaves][None, None, :, None] * d / r[..., None] ** 3, axis=2)
+        dc = c[None, None, :] - tgt
+        a_mono = mm * dc / np.linalg.norm(dc, axis=-1)[..., None] ** 3
+        want = np.max(np.linalg.norm(a - a_mono, axis=-1), axis=1) * s * s / mm
+        assert np.allclose(err[row], want, rtol=1e-9, atol=1e-14), e
+        # far field: the monopole's error is the quadrupole term, -(Q r)/r^5 + 5/2 (r.Q.r) r / r^7
+        # for the reference's traceless Q = sum m (3 x x - r^2 I), up to O(1/r^5)
+        rv = (rho[-1] * s) * u
+        rn = np.linalg.norm(rv, axis=1)
+        Qe = Q[e]
+        aq = -(rv @ Qe) / rn[:, None] ** 5 + 2.5 * np.einsum("ij,jk,ik->i", rv, Qe, rv)[:, None] * rv / rn[:, None] ** 7
+        far = np.max(np.linalg.norm(aq, axis=1)) * s * s / mm
+        assert abs(err[row, -1] - far) <= 0.2 * far + 1e-12, (e, err[row, -1], far)
+
+
+def test_labels_cross_the_tolerance():
+    x, m, h = case("plummer_3e4")
+    t = engine.DeviceTree(x, m, np.full(len(m), 0.01), quadrupole=True)
+    rng = np.random.default_rng(6)
+    nodes = _dev((rng.choice(t.nnodes, 300, replace=False) + t.n).astype(np.int32), torch.int32)
+    err = learned.error_curves(t, nodes)
+    rows = np.repeat(np.arange(300), 3).astype(np.int32)
+    eps = np.tile([1e-4, 1e-3, 1e-2], 300)
+    xs, ys, vs = learned.samples(t, nodes, err, rows, eps)
+    ys, vs, E = ys.cpu().numpy(), vs.cpu().numpy(), err.cpu().numpy()
+    rho = learned.radii_ladder()
+    S, M = t.sizes.cpu().numpy(), t.masses.cpu().numpy()
+    nn = nodes.cpu().numpy()
+    assert np.mean(vs == 1) > 0.9
+    for i in np.nonzero(vs == 1)[0]:
+        e = nn[rows[i]]
+        ea = E[rows[i]] * M[e] / S[e] ** 2
+        rstar = 10 ** ys[i]
+        assert np.all(ea[rho > rstar * (1 + 1e-5)] <= eps[i]), i
+        above = rho < rstar * (1 - 1e-5)
+        if np.any(above):
+            assert ea[np.nonzero(above)[0][-1]] > eps[i], i
+
+
+def test_training_is_deterministic_and_learns():
+    x, m = plummer_truncated(50000, seed=8)
+    t = engine.DeviceTree(x, m, np.full(len(m), 0.01), quadrupole=True)
+    xs, ys, vs = learned.training_set(t, [3e-4, 1e-3, 3e-3], nodes_per_tree=4000, seed=1)
+    keep = vs > 0
+    xs, ys = xs[keep].contiguous(), ys[keep].contiguous()
+    p1, l1 = learned.train(xs, ys, steps=600, batch=2048, seed=3)
+    p2, l2 = learned.train(xs, ys, steps=600, batch=2048, seed=3)
+    assert torch.equal(p1, p2) and np.array_equal(l1, l2)
+    print(f"[learned] loss {l1[:20].mean():.4f} -> {l1[-50:].mean():.4f}", flush=True)
+    assert l1[-50:].mean() < 0.3 * l1[:20].mean()
+
+
+def test_guard_kept_when_radii_vanish():
+    """With radii ~ 0 the learned walk is the geometric walk at the guard theta, bit for bit."""
+    x, m, h = case("plummer_1e5_h")
+    t = engine.DeviceTree(x, m, h, quadrupole=True)
+    model = learned.OpeningMLP(np.zeros(mlp_ref.NPARAM, dtype=np.float32), margin=-20.0)
+    a = engine.walk(t, x, h, t.order, PTG_POT | PTG_ACC, group_size=64, theta=0.9, precision="float32",
+                    criterion="learned", model=model, eps=1e-3)
+    b = engine.walk(t, x, h, t.order, PTG_POT | PTG_ACC, group_size=64, theta=0.9, precision="float32")
+    assert torch.equal(a, b)
+
+
+def _rms(a, d):
+    return float(np.sqrt(np.mean(np.sum((a - d) ** 2, 1)) / np.mean(np.sum(d * d, 1))))
+
+
+def test_trained_network_beats_geometric_criterion():
+    """Train on Plummer + clumpy Gaussian mixtures (N=1e5 each), evaluate on a held-out Plummer
+    sphere: at an RMS force error no worse than theta=0.7's, the learned criterion needs fewer
+    interactions than the geometric criterion at theta=0.7."""
+    sets = []
+    x1, m1 = plummer_truncated(100000, seed=31)
+    x2, m2 = gaussian_mixture(100000, seed=32, ncomp=64)
+    for x, m in ((x1, m1), (x2, m2)):
+        sets.append(engine.DeviceTree(x, m, np.full(len(m), 0.01), quadrupole=True))
+    X, Y = [], []
+    for i, t in enumerate(sets):
+        xs, ys, vs = learned.training_set(t, np.geomspace(1e-4, 1e-2, 5), nodes_per_tree=8000, seed=10 + i)
+        X.append(xs[vs > 0])
+        Y.append(ys[vs > 0])
+    X, Y = torch.cat(X).contiguous(), torch.cat(Y).contiguous()
+    p, losses = learned.train(X, Y, steps=1500, batch=4096, seed=0)
+    res = (learned.OpeningMLP(p.cpu().numpy()).forward(X) - Y).cpu().numpy()
+    model = learned.OpeningMLP(p.cpu().numpy(), margin=float(max(0.0, np.quantile(-res, 0.98))))
+
+    x, m = plummer_truncated(200000, seed=77)
+    h = np.full(len(m), 0.01)
+    t = engine.DeviceTree(x, m, h, quadrupole=True)
+    idx = np.random.default_rng(0).choice(len(x), 3000, replace=False)
+    d = engine.direct(x[idx], h[idx], x, m, h, PTG_ACC).cpu().numpy()
+
+    def run(**kw):
+        ctr = torch.zeros(2, dtype=torch.int64, device="cuda")
+        a = engine.walk(t, x, h, t.order, PTG_ACC, group_size=64, precision="float32", counters=ctr, **kw)
+        return _rms(a.cpu().numpy()[idx], d), int(ctr[0].item())
+
+    e_geo, n_geo = run(theta=0.7)
+    fs = engine.field_scale(t)
+    best = None
+    for alpha in (3e-3, 2e-3, 1.5e-3, 1e-3, 7e-4, 5e-4, 3e-4):
+        e_l, n_l = run(theta=2.0, criterion="learned", model=model, eps=alpha * fs)
+        print(f"[learned] alpha={alpha} rms={e_l:.3e} int={n_l} (theta=0.7: rms={e_geo:.3e} int={n_geo})",
+              flush=True)
+        if e_l <= e_geo:
+            best = (alpha, e_l, n_l)
+            break
+    assert best is not None, "no tolerance reached theta=0.7's accuracy"
+    print(f"[learned] gain: {n_geo / best[2]:.2f}x fewer interactions at rms {best[1]:.3e} <= {e_geo:.3e}",
+          flush=True)
+    assert best[2] < 0.8 * n_geo
