@@ -338,7 +338,7 @@ def main():
         return acc
 
     quad = bool(args.quadrupole)
-    theta = args.theta if args.theta is not None else (2.0 if args.criterion == "error" else 0.7)
+    theta = args.theta if args.theta is not None else (0.7 if args.criterion == "theta" else 2.0)
     main = run_config(args.criterion, theta, args.alpha, quad)
     plain = None
     if not args.no_plain and not (args.criterion == "theta" and theta == 0.7 and not quad):

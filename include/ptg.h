@@ -231,7 +231,8 @@ ptg_status ptg_mlp_adam(float *params, float *m, float *v, const float *grad, in
 /* y[i] = MLP(x[i]); bf16 != 0: weights and activations rounded to bfloat16, FP32 sums. */
 ptg_status ptg_mlp_forward(const float *params, const float *x, int64_t count, int bf16, float *y, void *stream);
 /* Fused inference: writes size * 10^(MLP(features(node, eps)) + margin) into aux[3] of every node
- * slot of an FP32 pack (0 for particles and massless nodes), for PTG_CRIT_RADIUS walks. */
+ * slot of an FP32 pack from ptg_pack_walk (0 for massless nodes; particle slots keep the pack's 0),
+ * for PTG_CRIT_RADIUS walks. */
 ptg_status ptg_open_radii(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_walk_tree *wt,
                           const float *params, float margin, double eps, int bf16, void *stream);
 

@@ -75,8 +75,11 @@ __device__ bool node_features(const ptg_tree_arrays &t, int64_t ne, int64_t e, d
         m3 += mc * d * d * d;
         dmax = fmax(dmax, d);
     }
-    const double rq = pow(3.5 * qn / eps, 0.25);
-    f[0] = (float)(clampd(log10(fmax(rq / s, 1e-3)), -2.0, 3.0) * 0.5);
+    // log10(r_Q / s) = log10(3.5 |Q| / (eps s^4)) / 4, r_Q = (3.5 |Q| / eps)^(1/4); the ratios are
+    // formed in FP64 and clamped before the FP32 logarithms
+    const double s2 = s * s;
+    const float lq = log10f((float)clampd(3.5 * qn / (eps * s2 * s2), 1e-12, 1e12));
+    f[0] = fminf(fmaxf(0.25f * lq, -2.f), 3.f) * 0.5f;
     f[1] = (float)clampd(t.deltas[e] / s, 0.0, 2.0);
     f[2] = (float)clampd(qn / (M * s * s), 0.0, 2.0);
     f[3] = (float)clampd(t.softenings[e] / s, 0.0, 4.0);
@@ -85,30 +88,49 @@ __device__ bool node_features(const ptg_tree_arrays &t, int64_t ne, int64_t e, d
     f[6] = (float)clampd(m2 / (M * s * s), 0.0, 2.0);
     f[7] = (float)clampd(m3 / (M * s * s * s), 0.0, 2.0);
     f[8] = (float)clampd(dmax / s, 0.0, 2.0);
-    f[9] = (float)(clampd(log10(M / (eps * s * s)), -4.0, 12.0) * 0.25);
+    f[9] = log10f((float)clampd(M / (eps * s2), 1e-4, 1e12)) * 0.25f;
     return true;
 }
 
 // ============================================================================================
-// MLP forward (FP32 and BF16 weights), parameters in LDS
+// MLP forward.  The hidden activation is tanh(z) = 1 - 2 / (exp(2z) + 1) with the hardware exp2 and
+// reciprocal (absolute error ~1e-7; saturates correctly for large |z|); training and inference use
+// the same function.
 // ============================================================================================
-__device__ __forceinline__ float mlp_forward_f32(const float *P, const float *x)
+__device__ __forceinline__ float act(float z)
 {
-    float h1[MLP_H];
+    const float e = __builtin_amdgcn_exp2f(z * 2.8853900817779268f);  // exp(2z) = 2^(2z log2 e)
+    return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
+}
+
+// FP32 forward with the parameters read straight from global memory: every lane reads the same
+// address, so the compiler turns the fully unrolled, constant-offset loads into scalar loads and
+// the FMAs take their weight from an SGPR.  Loops run input-major over the [in][out] rows.
+__device__ __forceinline__ float mlp_forward_f32(const float *__restrict__ P, const float *x)
+{
+    float z[MLP_H], h[MLP_H];
+#pragma unroll
+    for (int j = 0; j < MLP_H; j++) z[j] = P[OFF_B1 + j];
+#pragma unroll
+    for (int k = 0; k < MLP_IN; k++) {
+        const float xk = x[k];
+#pragma unroll
+        for (int j = 0; j < MLP_H; j++) z[j] = fmaf(xk, P[OFF_W1 + k * MLP_H + j], z[j]);
+    }
 #pragma unroll
     for (int j = 0; j < MLP_H; j++) {
-        float z = P[OFF_B1 + j];
+        h[j] = act(z[j]);
+        z[j] = P[OFF_B2 + j];
+    }
 #pragma unroll
-        for (int k = 0; k < MLP_IN; k++) z = fmaf(x[k], P[OFF_W1 + k * MLP_H + j], z);
-        h1[j] = tanhf(z);
+    for (int i = 0; i < MLP_H; i++) {
+        const float hi = h[i];
+#pragma unroll
+        for (int j = 0; j < MLP_H; j++) z[j] = fmaf(hi, P[OFF_W2 + i * MLP_H + j], z[j]);
     }
     float y = P[OFF_B3];
-    for (int j = 0; j < MLP_H; j++) {
-        float z = P[OFF_B2 + j];
 #pragma unroll
-        for (int i = 0; i < MLP_H; i++) z = fmaf(h1[i], P[OFF_W2 + i * MLP_H + j], z);
-        y = fmaf(tanhf(z), P[OFF_W3 + j], y);
-    }
+    for (int j = 0; j < MLP_H; j++) y = fmaf(act(z[j]), P[OFF_W3 + j], y);
     return y;
 }
 
@@ -136,14 +158,14 @@ __device__ __forceinline__ float mlp_forward_bf16(const Bf16Params &Q, const flo
             z0 = __builtin_amdgcn_fdot2_f32_bf16(xp[k], Q.w1t[j][k], z0, false);
             z1 = __builtin_amdgcn_fdot2_f32_bf16(xp[k], Q.w1t[j + 1][k], z1, false);
         }
-        hp[j / 2] = bf16x2{(__bf16)tanhf(z0), (__bf16)tanhf(z1)};
+        hp[j / 2] = bf16x2{(__bf16)act(z0), (__bf16)act(z1)};
     }
     float y = Q.b3;
     for (int j = 0; j < MLP_H; j++) {
         float z = Q.b2[j];
 #pragma unroll
         for (int i = 0; i < MLP_H / 2; i++) z = __builtin_amdgcn_fdot2_f32_bf16(hp[i], Q.w2t[j][i], z, false);
-        y = fmaf(tanhf(z), Q.w3[j], y);
+        y = fmaf(act(z), Q.w3[j], y);
     }
     return y;
 }
@@ -430,14 +452,14 @@ __global__ __launch_bounds__(TR_B) void k_mlp_grad(const float *P, const float *
     for (int j = 0; j < MLP_H; j++) {
         float z = S.P[OFF_B1 + j];
         for (int k = 0; k < MLP_IN; k++) z = fmaf(x[k], S.P[OFF_W1 + k * MLP_H + j], z);
-        h1[j] = tanhf(z);
+        h1[j] = act(z);
         S.h1[b][j] = h1[j];
     }
     float yhat = S.P[OFF_B3];
     for (int j = 0; j < MLP_H; j++) {
         float z = S.P[OFF_B2 + j];
         for (int i = 0; i < MLP_H; i++) z = fmaf(h1[i], S.P[OFF_W2 + i * MLP_H + j], z);
-        const float h2 = tanhf(z);
+        const float h2 = act(z);
         S.h2[b][j] = h2;
         yhat = fmaf(h2, S.P[OFF_W3 + j], yhat);
     }
@@ -516,18 +538,19 @@ __global__ __launch_bounds__(256) void k_adam(float *P, float *m, float *v, cons
     P[p] -= lr * mh / (sqrtf(vh) + eps);
 }
 
-__global__ __launch_bounds__(256) void k_mlp_forward(const float *P, const float *X, int64_t count, int bf16, float *Y)
+__global__ __launch_bounds__(256) void k_mlp_forward(const float *__restrict__ P, const float *X, int64_t count,
+                                                     int bf16, float *Y)
 {
-    __shared__ float sP[MLP_NP];
     __shared__ Bf16Params Q;
-    if (bf16) load_params_bf16(P, Q);
-    else load_params_f32(P, sP);
-    __syncthreads();
+    if (bf16) {
+        load_params_bf16(P, Q);
+        __syncthreads();
+    }
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     float x[MLP_IN];
     for (int k = 0; k < MLP_IN; k++) x[k] = X[i * MLP_IN + k];
-    Y[i] = bf16 ? mlp_forward_bf16(Q, x) : mlp_forward_f32(sP, x);
+    Y[i] = bf16 ? mlp_forward_bf16(Q, x) : mlp_forward_f32(P, x);
 }
 
 __global__ __launch_bounds__(256) void k_node_features(const ptg_tree_arrays t, int64_t ne, const int32_t *nodes,
@@ -544,14 +567,14 @@ __global__ __launch_bounds__(256) void k_node_features(const ptg_tree_arrays t, 
 // Fused radius pass: features + MLP for every node, radius = size * 10^(yhat + margin) into the
 // FP32 pack's aux[3] at the node's walk slot (0 for massless nodes and for particles).
 __global__ __launch_bounds__(256) void k_open_radii(const ptg_tree_arrays t, int64_t n, int64_t nnodes, float *aux,
-                                                    int64_t nslots, const float *P, float margin, double eps,
-                                                    int bf16, float rho_max)
+                                                    int64_t nslots, const float *__restrict__ P, float margin,
+                                                    double eps, int bf16, float rho_max)
 {
-    __shared__ float sP[MLP_NP];
     __shared__ Bf16Params Q;
-    if (bf16) load_params_bf16(P, Q);
-    else load_params_f32(P, sP);
-    __syncthreads();
+    if (bf16) {
+        load_params_bf16(P, Q);
+        __syncthreads();
+    }
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nnodes) return;
     const int64_t e = n + a;
@@ -560,20 +583,13 @@ __global__ __launch_bounds__(256) void k_open_radii(const ptg_tree_arrays t, int
     float f[MLP_IN];
     float radius = 0.f;
     if (node_features(t, n + nnodes, e, eps, f)) {
-        const float yhat = (bf16 ? mlp_forward_bf16(Q, f) : mlp_forward_f32(sP, f)) + margin;
+        const float yhat = (bf16 ? mlp_forward_bf16(Q, f) : mlp_forward_f32(P, f)) + margin;
         const float rho = fminf(exp10f(yhat), rho_max);
         radius = (float)(t.sizes[e] * (double)rho);
     }
     aux[4 * sl + 3] = radius;
 }
 
-__global__ __launch_bounds__(256) void k_zero_particle_aux3(const int32_t *slot, int64_t n, float *aux, int64_t nslots)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t sl = slot[i];
-    if (sl >= 0 && sl < nslots) aux[4 * sl + 3] = 0.f;
-}
 
 // ============================================================================================
 // C ABI
@@ -702,7 +718,6 @@ extern "C" ptg_status ptg_open_radii(const ptg_tree_arrays *tree, int64_t n, int
     }
     k_open_radii<<<lgrid(nnodes), 256, 0, s>>>(*tree, n, nnodes, (float *)wt->aux, wt->nslots, params, margin, eps,
                                                bf16 != 0, 1e4f);
-    k_zero_particle_aux3<<<lgrid(n), 256, 0, s>>>(tree->slot, n, (float *)wt->aux, wt->nslots);
     PTG_LAUNCH_CHECK();
     return PTG_OK;
 }

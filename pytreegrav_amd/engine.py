@@ -162,7 +162,9 @@ class DeviceTree:
 
         key = ("learned", model.key(), float(eps), bool(quadrupole), bool(bf16))
         if key not in self._packs:
-            p = self.pack(PTG_F32, quadrupole, cache=False)
+            # take over the plain FP32 pack if there is one (its aux[3] is overwritten, so it leaves
+            # the plain cache and is rebuilt there when needed again)
+            p = self._packs.pop((PTG_F32, bool(quadrupole)), None) or self.pack(PTG_F32, quadrupole, cache=False)
             open_radii(self, p, model, eps, bf16=bf16)
             self._packs[key] = p
         return self._packs[key]
