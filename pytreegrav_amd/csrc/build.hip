@@ -358,13 +358,45 @@ __device__ __forceinline__ uint64_t morton_key(double x, double y, double z, con
     return ptg_spread_bits(q[0]) | (ptg_spread_bits(q[1]) << 1) | (ptg_spread_bits(q[2]) << 2);
 }
 
-__global__ __launch_bounds__(256) void k_keys(const double *pos, int64_t n, const Geom *g, uint64_t *keys,
-                                              uint32_t *vals)
+struct Counters {
+    int32_t ncollide;     // adjacent pairs with equal 63-bit keys
+    int32_t nnodes;       // total internal nodes
+    int32_t nsub;         // particles in collision runs
+    int32_t coincident;   // bit-identical extended keys exist
+    int32_t max_level;
+    int32_t nonfinite;    // some position, mass or softening is NaN or infinite
+    int32_t badforest;    // forest offsets do not describe systems of >= 2 particles covering [0, n)
+    int32_t pad[9];
+};
+
+// Morton keys in input order; with rec, also the particle record (x, y, z, m, h in a 64-byte slot)
+// that the Morton-order gather reads with one cache line per particle, and the finiteness flag
+#define REC 8
+__device__ __forceinline__ bool store_record(const double *pos, const double *m, const double *h, int64_t i,
+                                             double *rec)
+{
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2], mi = m[i], hi = h[i];
+    double *r = rec + REC * i;
+    r[0] = x;
+    r[1] = y;
+    r[2] = z;
+    r[3] = mi;
+    r[4] = hi;
+    return !isfinite(x) || !isfinite(y) || !isfinite(z) || !isfinite(mi) || !isfinite(hi);
+}
+
+__global__ __launch_bounds__(256) void k_keys(const double *pos, const double *m, const double *h, int64_t n,
+                                              const Geom *g, uint64_t *keys, uint32_t *vals, double *rec,
+                                              Counters *c)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    keys[i] = morton_key(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], g->cmin, g->inv);
-    vals[i] = (uint32_t)i;
+    bool bad = false;
+    if (i < n) {
+        keys[i] = morton_key(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], g->cmin, g->inv);
+        vals[i] = (uint32_t)i;
+        if (rec) bad = store_record(pos, m, h, i, rec);
+    }
+    if (c && ptg_ballot(bad) && ptg_lane() == 0) c->nonfinite = 1;
 }
 
 // ---- forest: several independent systems built in one pass ----
@@ -423,8 +455,9 @@ __global__ __launch_bounds__(256) void k_bbox_sys(const double *pos, const int64
     for (int k = 0; k < 3; k++) g->cmin[k] = g->center[k] - 0.5 * sz;
 }
 
-__global__ __launch_bounds__(256) void k_keys_sys(const double *pos, int64_t n, const Geom *geoms,
-                                                  const uint32_t *sys, uint64_t *keys, uint64_t *kU, uint32_t *vals)
+__global__ __launch_bounds__(256) void k_keys_sys(const double *pos, const double *m, const double *h, int64_t n,
+                                                  const Geom *geoms, const uint32_t *sys, uint64_t *keys,
+                                                  uint64_t *kU, uint32_t *vals, double *rec)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -433,6 +466,7 @@ __global__ __launch_bounds__(256) void k_keys_sys(const double *pos, int64_t n, 
     keys[i] = k;
     kU[i] = k;
     vals[i] = (uint32_t)i;
+    if (rec) store_record(pos, m, h, i, rec);
 }
 
 __global__ __launch_bounds__(256) void k_gather_sys_key(const uint32_t *sys, const uint32_t *vals, int64_t n,
@@ -461,16 +495,6 @@ __global__ __launch_bounds__(256) void k_forest_cubes(const Geom *geoms, int64_t
 // ============================================================================================
 // build workspace
 // ============================================================================================
-struct Counters {
-    int32_t ncollide;     // adjacent pairs with equal 63-bit keys
-    int32_t nnodes;       // total internal nodes
-    int32_t nsub;         // particles in collision runs
-    int32_t coincident;   // bit-identical extended keys exist
-    int32_t max_level;
-    int32_t nonfinite;    // some position, mass or softening is NaN or infinite
-    int32_t badforest;    // forest offsets do not describe systems of >= 2 particles covering [0, n)
-    int32_t pad[9];
-};
 
 __global__ __launch_bounds__(256) void k_check_finite(const double *pos, const double *m, const double *h, int64_t n,
                                                       Counters *c)
@@ -507,6 +531,7 @@ struct BuildWS {
     uint64_t *kU;
     const int64_t *soff;
     int64_t nsys;
+    double *rec;  // (n, 8) particle records in input order
 };
 
 static bool carve_build(PtgArena &A, int64_t n, BuildWS &w, int64_t nsys = 1)
@@ -537,6 +562,7 @@ static bool carve_build(PtgArena &A, int64_t n, BuildWS &w, int64_t nsys = 1)
     w.tkB = A.take<uint64_t>(n);
     w.perm = A.take<uint32_t>(n);
     w.permB = A.take<uint32_t>(n);
+    w.rec = A.take<double>(REC * n);
     w.nsys = nsys;
     w.soff = nullptr;
     w.geoms = A.take<Geom>(nsys);
@@ -778,13 +804,16 @@ __global__ void k_u32_to_i64(const uint32_t *a, int64_t n, int64_t *b)
 
 static dim3 grid1(int64_t n, int bs = 256) { return dim3((unsigned)std::max<int64_t>(1, ptg_cdiv(n, bs))); }
 
-// bbox + keys + sort; result (sorted keys, order) in (w.kA, w.vA)
-static ptg_status sort_by_keys(const double *pos, int64_t n, BuildWS &w, hipStream_t s)
+// bbox + keys + sort; result (sorted keys, order) in (w.kA, w.vA).  With m and h, also the particle
+// records and the finiteness flag.
+static ptg_status sort_by_keys(const double *pos, const double *m, const double *h, int64_t n, BuildWS &w,
+                               hipStream_t s)
 {
     int nb = (int)std::min<int64_t>(BB_BLOCKS, std::max<int64_t>(1, ptg_cdiv(n, 1024)));
     k_bbox_partial<<<nb, 256, 0, s>>>(pos, n, w.part);
     k_bbox_final<<<1, 256, 0, s>>>(w.part, nb, w.geom);
-    k_keys<<<grid1(n), 256, 0, s>>>(pos, n, w.geom, w.kA, w.vA);
+    const bool rec = m && h;
+    k_keys<<<grid1(n), 256, 0, s>>>(pos, m, h, n, w.geom, w.kA, w.vA, rec ? w.rec : nullptr, rec ? w.ctr : nullptr);
     PTG_LAUNCH_CHECK();
     bool in_first;
     ptg_status st = radix_sort(w.kA, w.vA, w.kB, w.vB, n, 64, w.hist, w.bsum, &in_first, s);
@@ -805,11 +834,12 @@ static int sys_bits(int64_t nsys)
 
 // forest: per-system cubes and keys, stable sort by key, then a stable sort by system, which leaves
 // every system's particles in its own input range sorted by its own keys; result in (w.kA, w.vA)
-static ptg_status sort_by_keys_forest(const double *pos, int64_t n, BuildWS &w, hipStream_t s)
+static ptg_status sort_by_keys_forest(const double *pos, const double *m, const double *h, int64_t n, BuildWS &w,
+                                      hipStream_t s)
 {
     k_sys_of<<<grid1(n), 256, 0, s>>>(w.soff, w.nsys, n, w.sys);
     k_bbox_sys<<<(unsigned)w.nsys, 256, 0, s>>>(pos, w.soff, w.geoms);
-    k_keys_sys<<<grid1(n), 256, 0, s>>>(pos, n, w.geoms, w.sys, w.kA, w.kU, w.vA);
+    k_keys_sys<<<grid1(n), 256, 0, s>>>(pos, m, h, n, w.geoms, w.sys, w.kA, w.kU, w.vA, m && h ? w.rec : nullptr);
     PTG_LAUNCH_CHECK();
     bool in_first;
     ptg_status st = radix_sort(w.kA, w.vA, w.kB, w.vB, n, 64, w.hist, w.bsum, &in_first, s);
@@ -858,7 +888,7 @@ extern "C" ptg_status ptg_morton_order(const double *pos, int64_t n, int64_t *or
         ptg_set_error("ptg_morton_order: workspace too small (%zu < %zu)", ws_bytes, ptg_build_workspace_bytes(n, 1));
         return PTG_ERR_WORKSPACE;
     }
-    ptg_status st = sort_by_keys(pos, n, w, s);
+    ptg_status st = sort_by_keys(pos, nullptr, nullptr, n, w, s);
     if (st) return st;
     k_u32_to_i64<<<grid1(n), 256, 0, s>>>(w.vA, n, order);
     PTG_LAUNCH_CHECK();
@@ -883,8 +913,8 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
                                      void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
-    if (n < 1 || !pos || !info) {
-        ptg_set_error("ptg_build_sort: need n >= 1 particles and non-null pointers");
+    if (n < 1 || !pos || !m || !h || !info) {
+        ptg_set_error("ptg_build_sort: need n >= 1 particles and non-null pos, m, h and info");
         return PTG_ERR_ARG;
     }
     if (n > 0x3fffffffLL) {
@@ -903,23 +933,29 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
     const bool F = nsys > 1;
     if (F) w.soff = forest->offsets;
     k_zero_counters<<<1, 64, 0, s>>>(w.ctr);
-    k_check_finite<<<(unsigned)std::min<int64_t>(1024, ptg_cdiv(n, 256)), 256, 0, s>>>(pos, m, h, n, w.ctr);
-    if (forest) k_check_offsets<<<grid1(nsys + 1), 256, 0, s>>>(forest->offsets, nsys, n, w.ctr);
     Counters hc;
-    PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
-    PTG_HIP_CHECK(hipStreamSynchronize(s));
-    if (hc.nonfinite) {
+    auto nonfinite = [&]() {
         info->n = n;
         info->nnodes = 0;
         info->nonfinite = 1;
         ptg_set_error("ptg_build_sort: non-finite position, mass or softening");
         return PTG_ERR_ARG;
+    };
+    if (forest) {
+        // forests: the offsets index the per-system kernels, so they (and the inputs) are checked
+        // before anything else runs; a single tree checks finiteness inside the key pass instead
+        k_check_finite<<<(unsigned)std::min<int64_t>(1024, ptg_cdiv(n, 256)), 256, 0, s>>>(pos, m, h, n, w.ctr);
+        if (forest) k_check_offsets<<<grid1(nsys + 1), 256, 0, s>>>(forest->offsets, nsys, n, w.ctr);
+        PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+        PTG_HIP_CHECK(hipStreamSynchronize(s));
+        if (hc.nonfinite) return nonfinite();
+        if (hc.badforest) {
+            ptg_set_error("ptg_build_sort: forest offsets must start at 0, end at n and give every system >= 2 "
+                          "particles");
+            return PTG_ERR_ARG;
+        }
     }
-    if (hc.badforest) {
-        ptg_set_error("ptg_build_sort: forest offsets must start at 0, end at n and give every system >= 2 particles");
-        return PTG_ERR_ARG;
-    }
-    st = F ? sort_by_keys_forest(pos, n, w, s) : sort_by_keys(pos, n, w, s);
+    st = F ? sort_by_keys_forest(pos, m, h, n, w, s) : sort_by_keys(pos, m, h, n, w, s);
     if (st) return st;
     if (!F) PTG_HIP_CHECK(hipMemcpyAsync(w.geoms, w.geom, sizeof(Geom), hipMemcpyDeviceToDevice, s));
     if (forest) {
@@ -931,6 +967,7 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
         PTG_LAUNCH_CHECK();
         PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
         PTG_HIP_CHECK(hipStreamSynchronize(s));
+        if (hc.nonfinite) return nonfinite();
         if (hc.ncollide > 0) {
             // collision runs: compute extended keys and re-sort each run by them (stable)
             k_collision_flags<<<grid1(n), 256, 0, s>>>(w.kA, w.sys, n, w.flag);
@@ -979,6 +1016,7 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
     PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
     PTG_HIP_CHECK(hipMemcpyAsync(&hg, w.geoms, sizeof(Geom), hipMemcpyDeviceToHost, s));
     PTG_HIP_CHECK(hipStreamSynchronize(s));
+    if (hc.nonfinite) return nonfinite();
     info->n = n;
     info->nnodes = n >= 2 ? hc.nnodes : 1;
     info->has_collisions = hc.ncollide > 0;
@@ -1090,18 +1128,19 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
 }
 
 // particle slots: gather into Morton order (octree.py:577-603) and link to the next element
-__global__ __launch_bounds__(256) void k_particles(const double *pos, const double *m, const double *h,
-                                                   const uint32_t *order, const int32_t *lev, const int32_t *off,
-                                                   const uint32_t *sys, int64_t n, const ptg_tree_arrays t)
+__global__ __launch_bounds__(256) void k_particles(const double *rec, const uint32_t *order, const int32_t *lev,
+                                                   const int32_t *off, const uint32_t *sys, int64_t n,
+                                                   const ptg_tree_arrays t)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int64_t o = order[i];
-    t.coordinates[3 * i] = pos[3 * o];
-    t.coordinates[3 * i + 1] = pos[3 * o + 1];
-    t.coordinates[3 * i + 2] = pos[3 * o + 2];
-    t.masses[i] = m[o];
-    t.softenings[i] = h[o];
+    const double *r = rec + REC * o;
+    t.coordinates[3 * i] = r[0];
+    t.coordinates[3 * i + 1] = r[1];
+    t.coordinates[3 * i + 2] = r[2];
+    t.masses[i] = r[3];
+    t.softenings[i] = r[4];
     t.sizes[i] = 0.0;
     t.deltas[i] = 0.0;
     t.treewalk_indices[i] = o;
@@ -1127,9 +1166,6 @@ __global__ void k_single_root(const Geom *g, const ptg_tree_arrays t)
 // ============================================================================================
 struct NodeWS {
     double *acc;        // (nnodes, 3) running sum of m*x, indexed by node - n
-    int32_t *parent;    // (nnodes)
-    int32_t *nodekids;  // (nnodes) number of node children
-    int32_t *arrive;    // (nnodes) unused by the level-synchronous pass; kept zero
     int32_t *err;       // [0] corrupt-chain flag
     int32_t *bsum;
     uint64_t *lvA, *lvB;   // node levels (sort keys)
@@ -1143,9 +1179,6 @@ static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, int64_t max_leve
 {
     int64_t nh = (int64_t)RS_BINS * rs_nwt(nnodes);
     w.acc = A.take<double>(3 * nnodes);
-    w.parent = A.take<int32_t>(nnodes);
-    w.nodekids = A.take<int32_t>(nnodes);
-    w.arrive = A.take<int32_t>(nnodes);
     w.err = A.take<int32_t>(64);
     w.bsum = A.take<int32_t>(scan_blocks(std::max<int64_t>(nnodes, nh)) + 64);
     w.lvA = A.take<uint64_t>(nnodes);
@@ -1181,16 +1214,12 @@ __device__ __forceinline__ int node_children(const int32_t *first, const int32_t
     return nk;
 }
 
-__global__ __launch_bounds__(256) void k_node_init(int64_t nnodes, NodeWS w)
+__global__ __launch_bounds__(256) void k_node_init(NodeWS w)
 {
-    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < 64) w.err[a] = 0;
-    if (a >= nnodes) return;
-    w.parent[a] = -1;
-    w.arrive[a] = 0;
-    w.nodekids[a] = 0;
+    if (threadIdx.x < 64) w.err[threadIdx.x] = 0;
 }
 
+// child counts (the moments pass, or the particle fold, writes every node moment afterwards)
 __global__ __launch_bounds__(256) void k_link(int64_t n, int64_t nnodes, const ptg_tree_arrays t, NodeWS w)
 {
     int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1202,19 +1231,7 @@ __global__ __launch_bounds__(256) void k_link(int64_t n, int64_t nnodes, const p
         w.err[0] = 1;
         nk = 0;
     }
-    int nn = 0;
-    for (int q = 0; q < nk; q++)
-        if (kids[q] >= n) {
-            w.parent[kids[q] - n] = (int32_t)idx;
-            nn++;
-        }
     t.nchild[idx] = nk;
-    w.nodekids[a] = nn;
-    t.masses[idx] = 0.0;
-    t.softenings[idx] = 0.0;
-    t.deltas[idx] = 0.0;
-    if (t.quadrupoles)
-        for (int k = 0; k < 9; k++) t.quadrupoles[9 * idx + k] = 0.0;
 }
 
 // walk slots: the root of system s sits at slot s, child q of node idx at slot child_base[idx] + q;
@@ -1311,6 +1328,8 @@ __device__ void node_moments(int64_t idx, int64_t n, int64_t nnodes, const ptg_t
         t.coordinates[3 * idx] = c0;
         t.coordinates[3 * idx + 1] = c1;
         t.coordinates[3 * idx + 2] = c2;
+    } else {
+        t.deltas[idx] = 0.0;  // a massless node keeps its geometric centre
     }
     if (!quad) return;
     double Q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -1385,6 +1404,9 @@ __global__ __launch_bounds__(256) void k_fold_particles(int64_t n, int64_t nnode
     }
     t.masses[idx] = m;
     t.softenings[idx] = hs;
+    t.deltas[idx] = 0.0;
+    if (t.quadrupoles)
+        for (int k = 0; k < 9; k++) t.quadrupoles[9 * idx + k] = 0.0;
 }
 
 struct LoadNChild {
@@ -1445,7 +1467,7 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     const bool F = nsys > 1;
     if (F) w.soff = forest->offsets;
     KeyView kv{w.kA, info->has_collisions ? w.ext : nullptr, w.sys, n};
-    k_particles<<<grid1(n), 256, 0, s>>>(pos, m, h, w.vA, w.lev, w.off, w.sys, n, t);
+    k_particles<<<grid1(n), 256, 0, s>>>(w.rec, w.vA, w.lev, w.off, w.sys, n, t);
     if (n == 1) {
         k_single_root<<<1, 1, 0, s>>>(w.geoms, t);
         PTG_HIP_CHECK(hipMemsetAsync(nw.lvA, 0, sizeof(uint64_t), s));
@@ -1455,7 +1477,7 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     }
     if (F) k_forest_roots<<<grid1(nsys), 256, 0, s>>>(w.soff, nsys, w.off, n, forest->roots);
     else if (forest) k_fill_i32<<<1, 64, 0, s>>>(forest->roots, 1, (int32_t)n);
-    k_node_init<<<grid1(std::max<int64_t>(nnodes, 64)), 256, 0, s>>>(nnodes, nw);
+    k_node_init<<<1, 64, 0, s>>>(nw);
     k_link<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t, nw);
     PTG_LAUNCH_CHECK();
     // sibling-contiguous walk slots: exclusive scan of child counts in node order

@@ -228,7 +228,8 @@ def target_groups(pos, perm, group_size, tree=None, mode="cells"):
 
 
 def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7, G=1.0, precision="float64",
-         counters=None, grouping=None, groups=None, criterion="theta", eps=None, model=None, bf16=False):
+         counters=None, grouping=None, groups=None, criterion="theta", eps=None, model=None, bf16=False,
+         _events_end=None):
     """Grouped walk of `tree` at targets pos[perm[t]]; returns an (nt, W) float64 device tensor in the
     original target order (FieldsTarget_grouped, grouped_treewalk.py:350-397).
 
@@ -297,7 +298,73 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
                      int(group_size), ptr(goff), ptr(gcnt), float(theta), _lib.ctypes.byref(crit), float(G),
                      int(prec == PTG_F64), ptr(out),
                      ptr(counters), ptr(ws), ws.numel(), stream()), "ptg_walk")
+    if _events_end is not None:
+        _events_end.record()
     check(L.ptg_walk_check(ptr(ws), stream()), "ptg_walk")
+    return out
+
+
+_side_streams = {}
+
+
+def _side_stream():
+    d = torch.cuda.current_device()
+    if d not in _side_streams:
+        _side_streams[d] = torch.cuda.Stream()  # default priority
+    return _side_streams[d]
+
+
+def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, group_size=64, criterion="theta",
+                alpha=None, model=None, bf16=False, counters=None, walk_events=None):
+    """Self-gravity FP32 fields of the particles a tree was built from, in their input order.
+
+    The cell-aligned target groups are formed on a side stream while the main stream packs the
+    tree, runs the field-scale sample walk and (criterion "learned") the radius pass; the walk
+    waits for both.  criterion "error"/"learned" take the per-node tolerance alpha x (RMS field)
+    and need a tree built with quadrupoles; theta defaults to 0.7 ("theta") or 2.0 (the guard of
+    the other two).  walk_events: optional (start, end) torch.cuda.Event pair recorded around the
+    walk launch alone."""
+    L = _lib.load()
+    pos = to_dev(pos, (-1, 3))
+    n = pos.shape[0]
+    soft = to_dev(soft, (n,))
+    if n != tree.n:
+        raise ValueError("self_fields evaluates the tree's own particles")
+    if criterion not in ("theta", "error", "learned"):
+        raise ValueError(f"criterion must be 'theta', 'error' or 'learned', got {criterion!r}")
+    if theta is None:
+        theta = 0.7 if criterion == "theta" else 2.0
+    if criterion != "theta":
+        if alpha is None or not alpha > 0:
+            raise ValueError(f"criterion={criterion!r} needs alpha > 0")
+        if criterion == "learned" and model is None:
+            from .learned import OpeningMLP
+
+            model = OpeningMLP.load()
+    perm = tree.order
+    dev = device()
+    main = torch.cuda.current_stream()
+    side = _side_stream()
+    offsets = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    count = torch.empty(1, dtype=torch.int32, device=dev)
+    gws = workspace(L.ptg_groups_workspace_bytes(n, tree.nsys))
+    side.wait_stream(main)
+    with torch.cuda.stream(side):
+        check(L.ptg_target_groups(ptr(pos), ptr(perm), n, tree.forest_ref, int(group_size), _lib.PTG_GROUPS_CELLS,
+                                  ptr(offsets), ptr(count), ptr(gws), gws.numel(), stream()), "ptg_target_groups")
+    eps = None
+    tree.pack(PTG_F32, quadrupole)
+    if criterion != "theta":
+        eps = alpha * field_scale(tree)
+        if criterion == "learned":
+            tree.pack_learned(model, eps, quadrupole, bf16)
+    main.wait_stream(side)
+    if walk_events is not None:
+        walk_events[0].record()
+    out = walk(tree, pos, soft, perm, mask, quadrupole=quadrupole, group_size=group_size, theta=theta, G=G,
+               precision="float32", counters=counters, groups=(offsets, count), criterion=criterion, eps=eps,
+               model=model, bf16=bf16, _events_end=walk_events[1] if walk_events is not None else None)
+    del gws  # freed on the main stream, after it has waited for the side stream
     return out
 
 

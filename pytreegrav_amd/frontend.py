@@ -272,9 +272,21 @@ def TidalTensorTarget_grouped(pos, soft, tree, group_size=8, G=1.0, theta=0.7, q
 # shared drivers
 # ---------------------------------------------------------------------------------------------
 def _self_fields(pos, m, softening, mask, G, theta, tree, return_tree, parallel, method, quadrupole, group_size,
-                 precision, adaptive_n):
-    """Potential/Accel/TidalTensor (frontend.py:145-293, 421-560, 688-789)."""
+                 precision, adaptive_n, criterion="theta", alpha=None):
+    """Potential/Accel/TidalTensor (frontend.py:145-293, 421-560, 688-789).
+
+    criterion "error"/"learned" (float32 tree walks only, no reference counterpart): accept a node
+    only when its estimated (analytic or learned) monopole error is below alpha x the RMS field;
+    theta then defaults to 2.0 and acts as the reference's geometric guard."""
     valueTestMethod(method)
+    if criterion not in ("theta", "error", "learned"):
+        raise ValueError(f"criterion must be 'theta', 'error' or 'learned', got {criterion!r}")
+    if criterion != "theta":
+        if precision != "float32":
+            raise ValueError(f"criterion={criterion!r} runs on the float32 walk (device='cuda' or precision='float32')")
+        if alpha is None or not alpha > 0:
+            raise ValueError(f"criterion={criterion!r} needs alpha > 0 (per-node tolerance relative to the RMS field)")
+        method = "tree"
     on_dev = _is_dev(pos)
     pos = _atleast_2d(_f64(pos))
     m = _atleast_1d(_f64(m))
@@ -290,6 +302,18 @@ def _self_fields(pos, m, softening, mask, G, theta, tree, return_tree, parallel,
         out = engine.direct(pos, softening, pos, m, softening, mask, G=G, self_mode=True, precision=precision)
         if return_tree:
             tree = None
+    elif criterion != "theta":
+        if tree is None:
+            tree = ConstructTree(pos, m, softening, quadrupole=True)  # |Q| drives the criterion
+            out = engine.self_fields(tree.dev, pos, softening, mask, theta=theta, G=G, quadrupole=quadrupole,
+                                     group_size=64 if group_size == "auto" else group_size, criterion=criterion,
+                                     alpha=alpha)
+        else:
+            checkTreeQuadrupoles(tree, True)
+            eps = alpha * engine.field_scale(tree.dev)
+            out = engine.walk(tree.dev, pos, softening, _morton_order(pos), mask, quadrupole=quadrupole,
+                              group_size=group_size, theta=theta, G=G, precision=precision, criterion=criterion,
+                              eps=eps, model=_learned_model() if criterion == "learned" else None)
     else:
         if tree is None:
             tree = ConstructTree(pos, m, softening, quadrupole=quadrupole)
@@ -343,28 +367,48 @@ def _ret(val, tree, return_tree):
     return (val, tree) if return_tree else val
 
 
+_model = {}
+
+
+def _learned_model():
+    if "m" not in _model:
+        from .learned import OpeningMLP
+
+        _model["m"] = OpeningMLP.load()
+    return _model["m"]
+
+
 # ---------------------------------------------------------------------------------------------
 # public API (frontend.py:145-901)
 # ---------------------------------------------------------------------------------------------
 def Potential(pos, m, softening=None, G=1.0, theta=0.7, tree=None, return_tree=False, parallel=False,
-              method="adaptive", quadrupole=False, group_size=None, device="cpu", precision=None):
+              method="adaptive", quadrupole=False, group_size=None, device="cpu", precision=None, criterion="theta",
+              alpha=None):
     """Gravitational potential at the particle positions, shape (N,) (frontend.py:145-293)."""
     valueTestMethod(method)
     _check_device(device)
     prec = _precision(precision, device)
+    if criterion != "theta" and theta == 0.7:
+        theta = 2.0
     out, tree, on_dev = _self_fields(pos, m, softening, PTG_POT, G, theta, tree, return_tree, parallel, method,
-                                     quadrupole, _group(group_size, prec), prec, None)
+                                     quadrupole, _group(group_size, prec), prec, None, criterion, alpha)
     return _ret(_finish(out[:, 0], on_dev), tree, return_tree)
 
 
 def Accel(pos, m, softening=None, G=1.0, theta=0.7, tree=None, return_tree=False, parallel=False,
-          method="adaptive", quadrupole=False, group_size=None, device="cpu", precision=None):
-    """Gravitational acceleration at the particle positions, shape (N,3) (frontend.py:421-560)."""
+          method="adaptive", quadrupole=False, group_size=None, device="cpu", precision=None, criterion="theta",
+          alpha=None):
+    """Gravitational acceleration at the particle positions, shape (N,3) (frontend.py:421-560).
+
+    criterion="learned" / "error" with alpha (float32 only): the learned or analytic per-node error
+    bound instead of the geometric criterion (theta then defaults to 2.0, the guard)."""
     valueTestMethod(method)
     _check_device(device)
     prec = _precision(precision, device)
+    if criterion != "theta" and theta == 0.7:
+        theta = 2.0
     out, tree, on_dev = _self_fields(pos, m, softening, PTG_ACC, G, theta, tree, return_tree, parallel, method,
-                                     quadrupole, _group(group_size, prec), prec, None)
+                                     quadrupole, _group(group_size, prec), prec, None, criterion, alpha)
     return _ret(_finish(out, on_dev), tree, return_tree)
 
 

@@ -225,3 +225,29 @@ def test_trained_network_beats_geometric_criterion():
     print(f"[learned] gain: {n_geo / best[2]:.2f}x fewer interactions at rms {best[1]:.3e} <= {e_geo:.3e}",
           flush=True)
     assert best[2] < 0.8 * n_geo
+
+
+def test_frontend_criteria_and_self_fields_pipeline():
+    """Accel/Potential with criterion='learned'/'error' (the shipped model): accurate against FP64
+    direct summation, and equal bit for bit to the same walk issued step by step."""
+    from pytreegrav_amd import Accel, ConstructTree, Potential
+
+    x, m, h = case("plummer_1e5_h")
+    idx = np.random.default_rng(3).choice(len(x), 2000, replace=False)
+    d = engine.direct(x[idx], h[idx], x, m, h, PTG_ACC).cpu().numpy()
+    for crit, alpha in (("learned", 1.6e-3), ("error", 8.5e-4)):
+        a = Accel(x, m, h, device="cuda", criterion=crit, alpha=alpha)
+        p = Potential(x, m, h, device="cuda", criterion=crit, alpha=alpha)
+        e = _rms(a[idx], d)
+        print(f"[learned] frontend {crit}: accel rms {e:.3e}", flush=True)
+        assert e < 3e-3 and np.all(np.isfinite(p))
+        t = ConstructTree(x, m, h, quadrupole=True)
+        eps = alpha * engine.field_scale(t.dev)
+        kw = dict(model=learned.OpeningMLP.load()) if crit == "learned" else {}
+        b = engine.walk(t.dev, x, h, t.dev.order, PTG_ACC, group_size=64, theta=2.0, precision="float32",
+                        criterion=crit, eps=eps, **kw).cpu().numpy()
+        assert np.array_equal(a, b)
+    with pytest.raises(ValueError):
+        Accel(x, m, h, device="cpu", criterion="learned", alpha=1e-3)  # float64 walk has no learned form
+    with pytest.raises(ValueError):
+        Accel(x, m, h, device="cuda", criterion="learned")  # alpha required
