@@ -79,6 +79,7 @@ typedef struct {
     int32_t *nchild;          /* (n+nnodes): number of children (0 for particles) */
     int32_t *child_base;      /* (n+nnodes): walk slot of the first child (sibling-contiguous layout) */
     int32_t *slot;            /* (n+nnodes): walk slot of each element (the root is slot 0) */
+    uint64_t *keys;           /* (n) or NULL: 63-bit Morton key of each particle slot (of its system's cube) */
 } ptg_tree_arrays;
 
 /* Forest: several independent systems (no reference counterpart; the reference builds one Octree
@@ -118,13 +119,15 @@ ptg_status ptg_morton_order(const double *pos, int64_t n, int64_t *order, void *
  * Morton order.  Writes offsets[0..G] (capacity n+1) and the group count G to the device scalar
  * *ngroups; asynchronous.  With a forest the targets are the forest's own particles (perm = its
  * treewalk_indices): runs restart and cells are cut at every system start, and each system's
- * cube defines its cells, so every system is grouped exactly as if it were alone. */
+ * cube defines its cells, so every system is grouped exactly as if it were alone.  keys
+ * (nullable): the targets' Morton keys in sorted order when the caller has them (the tree's own
+ * particles: ptg_tree_arrays.keys), which skips the bounding cube and key pass. */
 #define PTG_GROUPS_RUNS 0
 #define PTG_GROUPS_CELLS 1
 size_t ptg_groups_workspace_bytes(int64_t n, int64_t nsys);
-ptg_status ptg_target_groups(const double *pos, const int64_t *perm, int64_t n, const ptg_forest *forest,
-                             int group_size, int mode, int32_t *offsets, int32_t *ngroups, void *ws,
-                             size_t ws_bytes, void *stream);
+ptg_status ptg_target_groups(const double *pos, const int64_t *perm, const uint64_t *keys, int64_t n,
+                             const ptg_forest *forest, int group_size, int mode, int32_t *offsets, int32_t *ngroups,
+                             void *ws, size_t ws_bytes, void *stream);
 
 /* ------------------------------------------------------------------------------------------
  * Walk tree: the sibling-contiguous packed copy of the tree the walk reads (one slot per

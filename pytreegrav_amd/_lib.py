@@ -48,6 +48,7 @@ class TreeArrays(_c.Structure):
         ("nchild", _vp),
         ("child_base", _vp),
         ("slot", _vp),
+        ("keys", _vp),
     ]
 
 
@@ -85,7 +86,8 @@ SIGNATURES = {
     "ptg_build_sort": (_c.c_int, [_vp, _vp, _vp, _i64, _F, _vp, _c.c_size_t, _c.POINTER(BuildInfo), _vp]),
     "ptg_node_workspace_bytes": (_c.c_size_t, [_i64, _i64, _c.c_int]),
     "ptg_groups_workspace_bytes": (_c.c_size_t, [_i64, _i64]),
-    "ptg_target_groups": (_c.c_int, [_vp, _vp, _i64, _F, _c.c_int, _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),
+    "ptg_target_groups": (_c.c_int, [_vp, _vp, _vp, _i64, _F, _c.c_int, _c.c_int, _vp, _vp, _vp, _c.c_size_t,
+                                     _vp]),
     "ptg_build_tree": (_c.c_int, [_vp, _vp, _vp, _c.POINTER(BuildInfo), _F, _c.c_int, _c.c_int, _vp, _c.c_size_t,
                                   _vp, _c.c_size_t, _c.POINTER(TreeArrays), _vp]),
     "ptg_morton_order": (_c.c_int, [_vp, _i64, _vp, _vp, _c.c_size_t, _vp]),

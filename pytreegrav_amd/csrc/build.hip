@@ -1128,9 +1128,9 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
 }
 
 // particle slots: gather into Morton order (octree.py:577-603) and link to the next element
-__global__ __launch_bounds__(256) void k_particles(const double *rec, const uint32_t *order, const int32_t *lev,
-                                                   const int32_t *off, const uint32_t *sys, int64_t n,
-                                                   const ptg_tree_arrays t)
+__global__ __launch_bounds__(256) void k_particles(const double *rec, const uint32_t *order, const uint64_t *keys,
+                                                   const int32_t *lev, const int32_t *off, const uint32_t *sys,
+                                                   int64_t n, const ptg_tree_arrays t)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1144,6 +1144,7 @@ __global__ __launch_bounds__(256) void k_particles(const double *rec, const uint
     t.sizes[i] = 0.0;
     t.deltas[i] = 0.0;
     t.treewalk_indices[i] = o;
+    if (t.keys) t.keys[i] = keys[i];
     t.first_subnode[i] = -1;
     t.next_branch[i] = n == 1 ? -1 : head_of(i + 1, n, lev, off, sys);
     t.nchild[i] = 0;
@@ -1467,7 +1468,7 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     const bool F = nsys > 1;
     if (F) w.soff = forest->offsets;
     KeyView kv{w.kA, info->has_collisions ? w.ext : nullptr, w.sys, n};
-    k_particles<<<grid1(n), 256, 0, s>>>(w.rec, w.vA, w.lev, w.off, w.sys, n, t);
+    k_particles<<<grid1(n), 256, 0, s>>>(w.rec, w.vA, w.kA, w.lev, w.off, w.sys, n, t);
     if (n == 1) {
         k_single_root<<<1, 1, 0, s>>>(w.geoms, t);
         PTG_HIP_CHECK(hipMemsetAsync(nw.lvA, 0, sizeof(uint64_t), s));
@@ -1757,9 +1758,9 @@ extern "C" size_t ptg_groups_workspace_bytes(int64_t n, int64_t nsys)
     return A.used;
 }
 
-extern "C" ptg_status ptg_target_groups(const double *pos, const int64_t *perm, int64_t n, const ptg_forest *forest,
-                                        int group_size, int mode, int32_t *offsets, int32_t *ngroups, void *ws,
-                                        size_t ws_bytes, void *stream)
+extern "C" ptg_status ptg_target_groups(const double *pos, const int64_t *perm, const uint64_t *keys, int64_t n,
+                                        const ptg_forest *forest, int group_size, int mode, int32_t *offsets,
+                                        int32_t *ngroups, void *ws, size_t ws_bytes, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (n < 1 || !pos || !perm || !offsets || !ngroups || group_size < 1 || group_size > 256) {
@@ -1794,17 +1795,20 @@ extern "C" ptg_status ptg_target_groups(const double *pos, const int64_t *perm, 
         PTG_LAUNCH_CHECK();
         return PTG_OK;
     }
-    if (nsys > 1) {
-        // targets are the forest's own particles: system s owns input indices and sorted positions
-        // [offsets[s], offsets[s+1])
-        k_bbox_sys<<<(unsigned)nsys, 256, 0, s>>>(pos, soff, w.geom);
-    } else {
-        int nb = (int)std::min<int64_t>(BB_BLOCKS, std::max<int64_t>(1, ptg_cdiv(n, 1024)));
-        k_bbox_partial<<<nb, 256, 0, s>>>(pos, n, w.part);
-        k_bbox_final<<<1, 256, 0, s>>>(w.part, nb, w.geom);
+    if (!keys) {
+        if (nsys > 1) {
+            // targets are the forest's own particles: system s owns input indices and sorted
+            // positions [offsets[s], offsets[s+1])
+            k_bbox_sys<<<(unsigned)nsys, 256, 0, s>>>(pos, soff, w.geom);
+        } else {
+            int nb = (int)std::min<int64_t>(BB_BLOCKS, std::max<int64_t>(1, ptg_cdiv(n, 1024)));
+            k_bbox_partial<<<nb, 256, 0, s>>>(pos, n, w.part);
+            k_bbox_final<<<1, 256, 0, s>>>(w.part, nb, w.geom);
+        }
+        k_sorted_keys<<<grid1(n), 256, 0, s>>>(pos, perm, n, w.geom, soff, nsys, w.keys);
+        keys = w.keys;
     }
-    k_sorted_keys<<<grid1(n), 256, 0, s>>>(pos, perm, n, w.geom, soff, nsys, w.keys);
-    k_cell_cuts<<<grid1(n), 256, 0, s>>>(w.keys, n, soff, nsys, 4 * (int64_t)group_size, w.pflag);
+    k_cell_cuts<<<grid1(n), 256, 0, s>>>(keys, n, soff, nsys, 4 * (int64_t)group_size, w.pflag);
     PTG_LAUNCH_CHECK();
     st = excl_scan(LoadArr{w.pflag}, n, w.pex, w.bsum, nullptr, s);
     if (st) return st;

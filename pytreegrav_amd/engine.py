@@ -125,11 +125,12 @@ class DeviceTree:
         self.nchild = torch.empty(ne, **i32)
         self.child_base = torch.empty(ne, **i32)
         self.slot = torch.empty(ne, **i32)
+        self.keys = torch.empty(n, dtype=torch.int64, device=dev)  # 63-bit Morton keys of the particle slots
         self.arrays = TreeArrays(
             ptr(self.coordinates).value, ptr(self.masses).value, ptr(self.softenings).value, ptr(self.sizes).value,
             ptr(self.deltas).value, ptr(self.quadrupoles).value if quadrupole else None,
             ptr(self.first_subnode).value, ptr(self.next_branch).value, ptr(self.order).value,
-            ptr(self.nchild).value, ptr(self.child_base).value, ptr(self.slot).value)
+            ptr(self.nchild).value, ptr(self.child_base).value, ptr(self.slot).value, ptr(self.keys).value)
         nws = workspace(L.ptg_node_workspace_bytes(n, nn, int(info.max_level)))
         check(L.ptg_build_tree(ptr(pos), ptr(m), ptr(h), _lib.ctypes.byref(info), fref, int(quadrupole),
                                int(compute_moments), ptr(ws), ws.numel(), ptr(nws), nws.numel(),
@@ -207,10 +208,11 @@ def morton_order(pos):
 _walk_ws = {}
 
 
-def target_groups(pos, perm, group_size, tree=None, mode="cells"):
+def target_groups(pos, perm, group_size, tree=None, mode="cells", keys=None):
     """Groups of the Morton-sorted targets pos[perm[t]]: (offsets, count) device int32 tensors for
     ptg_walk (ptg_target_groups).  mode "cells": cell-aligned groups; "fixed": runs of group_size.
-    tree: the forest whose own particles are the targets (groups then respect its systems)."""
+    tree: the forest whose own particles are the targets (groups then respect its systems).
+    keys: the targets' sorted Morton keys if known (tree.keys for a tree's own particles)."""
     L = _lib.load()
     pos = to_dev(pos, (-1, 3))
     n = pos.shape[0]
@@ -222,8 +224,8 @@ def target_groups(pos, perm, group_size, tree=None, mode="cells"):
         raise ValueError(f"mode must be 'cells' or 'fixed', got {mode!r}")
     m = _lib.PTG_GROUPS_CELLS if mode == "cells" else _lib.PTG_GROUPS_RUNS
     ws = workspace(L.ptg_groups_workspace_bytes(n, nsys))
-    check(L.ptg_target_groups(ptr(pos), ptr(perm), n, fref, int(group_size), m, ptr(offsets), ptr(count), ptr(ws),
-                              ws.numel(), stream()), "ptg_target_groups")
+    check(L.ptg_target_groups(ptr(pos), ptr(perm), ptr(keys), n, fref, int(group_size), m, ptr(offsets), ptr(count),
+                              ptr(ws), ws.numel(), stream()), "ptg_target_groups")
     return offsets, count
 
 
@@ -262,7 +264,9 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     if tree.nsys > 1 and (nt != tree.n or perm.data_ptr() != tree.order.data_ptr()):
         raise ValueError("a forest is walked at its own particles (perm = tree.order)")
     if groups is None and (grouping == "cells" or tree.nsys > 1):
-        groups = target_groups(pos, perm, group_size, tree=tree if forest else None, mode=grouping)
+        own = nt == tree.n and perm.data_ptr() == tree.order.data_ptr()
+        groups = target_groups(pos, perm, group_size, tree=tree if forest else None, mode=grouping,
+                               keys=tree.keys if own else None)
     goff, gcnt = groups if groups is not None else (None, None)
     crit = _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
     if criterion == "error":
@@ -350,8 +354,9 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
     gws = workspace(L.ptg_groups_workspace_bytes(n, tree.nsys))
     side.wait_stream(main)
     with torch.cuda.stream(side):
-        check(L.ptg_target_groups(ptr(pos), ptr(perm), n, tree.forest_ref, int(group_size), _lib.PTG_GROUPS_CELLS,
-                                  ptr(offsets), ptr(count), ptr(gws), gws.numel(), stream()), "ptg_target_groups")
+        check(L.ptg_target_groups(ptr(pos), ptr(perm), ptr(tree.keys), n, tree.forest_ref, int(group_size),
+                                  _lib.PTG_GROUPS_CELLS, ptr(offsets), ptr(count), ptr(gws), gws.numel(), stream()),
+              "ptg_target_groups")
     eps = None
     tree.pack(PTG_F32, quadrupole)
     if criterion != "theta":

@@ -5,7 +5,8 @@ The reference solves one particle set per call (frontend.Accel / Potential, src/
 frontend.py:391-520, building one Octree each).  Here systems of uneven size are concatenated;
 every system still gets its own bounding cube, Morton keys, octree, target groups and FP32 origin,
 so each system's result equals what the single-system path gives for it alone, bit for bit.  The
-walk stage (pack, groups, walk) can be captured once into a HIP graph and replayed.
+walk stage (groups and walk; the pack is made once before) can be captured into a HIP graph and
+replayed.
 """
 
 import numpy as np
@@ -49,8 +50,8 @@ class Forest:
         p = t.pack(self.prec, self.quadrupole)
         mode = _lib.PTG_GROUPS_CELLS if self.prec == PTG_F32 else _lib.PTG_GROUPS_RUNS
         gws = self._gws
-        check(L.ptg_target_groups(ptr(self.pos), ptr(t.order), self.n, t.forest_ref, self.group_size, mode,
-                                  ptr(goff), ptr(gcnt), ptr(gws), gws.numel(), stream()), "ptg_target_groups")
+        check(L.ptg_target_groups(ptr(self.pos), ptr(t.order), ptr(t.keys), self.n, t.forest_ref, self.group_size,
+                                  mode, ptr(goff), ptr(gcnt), ptr(gws), gws.numel(), stream()), "ptg_target_groups")
         crit = _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
         check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), t.forest_ref, ptr(self.pos), ptr(self.h), ptr(t.order), self.n,
                          int(mask), int(self.quadrupole), self.group_size, ptr(goff), ptr(gcnt), float(theta),

@@ -142,6 +142,10 @@ def test_cell_groups_partition_targets(name, gs):
     n = len(x)
     assert off[0] == 0 and off[G] == n and np.all(np.diff(off) >= 1) and np.all(np.diff(off) <= gs)
     keys = oracle.morton_keys(x[t.TreewalkIndices], *oracle.bounding_cube(x))
+    # the tree's own keys are the oracle's, and grouping from them gives the same groups
+    assert np.array_equal(t.dev.keys.cpu().numpy(), keys)
+    off_k, cnt_k = engine.target_groups(x, t.dev.order, gs, keys=t.dev.keys)
+    assert int(cnt_k.item()) == G and np.array_equal(off_k.cpu().numpy()[:G + 1], off)
     for g in range(min(G, 2000)):
         a, b = off[g], off[g + 1] - 1
         if a == b:
