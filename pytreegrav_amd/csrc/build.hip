@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "ptg_common.hpp"
 
@@ -369,32 +370,17 @@ struct Counters {
     int32_t pad[9];
 };
 
-// Morton keys in input order; with rec, also the particle record (x, y, z, m, h in a 64-byte slot)
-// that the Morton-order gather reads with one cache line per particle, and the finiteness flag
-#define REC 8
-__device__ __forceinline__ bool store_record(const double *pos, const double *m, const double *h, int64_t i,
-                                             double *rec)
-{
-    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2], mi = m[i], hi = h[i];
-    double *r = rec + REC * i;
-    r[0] = x;
-    r[1] = y;
-    r[2] = z;
-    r[3] = mi;
-    r[4] = hi;
-    return !isfinite(x) || !isfinite(y) || !isfinite(z) || !isfinite(mi) || !isfinite(hi);
-}
-
+// Morton keys in input order; with m and h (and c), also the finiteness flag of every input
 __global__ __launch_bounds__(256) void k_keys(const double *pos, const double *m, const double *h, int64_t n,
-                                              const Geom *g, uint64_t *keys, uint32_t *vals, double *rec,
-                                              Counters *c)
+                                              const Geom *g, uint64_t *keys, uint32_t *vals, Counters *c)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool bad = false;
     if (i < n) {
-        keys[i] = morton_key(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], g->cmin, g->inv);
+        const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        keys[i] = morton_key(x, y, z, g->cmin, g->inv);
         vals[i] = (uint32_t)i;
-        if (rec) bad = store_record(pos, m, h, i, rec);
+        if (c) bad = !isfinite(x) || !isfinite(y) || !isfinite(z) || !isfinite(m[i]) || !isfinite(h[i]);
     }
     if (c && ptg_ballot(bad) && ptg_lane() == 0) c->nonfinite = 1;
 }
@@ -455,9 +441,8 @@ __global__ __launch_bounds__(256) void k_bbox_sys(const double *pos, const int64
     for (int k = 0; k < 3; k++) g->cmin[k] = g->center[k] - 0.5 * sz;
 }
 
-__global__ __launch_bounds__(256) void k_keys_sys(const double *pos, const double *m, const double *h, int64_t n,
-                                                  const Geom *geoms, const uint32_t *sys, uint64_t *keys,
-                                                  uint64_t *kU, uint32_t *vals, double *rec)
+__global__ __launch_bounds__(256) void k_keys_sys(const double *pos, int64_t n, const Geom *geoms,
+                                                  const uint32_t *sys, uint64_t *keys, uint64_t *kU, uint32_t *vals)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -466,7 +451,6 @@ __global__ __launch_bounds__(256) void k_keys_sys(const double *pos, const doubl
     keys[i] = k;
     kU[i] = k;
     vals[i] = (uint32_t)i;
-    if (rec) store_record(pos, m, h, i, rec);
 }
 
 __global__ __launch_bounds__(256) void k_gather_sys_key(const uint32_t *sys, const uint32_t *vals, int64_t n,
@@ -531,7 +515,6 @@ struct BuildWS {
     uint64_t *kU;
     const int64_t *soff;
     int64_t nsys;
-    double *rec;  // (n, 8) particle records in input order
 };
 
 static bool carve_build(PtgArena &A, int64_t n, BuildWS &w, int64_t nsys = 1)
@@ -562,7 +545,6 @@ static bool carve_build(PtgArena &A, int64_t n, BuildWS &w, int64_t nsys = 1)
     w.tkB = A.take<uint64_t>(n);
     w.perm = A.take<uint32_t>(n);
     w.permB = A.take<uint32_t>(n);
-    w.rec = A.take<double>(REC * n);
     w.nsys = nsys;
     w.soff = nullptr;
     w.geoms = A.take<Geom>(nsys);
@@ -804,16 +786,15 @@ __global__ void k_u32_to_i64(const uint32_t *a, int64_t n, int64_t *b)
 
 static dim3 grid1(int64_t n, int bs = 256) { return dim3((unsigned)std::max<int64_t>(1, ptg_cdiv(n, bs))); }
 
-// bbox + keys + sort; result (sorted keys, order) in (w.kA, w.vA).  With m and h, also the particle
-// records and the finiteness flag.
+// bbox + keys + sort; result (sorted keys, order) in (w.kA, w.vA).  With m and h, also the
+// finiteness flag.
 static ptg_status sort_by_keys(const double *pos, const double *m, const double *h, int64_t n, BuildWS &w,
                                hipStream_t s)
 {
     int nb = (int)std::min<int64_t>(BB_BLOCKS, std::max<int64_t>(1, ptg_cdiv(n, 1024)));
     k_bbox_partial<<<nb, 256, 0, s>>>(pos, n, w.part);
     k_bbox_final<<<1, 256, 0, s>>>(w.part, nb, w.geom);
-    const bool rec = m && h;
-    k_keys<<<grid1(n), 256, 0, s>>>(pos, m, h, n, w.geom, w.kA, w.vA, rec ? w.rec : nullptr, rec ? w.ctr : nullptr);
+    k_keys<<<grid1(n), 256, 0, s>>>(pos, m, h, n, w.geom, w.kA, w.vA, m && h ? w.ctr : nullptr);
     PTG_LAUNCH_CHECK();
     bool in_first;
     ptg_status st = radix_sort(w.kA, w.vA, w.kB, w.vB, n, 64, w.hist, w.bsum, &in_first, s);
@@ -839,7 +820,7 @@ static ptg_status sort_by_keys_forest(const double *pos, const double *m, const 
 {
     k_sys_of<<<grid1(n), 256, 0, s>>>(w.soff, w.nsys, n, w.sys);
     k_bbox_sys<<<(unsigned)w.nsys, 256, 0, s>>>(pos, w.soff, w.geoms);
-    k_keys_sys<<<grid1(n), 256, 0, s>>>(pos, m, h, n, w.geoms, w.sys, w.kA, w.kU, w.vA, m && h ? w.rec : nullptr);
+    k_keys_sys<<<grid1(n), 256, 0, s>>>(pos, n, w.geoms, w.sys, w.kA, w.kU, w.vA);
     PTG_LAUNCH_CHECK();
     bool in_first;
     ptg_status st = radix_sort(w.kA, w.vA, w.kB, w.vB, n, 64, w.hist, w.bsum, &in_first, s);
@@ -1128,19 +1109,19 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
 }
 
 // particle slots: gather into Morton order (octree.py:577-603) and link to the next element
-__global__ __launch_bounds__(256) void k_particles(const double *rec, const uint32_t *order, const uint64_t *keys,
-                                                   const int32_t *lev, const int32_t *off, const uint32_t *sys,
-                                                   int64_t n, const ptg_tree_arrays t)
+__global__ __launch_bounds__(256) void k_particles(const double *pos, const double *m, const double *h,
+                                                   const uint32_t *order, const uint64_t *keys, const int32_t *lev,
+                                                   const int32_t *off, const uint32_t *sys, int64_t n,
+                                                   const ptg_tree_arrays t)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int64_t o = order[i];
-    const double *r = rec + REC * o;
-    t.coordinates[3 * i] = r[0];
-    t.coordinates[3 * i + 1] = r[1];
-    t.coordinates[3 * i + 2] = r[2];
-    t.masses[i] = r[3];
-    t.softenings[i] = r[4];
+    t.coordinates[3 * i] = pos[3 * o];
+    t.coordinates[3 * i + 1] = pos[3 * o + 1];
+    t.coordinates[3 * i + 2] = pos[3 * o + 2];
+    t.masses[i] = m[o];
+    t.softenings[i] = h[o];
     t.sizes[i] = 0.0;
     t.deltas[i] = 0.0;
     t.treewalk_indices[i] = o;
@@ -1165,8 +1146,13 @@ __global__ void k_single_root(const Geom *g, const ptg_tree_arrays t)
 // ============================================================================================
 // links and moments
 // ============================================================================================
+// a child's moments as its parent reads them: mass, sum of m*x (nodes), softening, centre of mass
+struct MomRec {
+    double m, a0, a1, a2, h, c0, c1, c2;
+};
+
 struct NodeWS {
-    double *acc;        // (nnodes, 3) running sum of m*x, indexed by node - n
+    double *acc;        // (nnodes, 3) sum of m*x of each node, by node offset (element - n)
     int32_t *err;       // [0] corrupt-chain flag
     int32_t *bsum;
     uint64_t *lvA, *lvB;   // node levels (sort keys)
@@ -1266,51 +1252,52 @@ __global__ __launch_bounds__(256) void k_slots(int64_t n, int64_t nnodes, int64_
     }
 }
 
-template <typename T> __device__ __forceinline__ T ld_agent(const T *p) { return *p; }
+// Bottom-up moments, level by level from the deepest up: every node of a level is computed after
+// all of its children (a kernel boundary, or a workgroup barrier for runs of small levels, orders
+// the levels), sums run in a fixed order, so the result is deterministic and needs no device-scope
+// fences or FP64 atomics.
+__global__ __launch_bounds__(256) void k_level_starts(const uint64_t *lv, int64_t nnodes, int max_level,
+                                                      int32_t *lvstart)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nnodes) return;
+    // lvstart[L] = first sorted position with level >= L
+    int64_t a = i == 0 ? -1 : (int64_t)lv[i - 1];
+    int64_t b = i == nnodes ? (int64_t)max_level + 1 : (int64_t)lv[i];
+    for (int64_t L = a + 1; L <= b && L <= max_level + 1; L++) lvstart[L] = (int32_t)i;
+}
+
 
 // One node's moments, in the reference's summation order: particle children in digit order
 // (folded during the split, octree.py:956-966), then node children in descending index order,
 // i.e. descending digit order (RollupMoments' reverse scan, octree.py:212-231).  Zero-mass node
 // children contribute nothing (:215-216).  Quadrupoles follow RollupQuadrupoles (:252-292).
-__device__ void node_moments(int64_t idx, int64_t n, int64_t nnodes, const ptg_tree_arrays &t, NodeWS &w, bool quad)
+// K[0..nk): the children's records in digit order (the records at slots child_base + k).
+__device__ void node_moments(int64_t idx, int nk, const MomRec *K, const int32_t *kn, const double (*KQ)[9], int64_t n,
+                             const ptg_tree_arrays &t, NodeWS &w, bool quad)
 {
-    int32_t kids[8];
-    int nk = t.nchild[idx];
-    int64_t base = t.child_base[idx];
-    if (nk < 0 || nk > 8 || base < 1 || base + nk > n + nnodes) {
-        w.err[0] = 1;
-        return;
-    }
-    for (int q = 0; q < nk; q++) kids[q] = w.elem[base + q];
     double m = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, hs = 0.0;
-    for (int q = 0; q < nk; q++) {
-        int32_t c = kids[q];
-        if (c >= n) continue;
-        double mi = t.masses[c];
+    for (int k = 0; k < nk; k++) {
+        if (kn[k] >= 0) continue;
+        double mi = K[k].m;
         m += mi;
-        a0 += mi * t.coordinates[3 * (int64_t)c];
-        a1 += mi * t.coordinates[3 * (int64_t)c + 1];
-        a2 += mi * t.coordinates[3 * (int64_t)c + 2];
-        double hc = t.softenings[c];
+        a0 += mi * K[k].c0;
+        a1 += mi * K[k].c1;
+        a2 += mi * K[k].c2;
+        double hc = K[k].h;
         if (!(hs >= hc)) hs = hc;
     }
-    for (int q = nk - 1; q >= 0; q--) {
-        int32_t c = kids[q];
-        if (c < n) continue;
-        double mc = ld_agent(t.masses + c);
+    for (int k = nk - 1; k >= 0; k--) {
+        if (kn[k] < 0) continue;
+        double mc = K[k].m;
         if (mc == 0.0) continue;
-        int64_t ca = c - n;
-        a0 += ld_agent(w.acc + 3 * ca);
-        a1 += ld_agent(w.acc + 3 * ca + 1);
-        a2 += ld_agent(w.acc + 3 * ca + 2);
+        a0 += K[k].a0;
+        a1 += K[k].a1;
+        a2 += K[k].a2;
         m += mc;
-        double hc = ld_agent(t.softenings + c);
+        double hc = K[k].h;
         if (!(hs >= hc)) hs = hc;
     }
-    int64_t ia = idx - n;
-    w.acc[3 * ia] = a0;
-    w.acc[3 * ia + 1] = a1;
-    w.acc[3 * ia + 2] = a2;
     t.masses[idx] = m;
     t.softenings[idx] = hs;
     double com0 = t.coordinates[3 * idx], com1 = t.coordinates[3 * idx + 1], com2 = t.coordinates[3 * idx + 2];
@@ -1332,6 +1319,10 @@ __device__ void node_moments(int64_t idx, int64_t n, int64_t nnodes, const ptg_t
     } else {
         t.deltas[idx] = 0.0;  // a massless node keeps its geometric centre
     }
+    const int64_t ia = idx - n;
+    w.acc[3 * ia] = a0;
+    w.acc[3 * ia + 1] = a1;
+    w.acc[3 * ia + 2] = a2;
     if (!quad) return;
     double Q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     auto shift = [&](double mm, double r0, double r1, double r2) {
@@ -1346,45 +1337,103 @@ __device__ void node_moments(int64_t idx, int64_t n, int64_t nnodes, const ptg_t
         Q[7] += mm * 3.0 * r2 * r1;
         Q[8] += mm * (3.0 * r2 * r2 - rsq);
     };
-    for (int q = 0; q < nk; q++) {
-        int32_t c = kids[q];
-        if (c >= n) continue;
-        shift(t.masses[c], t.coordinates[3 * (int64_t)c] - com0, t.coordinates[3 * (int64_t)c + 1] - com1,
-              t.coordinates[3 * (int64_t)c + 2] - com2);
+    for (int k = 0; k < nk; k++) {
+        if (kn[k] >= 0) continue;
+        shift(K[k].m, K[k].c0 - com0, K[k].c1 - com1, K[k].c2 - com2);
     }
-    for (int q = nk - 1; q >= 0; q--) {
-        int32_t c = kids[q];
-        if (c < n) continue;
-        double mc = ld_agent(t.masses + c);
+    for (int k = nk - 1; k >= 0; k--) {
+        if (kn[k] < 0) continue;
+        double mc = K[k].m;
         if (mc == 0.0) continue;
-        for (int k = 0; k < 9; k++) Q[k] += ld_agent(t.quadrupoles + 9 * (int64_t)c + k);
-        shift(mc, ld_agent(t.coordinates + 3 * (int64_t)c) - com0, ld_agent(t.coordinates + 3 * (int64_t)c + 1) - com1,
-              ld_agent(t.coordinates + 3 * (int64_t)c + 2) - com2);
+        for (int c = 0; c < 9; c++) Q[c] += KQ[k][c];
+        shift(mc, K[k].c0 - com0, K[k].c1 - com1, K[k].c2 - com2);
     }
-    for (int k = 0; k < 9; k++) t.quadrupoles[9 * idx + k] = Q[k];
+    for (int c = 0; c < 9; c++) t.quadrupoles[9 * idx + c] = Q[c];
 }
 
-// Bottom-up moments, one launch per level from the deepest up: every node of a level is computed
-// after all of its children (a kernel boundary orders the levels), sums run in a fixed order, so
-// the result is deterministic and needs no device-scope fences or FP64 atomics.
-__global__ __launch_bounds__(256) void k_level_starts(const uint64_t *lv, int64_t nnodes, int max_level,
-                                                      int32_t *lvstart)
+// Eight lanes per node: lane q loads the moments of child q into LDS, so a node's children are
+// fetched in parallel; the node's first lane then sums them.  A wave does eight nodes.
+struct CoopLds {
+    MomRec K[64];
+    double Q[64][9];
+    int32_t kn[64];
+};
+#define MC_WAVES 4
+
+__device__ void coop_moments(int64_t qi, bool valid, int64_t n, int64_t nnodes, const uint32_t *byid,
+                             const ptg_tree_arrays &t, NodeWS &w, bool quad, CoopLds &S)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > nnodes) return;
-    // lvstart[L] = first sorted position with level >= L
-    int64_t a = i == 0 ? -1 : (int64_t)lv[i - 1];
-    int64_t b = i == nnodes ? (int64_t)max_level + 1 : (int64_t)lv[i];
-    for (int64_t L = a + 1; L <= b && L <= max_level + 1; L++) lvstart[L] = (int32_t)i;
+    const int lane = ptg_lane(), q = lane & 7, lead = lane & ~7;
+    const int64_t ne = n + nnodes;
+    const int64_t idx = valid ? n + (int64_t)byid[qi] : 0;
+    int nk = valid ? t.nchild[idx] : 0;
+    const int64_t base = valid ? t.child_base[idx] : 0;
+    const bool bad = valid && (nk < 0 || nk > 8 || base < 1 || base + nk > ne);
+    if (bad) {
+        if (q == 0) w.err[0] = 1;
+        nk = 0;
+    }
+    if (q < nk) {
+        int64_t c = w.elem[base + q];
+        if (c < 0 || c >= ne) {  // a malformed slot map is reported, never followed
+            w.err[0] = 1;
+            c = 0;
+        }
+        MomRec r;
+        r.m = t.masses[c];
+        r.h = t.softenings[c];
+        r.c0 = t.coordinates[3 * c];
+        r.c1 = t.coordinates[3 * c + 1];
+        r.c2 = t.coordinates[3 * c + 2];
+        r.a0 = r.a1 = r.a2 = 0.0;
+        if (c >= n) {
+            const int64_t ca = c - n;
+            r.a0 = w.acc[3 * ca];
+            r.a1 = w.acc[3 * ca + 1];
+            r.a2 = w.acc[3 * ca + 2];
+            if (quad)
+                for (int k = 0; k < 9; k++) S.Q[lane][k] = t.quadrupoles[9 * c + k];
+        }
+        S.K[lane] = r;
+        S.kn[lane] = c >= n ? (int32_t)(c - n) : -1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (q == 0 && valid && !bad) node_moments(idx, nk, S.K + lead, S.kn + lead, S.Q + lead, n, t, w, quad);
+    __builtin_amdgcn_wave_barrier();
 }
 
-__global__ __launch_bounds__(256) void k_moments_level(int64_t n, int64_t nnodes, int L, const int32_t *lvstart,
-                                                       const uint32_t *byid, const ptg_tree_arrays t, NodeWS w,
-                                                       int quad)
+// one level, eight nodes per wave
+__global__ __launch_bounds__(64 * MC_WAVES) void k_moments_level(int64_t n, int64_t nnodes, int L,
+                                                                 const int32_t *lvstart, const uint32_t *byid,
+                                                                 const ptg_tree_arrays t, NodeWS w, int quad)
 {
-    int64_t lo = lvstart[L], hi = lvstart[L + 1];
-    for (int64_t q = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < hi; q += (int64_t)gridDim.x * blockDim.x)
-        node_moments(n + byid[q], n, nnodes, t, w, quad != 0);
+    __shared__ CoopLds S[MC_WAVES];
+    const int wave = threadIdx.x >> 6, j = ptg_lane() >> 3;
+    const int64_t lo = lvstart[L], hi = lvstart[L + 1];
+    const int64_t stride = (int64_t)gridDim.x * MC_WAVES * 8;
+    for (int64_t b = lo + ((int64_t)blockIdx.x * MC_WAVES + wave) * 8; b < hi; b += stride) {
+        const int64_t qi = b + j;
+        coop_moments(qi, qi < hi, n, nnodes, byid, t, w, quad != 0, S[wave]);
+    }
+}
+
+// levels Lhi down to Llo (each with few nodes) in one block, a barrier between levels: all waves
+// of a workgroup share one CU's L1, so the barrier orders one level's writes before the next
+// level's reads
+__global__ __launch_bounds__(64 * MC_WAVES) void k_moments_small(int64_t n, int64_t nnodes, int Lhi, int Llo,
+                                                                 const int32_t *lvstart, const uint32_t *byid,
+                                                                 const ptg_tree_arrays t, NodeWS w, int quad)
+{
+    __shared__ CoopLds S[MC_WAVES];
+    const int wave = threadIdx.x >> 6, j = ptg_lane() >> 3;
+    for (int L = Lhi; L >= Llo; L--) {
+        const int64_t lo = lvstart[L], hi = lvstart[L + 1];
+        for (int64_t b = lo + (int64_t)wave * 8; b < hi; b += MC_WAVES * 8) {
+            const int64_t qi = b + j;
+            coop_moments(qi, qi < hi, n, nnodes, byid, t, w, quad != 0, S[wave]);
+        }
+        __syncthreads();
+    }
 }
 
 // compute_moments=False: the split has still folded particle children into their parent's mass
@@ -1468,7 +1517,7 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     const bool F = nsys > 1;
     if (F) w.soff = forest->offsets;
     KeyView kv{w.kA, info->has_collisions ? w.ext : nullptr, w.sys, n};
-    k_particles<<<grid1(n), 256, 0, s>>>(w.rec, w.vA, w.kA, w.lev, w.off, w.sys, n, t);
+    k_particles<<<grid1(n), 256, 0, s>>>(pos, m, h, w.vA, w.kA, w.lev, w.off, w.sys, n, t);
     if (n == 1) {
         k_single_root<<<1, 1, 0, s>>>(w.geoms, t);
         PTG_HIP_CHECK(hipMemsetAsync(nw.lvA, 0, sizeof(uint64_t), s));
@@ -1500,8 +1549,26 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
         const uint32_t *byid = in_first ? nw.idA : nw.idB;
         k_level_starts<<<grid1(nnodes + 1), 256, 0, s>>>(lv, nnodes, max_level, nw.lvstart);
         PTG_LAUNCH_CHECK();
-        for (int L = max_level; L >= 0; L--)
-            k_moments_level<<<512, 256, 0, s>>>(n, nnodes, L, nw.lvstart, byid, t, nw, quadrupole);
+        // per-level node counts decide the launches: runs of small levels go to one block, the
+        // others get a grid each
+        std::vector<int32_t> ls(max_level + 2);
+        PTG_HIP_CHECK(hipMemcpyAsync(ls.data(), nw.lvstart, (max_level + 2) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        PTG_HIP_CHECK(hipStreamSynchronize(s));
+        const int64_t small = 512;
+        int L = max_level;
+        while (L >= 0) {
+            if (ls[L + 1] - ls[L] <= small) {
+                int Llo = L;
+                while (Llo - 1 >= 0 && ls[Llo] - ls[Llo - 1] <= small) Llo--;
+                k_moments_small<<<1, 64 * MC_WAVES, 0, s>>>(n, nnodes, L, Llo, nw.lvstart, byid, t, nw, quadrupole);
+                L = Llo - 1;
+            } else {
+                const int64_t cnt = ls[L + 1] - ls[L];
+                const unsigned blocks = (unsigned)std::min<int64_t>(4096, ptg_cdiv(cnt, MC_WAVES * 8));
+                k_moments_level<<<blocks, 64 * MC_WAVES, 0, s>>>(n, nnodes, L, nw.lvstart, byid, t, nw, quadrupole);
+                L--;
+            }
+        }
     } else {
         k_fold_particles<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t);
     }
