@@ -44,9 +44,10 @@ FLOPS_PER_INTERACTION = 20  # monopole potential + acceleration
 # 1.005e-6 + 1.124e-6 s/particle (examples/benchmark_scaling.json:51,131)
 REF_PUBLISHED = 1.0 / (1.005e-6 + 1.124e-6)
 # Calibrated HBM bytes per walk launch from two rocprofv3 --pmc passes (FETCH_SIZE x2.000,
-# WRITE_SIZE x1.000, factors from a 2048 MiB device copy); see
-# profiles/k_walk_plummer1e7_hbm_traffic.md.  Keyed by (N, group_size, criterion, theta, quadrupole).
-WALK_TRAFFIC = {(10_000_000, 64, "theta", 0.7, False): 8.329e9}
+# WRITE_SIZE x1.000, factors from a 2048 MiB device copy); see profiles/k_walk_*_hbm_traffic.md.
+# Keyed by (N, group_size, criterion, theta, quadrupole).
+WALK_TRAFFIC = {(10_000_000, 64, "theta", 0.7, False): 8.329e9,
+                (10_000_000, 64, "learned", 2.0, False): 5.545e9}
 
 
 def parse():
