@@ -103,9 +103,13 @@ size_t ptg_build_workspace_bytes(int64_t n, int64_t nsys);
 ptg_status ptg_build_sort(const double *pos, const double *m, const double *h, int64_t n, const ptg_forest *forest,
                           void *ws, size_t ws_bytes, ptg_build_info *info, void *stream);
 size_t ptg_node_workspace_bytes(int64_t n, int64_t nnodes, int max_level);
+/* particles_ready (nullable hipEvent_t): recorded on the stream as soon as the particle slots
+ * (coordinates, masses, softenings, treewalk_indices, keys) are written, so a caller can start
+ * work that needs only those (target groups) on another stream while nodes and moments are built. */
 ptg_status ptg_build_tree(const double *pos, const double *m, const double *h, const ptg_build_info *info,
                           const ptg_forest *forest, int quadrupole, int compute_moments, void *ws, size_t ws_bytes,
-                          void *node_ws, size_t node_ws_bytes, const ptg_tree_arrays *tree, void *stream);
+                          void *node_ws, size_t node_ws_bytes, const ptg_tree_arrays *tree, void *particles_ready,
+                          void *stream);
 
 /* Morton permutation of arbitrary points: replaces grouped_treewalk._morton_order (:269-275).
  * Uses a ptg_build_workspace_bytes(n, 1) workspace; asynchronous. */

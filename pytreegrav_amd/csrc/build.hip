@@ -1482,7 +1482,7 @@ __global__ __launch_bounds__(256) void k_forest_roots(const int64_t *soff, int64
 extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const double *h, const ptg_build_info *info,
                                      const ptg_forest *forest, int quadrupole, int compute_moments, void *ws,
                                      size_t ws_bytes, void *node_ws, size_t node_ws_bytes,
-                                     const ptg_tree_arrays *tree, void *stream)
+                                     const ptg_tree_arrays *tree, void *particles_ready, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!pos || !m || !h || !info || !tree || info->n < 1) {
@@ -1518,6 +1518,7 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     if (F) w.soff = forest->offsets;
     KeyView kv{w.kA, info->has_collisions ? w.ext : nullptr, w.sys, n};
     k_particles<<<grid1(n), 256, 0, s>>>(pos, m, h, w.vA, w.kA, w.lev, w.off, w.sys, n, t);
+    if (particles_ready) PTG_HIP_CHECK(hipEventRecord((hipEvent_t)particles_ready, s));
     if (n == 1) {
         k_single_root<<<1, 1, 0, s>>>(w.geoms, t);
         PTG_HIP_CHECK(hipMemsetAsync(nw.lvA, 0, sizeof(uint64_t), s));

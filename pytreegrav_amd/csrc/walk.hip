@@ -10,8 +10,8 @@
 #include "ptg_common.hpp"
 
 #define WALK_WAVES 4
-#define SCAP 1024      // LDS traversal stack entries per wave
-#define SPILL 512      // entries moved to / from global memory at a time
+#define SCAP 1024      // LDS traversal stack entries per wave (>= SPILL + the 512 children one step can push)
+#define SPILL 512      // entries moved to / from global memory at a time (>= 512, the most one step pushes)
 #define GCAP 16384     // global spill entries per wave
 #define ICAP 128       // interaction-list entries per list and wave
 #define NXCD 8
@@ -24,7 +24,7 @@ template <typename T> struct alignas(4 * sizeof(T)) V4 {
 template <typename T, bool QUAD> struct WaveLds {
     int stack[SCAP];
     V4<T> nodes[ICAP];  // accepted nodes: x y z m
-    T nh[ICAP];         // node softening (used by the exact path)
+    T nh[sizeof(T) == 8 ? ICAP : 1];  // node softening (FP64 exact path only)
     T nq[QUAD ? 6 * ICAP : 1];
     V4<T> parts[ICAP];  // particles: x y z m
     T ph[ICAP];
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                 if (acc_node) {
                     const int at = nn + (int)__popcll(bn & lt);
                     L.nodes[at] = X;
-                    L.nh[at] = X2.x;
+                    if (EXACT) L.nh[at] = X2.x;
                     if (QUAD) {
                         const T *q = A.quad + 8 * (int64_t)e;
 #pragma unroll
@@ -558,7 +558,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                     __builtin_amdgcn_wave_barrier();
                     if (lane < nn - 64) {  // reads [64, nn) and writes [0, nn-64): disjoint
                         L.nodes[lane] = L.nodes[64 + lane];
-                        L.nh[lane] = L.nh[64 + lane];
+                        if (EXACT) L.nh[lane] = L.nh[64 + lane];
                         if (QUAD)
                             for (int c = 0; c < 6; c++) L.nq[6 * lane + c] = L.nq[6 * (64 + lane) + c];
                     }

@@ -132,9 +132,14 @@ class DeviceTree:
             ptr(self.first_subnode).value, ptr(self.next_branch).value, ptr(self.order).value,
             ptr(self.nchild).value, ptr(self.child_base).value, ptr(self.slot).value, ptr(self.keys).value)
         nws = workspace(L.ptg_node_workspace_bytes(n, nn, int(info.max_level)))
+        # recorded by the build once the particle slots exist (order, keys): target groups can start
+        # on a side stream while the nodes and moments are still being built
+        self.particles_ready = torch.cuda.Event()
+        self.particles_ready.record()  # creates the event; the build records it again
         check(L.ptg_build_tree(ptr(pos), ptr(m), ptr(h), _lib.ctypes.byref(info), fref, int(quadrupole),
                                int(compute_moments), ptr(ws), ws.numel(), ptr(nws), nws.numel(),
-                               _lib.ctypes.byref(self.arrays), s), "ptg_build_tree")
+                               _lib.ctypes.byref(self.arrays), _lib._vp(self.particles_ready.cuda_event), s),
+              "ptg_build_tree")
         # ws / nws go back to torch's stream-ordered caching allocator here; later allocations on
         # this stream are ordered after the build kernels that use them
         self._packs = {}
@@ -349,18 +354,25 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
     dev = device()
     main = torch.cuda.current_stream()
     side = _side_stream()
-    offsets = torch.empty(n + 1, dtype=torch.int32, device=dev)
-    count = torch.empty(1, dtype=torch.int32, device=dev)
-    gws = workspace(L.ptg_groups_workspace_bytes(n, tree.nsys))
-    side.wait_stream(main)
+    # the side stream starts once the build has written the particle slots (the group pass reads
+    # only the tree's order and keys); its buffers come from the side stream's own pool (never
+    # memory the build may still be using) and are marked as used by the main stream, which reads
+    # the groups later
+    side.wait_stream(main) if getattr(tree, "particles_ready", None) is None else side.wait_event(tree.particles_ready)
     with torch.cuda.stream(side):
+        offsets = torch.empty(n + 1, dtype=torch.int32, device=dev)
+        count = torch.empty(1, dtype=torch.int32, device=dev)
+        gws = workspace(L.ptg_groups_workspace_bytes(n, tree.nsys))
         check(L.ptg_target_groups(ptr(pos), ptr(perm), ptr(tree.keys), n, tree.forest_ref, int(group_size),
                                   _lib.PTG_GROUPS_CELLS, ptr(offsets), ptr(count), ptr(gws), gws.numel(), stream()),
               "ptg_target_groups")
+    offsets.record_stream(main)
+    count.record_stream(main)
     eps = None
     tree.pack(PTG_F32, quadrupole)
     if criterion != "theta":
-        eps = alpha * field_scale(tree)
+        main.wait_stream(side)  # the sample groups are taken from the cell groups
+        eps = alpha * field_scale(tree, cells=(offsets, count))
         if criterion == "learned":
             tree.pack_learned(model, eps, quadrupole, bf16)
     main.wait_stream(side)
@@ -369,7 +381,6 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
     out = walk(tree, pos, soft, perm, mask, quadrupole=quadrupole, group_size=group_size, theta=theta, G=G,
                precision="float32", counters=counters, groups=(offsets, count), criterion=criterion, eps=eps,
                model=model, bf16=bf16, _events_end=walk_events[1] if walk_events is not None else None)
-    del gws  # freed on the main stream, after it has waited for the side stream
     return out
 
 
@@ -390,18 +401,29 @@ def direct(pos_t, soft_t, pos_s, m_s, h_s, mask, G=1.0, self_mode=False, precisi
     return out
 
 
-def field_scale(tree, groups=64, group_size=64, theta=1.0):
+def field_scale(tree, groups=64, group_size=64, theta=1.0, cells=None):
     """RMS |acceleration| (G=1) of the tree's own particles, estimated by a theta=1 FP32 walk of
-    `groups` runs of `group_size` Morton-consecutive particles spread evenly over the Morton order
-    (compact runs, so each sample group's walk costs what an ordinary group's does); turns a relative
-    accuracy target into the absolute tolerance of the error-bounded criterion."""
+    `groups` sample groups of `group_size` Morton-consecutive particles spread evenly over the
+    Morton order; turns a relative accuracy target into the absolute tolerance of the error-bounded
+    and learned criteria.  cells: the (offsets, count) cell-aligned groups of the tree's own
+    particles, if already formed: each sample is then one of those groups (its last target repeated
+    to fill group_size), so no sample straddles two distant cells, whose bounding box would make
+    its walk open most of the tree."""
     n = tree.n
     k = min(groups, max(1, n // group_size))
-    starts = (torch.arange(k, device=device(), dtype=torch.int64) * (n // k))
-    idx = (starts[:, None] + torch.arange(min(group_size, n), device=device(), dtype=torch.int64)[None, :]).reshape(-1)
+    gs = min(group_size, n)
+    j = torch.arange(gs, device=device(), dtype=torch.int64)[None, :]
+    if cells is not None:
+        off, cnt = cells
+        g = (torch.arange(k, device=device(), dtype=torch.int64) * cnt.to(torch.int64)) // k
+        a, b = off.index_select(0, g).to(torch.int64), off.index_select(0, g + 1).to(torch.int64)
+        idx = (a[:, None] + torch.minimum(j, (b - a - 1)[:, None])).reshape(-1)
+    else:
+        starts = (torch.arange(k, device=device(), dtype=torch.int64) * (n // k))
+        idx = (starts[:, None] + j).reshape(-1)
     pos = tree.coordinates.index_select(0, idx)  # particle slots [0, n) are in Morton order
     h = tree.softenings.index_select(0, idx)
-    a = walk(tree, pos, h, torch.arange(idx.shape[0], device=device()), PTG_ACC, group_size=min(group_size, n),
+    a = walk(tree, pos, h, torch.arange(idx.shape[0], device=device()), PTG_ACC, group_size=gs,
              theta=theta, precision="float32", grouping="fixed")
     return float(torch.sqrt(torch.mean(torch.sum(a * a, dim=1))).item())
 

@@ -242,7 +242,8 @@ def test_frontend_criteria_and_self_fields_pipeline():
         print(f"[learned] frontend {crit}: accel rms {e:.3e}", flush=True)
         assert e < 3e-3 and np.all(np.isfinite(p))
         t = ConstructTree(x, m, h, quadrupole=True)
-        eps = alpha * engine.field_scale(t.dev)
+        cells = engine.target_groups(x, t.dev.order, 64, keys=t.dev.keys)
+        eps = alpha * engine.field_scale(t.dev, cells=cells)
         kw = dict(model=learned.OpeningMLP.load()) if crit == "learned" else {}
         b = engine.walk(t.dev, x, h, t.dev.order, PTG_ACC, group_size=64, theta=2.0, precision="float32",
                         criterion=crit, eps=eps, **kw).cpu().numpy()
