@@ -568,8 +568,9 @@ __global__ __launch_bounds__(256) void k_node_features(const ptg_tree_arrays t, 
 // FP32 pack's aux[3] at the node's walk slot (0 for massless nodes and for particles).
 __global__ __launch_bounds__(256) void k_open_radii(const ptg_tree_arrays t, int64_t n, int64_t nnodes, float *aux,
                                                     int64_t nslots, const float *__restrict__ P, float margin,
-                                                    double eps, int bf16, float rho_max)
+                                                    double eps, const double *scale, int bf16, float rho_max)
 {
+    if (scale) eps *= *scale;
     __shared__ Bf16Params Q;
     if (bf16) {
         load_params_bf16(P, Q);
@@ -591,9 +592,42 @@ __global__ __launch_bounds__(256) void k_open_radii(const ptg_tree_arrays t, int
 }
 
 
+// root mean square over rows of the vector norm of columns [col, col + ncols) of an (rows, stride)
+// array, one block, fixed summation order
+__global__ __launch_bounds__(256) void k_rms_rows(const double *x, int64_t rows, int stride, int col, int ncols,
+                                                  double *out)
+{
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int64_t r = threadIdx.x; r < rows; r += blockDim.x)
+        for (int c = 0; c < ncols; c++) {
+            const double v = x[r * stride + col + c];
+            acc += v * v;
+        }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = rows > 0 ? sqrt(sh[0] / (double)rows) : 0.0;
+}
+
 // ============================================================================================
 // C ABI
 // ============================================================================================
+extern "C" ptg_status ptg_rms_rows(const double *x, int64_t rows, int stride, int col, int ncols, double *out,
+                                   void *stream)
+{
+    if (!x || !out || rows < 0 || stride < 1 || col < 0 || ncols < 1 || col + ncols > stride) {
+        ptg_set_error("ptg_rms_rows: bad arguments");
+        return PTG_ERR_ARG;
+    }
+    k_rms_rows<<<1, 256, 0, (hipStream_t)stream>>>(x, rows, stride, col, ncols, out);
+    PTG_LAUNCH_CHECK();
+    return PTG_OK;
+}
+
 extern "C" ptg_status ptg_node_features(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const int32_t *nodes,
                                         int64_t count, double eps, float *features, void *stream)
 {
@@ -704,7 +738,8 @@ extern "C" ptg_status ptg_mlp_forward(const float *params, const float *x, int64
 }
 
 extern "C" ptg_status ptg_open_radii(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_walk_tree *wt,
-                                     const float *params, float margin, double eps, int bf16, void *stream)
+                                     const float *params, float margin, double eps, const double *eps_scale, int bf16,
+                                     void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!tree || !wt || !params || n < 1 || nnodes < 1 || wt->nslots != n + nnodes || !wt->aux || !(eps > 0) ||
@@ -717,7 +752,7 @@ extern "C" ptg_status ptg_open_radii(const ptg_tree_arrays *tree, int64_t n, int
         return PTG_ERR_ARG;
     }
     k_open_radii<<<lgrid(nnodes), 256, 0, s>>>(*tree, n, nnodes, (float *)wt->aux, wt->nslots, params, margin, eps,
-                                               bf16 != 0, 1e4f);
+                                               eps_scale, bf16 != 0, 1e4f);
     PTG_LAUNCH_CHECK();
     return PTG_OK;
 }

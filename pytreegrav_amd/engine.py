@@ -161,17 +161,19 @@ class DeviceTree:
     def has_coincident(self):
         return bool(self.info.has_coincident)
 
-    def pack_learned(self, model, eps, quadrupole=False, bf16=False):
-        """FP32 walk copy whose node slots carry the learned opening radii at tolerance eps
-        (ptg_open_radii); cached per (model, eps, moment order, bf16)."""
+    def pack_learned(self, model, eps, quadrupole=False, bf16=False, scale=None):
+        """FP32 walk copy whose node slots carry the learned opening radii at tolerance eps (times
+        the device scalar `scale` if given) (ptg_open_radii); cached per (model, tolerance, moment
+        order, bf16)."""
         from .learned import open_radii
 
-        key = ("learned", model.key(), float(eps), bool(quadrupole), bool(bf16))
+        skey = None if scale is None else scale.data_ptr()
+        key = ("learned", model.key(), float(eps), skey, bool(quadrupole), bool(bf16))
         if key not in self._packs:
             # take over the plain FP32 pack if there is one (its aux[3] is overwritten, so it leaves
             # the plain cache and is rebuilt there when needed again)
             p = self._packs.pop((PTG_F32, bool(quadrupole)), None) or self.pack(PTG_F32, quadrupole, cache=False)
-            open_radii(self, p, model, eps, bf16=bf16)
+            open_radii(self, p, model, eps, bf16=bf16, scale=scale)
             self._packs[key] = p
         return self._packs[key]
 
@@ -236,7 +238,7 @@ def target_groups(pos, perm, group_size, tree=None, mode="cells", keys=None):
 
 def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7, G=1.0, precision="float64",
          counters=None, grouping=None, groups=None, criterion="theta", eps=None, model=None, bf16=False,
-         _events_end=None):
+         eps_scale=None, _events_end=None, _check=True, _ws_slot=0):
     """Grouped walk of `tree` at targets pos[perm[t]]; returns an (nt, W) float64 device tensor in the
     original target order (FieldsTarget_grouped, grouped_treewalk.py:350-397).
 
@@ -247,7 +249,7 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     dropped quadrupole term 3.5|Q|/r^4 by eps, an absolute acceleration tolerance for G=1; theta is
     then a loose cap; the tree must carry quadrupoles), or "learned" (FP32 only: accept a node only
     beyond the opening radius the MLP `model` predicts for tolerance eps; theta is the guard; bf16
-    runs the MLP with bfloat16 weights)."""
+    runs the MLP with bfloat16 weights; eps_scale: optional device float64 scalar multiplying eps)."""
     L = _lib.load()
     pos = to_dev(pos, (-1, 3))
     nt = pos.shape[0]
@@ -294,14 +296,11 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
         crit = _lib.Criterion(_lib.PTG_CRIT_RADIUS, 0, float(eps))
     elif criterion != "theta":
         raise ValueError(f"criterion must be 'theta', 'error' or 'learned', got {criterion!r}")
-    p = tree.pack_learned(model, eps, quadrupole, bf16) if criterion == "learned" else tree.pack(prec, quadrupole)
-    wsb = L.ptg_walk_workspace_bytes(nt, int(group_size))
-    key = (torch.cuda.current_device(), wsb)
-    ws = _walk_ws.get(key)
-    if ws is None:
-        ws = workspace(wsb)
-        _walk_ws.clear()
-        _walk_ws[key] = ws
+    if criterion == "learned":
+        p = tree.pack_learned(model, eps, quadrupole, bf16, scale=eps_scale)
+    else:
+        p = tree.pack(prec, quadrupole)
+    ws = _walk_workspace(L.ptg_walk_workspace_bytes(nt, int(group_size)), _ws_slot)
     check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), tree.forest_ref if forest else None, ptr(pos), ptr(soft), ptr(perm),
                      nt, int(mask), int(bool(quadrupole)),
                      int(group_size), ptr(goff), ptr(gcnt), float(theta), _lib.ctypes.byref(crit), float(G),
@@ -309,8 +308,27 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
                      ptr(counters), ptr(ws), ws.numel(), stream()), "ptg_walk")
     if _events_end is not None:
         _events_end.record()
-    check(L.ptg_walk_check(ptr(ws), stream()), "ptg_walk")
+    if _check:  # synchronises; a caller that skips it calls check_walk(_ws_slot) later
+        check(L.ptg_walk_check(ptr(ws), stream()), "ptg_walk")
     return out
+
+
+def _walk_workspace(nbytes, slot=0):
+    """Walk workspace per (device, slot), reused across walks; its overflow flag belongs to the last
+    walk run on it."""
+    key = (torch.cuda.current_device(), slot)
+    ws = _walk_ws.get(key)
+    if ws is None or ws.numel() < nbytes:
+        ws = workspace(nbytes)
+        _walk_ws[key] = ws
+    return ws
+
+
+def check_walk(slot=0):
+    """Overflow check of the last walk run on workspace `slot` (synchronises the stream)."""
+    ws = _walk_ws.get((torch.cuda.current_device(), slot))
+    if ws is not None:
+        check(_lib.load().ptg_walk_check(ptr(ws), stream()), "ptg_walk")
 
 
 _side_streams = {}
@@ -368,19 +386,24 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
               "ptg_target_groups")
     offsets.record_stream(main)
     count.record_stream(main)
-    eps = None
+    eps, scale = None, None
     tree.pack(PTG_F32, quadrupole)
     if criterion != "theta":
         main.wait_stream(side)  # the sample groups are taken from the cell groups
-        eps = alpha * field_scale(tree, cells=(offsets, count))
-        if criterion == "learned":
-            tree.pack_learned(model, eps, quadrupole, bf16)
+        if criterion == "learned":  # the tolerance stays on the device: no host round trip
+            eps, scale = alpha, field_scale(tree, cells=(offsets, count), on_device=True)
+            tree.pack_learned(model, eps, quadrupole, bf16, scale=scale)
+        else:
+            eps = alpha * field_scale(tree, cells=(offsets, count))
     main.wait_stream(side)
     if walk_events is not None:
         walk_events[0].record()
     out = walk(tree, pos, soft, perm, mask, quadrupole=quadrupole, group_size=group_size, theta=theta, G=G,
                precision="float32", counters=counters, groups=(offsets, count), criterion=criterion, eps=eps,
-               model=model, bf16=bf16, _events_end=walk_events[1] if walk_events is not None else None)
+               model=model, bf16=bf16, eps_scale=scale,
+               _events_end=walk_events[1] if walk_events is not None else None)
+    if scale is not None:
+        check_walk(1)  # the field-scale sample walk was not checked when it ran
     return out
 
 
@@ -401,14 +424,15 @@ def direct(pos_t, soft_t, pos_s, m_s, h_s, mask, G=1.0, self_mode=False, precisi
     return out
 
 
-def field_scale(tree, groups=64, group_size=64, theta=1.0, cells=None):
+def field_scale(tree, groups=64, group_size=64, theta=1.0, cells=None, on_device=False):
     """RMS |acceleration| (G=1) of the tree's own particles, estimated by a theta=1 FP32 walk of
     `groups` sample groups of `group_size` Morton-consecutive particles spread evenly over the
     Morton order; turns a relative accuracy target into the absolute tolerance of the error-bounded
     and learned criteria.  cells: the (offsets, count) cell-aligned groups of the tree's own
     particles, if already formed: each sample is then one of those groups (its last target repeated
     to fill group_size), so no sample straddles two distant cells, whose bounding box would make
-    its walk open most of the tree."""
+    its walk open most of the tree.  on_device: return a device float64 scalar instead of a float
+    (no host synchronisation)."""
     n = tree.n
     k = min(groups, max(1, n // group_size))
     gs = min(group_size, n)
@@ -424,8 +448,10 @@ def field_scale(tree, groups=64, group_size=64, theta=1.0, cells=None):
     pos = tree.coordinates.index_select(0, idx)  # particle slots [0, n) are in Morton order
     h = tree.softenings.index_select(0, idx)
     a = walk(tree, pos, h, torch.arange(idx.shape[0], device=device()), PTG_ACC, group_size=gs,
-             theta=theta, precision="float32", grouping="fixed")
-    return float(torch.sqrt(torch.mean(torch.sum(a * a, dim=1))).item())
+             theta=theta, precision="float32", grouping="fixed", _check=not on_device, _ws_slot=1)
+    rms = torch.empty(1, dtype=torch.float64, device=device())
+    check(_lib.load().ptg_rms_rows(ptr(a), a.shape[0], 3, 0, 3, ptr(rms), stream()), "ptg_rms_rows")
+    return rms if on_device else float(rms.item())
 
 
 # ---------------------------------------------------------------------------------------------
