@@ -28,13 +28,13 @@ class Forest:
     "float64": the reference's FP64 branch set with runs of group_size targets per system."""
 
     def __init__(self, pos, m, h, offsets, quadrupole=False, precision="float32", group_size=None):
-        self.pos = to_dev(pos, (-1, 3))
-        n = self.pos.shape[0]
-        self.m = to_dev(m, (n,))
-        self.h = to_dev(h, (n,))
+        n = len(pos)
         off = np.asarray(offsets.cpu() if isinstance(offsets, torch.Tensor) else offsets, dtype=np.int64)
         if off.ndim != 1 or off.size < 2 or off[0] != 0 or off[-1] != n or np.any(np.diff(off) < 2):
             raise ValueError("offsets must rise from 0 to n with at least two particles per system")
+        self.pos = to_dev(pos, (-1, 3))
+        self.m = to_dev(m, (n,))
+        self.h = to_dev(h, (n,))
         self.offsets = off
         self.n, self.nsys = n, off.size - 1
         self.quadrupole = bool(quadrupole)

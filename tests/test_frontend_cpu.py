@@ -91,3 +91,43 @@ def test_coincident_warning_messages():
         ptg.warn_if_coincident_positions(T(), np.full(3, 0.1))
     with pytest.warns(UserWarning, match="singular or garbage"):
         ptg.warn_if_coincident_positions(T(), np.zeros(3))
+
+
+@pytest.mark.parametrize("fn", [ptg.Potential, ptg.Accel])
+def test_criterion_validation(fn):
+    x, m, h = _x()
+    with pytest.raises(ValueError, match="criterion must be"):
+        fn(x, m, h, device="cuda", criterion="fmm", alpha=1e-3)
+    with pytest.raises(ValueError, match="float32 walk"):
+        fn(x, m, h, device="cpu", criterion="learned", alpha=1e-3)
+    with pytest.raises(ValueError, match="needs alpha"):
+        fn(x, m, h, device="cuda", criterion="error")
+
+
+def test_group_size_validation():
+    x, m, h = _x()
+    with pytest.raises(ValueError, match="group_size must be"):
+        ptg.Accel(x, m, h, device="cuda", group_size="fast")
+    with pytest.raises(ValueError, match="group_size must be between"):
+        ptg.Accel(x, m, h, device="cuda", group_size=512, method="tree")
+
+
+def test_forest_offsets_validation():
+    x, m, h = _x(10)
+    for off in ([0, 1, 10], [0, 5, 9], [1, 5, 10], [0, 10, 5, 10]):
+        with pytest.raises(ValueError, match="offsets must rise"):
+            ptg.Forest(x, m, h, off)
+
+
+def test_shipped_opening_model_layout():
+    import json
+    import os
+
+    from pytreegrav_amd import learned
+
+    with open(learned.DEFAULT_MODEL) as f:
+        d = json.load(f)
+    assert len(d["params"]) == 1441 and all(np.isfinite(d["params"]))
+    assert 0.0 <= d["margin"] < 0.5
+    assert d["meta"]["features"] == learned.FEATURES
+    assert os.path.getsize(learned.DEFAULT_MODEL) < 64 * 1024
