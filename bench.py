@@ -149,7 +149,7 @@ def run_batched(args):
     pos, mass, soft = (torch.from_numpy(a).to(dev) for a in (x, m, h))
     theta = args.theta if args.theta is not None else 0.7
     gs = args.group_size
-    ctr = torch.zeros(2, dtype=torch.int64, device=dev)
+    ctr = torch.zeros(3, dtype=torch.int64, device=dev)
 
     def step():
         F = Forest(pos, mass, soft, off, precision="float32", group_size=gs)
@@ -253,7 +253,7 @@ def main():
     mass = torch.from_numpy(m).to(dev)
     soft = torch.from_numpy(h).to(dev)
     mask = engine.PTG_POT | engine.PTG_ACC
-    ctr = torch.zeros(2, dtype=torch.int64, device=dev)
+    ctr = torch.zeros(3, dtype=torch.int64, device=dev)
 
     exact = None
     sel = None
@@ -297,7 +297,8 @@ def main():
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / args.steps
         walk_ms = float(np.mean([a.elapsed_time(b) for a, b in walk_times]))
-        r = {"dt": dt, "walk_ms": walk_ms, "interactions": int(ctr[0].item()), "visits": int(ctr[1].item())}
+        r = {"dt": dt, "walk_ms": walk_ms, "interactions": int(ctr[0].item()), "visits": int(ctr[1].item()),
+             "particle_interactions": int(ctr[2].item())}
         # tree build alone (same inputs), for the breakdown
         torch.cuda.synchronize()
         tb0 = time.perf_counter()
@@ -342,6 +343,7 @@ def main():
                  "step_speedup_of_headline": p["dt"] / main["dt"]}
     dt, walk_ms, build_ms = main["dt"], main["walk_ms"], main["build_ms"]
     interactions, visits, acc = main["interactions"], main["visits"], main["accuracy"]
+    pinter = main["particle_interactions"]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -385,6 +387,7 @@ def main():
             "walk_ms": walk_ms,
             "walk_only_particles_per_s": N / (walk_ms * 1e-3),
             "interactions_per_particle": interactions / N,
+            "near_field_particle_interactions_per_particle": pinter / N,
             "visits_per_target": visits / N,
         },
         "roofline": {

@@ -166,8 +166,9 @@ ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
  * consecutive sorted targets (1..256), or, when group_offsets/group_count (device arrays from
  * ptg_target_groups) are given, each group [offsets[g], offsets[g+1]).  exact != 0 evaluates every pair with the reference's
  * branches (r2 > 0 self skip, softened iff r < max(h_s, h_t)); exact == 0 (FP32 fast path) skips
- * those tests where the group geometry proves them false.  counters (nullable, device int64[2])
- * accumulate: [0] element-target interactions, [1] traversal visits.  With a forest (the same
+ * those tests where the group geometry proves them false.  counters (nullable, device int64[3])
+ * accumulate: [0] element-target interactions, [1] traversal visits, [2] the part of [0] taken
+ * through the particle list (near-field pairs whose softening is checked).  With a forest (the same
  * descriptor the tree was built and packed with) the targets are the forest's own particles
  * (perm = its treewalk_indices, nt = n, groups that never cross a system, e.g. from
  * ptg_target_groups with the forest or group_size-runs when every system size is a multiple of

@@ -371,7 +371,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
     const int64_t gwave = (int64_t)blockIdx.x * WALK_WAVES + wave;
     int *spill = A.spill + gwave * GCAP;
     const int xcd = blockIdx.x % NXCD;
-    int64_t n_int = 0, n_vis = 0;
+    int64_t n_int = 0, n_vis = 0, n_part = 0;
     const int64_t ngroups = A.goff ? (int64_t)__builtin_amdgcn_readfirstlane(*A.gcount) : A.ngroups;
 
     for (int attempt = 0; attempt < NXCD; attempt++) {
@@ -524,6 +524,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                 np += addp;
                 nn += addn;
                 n_int += (int64_t)(addp + addn) * mt;
+                n_part += (int64_t)addp * mt;
 
                 // push the children of opened nodes
                 const int cnt = open ? std::min(std::max(K.y, 0), 8) : 0;
@@ -610,6 +611,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
     if (lane == 0 && A.counters) {
         atomicAdd((unsigned long long *)&A.counters[0], (unsigned long long)n_int);
         atomicAdd((unsigned long long *)&A.counters[1], (unsigned long long)n_vis);
+        atomicAdd((unsigned long long *)&A.counters[2], (unsigned long long)n_part);
     }
 }
 

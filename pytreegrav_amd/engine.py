@@ -487,7 +487,7 @@ def autotune_group_size(tree, pos, soft, perm, mask, quadrupole=False, theta=0.7
         return _tuned[key]
     times, inter = {}, {}
     for gs in sizes:
-        ctr = torch.zeros(2, dtype=torch.int64, device=device())
+        ctr = torch.zeros(3, dtype=torch.int64, device=device())
         kw = dict(quadrupole=quadrupole, group_size=int(gs), theta=theta, precision=precision, grouping=grouping,
                   criterion=criterion, eps=eps, model=model, bf16=bf16)
         walk(tree, pos, soft, perm, mask, counters=ctr, **kw)  # warm-up (and interaction count)
