@@ -192,8 +192,11 @@ typedef struct {
 } ptg_criterion;
 
 size_t ptg_walk_workspace_bytes(int64_t nt, int group_size);
+/* targets_sorted != 0: pos and soft are already in sorted target order (pos[t], soft[t], e.g. the
+ * tree's own Morton-ordered coordinates and softenings for self gravity); perm then only places
+ * the outputs. */
 ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest, const double *pos, const double *soft,
-                    const int64_t *perm, int64_t nt, int fields, int quadrupole, int group_size,
+                    const int64_t *perm, int targets_sorted, int64_t nt, int fields, int quadrupole, int group_size,
                     const int32_t *group_offsets,
                     const int32_t *group_count, double theta, const ptg_criterion *criterion, double G,
                     int exact, double *out, int64_t *counters, void *ws, size_t ws_bytes, void *stream);
@@ -212,13 +215,14 @@ ptg_status ptg_walk_check(void *ws, void *stream);
 /* features[count][10] of the given node elements (indices >= n) at tolerance eps */
 ptg_status ptg_node_features(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const int32_t *nodes,
                              int64_t count, double eps, float *features, void *stream);
-/* Labels by direct summation: err[count][nradii] = worst monopole error over 32 directions at
- * distance rho_j * size from the node's COM (rho log-spaced from rho_lo to rho_hi), in units of
- * mass / size^2; rows of nodes with more than cap particles (or massless) are set to -1.  nradii
- * <= 64.  forest (nullable) gives the particle range ends of forest trees. */
+/* Labels by direct summation: err[count][nradii] = the monopole error over 32 directions at
+ * distance rho_j * size from the node's COM (rho log-spaced from rho_lo to rho_hi), reduced over
+ * the directions by its maximum (stat 0) or root mean square (stat 1), in units of mass / size^2;
+ * rows of nodes with more than cap particles (or massless) are set to -1.  nradii <= 64.  forest
+ * (nullable) gives the particle range ends of forest trees. */
 ptg_status ptg_node_errcurve(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_forest *forest,
                              const int32_t *nodes, int64_t count, int64_t cap, int nradii, double rho_lo,
-                             double rho_hi, double *err, void *stream);
+                             double rho_hi, int stat, double *err, void *stream);
 /* Training samples: sample i = (curve row rows[i] of node nodes[rows[i]], tolerance eps[i]) ->
  * x[i][10], label y[i] = log10 of the smallest distance (in sizes) beyond which the curve stays
  * within eps, valid[i] (0 skipped, 1 ok, 2 label clipped at rho_hi). */

@@ -93,12 +93,13 @@ SIGNATURES = {
     "ptg_morton_order": (_c.c_int, [_vp, _i64, _vp, _vp, _c.c_size_t, _vp]),
     "ptg_pack_walk": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _F, _c.POINTER(WalkTree), _vp]),
     "ptg_walk_workspace_bytes": (_c.c_size_t, [_i64, _c.c_int]),
-    "ptg_walk": (_c.c_int, [_c.POINTER(WalkTree), _F, _vp, _vp, _vp, _i64, _c.c_int, _c.c_int, _c.c_int, _vp, _vp,
+    "ptg_walk": (_c.c_int, [_c.POINTER(WalkTree), _F, _vp, _vp, _vp, _c.c_int, _i64, _c.c_int, _c.c_int, _c.c_int,
+                            _vp, _vp,
                             _dbl, _c.POINTER(Criterion), _dbl, _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),
     "ptg_walk_check": (_c.c_int, [_vp, _vp]),
     "ptg_node_features": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _vp, _i64, _dbl, _vp, _vp]),
     "ptg_node_errcurve": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _F, _vp, _i64, _i64, _c.c_int, _dbl, _dbl,
-                                     _vp, _vp]),
+                                     _c.c_int, _vp, _vp]),
     "ptg_mlp_samples": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _vp, _vp, _vp, _i64, _vp, _c.c_int, _dbl,
                                    _dbl, _vp, _vp, _vp, _vp]),
     "ptg_mlp_grad_workspace_bytes": (_c.c_size_t, [_i64]),

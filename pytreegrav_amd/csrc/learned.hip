@@ -256,7 +256,7 @@ struct EcForest {
 // particles, massless or malformed nodes).
 __global__ __launch_bounds__(EC_THREADS) void k_errcurve(const ptg_tree_arrays t, int64_t n, int64_t ne, EcForest fo,
                                                          const int32_t *nodes, int64_t count, int64_t cap, int nr,
-                                                         double rho_lo, double rho_hi, double *err)
+                                                         double rho_lo, double rho_hi, int stat, double *err)
 {
     __shared__ double px[EC_THREADS], py[EC_THREADS], pz[EC_THREADS], pm[EC_THREADS];
     __shared__ double emax[EC_DIRS * EC_MAXR];
@@ -353,7 +353,12 @@ __global__ __launch_bounds__(EC_THREADS) void k_errcurve(const ptg_tree_arrays t
     __syncthreads();
     for (int j = threadIdx.x; j < nr; j += blockDim.x) {
         double m = 0.0;
-        for (int d = 0; d < EC_DIRS; d++) m = fmax(m, emax[j * EC_DIRS + d]);
+        if (stat == 0) {
+            for (int d = 0; d < EC_DIRS; d++) m = fmax(m, emax[j * EC_DIRS + d]);
+        } else {
+            for (int d = 0; d < EC_DIRS; d++) m += emax[j * EC_DIRS + d] * emax[j * EC_DIRS + d];
+            m = sqrt(m / EC_DIRS);
+        }
         err[row * nr + j] = m;
     }
 }
@@ -644,12 +649,13 @@ extern "C" ptg_status ptg_node_features(const ptg_tree_arrays *tree, int64_t n, 
 
 extern "C" ptg_status ptg_node_errcurve(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
                                         const ptg_forest *forest, const int32_t *nodes, int64_t count, int64_t cap,
-                                        int nradii, double rho_lo, double rho_hi, double *err, void *stream)
+                                        int nradii, double rho_lo, double rho_hi, int stat, double *err, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!tree || n < 1 || nnodes < 1 || count < 0 || (count > 0 && (!nodes || !err)) || nradii < 2 ||
-        nradii > EC_MAXR || !(rho_lo > 0) || !(rho_hi > rho_lo) || cap < 1) {
-        ptg_set_error("ptg_node_errcurve: bad arguments (2 <= nradii <= %d, 0 < rho_lo < rho_hi, cap >= 1)", EC_MAXR);
+        nradii > EC_MAXR || !(rho_lo > 0) || !(rho_hi > rho_lo) || cap < 1 || (stat != 0 && stat != 1)) {
+        ptg_set_error("ptg_node_errcurve: bad arguments (2 <= nradii <= %d, 0 < rho_lo < rho_hi, cap >= 1, "
+                      "stat 0 or 1)", EC_MAXR);
         return PTG_ERR_ARG;
     }
     if (forest && (forest->nsys < 1 || !forest->offsets || !forest->roots)) {
@@ -661,7 +667,7 @@ extern "C" ptg_status ptg_node_errcurve(const ptg_tree_arrays *tree, int64_t n, 
     for (int64_t c0 = 0; c0 < count; c0 += 65535) {
         const int64_t c = std::min<int64_t>(65535, count - c0);
         k_errcurve<<<(unsigned)c, EC_THREADS, 0, s>>>(*tree, n, n + nnodes, fo, nodes + c0, c, cap, nradii, rho_lo,
-                                                      rho_hi, err + c0 * nradii);
+                                                      rho_hi, stat, err + c0 * nradii);
         PTG_LAUNCH_CHECK();
     }
     return PTG_OK;

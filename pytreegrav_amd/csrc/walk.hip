@@ -44,6 +44,7 @@ template <typename T> struct WalkArgs {
     const double *pos;
     const double *soft;
     const int64_t *perm;
+    int sorted;  // pos/soft are indexed by the sorted target position (perm maps the outputs only)
     int64_t nslots;
     int64_t nt;
     int64_t ngroups;
@@ -418,10 +419,11 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                 tval[j] = o < mt;
                 tidx[j] = tval[j] ? A.perm[a0 + o] : 0;
                 if (tval[j]) {
-                    tx[j] = (T)(A.pos[3 * tidx[j]] - ox);
-                    ty[j] = (T)(A.pos[3 * tidx[j] + 1] - oy);
-                    tz[j] = (T)(A.pos[3 * tidx[j] + 2] - oz);
-                    th[j] = (T)A.soft[tidx[j]];
+                    const int64_t src = A.sorted ? a0 + o : tidx[j];
+                    tx[j] = (T)(A.pos[3 * src] - ox);
+                    ty[j] = (T)(A.pos[3 * src + 1] - oy);
+                    tz[j] = (T)(A.pos[3 * src + 2] - oz);
+                    th[j] = (T)A.soft[src];
                     bmin0 = std::min(bmin0, tx[j]);
                     bmin1 = std::min(bmin1, ty[j]);
                     bmin2 = std::min(bmin2, tz[j]);
@@ -691,7 +693,8 @@ static ptg_status launch_walk(const WalkArgs<T> &a, int fields, bool quad, int64
 }
 
 extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest, const double *pos,
-                               const double *soft, const int64_t *perm, int64_t nt, int fields, int quadrupole,
+                               const double *soft, const int64_t *perm, int targets_sorted, int64_t nt, int fields,
+                               int quadrupole,
                                int group_size, const int32_t *group_offsets, const int32_t *group_count, double theta,
                                const ptg_criterion *criterion, double G, int exact, double *out, int64_t *counters,
                                void *ws, size_t ws_bytes, void *stream)
@@ -760,7 +763,7 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest
     ptg_status st;
     if (wt->prec == PTG_F32) {
         WalkArgs<float> a{(const V4<float> *)wt->xyzm, (const V4<float> *)wt->aux, (const int2 *)wt->kids,
-                          (const float *)wt->quad, pos, soft, perm, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
+                          (const float *)wt->quad, pos, soft, perm, targets_sorted != 0, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
                           group_size, (float)theta, crit, (float)eps, G, wt->origin[0], wt->origin[1], wt->origin[2],
                           fn, fo, fc, out, counters, ctl, spill};
         if (exact) {
@@ -770,7 +773,7 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest
         st = launch_walk<float, false>(a, fields, quadrupole != 0, blocks, s);
     } else if (wt->prec == PTG_F64) {
         WalkArgs<double> a{(const V4<double> *)wt->xyzm, (const V4<double> *)wt->aux, (const int2 *)wt->kids,
-                           (const double *)wt->quad, pos, soft, perm, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
+                           (const double *)wt->quad, pos, soft, perm, targets_sorted != 0, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
                            group_size, theta, crit, eps, G, wt->origin[0], wt->origin[1], wt->origin[2],
                            fn, fo, fc, out, counters, ctl, spill};
         st = launch_walk<double, true>(a, fields, quadrupole != 0, blocks, s);

@@ -238,7 +238,7 @@ def target_groups(pos, perm, group_size, tree=None, mode="cells", keys=None):
 
 def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7, G=1.0, precision="float64",
          counters=None, grouping=None, groups=None, criterion="theta", eps=None, model=None, bf16=False,
-         eps_scale=None, _events_end=None, _check=True, _ws_slot=0):
+         eps_scale=None, sorted_targets=False, _events_end=None, _check=True, _ws_slot=0):
     """Grouped walk of `tree` at targets pos[perm[t]]; returns an (nt, W) float64 device tensor in the
     original target order (FieldsTarget_grouped, grouped_treewalk.py:350-397).
 
@@ -249,7 +249,9 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     dropped quadrupole term 3.5|Q|/r^4 by eps, an absolute acceleration tolerance for G=1; theta is
     then a loose cap; the tree must carry quadrupoles), or "learned" (FP32 only: accept a node only
     beyond the opening radius the MLP `model` predicts for tolerance eps; theta is the guard; bf16
-    runs the MLP with bfloat16 weights; eps_scale: optional device float64 scalar multiplying eps)."""
+    runs the MLP with bfloat16 weights; eps_scale: optional device float64 scalar multiplying eps).
+    sorted_targets: pos and soft are already in the sorted order (pos[t] is target perm[t]), as the
+    tree's own coordinates and softenings are for self gravity; perm then only places the output."""
     L = _lib.load()
     pos = to_dev(pos, (-1, 3))
     nt = pos.shape[0]
@@ -302,7 +304,7 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
         p = tree.pack(prec, quadrupole)
     ws = _walk_workspace(L.ptg_walk_workspace_bytes(nt, int(group_size)), _ws_slot)
     check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), tree.forest_ref if forest else None, ptr(pos), ptr(soft), ptr(perm),
-                     nt, int(mask), int(bool(quadrupole)),
+                     int(bool(sorted_targets)), nt, int(mask), int(bool(quadrupole)),
                      int(group_size), ptr(goff), ptr(gcnt), float(theta), _lib.ctypes.byref(crit), float(G),
                      int(prec == PTG_F64), ptr(out),
                      ptr(counters), ptr(ws), ws.numel(), stream()), "ptg_walk")
@@ -398,9 +400,12 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
     main.wait_stream(side)
     if walk_events is not None:
         walk_events[0].record()
-    out = walk(tree, pos, soft, perm, mask, quadrupole=quadrupole, group_size=group_size, theta=theta, G=G,
-               precision="float32", counters=counters, groups=(offsets, count), criterion=criterion, eps=eps,
-               model=model, bf16=bf16, eps_scale=scale,
+    # targets read from the tree's Morton-ordered copies (the same doubles as pos[perm]) instead of
+    # gathering pos through the permutation
+    out = walk(tree, tree.coordinates[:n], tree.softenings[:n], perm, mask, quadrupole=quadrupole,
+               group_size=group_size, theta=theta, G=G, precision="float32", counters=counters,
+               groups=(offsets, count), criterion=criterion, eps=eps, model=model, bf16=bf16, eps_scale=scale,
+               sorted_targets=True,
                _events_end=walk_events[1] if walk_events is not None else None)
     if scale is not None:
         check_walk(1)  # the field-scale sample walk was not checked when it ran

@@ -53,7 +53,9 @@ class Forest:
         check(L.ptg_target_groups(ptr(self.pos), ptr(t.order), ptr(t.keys), self.n, t.forest_ref, self.group_size,
                                   mode, ptr(goff), ptr(gcnt), ptr(gws), gws.numel(), stream()), "ptg_target_groups")
         crit = _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
-        check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), t.forest_ref, ptr(self.pos), ptr(self.h), ptr(t.order), self.n,
+        # targets read from the tree's Morton-ordered coordinates and softenings (sorted order)
+        check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), t.forest_ref, ptr(t.coordinates), ptr(t.softenings),
+                         ptr(t.order), 1, self.n,
                          int(mask), int(self.quadrupole), self.group_size, ptr(goff), ptr(gcnt), float(theta),
                          _lib.ctypes.byref(crit), float(G), int(self.prec == PTG_F64), ptr(out), ptr(counters),
                          ptr(ws), ws.numel(), stream()), "ptg_walk")

@@ -93,14 +93,15 @@ def node_features(tree, nodes, eps):
     return x
 
 
-def error_curves(tree, nodes, cap=4096, nradii=NRADII, rho_lo=RHO_LO, rho_hi=RHO_HI):
-    """(m, nradii) float64 worst monopole error over 32 directions at rho_j * size (units M/size^2);
-    rows of skipped nodes are -1."""
+def error_curves(tree, nodes, cap=4096, nradii=NRADII, rho_lo=RHO_LO, rho_hi=RHO_HI, stat="max"):
+    """(m, nradii) float64 monopole error over 32 directions at rho_j * size (units M/size^2),
+    their maximum (stat "max") or root mean square ("rms"); rows of skipped nodes are -1."""
     nodes = torch.as_tensor(nodes, dtype=torch.int32, device=device()).contiguous()
     err = torch.empty((nodes.numel(), nradii), dtype=torch.float64, device=device())
     check(_lib.load().ptg_node_errcurve(_lib.ctypes.byref(tree.arrays), tree.n, tree.nnodes, tree.forest_ref,
                                         ptr(nodes), nodes.numel(), int(cap), int(nradii), float(rho_lo),
-                                        float(rho_hi), ptr(err), stream()), "ptg_node_errcurve")
+                                        float(rho_hi), {"max": 0, "rms": 1}[stat], ptr(err), stream()),
+          "ptg_node_errcurve")
     return err
 
 
@@ -165,14 +166,14 @@ def open_radii(tree, pack, model, eps, bf16=False, scale=None):
                                      float(eps), ptr(scale), int(bool(bf16)), stream()), "ptg_open_radii")
 
 
-def training_set(tree, alphas, nodes_per_tree=20000, cap=4096, seed=0, fs=None):
+def training_set(tree, alphas, nodes_per_tree=20000, cap=4096, seed=0, fs=None, stat="max"):
     """Samples from one tree: `nodes_per_tree` random nodes, each paired with every tolerance
     alpha * (RMS field of the tree's particles).  Returns (x, y, valid) device tensors."""
     rng = np.random.default_rng(seed)
     k = min(nodes_per_tree, tree.nnodes)
     nodes = np.sort(rng.choice(tree.nnodes, size=k, replace=False)).astype(np.int32) + tree.n
     nodes_d = torch.from_numpy(nodes).to(device())
-    err = error_curves(tree, nodes_d, cap=cap)
+    err = error_curves(tree, nodes_d, cap=cap, stat=stat)
     fs = field_scale(tree) if fs is None else fs
     rows = np.repeat(np.arange(k, dtype=np.int32), len(alphas))
     eps = np.tile(np.asarray(alphas, dtype=np.float64) * fs, k)
@@ -197,13 +198,16 @@ def main(argv=None):
     ap.add_argument("--under", type=float, default=4.0)
     ap.add_argument("--cap", type=int, default=8192)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--stat", choices=("max", "rms"), default="max",
+                    help="label statistic over the 32 directions: worst case or root mean square")
     args = ap.parse_args(argv)
     alphas = np.geomspace(1e-4, 1e-2, 7)
     t0 = time.time()
     X, Y, V = [], [], []
     for i, (x, m, h) in enumerate(_datasets(args.N, args.seed)):
         tree = DeviceTree(x, m, h, quadrupole=True)
-        xs, ys, vs = training_set(tree, alphas, nodes_per_tree=args.nodes, cap=args.cap, seed=args.seed + i)
+        xs, ys, vs = training_set(tree, alphas, nodes_per_tree=args.nodes, cap=args.cap, seed=args.seed + i,
+                                  stat=args.stat)
         X.append(xs)
         Y.append(ys)
         V.append(vs)
@@ -228,7 +232,7 @@ def main(argv=None):
     model.margin = margin
     model.meta = {"train_seconds": round(time.time() - t1, 2), "samples": int(X.shape[0]), "N": args.N,
                   "alphas": [float(a) for a in alphas], "steps": args.steps, "batch": args.batch,
-                  "under_weight": args.under, "final_loss": float(losses[-1]),
+                  "under_weight": args.under, "label_statistic": args.stat, "final_loss": float(losses[-1]),
                   "val_rmse": float(np.sqrt(np.mean(res ** 2))), "val_under_frac_at_margin":
                   float(np.mean(res + margin < 0)), "features": FEATURES,
                   "data": "Plummer + 64-component Gaussian mixture, uniform node sample, h=0.01"}
