@@ -47,7 +47,7 @@ REF_PUBLISHED = 1.0 / (1.005e-6 + 1.124e-6)
 # WRITE_SIZE x1.000, factors from a 2048 MiB device copy); see profiles/k_walk_*_hbm_traffic.md.
 # Keyed by (N, group_size, criterion, theta, quadrupole).
 WALK_TRAFFIC = {(10_000_000, 64, "theta", 0.7, False): 8.329e9,
-                (10_000_000, 64, "learned", 2.0, False): 5.545e9}
+                (10_000_000, 64, "learned", 2.0, False): 2.720e9}
 
 
 def parse():
