@@ -244,14 +244,17 @@ ptg_status ptg_mlp_adam(float *params, float *m, float *v, const float *grad, in
 ptg_status ptg_mlp_forward(const float *params, const float *x, int64_t count, int bf16, float *y, void *stream);
 /* Fused inference: writes size * 10^(MLP(features(node, eps)) + margin) into aux[3] of every node
  * slot of an FP32 pack from ptg_pack_walk (0 for massless nodes; particle slots keep the pack's 0),
- * for PTG_CRIT_RADIUS walks.  The tolerance is eps * (*eps_scale) when eps_scale (a device scalar,
- * e.g. the RMS field from ptg_rms_rows) is given, so it never has to visit the host. */
-ptg_status ptg_open_radii(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_walk_tree *wt,
-                          const float *params, float margin, double eps, const double *eps_scale, int bf16,
-                          void *stream);
-/* out[0] = sqrt(mean over rows r of sum_{c < ncols} x[r*stride + col + c]^2) (no reference
- * counterpart: the RMS field that scales the criteria's tolerance), one block, device scalar. */
-ptg_status ptg_rms_rows(const double *x, int64_t rows, int stride, int col, int ncols, double *out, void *stream);
+ * for PTG_CRIT_RADIUS walks.  The tolerance is eps * eps_scale[s] when eps_scale (device, e.g. the
+ * RMS field from ptg_rms_rows) is given: s = 0 for one tree, the node's system for a forest (one
+ * scale per system), so the tolerance never has to visit the host. */
+ptg_status ptg_open_radii(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_forest *forest,
+                          const ptg_walk_tree *wt, const float *params, float margin, double eps,
+                          const double *eps_scale, int bf16, void *stream);
+/* out[s] = sqrt(mean over the rows r of segment s of sum_{c < ncols} x[r*stride + col + c]^2) for
+ * nseg segments of rows / nseg consecutive rows (no reference counterpart: the RMS field, per
+ * system of a forest, that scales the criteria's tolerance); device results. */
+ptg_status ptg_rms_rows(const double *x, int64_t rows, int stride, int col, int ncols, int64_t nseg, double *out,
+                        void *stream);
 
 /* ------------------------------------------------------------------------------------------
  * Direct summation, LDS-tiled: replaces bruteforce.py.  self_mode != 0: targets are the sources

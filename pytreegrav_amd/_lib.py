@@ -107,9 +107,9 @@ SIGNATURES = {
                                 _vp]),
     "ptg_mlp_adam": (_c.c_int, [_vp, _vp, _vp, _vp, _i64, _c.c_float, _c.c_float, _c.c_float, _c.c_float, _vp]),
     "ptg_mlp_forward": (_c.c_int, [_vp, _vp, _i64, _c.c_int, _vp, _vp]),
-    "ptg_open_radii": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _c.POINTER(WalkTree), _vp, _c.c_float, _dbl,
-                                  _vp, _c.c_int, _vp]),
-    "ptg_rms_rows": (_c.c_int, [_vp, _i64, _c.c_int, _c.c_int, _c.c_int, _vp, _vp]),
+    "ptg_open_radii": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _F, _c.POINTER(WalkTree), _vp, _c.c_float,
+                                  _dbl, _vp, _c.c_int, _vp]),
+    "ptg_rms_rows": (_c.c_int, [_vp, _i64, _c.c_int, _c.c_int, _c.c_int, _i64, _vp, _vp]),
     "ptg_direct": (_c.c_int, [_vp, _vp, _i64, _vp, _vp, _vp, _i64, _c.c_int, _c.c_int, _c.c_int, _dbl, _vp, _vp]),
 }
 

@@ -1420,15 +1420,16 @@ __global__ __launch_bounds__(64 * MC_WAVES) void k_moments_level(int64_t n, int6
 // levels Lhi down to Llo (each with few nodes) in one block, a barrier between levels: all waves
 // of a workgroup share one CU's L1, so the barrier orders one level's writes before the next
 // level's reads
-__global__ __launch_bounds__(64 * MC_WAVES) void k_moments_small(int64_t n, int64_t nnodes, int Lhi, int Llo,
+#define MS_WAVES 16
+__global__ __launch_bounds__(64 * MS_WAVES) void k_moments_small(int64_t n, int64_t nnodes, int Lhi, int Llo,
                                                                  const int32_t *lvstart, const uint32_t *byid,
                                                                  const ptg_tree_arrays t, NodeWS w, int quad)
 {
-    __shared__ CoopLds S[MC_WAVES];
+    __shared__ CoopLds S[MS_WAVES];
     const int wave = threadIdx.x >> 6, j = ptg_lane() >> 3;
     for (int L = Lhi; L >= Llo; L--) {
         const int64_t lo = lvstart[L], hi = lvstart[L + 1];
-        for (int64_t b = lo + (int64_t)wave * 8; b < hi; b += MC_WAVES * 8) {
+        for (int64_t b = lo + (int64_t)wave * 8; b < hi; b += MS_WAVES * 8) {
             const int64_t qi = b + j;
             coop_moments(qi, qi < hi, n, nnodes, byid, t, w, quad != 0, S[wave]);
         }
@@ -1561,7 +1562,7 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
             if (ls[L + 1] - ls[L] <= small) {
                 int Llo = L;
                 while (Llo - 1 >= 0 && ls[Llo] - ls[Llo - 1] <= small) Llo--;
-                k_moments_small<<<1, 64 * MC_WAVES, 0, s>>>(n, nnodes, L, Llo, nw.lvstart, byid, t, nw, quadrupole);
+                k_moments_small<<<1, 64 * MS_WAVES, 0, s>>>(n, nnodes, L, Llo, nw.lvstart, byid, t, nw, quadrupole);
                 L = Llo - 1;
             } else {
                 const int64_t cnt = ls[L + 1] - ls[L];

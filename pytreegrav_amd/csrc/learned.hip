@@ -573,9 +573,9 @@ __global__ __launch_bounds__(256) void k_node_features(const ptg_tree_arrays t, 
 // FP32 pack's aux[3] at the node's walk slot (0 for massless nodes and for particles).
 __global__ __launch_bounds__(256) void k_open_radii(const ptg_tree_arrays t, int64_t n, int64_t nnodes, float *aux,
                                                     int64_t nslots, const float *__restrict__ P, float margin,
-                                                    double eps, const double *scale, int bf16, float rho_max)
+                                                    double eps, const double *scale, EcForest fo, int bf16,
+                                                    float rho_max)
 {
-    if (scale) eps *= *scale;
     __shared__ Bf16Params Q;
     if (bf16) {
         load_params_bf16(P, Q);
@@ -586,6 +586,19 @@ __global__ __launch_bounds__(256) void k_open_radii(const ptg_tree_arrays t, int
     const int64_t e = n + a;
     const int64_t sl = t.slot[e];
     if (sl < 0 || sl >= nslots) return;
+    if (scale) {  // one scale, or one per system of a forest
+        int64_t sy = 0;
+        if (fo.nsys > 1) {
+            int64_t lo = 0, hi = fo.nsys;
+            while (hi - lo > 1) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (fo.roots[mid] <= e) lo = mid;
+                else hi = mid;
+            }
+            sy = lo;
+        }
+        eps *= scale[sy];
+    }
     float f[MLP_IN];
     float radius = 0.f;
     if (node_features(t, n + nnodes, e, eps, f)) {
@@ -598,11 +611,13 @@ __global__ __launch_bounds__(256) void k_open_radii(const ptg_tree_arrays t, int
 
 
 // root mean square over rows of the vector norm of columns [col, col + ncols) of an (rows, stride)
-// array, one block, fixed summation order
+// array; block b reduces segment b of rows / gridDim.x consecutive rows; fixed summation order
 __global__ __launch_bounds__(256) void k_rms_rows(const double *x, int64_t rows, int stride, int col, int ncols,
                                                   double *out)
 {
     __shared__ double sh[256];
+    x += (int64_t)blockIdx.x * rows * stride;
+    out += blockIdx.x;
     double acc = 0.0;
     for (int64_t r = threadIdx.x; r < rows; r += blockDim.x)
         for (int c = 0; c < ncols; c++) {
@@ -621,14 +636,15 @@ __global__ __launch_bounds__(256) void k_rms_rows(const double *x, int64_t rows,
 // ============================================================================================
 // C ABI
 // ============================================================================================
-extern "C" ptg_status ptg_rms_rows(const double *x, int64_t rows, int stride, int col, int ncols, double *out,
-                                   void *stream)
+extern "C" ptg_status ptg_rms_rows(const double *x, int64_t rows, int stride, int col, int ncols, int64_t nseg,
+                                   double *out, void *stream)
 {
-    if (!x || !out || rows < 0 || stride < 1 || col < 0 || ncols < 1 || col + ncols > stride) {
-        ptg_set_error("ptg_rms_rows: bad arguments");
+    if (!x || !out || rows < 0 || stride < 1 || col < 0 || ncols < 1 || col + ncols > stride || nseg < 1 ||
+        nseg > 0x7fffffff || rows % nseg) {
+        ptg_set_error("ptg_rms_rows: bad arguments (rows must be a multiple of nseg)");
         return PTG_ERR_ARG;
     }
-    k_rms_rows<<<1, 256, 0, (hipStream_t)stream>>>(x, rows, stride, col, ncols, out);
+    k_rms_rows<<<(unsigned)nseg, 256, 0, (hipStream_t)stream>>>(x, rows / nseg, stride, col, ncols, out);
     PTG_LAUNCH_CHECK();
     return PTG_OK;
 }
@@ -743,9 +759,9 @@ extern "C" ptg_status ptg_mlp_forward(const float *params, const float *x, int64
     return PTG_OK;
 }
 
-extern "C" ptg_status ptg_open_radii(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_walk_tree *wt,
-                                     const float *params, float margin, double eps, const double *eps_scale, int bf16,
-                                     void *stream)
+extern "C" ptg_status ptg_open_radii(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_forest *forest,
+                                     const ptg_walk_tree *wt, const float *params, float margin, double eps,
+                                     const double *eps_scale, int bf16, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!tree || !wt || !params || n < 1 || nnodes < 1 || wt->nslots != n + nnodes || !wt->aux || !(eps > 0) ||
@@ -757,8 +773,13 @@ extern "C" ptg_status ptg_open_radii(const ptg_tree_arrays *tree, int64_t n, int
         ptg_set_error("ptg_open_radii: the learned criterion runs on the FP32 walk (prec=PTG_F32 pack)");
         return PTG_ERR_ARG;
     }
+    if (forest && (forest->nsys < 1 || (forest->nsys > 1 && !forest->roots))) {
+        ptg_set_error("ptg_open_radii: bad forest");
+        return PTG_ERR_ARG;
+    }
+    EcForest fo{forest ? forest->nsys : 1, forest ? forest->offsets : nullptr, forest ? forest->roots : nullptr};
     k_open_radii<<<lgrid(nnodes), 256, 0, s>>>(*tree, n, nnodes, (float *)wt->aux, wt->nslots, params, margin, eps,
-                                               eps_scale, bf16 != 0, 1e4f);
+                                               eps_scale, fo, bf16 != 0, 1e4f);
     PTG_LAUNCH_CHECK();
     return PTG_OK;
 }

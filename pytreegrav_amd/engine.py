@@ -174,6 +174,8 @@ class DeviceTree:
             # the plain cache and is rebuilt there when needed again)
             p = self._packs.pop((PTG_F32, bool(quadrupole)), None) or self.pack(PTG_F32, quadrupole, cache=False)
             open_radii(self, p, model, eps, bf16=bf16, scale=scale)
+            for k in [k for k in self._packs if k[0] == "learned"]:  # one learned pack per tree
+                del self._packs[k]
             self._packs[key] = p
         return self._packs[key]
 
@@ -455,7 +457,7 @@ def field_scale(tree, groups=64, group_size=64, theta=1.0, cells=None, on_device
     a = walk(tree, pos, h, torch.arange(idx.shape[0], device=device()), PTG_ACC, group_size=gs,
              theta=theta, precision="float32", grouping="fixed", _check=not on_device, _ws_slot=1)
     rms = torch.empty(1, dtype=torch.float64, device=device())
-    check(_lib.load().ptg_rms_rows(ptr(a), a.shape[0], 3, 0, 3, ptr(rms), stream()), "ptg_rms_rows")
+    check(_lib.load().ptg_rms_rows(ptr(a), a.shape[0], 3, 0, 3, 1, ptr(rms), stream()), "ptg_rms_rows")
     return rms if on_device else float(rms.item())
 
 

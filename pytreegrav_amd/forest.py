@@ -6,15 +6,19 @@ frontend.py:391-520, building one Octree each).  Here systems of uneven size are
 every system still gets its own bounding cube, Morton keys, octree, target groups and FP32 origin,
 so each system's result equals what the single-system path gives for it alone, bit for bit.  The
 walk stage (groups and walk; the pack is made once before) can be captured into a HIP graph and
-replayed.
+replayed.  The learned opening criterion is available per system: each system's tolerance is
+alpha x its own RMS field, estimated by one sample walk over all systems.
 """
+
+import time
 
 import numpy as np
 import torch
 
 from . import _lib
-from ._lib import PTG_F32, PTG_F64, check, ptr
-from .engine import DeviceTree, device, fields_mask, field_width, stream, to_dev, unpack_fields, workspace
+from ._lib import PTG_ACC, PTG_F32, PTG_F64, check, ptr
+from .engine import (TUNE_SIZES, DeviceTree, device, fields_mask, field_width, stream, to_dev, unpack_fields,
+                     workspace)
 
 
 def _prec(precision):
@@ -25,13 +29,18 @@ class Forest:
     """nsys independent systems: particles [offsets[s], offsets[s+1]) of (pos, m, h) form system s.
 
     precision "float32" (default): FP32 walk with cell-aligned groups of up to group_size targets;
-    "float64": the reference's FP64 branch set with runs of group_size targets per system."""
+    "float64": the reference's FP64 branch set with runs of group_size targets per system.
+    group_size "auto": chosen by timing the walk stage at 32/64/128/256 (autotune()).
+    criterion "learned" (FP32): the tree is built with quadrupoles for the opening-radius features."""
 
-    def __init__(self, pos, m, h, offsets, quadrupole=False, precision="float32", group_size=None):
+    def __init__(self, pos, m, h, offsets, quadrupole=False, precision="float32", group_size=None,
+                 criterion="theta"):
         n = len(pos)
         off = np.asarray(offsets.cpu() if isinstance(offsets, torch.Tensor) else offsets, dtype=np.int64)
         if off.ndim != 1 or off.size < 2 or off[0] != 0 or off[-1] != n or np.any(np.diff(off) < 2):
             raise ValueError("offsets must rise from 0 to n with at least two particles per system")
+        if criterion not in ("theta", "learned"):
+            raise ValueError(f"criterion must be 'theta' or 'learned', got {criterion!r}")
         self.pos = to_dev(pos, (-1, 3))
         self.m = to_dev(m, (n,))
         self.h = to_dev(h, (n,))
@@ -39,65 +48,169 @@ class Forest:
         self.n, self.nsys = n, off.size - 1
         self.quadrupole = bool(quadrupole)
         self.prec = _prec(precision)
-        self.group_size = int(group_size) if group_size else (64 if self.prec == PTG_F32 else 8)
-        self.tree = DeviceTree(self.pos, self.m, self.h, quadrupole=quadrupole, offsets=off)
+        if criterion == "learned" and self.prec != PTG_F32:
+            raise ValueError("criterion='learned' runs on the float32 walk")
+        self.criterion = criterion
+        self.auto = group_size == "auto"
+        self.group_size = 64 if self.auto or not group_size else int(group_size)
+        if self.prec == PTG_F64 and not group_size:
+            self.group_size = 8
+        self.tree = DeviceTree(self.pos, self.m, self.h, quadrupole=quadrupole or criterion == "learned", offsets=off)
         self._graphs = {}
+        self.tuning = None
 
-    def _stage(self, mask, theta, G, out, ws, goff, gcnt, counters=None):
-        """Enqueue pack, groups and walk (no allocation, no host synchronisation)."""
-        L = _lib.load()
+    # ---------------------------------------------------------------------------------------------
+    def _groups(self, gs, goff, gcnt, gws):
         t = self.tree
-        p = t.pack(self.prec, self.quadrupole)
         mode = _lib.PTG_GROUPS_CELLS if self.prec == PTG_F32 else _lib.PTG_GROUPS_RUNS
-        gws = self._gws
-        check(L.ptg_target_groups(ptr(self.pos), ptr(t.order), ptr(t.keys), self.n, t.forest_ref, self.group_size,
-                                  mode, ptr(goff), ptr(gcnt), ptr(gws), gws.numel(), stream()), "ptg_target_groups")
-        crit = _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
-        # targets read from the tree's Morton-ordered coordinates and softenings (sorted order)
-        check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), t.forest_ref, ptr(t.coordinates), ptr(t.softenings),
-                         ptr(t.order), 1, self.n,
-                         int(mask), int(self.quadrupole), self.group_size, ptr(goff), ptr(gcnt), float(theta),
-                         _lib.ctypes.byref(crit), float(G), int(self.prec == PTG_F64), ptr(out), ptr(counters),
-                         ptr(ws), ws.numel(), stream()), "ptg_walk")
+        check(_lib.load().ptg_target_groups(ptr(self.pos), ptr(t.order), ptr(t.keys), self.n, t.forest_ref, gs, mode,
+                                            ptr(goff), ptr(gcnt), ptr(gws), gws.numel(), stream()),
+              "ptg_target_groups")
 
-    def _buffers(self, mask):
+    def _walk(self, p, mask, theta, G, out, ws, gs, goff, gcnt, crit, counters=None):
+        t = self.tree
+        # targets read from the tree's Morton-ordered coordinates and softenings (sorted order)
+        check(_lib.load().ptg_walk(_lib.ctypes.byref(p["wt"]), t.forest_ref, ptr(t.coordinates), ptr(t.softenings),
+                                   ptr(t.order), 1, self.n, int(mask), int(self.quadrupole), gs, ptr(goff),
+                                   ptr(gcnt), float(theta), _lib.ctypes.byref(crit), float(G),
+                                   int(self.prec == PTG_F64), ptr(out), ptr(counters), ptr(ws), ws.numel(),
+                                   stream()), "ptg_walk")
+
+    def _buffers(self, mask, gs):
         L = _lib.load()
         dev = device()
-        self._gws = workspace(L.ptg_groups_workspace_bytes(self.n, self.nsys))
-        ws = workspace(L.ptg_walk_workspace_bytes(self.n, self.group_size))
+        gws = workspace(L.ptg_groups_workspace_bytes(self.n, self.nsys))
+        ws = workspace(L.ptg_walk_workspace_bytes(self.n, gs))
         out = torch.empty((self.n, field_width(mask)), dtype=torch.float64, device=dev)
         goff = torch.empty(self.n + 1, dtype=torch.int32, device=dev)
         gcnt = torch.empty(1, dtype=torch.int32, device=dev)
-        return out, ws, goff, gcnt
+        return out, ws, goff, gcnt, gws
 
-    def evaluate(self, potential=True, accel=True, tidal=False, theta=0.7, G=1.0, counters=None):
+    def system_field_scales(self, goff, gcnt, samples=8, gs=64):
+        """RMS |acceleration| (G=1) of each system, a device (nsys,) float64 array: a theta=1 FP32
+        walk of `samples` of each system's cell-aligned groups (each padded to gs targets by
+        repeating its last target), then one RMS per system (ptg_rms_rows)."""
+        t = self.tree
+        dev = device()
+        G = int(gcnt.item())
+        starts = goff[:G].to(torch.int64)
+        soff = t.sys_offsets
+        gsys = torch.searchsorted(soff, starts, right=True) - 1
+        sys_ids = torch.arange(self.nsys, device=dev, dtype=torch.int64)
+        gfirst = torch.searchsorted(gsys, sys_ids)
+        gnum = torch.bincount(gsys, minlength=self.nsys)
+        k = torch.arange(samples, device=dev, dtype=torch.int64)
+        g = gfirst[:, None] + (k[None, :] * gnum[:, None]) // samples  # (nsys, samples)
+        g = g.reshape(-1)
+        a = goff.index_select(0, g).to(torch.int64)
+        b = goff.index_select(0, g + 1).to(torch.int64)
+        j = torch.arange(gs, device=dev, dtype=torch.int64)[None, :]
+        idx = (a[:, None] + torch.minimum(j, (b - a - 1)[:, None])).reshape(-1)
+        pos = t.coordinates.index_select(0, idx)
+        h = t.softenings.index_select(0, idx)
+        nt = idx.numel()
+        # the samples form a forest of their own: system s owns sample targets [s, s+1) * samples * gs
+        soff_s = torch.arange(self.nsys + 1, device=dev, dtype=torch.int64) * (samples * gs)
+        fs = _lib.Forest(self.nsys, ptr(soff_s).value, ptr(t.roots).value, ptr(t.cubes).value)
+        goff_s = (torch.arange(self.nsys * samples + 1, device=dev, dtype=torch.int64) * gs).to(torch.int32)
+        gcnt_s = torch.full((1,), self.nsys * samples, dtype=torch.int32, device=dev)
+        out = torch.empty((nt, 3), dtype=torch.float64, device=dev)
+        L = _lib.load()
+        ws = workspace(L.ptg_walk_workspace_bytes(nt, gs))
+        p = t.pack(PTG_F32, False)
+        crit = _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
+        perm = torch.arange(nt, device=dev, dtype=torch.int64)
+        check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), _lib.ctypes.byref(fs), ptr(pos), ptr(h), ptr(perm), 1, nt,
+                         PTG_ACC, 0, gs, ptr(goff_s), ptr(gcnt_s), 1.0, _lib.ctypes.byref(crit), 1.0, 0, ptr(out),
+                         None, ptr(ws), ws.numel(), stream()), "ptg_walk")
+        check(L.ptg_walk_check(ptr(ws), stream()), "ptg_walk")
+        rms = torch.empty(self.nsys, dtype=torch.float64, device=dev)
+        check(L.ptg_rms_rows(ptr(out), nt, 3, 0, 3, self.nsys, ptr(rms), stream()), "ptg_rms_rows")
+        return rms
+
+    def _prepare(self, gs, theta, alpha, eps, model, bf16, goff, gcnt, gws):
+        """Groups, and for the learned criterion the opening radii; returns (pack, criterion, theta)."""
+        self._groups(gs, goff, gcnt, gws)
+        if self.criterion != "learned":
+            return self.tree.pack(self.prec, self.quadrupole), _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0), \
+                (0.7 if theta is None else theta)
+        if model is None:
+            from .learned import OpeningMLP
+
+            model = OpeningMLP.load()
+        if eps is not None:  # explicit per-system tolerances
+            scale = torch.as_tensor(np.broadcast_to(np.asarray(eps, dtype=np.float64), (self.nsys,)).copy()).to(device())
+            base = 1.0
+        else:
+            if alpha is None or not alpha > 0:
+                raise ValueError("criterion='learned' needs alpha > 0 (or per-system eps)")
+            scale, base = self.system_field_scales(goff, gcnt), float(alpha)
+        p = self.tree.pack_learned(model, base, self.quadrupole, bf16, scale=scale)
+        self._scale = scale  # keeps the scale alive while the pack is cached under its address
+        return p, _lib.Criterion(_lib.PTG_CRIT_RADIUS, 0, 0.0), (2.0 if theta is None else theta)
+
+    def evaluate(self, potential=True, accel=True, tidal=False, theta=None, G=1.0, counters=None, alpha=None,
+                 eps=None, model=None, bf16=False):
         """Fields of every particle from its own system, in input order: dict of device views
-        ("potential" (n,), "accel" (n,3), "tidal" (n,3,3))."""
+        ("potential" (n,), "accel" (n,3), "tidal" (n,3,3)).  theta defaults to 0.7 (geometric
+        criterion) or 2.0 (the guard of the learned criterion, whose per-system tolerance is
+        alpha x the system's RMS field, or the given per-system eps)."""
+        if self.auto and self.tuning is None:
+            self.autotune(potential, accel, tidal, theta, G, alpha=alpha, eps=eps, model=model, bf16=bf16)
+        gs = self.group_size
         mask = fields_mask(potential, accel, tidal)
-        out, ws, goff, gcnt = self._buffers(mask)
-        self.tree.pack(self.prec, self.quadrupole)
-        self._stage(mask, theta, G, out, ws, goff, gcnt, counters)
+        out, ws, goff, gcnt, gws = self._buffers(mask, gs)
+        p, crit, th = self._prepare(gs, theta, alpha, eps, model, bf16, goff, gcnt, gws)
+        self._walk(p, mask, th, G, out, ws, gs, goff, gcnt, crit, counters)
         check(_lib.load().ptg_walk_check(ptr(ws), stream()), "ptg_walk")
         return unpack_fields(out, mask)
 
-    def capture(self, potential=True, accel=True, tidal=False, theta=0.7, G=1.0):
-        """Capture the walk stage (groups + walk over all systems) into a HIP graph.  Returns
+    def autotune(self, potential=True, accel=True, tidal=False, theta=None, G=1.0, sizes=TUNE_SIZES, **kw):
+        """Pick the group size by timing the walk stage (groups + walk, HIP events, best of two) at
+        each candidate; records {"choice", "ms"} in self.tuning."""
+        mask = fields_mask(potential, accel, tidal)
+        times = {}
+        for gs in sizes:
+            out, ws, goff, gcnt, gws = self._buffers(mask, gs)
+            p, crit, th = self._prepare(gs, theta, kw.get("alpha"), kw.get("eps"), kw.get("model"),
+                                        kw.get("bf16", False), goff, gcnt, gws)
+            best = float("inf")
+            for _ in range(2):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                self._groups(gs, goff, gcnt, gws)
+                self._walk(p, mask, th, G, out, ws, gs, goff, gcnt, crit)
+                e1.record()
+                torch.cuda.synchronize()
+                best = min(best, e0.elapsed_time(e1))
+            check(_lib.load().ptg_walk_check(ptr(ws), stream()), "ptg_walk")
+            times[int(gs)] = best
+        self.group_size = min(times, key=times.get)
+        self.tuning = {"choice": self.group_size, "ms": times, "when": time.time()}
+        return self.tuning
+
+    def capture(self, potential=True, accel=True, tidal=False, theta=None, G=1.0, alpha=None, eps=None, model=None,
+                bf16=False):
+        """Capture the walk stage (groups + walk over all systems) into a HIP graph.  The pack (and,
+        for the learned criterion, the opening radii) are made once before the capture.  Returns
         (replay, out): replay() re-runs the stage and checks the traversal overflow flag; out is
         the (n, W) result buffer every replay rewrites."""
+        gs = self.group_size
         mask = fields_mask(potential, accel, tidal)
-        out, ws, goff, gcnt = self._buffers(mask)
-        self.tree.pack(self.prec, self.quadrupole)  # the pack is built once, outside the graph
+        out, ws, goff, gcnt, gws = self._buffers(mask, gs)
+        p, crit, th = self._prepare(gs, theta, alpha, eps, model, bf16, goff, gcnt, gws)
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):  # warm-up on the capture stream
-            self._stage(mask, theta, G, out, ws, goff, gcnt)
+            self._groups(gs, goff, gcnt, gws)
+            self._walk(p, mask, th, G, out, ws, gs, goff, gcnt, crit)
         torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g, stream=s):
-            self._stage(mask, theta, G, out, ws, goff, gcnt)
-        keep = (g, out, ws, goff, gcnt, self._gws)
-        self._graphs[(mask, theta, G)] = keep
+            self._groups(gs, goff, gcnt, gws)
+            self._walk(p, mask, th, G, out, ws, gs, goff, gcnt, crit)
+        self._graphs[(mask, th, G)] = (g, out, ws, goff, gcnt, gws, p)
 
         def replay():
             g.replay()
@@ -111,12 +224,13 @@ class Forest:
         return [arr[self.offsets[s]:self.offsets[s + 1]] for s in range(self.nsys)]
 
 
-def BatchedAccelPotential(pos, m, h, offsets, theta=0.7, G=1.0, quadrupole=False, precision="float32",
-                          group_size=None, return_device=False):
+def BatchedAccelPotential(pos, m, h, offsets, theta=None, G=1.0, quadrupole=False, precision="float32",
+                          group_size=None, return_device=False, criterion="theta", alpha=None):
     """Accel and potential of many independent systems in one batched build and walk: returns
     (accel (n,3), potential (n,)) in input order, each particle feeling only its own system."""
-    f = Forest(pos, m, h, offsets, quadrupole=quadrupole, precision=precision, group_size=group_size)
-    r = f.evaluate(potential=True, accel=True, theta=theta, G=G)
+    f = Forest(pos, m, h, offsets, quadrupole=quadrupole, precision=precision, group_size=group_size,
+               criterion=criterion)
+    r = f.evaluate(potential=True, accel=True, theta=theta, G=G, alpha=alpha)
     if return_device:
         return r["accel"], r["potential"]
     return r["accel"].cpu().numpy(), r["potential"].cpu().numpy()

@@ -160,8 +160,9 @@ def train(x, y, steps=4000, batch=4096, lr=3e-3, lr_end=2e-4, under=4.0, seed=0,
 
 def open_radii(tree, pack, model, eps, bf16=False, scale=None):
     """Write the learned opening radius of every node into the FP32 pack (ptg_open_radii).  The
-    tolerance is eps, or eps * scale for a device float64 scalar `scale` (no host round trip)."""
-    check(_lib.load().ptg_open_radii(_lib.ctypes.byref(tree.arrays), tree.n, tree.nnodes,
+    tolerance is eps, or eps * scale for a device float64 `scale` (no host round trip): one value,
+    or one per system of a forest tree."""
+    check(_lib.load().ptg_open_radii(_lib.ctypes.byref(tree.arrays), tree.n, tree.nnodes, tree.forest_ref,
                                      _lib.ctypes.byref(pack["wt"]), ptr(model.params), float(model.margin),
                                      float(eps), ptr(scale), int(bool(bf16)), stream()), "ptg_open_radii")
 

@@ -148,3 +148,45 @@ def test_forest_rejects_bad_offsets():
     # the library checks the device offsets itself too
     with pytest.raises(PtgError):
         engine.DeviceTree(x, m, h, offsets=np.array([0, 5000, 5001, len(x)]))
+
+
+def test_forest_learned_equals_single_learned_walks():
+    """With the same per-system tolerance, the learned-criterion forest walk equals each system's
+    single-tree learned walk bit for bit; with alpha the per-system tolerances come from one
+    sample walk and the fields match FP64 direct summation."""
+    from pytreegrav_amd import learned
+
+    systems = _systems()[:8]
+    x, m, h, off = _concat(systems)
+    model = learned.OpeningMLP.load()
+    eps = np.array([1e-3 * (k + 1) for k in range(len(systems))])
+    F = Forest(x, m, h, off, precision="float32", criterion="learned")
+    r = F.evaluate(potential=True, accel=True, eps=eps, model=model)
+    fa, fp = r["accel"].cpu().numpy(), r["potential"].cpu().numpy()
+    for s, (xs, ms, hs) in enumerate(systems):
+        u = engine.DeviceTree(xs, ms, hs, quadrupole=True)
+        o = engine.walk(u, xs, hs, u.order, PTG_POT | PTG_ACC, group_size=64, theta=2.0, precision="float32",
+                        criterion="learned", eps=float(eps[s]), model=model).cpu().numpy()
+        assert np.array_equal(fp[off[s]:off[s + 1]], o[:, 0]), s
+        assert np.array_equal(fa[off[s]:off[s + 1]], o[:, 1:4]), s
+    a = Forest(x, m, h, off, criterion="learned").evaluate(potential=False, accel=True, alpha=1.6e-3)["accel"]
+    a = a.cpu().numpy()
+    for s, (xs, ms, hs) in enumerate(systems[2:], start=2):
+        d = engine.direct(xs, hs, xs, ms, hs, PTG_ACC, self_mode=True).cpu().numpy()
+        err = np.sqrt(np.mean(np.sum((a[off[s]:off[s + 1]] - d) ** 2, 1)) / np.mean(np.sum(d * d, 1)))
+        assert err < 4e-3, (s, err)
+
+
+def test_forest_autotune_and_learned_capture():
+    systems = _systems()[:8]
+    x, m, h, off = _concat(systems)
+    F = Forest(x, m, h, off, group_size="auto", criterion="learned")
+    r = F.evaluate(potential=True, accel=True, alpha=1.6e-3)
+    assert F.tuning["choice"] in engine.TUNE_SIZES and F.group_size == F.tuning["choice"]
+    replay, out = F.capture(potential=True, accel=True, alpha=1.6e-3)
+    ref = F.evaluate(potential=True, accel=True, alpha=1.6e-3)
+    for _ in range(3):
+        out.zero_()
+        replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out[:, 0], ref["potential"]) and torch.equal(out[:, 1:4], ref["accel"])
