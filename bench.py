@@ -147,13 +147,15 @@ def run_batched(args):
     off = np.concatenate([[0], np.cumsum([len(s[0]) for s in systems])])
     N = len(x)
     pos, mass, soft = (torch.from_numpy(a).to(dev) for a in (x, m, h))
-    theta = args.theta if args.theta is not None else 0.7
+    crit = "learned" if args.criterion == "learned" else "theta"
+    theta = args.theta if args.theta is not None else (0.7 if crit == "theta" else 2.0)
+    alpha = (args.alpha or 1.6e-3) if crit == "learned" else None
     gs = args.group_size
     ctr = torch.zeros(3, dtype=torch.int64, device=dev)
 
     def step():
-        F = Forest(pos, mass, soft, off, precision="float32", group_size=gs)
-        return F, F.evaluate(potential=True, accel=True, theta=theta)
+        F = Forest(pos, mass, soft, off, precision="float32", group_size=gs, criterion=crit)
+        return F, F.evaluate(potential=True, accel=True, theta=theta, alpha=alpha)
 
     for _ in range(args.warmup):
         step()
@@ -165,9 +167,9 @@ def run_batched(args):
     dt = (time.perf_counter() - t0) / args.steps
     # walk stage alone: eager, and as a replayed HIP graph
     ctr.zero_()
-    F.evaluate(potential=True, accel=True, theta=theta, counters=ctr)
+    F.evaluate(potential=True, accel=True, theta=theta, counters=ctr, alpha=alpha)
     interactions = int(ctr[0].item())
-    replay, out = F.capture(potential=True, accel=True, theta=theta)
+    replay, out = F.capture(potential=True, accel=True, theta=theta, alpha=alpha)
     replay()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -204,8 +206,8 @@ def run_batched(args):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "batched_plummer_clusters", "systems": len(systems), "N_total": N,
-                   "N_range": [int(np.diff(off).min()), int(np.diff(off).max())], "criterion": "theta",
-                   "theta": theta, "group_size": gs, "grouping": "cell-aligned per system",
+                   "N_range": [int(np.diff(off).min()), int(np.diff(off).max())], "criterion": crit,
+                   "theta": theta, "alpha": alpha, "group_size": gs, "grouping": "cell-aligned per system",
                    "softening": "0.01 x scale radius", "scale_radius": "log-uniform 0.1-10"},
         "breakdown": {"walk_stage_graph_ms": graph_ms, "graph_replay_equals_eager": graph_equal,
                       "interactions_per_particle": interactions / N},
