@@ -9,8 +9,12 @@ in FP32 (node-relative coordinates, FP32 accumulation), results scattered back t
 The headline configuration uses the learned opening criterion: a node is accepted beyond the
 opening radius the shipped MLP predicts for a per-node tolerance of alpha x the RMS field (the RMS
 field is estimated per step by a 4096-target sample walk; the radii are computed per step by the
-fused inference pass), with the reference's geometric test kept as the guard at theta=2.  The same
-run also times the plain theta=0.7 geometric walk ("plain_theta_0.7") under the same accuracy gate.  Inputs (positions, masses, softenings) are resident in HBM
+fused inference pass), with the reference's geometric test kept as the guard at theta=2, and the
+group expansion: an accepted element whose third-order Taylor expansion about the target group's
+centre is within expansion x the RMS field is folded into that expansion (evaluated once per
+target) instead of being evaluated per target.  The defaults (alpha 5e-4, expansion 4e-3) give an
+acceleration RMS error equal to the plain theta=0.7 walk's, which the same run also times
+("plain_theta_0.7") under the same accuracy gate.  Inputs (positions, masses, softenings) are resident in HBM
 before the timed region.  Particles: Plummer sphere, M=1, a=1, truncated at 100 scale radii,
 h=0.01, seed 0.
 
@@ -40,14 +44,17 @@ sys.path.insert(0, ROOT)
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 FLOPS_PER_INTERACTION = 20  # monopole potential + acceleration
+FLOPS_PER_FOLD = 60  # one element added to a group's third-order expansion (walk.hip exp_add)
+FLOPS_PER_EXPANSION_EVAL = 100  # potential + acceleration of the expansion at one target (exp_eval)
 # tree Potential + Accel at N=1e7 Plummer, theta=0.7, 32 threads, two calls:
 # 1.005e-6 + 1.124e-6 s/particle (examples/benchmark_scaling.json:51,131)
 REF_PUBLISHED = 1.0 / (1.005e-6 + 1.124e-6)
 # Calibrated HBM bytes per walk launch from two rocprofv3 --pmc passes (FETCH_SIZE x2.000,
 # WRITE_SIZE x1.000, factors from a 2048 MiB device copy); see profiles/k_walk_*_hbm_traffic.md.
-# Keyed by (N, group_size, criterion, theta, quadrupole).
-WALK_TRAFFIC = {(10_000_000, 64, "theta", 0.7, False): 8.329e9,
-                (10_000_000, 64, "learned", 2.0, False): 2.720e9}
+# Keyed by (N, group_size, criterion, theta, quadrupole, alpha, expansion).
+WALK_TRAFFIC = {(10_000_000, 64, "theta", 0.7, False, None, None): 8.329e9,
+                (10_000_000, 64, "learned", 2.0, False, 1.6e-3, None): 2.720e9,
+                (10_000_000, 64, "learned", 2.0, False, 5e-4, 4e-3): 3.078e9}
 
 
 def parse():
@@ -65,9 +72,14 @@ def parse():
                          "radius the trained MLP predicts for that tolerance; 'theta' is the reference's "
                          "geometric test")
     ap.add_argument("--alpha", type=float, default=None,
-                    help="per-node error tolerance relative to the RMS field (default 1.6e-3 learned, 8.5e-4 error)")
+                    help="per-node error tolerance relative to the RMS field (default 5e-4 learned, 8.5e-4 error)")
     ap.add_argument("--theta", type=float, default=None,
                     help="opening angle (criterion theta) or loose geometric cap (criterion error); default 0.7 / 2.0")
+    ap.add_argument("--expansion", type=float, default=None,
+                    help="group-expansion tolerance in units of the RMS field: accepted elements far enough "
+                         "from a group's centre are folded into a third-order expansion about it (default 4e-3 "
+                         "with the learned criterion, off otherwise)")
+    ap.add_argument("--no-expansion", action="store_true", help="walk without the group expansion")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain theta=0.7 comparison line")
     ap.add_argument("--group-size", type=int, default=64)
     ap.add_argument("--softening", type=float, default=0.01)
@@ -151,7 +163,7 @@ def run_batched(args):
     theta = args.theta if args.theta is not None else (0.7 if crit == "theta" else 2.0)
     alpha = (args.alpha or 1.6e-3) if crit == "learned" else None
     gs = args.group_size
-    ctr = torch.zeros(3, dtype=torch.int64, device=dev)
+    ctr = torch.zeros(4, dtype=torch.int64, device=dev)
 
     def step():
         F = Forest(pos, mass, soft, off, precision="float32", group_size=gs, criterion=crit)
@@ -255,7 +267,7 @@ def main():
     mass = torch.from_numpy(m).to(dev)
     soft = torch.from_numpy(h).to(dev)
     mask = engine.PTG_POT | engine.PTG_ACC
-    ctr = torch.zeros(3, dtype=torch.int64, device=dev)
+    ctr = torch.zeros(4, dtype=torch.int64, device=dev)
 
     exact = None
     sel = None
@@ -272,7 +284,7 @@ def main():
 
         model = OpeningMLP.load()
 
-    def run_config(criterion, theta, alpha, quad_walk=False):
+    def run_config(criterion, theta, alpha, quad_walk=False, expansion=None):
         """Time `steps` whole evaluations (build + pack + groups + walk) of one configuration; the
         HIP events bracket the walk launches alone (k_walk_init + k_walk)."""
         build_quads = quad_walk or criterion in ("error", "learned")
@@ -288,7 +300,7 @@ def main():
             return engine.self_fields(tree.dev, pos, soft, mask, theta=theta, quadrupole=quad_walk,
                                       group_size=args.group_size, criterion=criterion,
                                       alpha=alpha if criterion != "theta" else None, model=model,
-                                      counters=ctr if record else None, walk_events=ev)
+                                      counters=ctr if record else None, walk_events=ev, expansion=expansion)
 
         for _ in range(args.warmup):
             step(False)
@@ -300,7 +312,7 @@ def main():
         dt = (time.perf_counter() - t0) / args.steps
         walk_ms = float(np.mean([a.elapsed_time(b) for a, b in walk_times]))
         r = {"dt": dt, "walk_ms": walk_ms, "interactions": int(ctr[0].item()), "visits": int(ctr[1].item()),
-             "particle_interactions": int(ctr[2].item())}
+             "particle_interactions": int(ctr[2].item()), "folds": int(ctr[3].item())}
         # tree build alone (same inputs), for the breakdown
         torch.cuda.synchronize()
         tb0 = time.perf_counter()
@@ -331,10 +343,14 @@ def main():
         return acc
 
     if args.alpha is None:
-        args.alpha = 1.6e-3 if args.criterion == "learned" else 8.5e-4
+        args.alpha = 5e-4 if args.criterion == "learned" else 8.5e-4
+    if args.no_expansion:
+        args.expansion = None
+    elif args.expansion is None and args.criterion == "learned" and not args.quadrupole:
+        args.expansion = 4e-3
     quad = bool(args.quadrupole)
     theta = args.theta if args.theta is not None else (0.7 if args.criterion == "theta" else 2.0)
-    main = run_config(args.criterion, theta, args.alpha, quad)
+    main = run_config(args.criterion, theta, args.alpha, quad, args.expansion)
     plain = None
     if not args.no_plain and not (args.criterion == "theta" and theta == 0.7 and not quad):
         p = run_config("theta", 0.7, 0.0, False)
@@ -345,13 +361,16 @@ def main():
                  "step_speedup_of_headline": p["dt"] / main["dt"]}
     dt, walk_ms, build_ms = main["dt"], main["walk_ms"], main["build_ms"]
     interactions, visits, acc = main["interactions"], main["visits"], main["accuracy"]
-    pinter = main["particle_interactions"]
+    pinter, folds = main["particle_interactions"], main["folds"]
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(x, m, h, 0.7, args.cpu_sample_frac, args.seed)
 
-    flops = FLOPS_PER_INTERACTION * interactions  # counters hold the last timed step
+    # counters hold the last timed step
+    flops = FLOPS_PER_INTERACTION * interactions
+    if args.expansion is not None:
+        flops += FLOPS_PER_FOLD * folds + FLOPS_PER_EXPANSION_EVAL * N
     achieved = flops / (walk_ms * 1e-3) / 1e12
     line = {
         "metric": "Accel+Potential particles/sec at N=1e7 Plummer, equal RMS force error vs FP64 direct",
@@ -373,6 +392,7 @@ def main():
             "softening": args.softening,
             "criterion": args.criterion,
             "theta": theta,
+            "group_expansion": args.expansion,
             "alpha": args.alpha if args.criterion in ("error", "learned") else None,
             "model": (model.meta.get("data", "") + f"; margin {model.margin:.3f}") if model is not None else None,
             "group_size": args.group_size,
@@ -390,6 +410,7 @@ def main():
             "walk_only_particles_per_s": N / (walk_ms * 1e-3),
             "interactions_per_particle": interactions / N,
             "near_field_particle_interactions_per_particle": pinter / N,
+            "folded_elements_per_group": (folds / (N / args.group_size)) if args.expansion is not None else None,
             "visits_per_target": visits / N,
         },
         "roofline": {
@@ -399,9 +420,12 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": WALK_TRAFFIC.get((N, args.group_size, args.criterion, theta, bool(args.quadrupole))),
+            "traffic": WALK_TRAFFIC.get((N, args.group_size, args.criterion, theta, bool(args.quadrupole),
+                                         args.alpha if args.criterion != "theta" else None, args.expansion)),
             "flops_per_launch": flops,
-            "convention": "20 flops per target-element interaction (monopole pot+acc)",
+            "convention": "20 flops per target-element interaction (monopole pot+acc)"
+                          + ("; 60 per element folded into a group expansion, 100 per target evaluation of it"
+                             if args.expansion is not None else ""),
         },
         "accuracy": acc,
         "plain_theta_0.7": plain,

@@ -166,9 +166,10 @@ ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
  * consecutive sorted targets (1..256), or, when group_offsets/group_count (device arrays from
  * ptg_target_groups) are given, each group [offsets[g], offsets[g+1]).  exact != 0 evaluates every pair with the reference's
  * branches (r2 > 0 self skip, softened iff r < max(h_s, h_t)); exact == 0 (FP32 fast path) skips
- * those tests where the group geometry proves them false.  counters (nullable, device int64[3])
+ * those tests where the group geometry proves them false.  counters (nullable, device int64[4])
  * accumulate: [0] element-target interactions, [1] traversal visits, [2] the part of [0] taken
- * through the particle list (near-field pairs whose softening is checked).  With a forest (the same
+ * through the particle list (near-field pairs whose softening is checked), [3] element-group pairs
+ * folded into group expansions (see ptg_criterion).  With a forest (the same
  * descriptor the tree was built and packed with) the targets are the forest's own particles
  * (perm = its treewalk_indices, nt = n, groups that never cross a system, e.g. from
  * ptg_target_groups with the forest or group_size-runs when every system size is a multiple of
@@ -185,10 +186,19 @@ ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
  * geometric test at `theta` (the guard) and r_min exceeds the opening radius stored in aux[3] of
  * its slot by ptg_open_radii (the learned criterion); eps is unused (it is baked into the radii). */
 #define PTG_CRIT_RADIUS 2
+/* group_expansion != 0 (no reference counterpart; FP32 walk, monopole interactions, potential and/or
+ * acceleration): an accepted element of mass M at distance D from the centre of a group's bounding
+ * box (R = its half diagonal, x = R / D < 1) whose truncation bound (M / D^2) x^3 (4 - 3x) / (1 - x)^2
+ * is below expansion_eps is not evaluated per target but folded into a third-order Taylor expansion
+ * of its field about the group centre, evaluated once per target after the traversal.
+ * expansion_eps is an absolute acceleration (G = 1), times eps_scale[s] (device, nullable; s = 0 for
+ * one tree, the group's system for a forest) when given. */
 typedef struct {
     int32_t criterion;
-    int32_t reserved;
+    int32_t group_expansion;
     double eps;
+    double expansion_eps;
+    const double *eps_scale;
 } ptg_criterion;
 
 size_t ptg_walk_workspace_bytes(int64_t nt, int group_size);

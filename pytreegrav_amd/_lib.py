@@ -70,7 +70,8 @@ class Forest(_c.Structure):
 
 
 class Criterion(_c.Structure):
-    _fields_ = [("criterion", _i32), ("reserved", _i32), ("eps", _dbl)]
+    _fields_ = [("criterion", _i32), ("group_expansion", _i32), ("eps", _dbl), ("expansion_eps", _dbl),
+                ("eps_scale", _c.c_void_p)]
 
 
 PTG_CRIT_THETA, PTG_CRIT_QERR, PTG_CRIT_RADIUS = 0, 1, 2

@@ -5,6 +5,12 @@
 // ballot/popcount compaction and pushes the children of opened nodes with a wave prefix sum.
 // When a list holds 64 entries the wave evaluates them against its targets, every lane reading the
 // same LDS entry (broadcast).  Persistent waves take groups from per-XCD counters.
+//
+// Group expansion (FP32 monopole walks, no reference counterpart): an accepted element far enough
+// from the group's centre that a third-order Taylor expansion of its field about that centre is
+// within the tolerance is not listed; the lane that accepted it adds its derivative tensors to the
+// lane's local-expansion coefficients, the wave sums them once per group and every target
+// evaluates the expansion at its offset from the centre.
 #include <algorithm>
 
 #include "ptg_common.hpp"
@@ -52,8 +58,11 @@ template <typename T> struct WalkArgs {
     const int32_t *gcount;  // device group count when goff is given
     int gs;
     T theta;
+    T inv_theta;  // FP32 path: 1 / theta
     int crit;  // 0: geometric (treewalk.py:22-25); 1: quadrupole error bound; 2: learned
     T eps;     // criterion 1/2: per-node acceleration error tolerance (already divided by 3.5)
+    T eps_exp;                 // group expansion: per-element tolerance of the expansion test
+    const double *eps_scale;   // nullable: eps_exp * eps_scale[system] (one scale per system)
     double G;
     double ox, oy, oz;
     // forest: group g walks the tree of the system s holding its first target (root at slot s),
@@ -322,6 +331,125 @@ __device__ __forceinline__ void interact_part_fast(Acc<T> &a, V4<T> s, T hs, T t
     }
 }
 
+// ---- group expansion ----
+// A point mass M at D = y - c from the group centre c acts on a target at c + u through
+// f(D - u) = 1/|D - u| = f - u.f1 + (1/2) u f2 u - (1/6) f3 uuu + O(|u|^4 / |D|^5) with the
+// derivative tensors of 1/r at D: f1 = -D/D^3, f2 = 3 D D / D^5 - I / D^3,
+// f3 = -15 D D D / D^7 + 3 (D I + I D + (I D)^T) / D^5.  Summed over the folded elements:
+// a0 = sum M f, g = -sum M f1, a2 = sum M f2, a3 = sum M f3; then at u
+//   phi = -a0 - u.g - (1/2) u a2 u + (1/6) a3 uuu,    acc = g + a2 u - (1/2) a3 uu.
+// a2 and a3 are trace-free (1/r is harmonic), so five and seven components are kept:
+// a2zz = -(xx + yy), a3xzz = -(xxx + xyy), a3yzz = -(xxy + yyy), a3zzz = -(xxz + yyz).
+// Error bound: the dropped terms of 1/|D - u| are |u|^l P_l(cos g) / D^(l+1), l >= 4, and the
+// gradient of the solid harmonic r^l P_l(cos) is at most l r^(l-1) long, so for |u| <= R < D the
+// acceleration misses by at most (M / D^2) sum_{l>=4} l x^(l-1) = (M / D^2) x^3 (4 - 3x) / (1 - x)^2
+// with x = R / D (the potential by (M / D) x^4 / (1 - x)); an element is folded when that is below
+// the tolerance.
+template <typename T> struct GroupExp {
+    T a0, gx, gy, gz;
+    T xx, yy, xy, xz, yz;
+    T xxx, xxy, xxz, xyy, xyz, yyy, yyz;
+};
+
+template <typename T>
+__device__ __forceinline__ void exp_add(GroupExp<T> &E, T M, T Dx, T Dy, T Dz, T rinv)
+{
+    const T rinv2 = rinv * rinv;
+    const T mr = M * rinv;
+    const T m3 = mr * rinv2;
+    E.a0 += mr;
+    E.gx += m3 * Dx;
+    E.gy += m3 * Dy;
+    E.gz += m3 * Dz;
+    const T p = T(3) * m3 * rinv2;
+    const T pDx = p * Dx, pDy = p * Dy;
+    E.xx += pDx * Dx - m3;
+    E.yy += pDy * Dy - m3;
+    E.xy += pDx * Dy;
+    E.xz += pDx * Dz;
+    E.yz += pDy * Dz;
+    const T q = T(-5) * p * rinv2;
+    const T qxx = q * Dx * Dx, qyy = q * Dy * Dy;
+    E.xxx += (qxx + T(3) * p) * Dx;
+    E.xxy += (qxx + p) * Dy;
+    E.xxz += (qxx + p) * Dz;
+    E.xyy += (qyy + p) * Dx;
+    E.xyz += q * Dx * Dy * Dz;
+    E.yyy += (qyy + T(3) * p) * Dy;
+    E.yyz += (qyy + p) * Dz;
+}
+
+__device__ __forceinline__ float wave_fsum(float v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+__device__ __forceinline__ void exp_wave_sum(GroupExp<float> &E)
+{
+    E.a0 = wave_fsum(E.a0);
+    E.gx = wave_fsum(E.gx);
+    E.gy = wave_fsum(E.gy);
+    E.gz = wave_fsum(E.gz);
+    E.xx = wave_fsum(E.xx);
+    E.yy = wave_fsum(E.yy);
+    E.xy = wave_fsum(E.xy);
+    E.xz = wave_fsum(E.xz);
+    E.yz = wave_fsum(E.yz);
+    E.xxx = wave_fsum(E.xxx);
+    E.xxy = wave_fsum(E.xxy);
+    E.xxz = wave_fsum(E.xxz);
+    E.xyy = wave_fsum(E.xyy);
+    E.xyz = wave_fsum(E.xyz);
+    E.yyy = wave_fsum(E.yyy);
+    E.yyz = wave_fsum(E.yyz);
+}
+
+template <typename T, int FIELDS>
+__device__ __forceinline__ void exp_eval(Acc<T> &a, const GroupExp<T> &E, T ux, T uy, T uz)
+{
+    const T zz = -(E.xx + E.yy);
+    const T xzz = -(E.xxx + E.xyy), yzz = -(E.xxy + E.yyy), zzz = -(E.xxz + E.yyz);
+    const T a2x = E.xx * ux + E.xy * uy + E.xz * uz;
+    const T a2y = E.xy * ux + E.yy * uy + E.yz * uz;
+    const T a2z = E.xz * ux + E.yz * uy + zz * uz;
+    const T uxx = ux * ux, uyy = uy * uy, uzz = uz * uz;
+    const T uxy = T(2) * ux * uy, uxz = T(2) * ux * uz, uyz = T(2) * uy * uz;
+    const T a3x = E.xxx * uxx + E.xyy * uyy + xzz * uzz + E.xxy * uxy + E.xxz * uxz + E.xyz * uyz;
+    const T a3y = E.xxy * uxx + E.yyy * uyy + yzz * uzz + E.xyy * uxy + E.xyz * uxz + E.yyz * uyz;
+    const T a3z = E.xxz * uxx + E.yyz * uyy + zzz * uzz + E.xyz * uxy + xzz * uxz + yzz * uyz;
+    if (NW<FIELDS>::POT)
+        a.phi += -E.a0 - (ux * E.gx + uy * E.gy + uz * E.gz) - T(0.5) * (ux * a2x + uy * a2y + uz * a2z) +
+                 T(1.0 / 6.0) * (ux * a3x + uy * a3y + uz * a3z);
+    if (NW<FIELDS>::ACC) {
+        a.gx += E.gx + a2x - T(0.5) * a3x;
+        a.gy += E.gy + a2y - T(0.5) * a3y;
+        a.gz += E.gz + a2z - T(0.5) * a3z;
+    }
+}
+
+// lanes below this one whose bit is set in m
+__device__ __forceinline__ int lanes_below(uint64_t m)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// exclusive prefix sum across the wave of v in [0, 15] (children counts), one ballot per bit;
+// tot receives the wave total
+__device__ __forceinline__ int small_excl_scan(int v, int &tot)
+{
+    int below = 0, t = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const uint64_t m = ptg_ballot((v >> b) & 1);
+        below += lanes_below(m) << b;
+        t += (int)__popcll(m) << b;
+    }
+    tot = t;
+    return below;
+}
+
 template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT>
 __device__ __forceinline__ void flush_nodes(const WaveLds<T, QUAD> &L, int cnt, Acc<T> (&acc)[TPL],
                                             const T (&tx)[TPL], const T (&ty)[TPL], const T (&tz)[TPL],
@@ -362,17 +490,18 @@ __device__ __forceinline__ void flush_parts(const WaveLds<T, QUAD> &L, int cnt, 
     }
 }
 
-template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT>
+template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT, bool EXP>
 __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
 {
+    static_assert(!EXP || (sizeof(T) == 4 && !QUAD && !EXACT && !NW<FIELDS>::TID),
+                  "the group expansion is an FP32 monopole potential/acceleration path");
     __shared__ WaveLds<T, QUAD> lds[WALK_WAVES];
     const int wave = threadIdx.x >> 6, lane = ptg_lane();
     WaveLds<T, QUAD> &L = lds[wave];
-    const uint64_t lt = ptg_lanemask_lt();
     const int64_t gwave = (int64_t)blockIdx.x * WALK_WAVES + wave;
     int *spill = A.spill + gwave * GCAP;
     const int xcd = blockIdx.x % NXCD;
-    int64_t n_int = 0, n_vis = 0, n_part = 0;
+    int64_t n_int = 0, n_vis = 0, n_part = 0, n_exp = 0;
     const int64_t ngroups = A.goff ? (int64_t)__builtin_amdgcn_readfirstlane(*A.gcount) : A.ngroups;
 
     for (int attempt = 0; attempt < NXCD; attempt++) {
@@ -455,14 +584,33 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
 #pragma unroll
             for (int j = 0; j < TPL; j++) acc[j] = Acc<T>{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
+            // group expansion: centre and radius of the group's box, this system's tolerance
+            T cx = 0, cy = 0, cz = 0, rg = 0, eps_x = 0;
+            GroupExp<T> E{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+            if constexpr (EXP) {
+                cx = T(0.5) * (bmin0 + bmax0);
+                cy = T(0.5) * (bmin1 + bmax1);
+                cz = T(0.5) * (bmin2 + bmax2);
+                const T hx = T(0.5) * (bmax0 - bmin0), hy = T(0.5) * (bmax1 - bmin1), hz = T(0.5) * (bmax2 - bmin2);
+                // rounded up by a relative 1e-6 so every target's offset from the centre is within rg
+                rg = sqrt(hx * hx + hy * hy + hz * hz) * T(1.000001);
+                eps_x = A.eps_exp * (A.eps_scale ? (T)A.eps_scale[root] : T(1));
+            }
+
             int sp = 1, gsp = 0, nn = 0, np = 0;
             bool overflow = false;
             if (lane == 0) L.stack[0] = root;  // the root of system s occupies walk slot s
             __builtin_amdgcn_wave_barrier();
 
-            for (;;) {
+            // pops up to 64 stack entries (one per lane, refilling from the spill region when the
+            // LDS stack is empty) and issues their loads; false once the traversal is done
+            int e = 0;
+            bool ve = false;
+            V4<T> X, X2;
+            int2 K;
+            auto pop = [&]() -> bool {
                 if (sp == 0) {
-                    if (gsp == 0) break;
+                    if (gsp == 0) return false;
                     const int k = std::min(gsp, SPILL);
                     for (int q = lane; q < k; q += 64) L.stack[q] = spill[gsp - k + q];
                     __builtin_amdgcn_wave_barrier();
@@ -471,49 +619,80 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                 }
                 const int k = std::min(sp, 64);
                 const int base = sp - k;
-                int e = lane < k ? L.stack[base + lane] : 0;
-                const bool ve = lane < k && e >= 0 && e < A.nslots;  // a corrupt link is dropped, not followed
+                e = lane < k ? L.stack[base + lane] : 0;
+                ve = lane < k && e >= 0 && e < A.nslots;  // a corrupt link is dropped, not followed
                 if (!ve) e = 0;
                 __builtin_amdgcn_wave_barrier();
                 sp = base;
                 n_vis += k;
+                X = A.xyzm[e];
+                X2 = A.aux[e];
+                K = A.kids[e];
+                return true;
+            };
 
-                const V4<T> X = A.xyzm[e];
-                const V4<T> X2 = A.aux[e];
-                const int2 K = A.kids[e];
+            bool more = pop();
+            while (more) {
                 const bool ispart = K.y == 0;
                 T dxm = X.x < bmin0 ? bmin0 - X.x : (X.x > bmax0 ? X.x - bmax0 : T(0));
                 T dym = X.y < bmin1 ? bmin1 - X.y : (X.y > bmax1 ? X.y - bmax1 : T(0));
                 T dzm = X.z < bmin2 ? bmin2 - X.z : (X.z > bmax2 ? X.z - bmax2 : T(0));
-                T r_min = sqrt(dxm * dxm + dym * dym + dzm * dzm);
-                T hh = X2.x > hmax ? X2.x : hmax;
-                T c1 = X2.y / A.theta + X2.z;
-                T c2 = hh + X2.y * T(0.6) + X2.z;
-                bool acc_node = ve && !ispart && r_min > (c1 > c2 ? c1 : c2);
-                // fast path: a particle farther from the group's box than every softening radius
-                // (and not inside it) needs neither the softening nor the self test -> node list
-                const bool far_part = !EXACT && ve && ispart && r_min > (X2.x > hmax ? X2.x : hmax);
-                if (A.crit == 1) {
-                    // error-bounded: also require the quadrupole term the monopole drops,
-                    // 3.5 |Q| / r^4, to stay below eps (theta is then only a loose cap)
-                    T r2m = r_min * r_min;
-                    acc_node = acc_node && X2.w < A.eps * r2m * r2m;
-                } else if (A.crit == 2) {
-                    // learned: r_min beyond the node's predicted opening radius
-                    acc_node = acc_node && r_min > X2.w;
+                const T r2m = dxm * dxm + dym * dym + dzm * dzm;
+                const T hh = X2.x > hmax ? X2.x : hmax;
+                bool acc_node, far_part = false;
+                if constexpr (EXACT) {
+                    // the reference's test as written: r_min > max(size/theta + delta, h + 0.6 size + delta)
+                    const T r_min = sqrt(r2m);
+                    const T c1 = X2.y / A.theta + X2.z;
+                    const T c2 = hh + X2.y * T(0.6) + X2.z;
+                    acc_node = ve && !ispart && r_min > (c1 > c2 ? c1 : c2);
+                } else {
+                    // the same test on squared distances (every threshold is >= 0), with 1/theta;
+                    // evaluated without branches (bitwise &) so the slot's aux record is one load
+                    // issued with the others, not a second round trip inside a branch
+                    const T c1 = X2.y * A.inv_theta + X2.z;
+                    const T c2 = hh + X2.y * T(0.6) + X2.z;
+                    const T c = c1 > c2 ? c1 : c2;
+                    acc_node = ve & !ispart & (r2m > c * c);
+                    // a particle farther from the group's box than every softening radius (and not
+                    // inside it) needs neither the softening nor the self test -> node list
+                    far_part = ve & ispart & (r2m > hh * hh);
+                    if (A.crit == 1) {
+                        // error-bounded: also require the quadrupole term the monopole drops,
+                        // 3.5 |Q| / r^4, to stay below eps (theta is then only a loose cap)
+                        acc_node = acc_node & (X2.w < A.eps * r2m * r2m);
+                    } else if (A.crit == 2) {
+                        // learned: r_min beyond the node's predicted opening radius
+                        acc_node = acc_node & (r2m > X2.w * X2.w);
+                    }
                 }
                 bool acc_part = ve && ispart && !far_part;
                 acc_node = acc_node || far_part;
                 bool open = ve && !ispart && !acc_node;
+                if constexpr (EXP) {
+                    if (acc_node) {
+                        // accepted elements are outside every softening radius and r_min > 0,
+                        // so D >= r_min > 0; fold if the truncation bound is within the tolerance
+                        const T Dx = X.x - cx, Dy = X.y - cy, Dz = X.z - cz;
+                        const T D2 = Dx * Dx + Dy * Dy + Dz * Dz;
+                        const T rinv = fast_rsqrt(D2);
+                        const T x = rg * rinv, omx = T(1) - x;
+                        if (x < T(1) && X.w * x * x * x * (T(4) - T(3) * x) < eps_x * D2 * omx * omx) {
+                            exp_add(E, X.w, Dx, Dy, Dz, rinv);
+                            acc_node = false;
+                            n_exp++;
+                        }
+                    }
+                }
 
                 const uint64_t bp = ptg_ballot(acc_part), bn = ptg_ballot(acc_node);
                 if (acc_part) {
-                    const int at = np + (int)__popcll(bp & lt);
+                    const int at = np + lanes_below(bp);
                     L.parts[at] = X;
                     L.ph[at] = X2.x;
                 }
                 if (acc_node) {
-                    const int at = nn + (int)__popcll(bn & lt);
+                    const int at = nn + lanes_below(bn);
                     L.nodes[at] = X;
                     if (EXACT) L.nh[at] = X2.x;
                     if (QUAD) {
@@ -530,8 +709,8 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
 
                 // push the children of opened nodes
                 const int cnt = open ? std::min(std::max(K.y, 0), 8) : 0;
-                const int incl = ptg_wave_incl_scan(cnt);
-                const int tot = __shfl(incl, 63, 64);
+                int tot;
+                const int below = small_excl_scan(cnt, tot);
                 if (sp + tot > SCAP) {
                     // move the bottom SPILL entries of the LDS stack to this wave's global region
                     if (gsp + SPILL > GCAP) {
@@ -550,11 +729,18 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                     sp -= SPILL;
                 }
                 {
-                    const int at = sp + incl - cnt;
-                    for (int c = 0; c < cnt; c++) L.stack[at + c] = K.x + c;
+                    // unrolled: eight masked LDS stores with immediate offsets, no per-lane loop
+                    int *dst = &L.stack[sp + below];
+#pragma unroll
+                    for (int c = 0; c < 8; c++)
+                        if (c < cnt) dst[c] = K.x + c;
                 }
                 sp += tot;
                 __builtin_amdgcn_wave_barrier();
+
+                // the next chunk's loads go out before the list flushes below, whose arithmetic
+                // hides their latency (this chunk's X, X2, K are no longer needed)
+                more = pop();
 
                 if (nn >= 64) {
                     flush_nodes<T, TPL, FIELDS, QUAD, EXACT>(L, 64, acc, tx, ty, tz, th);
@@ -586,6 +772,11 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
             flush_nodes<T, TPL, FIELDS, QUAD, EXACT>(L, nn, acc, tx, ty, tz, th);
             flush_parts<T, TPL, FIELDS, QUAD, EXACT>(L, np, acc, tx, ty, tz, th);
             __builtin_amdgcn_wave_barrier();
+            if constexpr (EXP) {
+                exp_wave_sum(E);
+#pragma unroll
+                for (int j = 0; j < TPL; j++) exp_eval<T, FIELDS>(acc[j], E, tx[j] - cx, ty[j] - cy, tz[j] - cz);
+            }
 
             // ---- write G * fields in the caller's order ----
             constexpr int W = NW<FIELDS>::W;
@@ -614,6 +805,11 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
         atomicAdd((unsigned long long *)&A.counters[0], (unsigned long long)n_int);
         atomicAdd((unsigned long long *)&A.counters[1], (unsigned long long)n_vis);
         atomicAdd((unsigned long long *)&A.counters[2], (unsigned long long)n_part);
+    }
+    if (EXP && A.counters) {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) n_exp += __shfl_xor(n_exp, d, 64);  // lanes fold separately
+        if (lane == 0) atomicAdd((unsigned long long *)&A.counters[3], (unsigned long long)n_exp);
     }
 }
 
@@ -652,30 +848,38 @@ extern "C" size_t ptg_walk_workspace_bytes(int64_t nt, int group_size)
     return ptg_align256(sizeof(WalkCtl)) + ptg_align256(waves * GCAP * sizeof(int));
 }
 
-template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT>
+template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT, bool EXP>
 static ptg_status launch_t(const WalkArgs<T> &a, int64_t max_blocks, hipStream_t s)
 {
     static int per_cu = 0;
     if (per_cu == 0) {
         int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_walk<T, TPL, FIELDS, QUAD, EXACT>, 64 * WALK_WAVES, 0) !=
-                hipSuccess || nb < 1)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_walk<T, TPL, FIELDS, QUAD, EXACT, EXP>, 64 * WALK_WAVES,
+                                                         0) != hipSuccess ||
+            nb < 1)
             nb = 1;
         per_cu = std::min(nb, WALK_MAX_BLOCKS_PER_CU);
     }
     int64_t blocks = std::min<int64_t>(max_blocks, (int64_t)num_cus() * per_cu);
-    k_walk<T, TPL, FIELDS, QUAD, EXACT><<<(unsigned)blocks, 64 * WALK_WAVES, 0, s>>>(a);
+    k_walk<T, TPL, FIELDS, QUAD, EXACT, EXP><<<(unsigned)blocks, 64 * WALK_WAVES, 0, s>>>(a);
     PTG_LAUNCH_CHECK();
     return PTG_OK;
 }
 
 template <typename T, int TPL, bool EXACT>
-static ptg_status launch_fq(const WalkArgs<T> &a, int fields, bool quad, int64_t blocks, hipStream_t s)
+static ptg_status launch_fq(const WalkArgs<T> &a, int fields, bool quad, bool expand, int64_t blocks, hipStream_t s)
 {
+    if constexpr (sizeof(T) == 4 && !EXACT) {
+        if (expand) {  // checked by the caller: monopole, potential and/or acceleration
+            if (fields == PTG_POT) return launch_t<T, TPL, PTG_POT, false, false, true>(a, blocks, s);
+            if (fields == PTG_ACC) return launch_t<T, TPL, PTG_ACC, false, false, true>(a, blocks, s);
+            return launch_t<T, TPL, PTG_POT | PTG_ACC, false, false, true>(a, blocks, s);
+        }
+    }
 #define PTG_F(F)                                                              \
     case F:                                                                   \
-        return quad ? launch_t<T, TPL, F, true, EXACT>(a, blocks, s)          \
-                    : launch_t<T, TPL, F, false, EXACT>(a, blocks, s);
+        return quad ? launch_t<T, TPL, F, true, EXACT, false>(a, blocks, s)   \
+                    : launch_t<T, TPL, F, false, EXACT, false>(a, blocks, s);
     switch (fields) {
         PTG_F(1) PTG_F(2) PTG_F(3) PTG_F(4) PTG_F(5) PTG_F(6) PTG_F(7)
     }
@@ -685,11 +889,11 @@ static ptg_status launch_fq(const WalkArgs<T> &a, int fields, bool quad, int64_t
 }
 
 template <typename T, bool EXACT>
-static ptg_status launch_walk(const WalkArgs<T> &a, int fields, bool quad, int64_t blocks, hipStream_t s)
+static ptg_status launch_walk(const WalkArgs<T> &a, int fields, bool quad, bool expand, int64_t blocks, hipStream_t s)
 {
-    if (a.gs <= 64) return launch_fq<T, 1, EXACT>(a, fields, quad, blocks, s);
-    if (a.gs <= 128) return launch_fq<T, 2, EXACT>(a, fields, quad, blocks, s);
-    return launch_fq<T, 4, EXACT>(a, fields, quad, blocks, s);
+    if (a.gs <= 64) return launch_fq<T, 1, EXACT>(a, fields, quad, expand, blocks, s);
+    if (a.gs <= 128) return launch_fq<T, 2, EXACT>(a, fields, quad, expand, blocks, s);
+    return launch_fq<T, 4, EXACT>(a, fields, quad, expand, blocks, s);
 }
 
 extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest, const double *pos,
@@ -747,6 +951,20 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest
         return PTG_ERR_ARG;
     }
     const double eps = crit == PTG_CRIT_QERR ? criterion->eps / 3.5 : 0.0;
+    const bool expand = criterion && criterion->group_expansion != 0;
+    if (expand) {
+        if (exact || wt->prec != PTG_F32 || quadrupole || (fields & PTG_TIDAL)) {
+            ptg_set_error("ptg_walk: the group expansion runs on the FP32 walk with monopole interactions, "
+                          "for the potential and/or the acceleration");
+            return PTG_ERR_ARG;
+        }
+        if (!(criterion->expansion_eps > 0)) {
+            ptg_set_error("ptg_walk: the group expansion needs expansion_eps > 0");
+            return PTG_ERR_ARG;
+        }
+    }
+    const double eps_exp = expand ? criterion->expansion_eps : 0.0;
+    const double *eps_scale = expand ? criterion->eps_scale : nullptr;
     if (nt == 0) return PTG_OK;
     if (ws_bytes < ptg_walk_workspace_bytes(nt, group_size) || !ws) {
         ptg_set_error("ptg_walk: workspace too small (%zu < %zu)", ws_bytes, ptg_walk_workspace_bytes(nt, group_size));
@@ -764,19 +982,19 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest
     if (wt->prec == PTG_F32) {
         WalkArgs<float> a{(const V4<float> *)wt->xyzm, (const V4<float> *)wt->aux, (const int2 *)wt->kids,
                           (const float *)wt->quad, pos, soft, perm, targets_sorted != 0, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
-                          group_size, (float)theta, crit, (float)eps, G, wt->origin[0], wt->origin[1], wt->origin[2],
+                          group_size, (float)theta, (float)(1.0 / theta), crit, (float)eps, (float)eps_exp, eps_scale, G, wt->origin[0], wt->origin[1], wt->origin[2],
                           fn, fo, fc, out, counters, ctl, spill};
         if (exact) {
             ptg_set_error("ptg_walk: the exact branch set is evaluated in FP64; use prec=PTG_F64");
             return PTG_ERR_ARG;
         }
-        st = launch_walk<float, false>(a, fields, quadrupole != 0, blocks, s);
+        st = launch_walk<float, false>(a, fields, quadrupole != 0, expand, blocks, s);
     } else if (wt->prec == PTG_F64) {
         WalkArgs<double> a{(const V4<double> *)wt->xyzm, (const V4<double> *)wt->aux, (const int2 *)wt->kids,
                            (const double *)wt->quad, pos, soft, perm, targets_sorted != 0, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
-                           group_size, theta, crit, eps, G, wt->origin[0], wt->origin[1], wt->origin[2],
+                           group_size, theta, 1.0 / theta, crit, eps, 0.0, nullptr, G, wt->origin[0], wt->origin[1], wt->origin[2],
                            fn, fo, fc, out, counters, ctl, spill};
-        st = launch_walk<double, true>(a, fields, quadrupole != 0, blocks, s);
+        st = launch_walk<double, true>(a, fields, quadrupole != 0, false, blocks, s);
     } else {
         ptg_set_error("ptg_walk: prec must be PTG_F32 or PTG_F64");
         return PTG_ERR_ARG;

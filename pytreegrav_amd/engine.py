@@ -240,7 +240,7 @@ def target_groups(pos, perm, group_size, tree=None, mode="cells", keys=None):
 
 def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7, G=1.0, precision="float64",
          counters=None, grouping=None, groups=None, criterion="theta", eps=None, model=None, bf16=False,
-         eps_scale=None, sorted_targets=False, _events_end=None, _check=True, _ws_slot=0):
+         eps_scale=None, sorted_targets=False, expansion=None, _events_end=None, _check=True, _ws_slot=0):
     """Grouped walk of `tree` at targets pos[perm[t]]; returns an (nt, W) float64 device tensor in the
     original target order (FieldsTarget_grouped, grouped_treewalk.py:350-397).
 
@@ -253,7 +253,11 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     beyond the opening radius the MLP `model` predicts for tolerance eps; theta is the guard; bf16
     runs the MLP with bfloat16 weights; eps_scale: optional device float64 scalar multiplying eps).
     sorted_targets: pos and soft are already in the sorted order (pos[t] is target perm[t]), as the
-    tree's own coordinates and softenings are for self gravity; perm then only places the output."""
+    tree's own coordinates and softenings are for self gravity; perm then only places the output.
+    expansion: tolerance (absolute acceleration, G=1, times eps_scale when given) of the group
+    expansion: accepted elements far enough from a group's centre are folded into a third-order
+    Taylor expansion about it instead of being evaluated per target (FP32, monopole interactions,
+    potential and/or acceleration)."""
     L = _lib.load()
     pos = to_dev(pos, (-1, 3))
     nt = pos.shape[0]
@@ -300,6 +304,17 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
         crit = _lib.Criterion(_lib.PTG_CRIT_RADIUS, 0, float(eps))
     elif criterion != "theta":
         raise ValueError(f"criterion must be 'theta', 'error' or 'learned', got {criterion!r}")
+    if expansion is not None:
+        if prec != PTG_F32 or quadrupole or (int(mask) & PTG_TIDAL):
+            raise ValueError("the group expansion runs on the float32 walk with monopole interactions, "
+                             "for the potential and/or the acceleration")
+        if not expansion > 0:
+            raise ValueError("expansion must be a positive tolerance")
+        if eps_scale is not None and eps_scale.numel() < (tree.nsys if forest else 1):
+            raise ValueError("eps_scale needs one value per system of the forest")
+        crit.group_expansion = 1
+        crit.expansion_eps = float(expansion)
+        crit.eps_scale = eps_scale.data_ptr() if eps_scale is not None else None
     if criterion == "learned":
         p = tree.pack_learned(model, eps, quadrupole, bf16, scale=eps_scale)
     else:
@@ -346,15 +361,16 @@ def _side_stream():
 
 
 def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, group_size=64, criterion="theta",
-                alpha=None, model=None, bf16=False, counters=None, walk_events=None):
+                alpha=None, model=None, bf16=False, counters=None, walk_events=None, expansion=None):
     """Self-gravity FP32 fields of the particles a tree was built from, in their input order.
 
     The cell-aligned target groups are formed on a side stream while the main stream packs the
     tree, runs the field-scale sample walk and (criterion "learned") the radius pass; the walk
     waits for both.  criterion "error"/"learned" take the per-node tolerance alpha x (RMS field)
     and need a tree built with quadrupoles; theta defaults to 0.7 ("theta") or 2.0 (the guard of
-    the other two).  walk_events: optional (start, end) torch.cuda.Event pair recorded around the
-    walk launch alone."""
+    the other two).  expansion: group-expansion tolerance in units of the RMS field (see walk);
+    None walks without it.  walk_events: optional (start, end) torch.cuda.Event pair recorded around
+    the walk launch alone."""
     L = _lib.load()
     pos = to_dev(pos, (-1, 3))
     n = pos.shape[0]
@@ -390,15 +406,20 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
               "ptg_target_groups")
     offsets.record_stream(main)
     count.record_stream(main)
-    eps, scale = None, None
+    eps, scale, exp_eps = None, None, None
     tree.pack(PTG_F32, quadrupole)
-    if criterion != "theta":
+    if criterion != "theta" or expansion is not None:
         main.wait_stream(side)  # the sample groups are taken from the cell groups
-        if criterion == "learned":  # the tolerance stays on the device: no host round trip
-            eps, scale = alpha, field_scale(tree, cells=(offsets, count), on_device=True)
-            tree.pack_learned(model, eps, quadrupole, bf16, scale=scale)
-        else:
-            eps = alpha * field_scale(tree, cells=(offsets, count))
+        if criterion == "error":
+            rms = field_scale(tree, cells=(offsets, count))
+            eps = alpha * rms
+            exp_eps = None if expansion is None else expansion * rms
+        else:  # the tolerances stay on the device: no host round trip
+            scale = field_scale(tree, cells=(offsets, count), on_device=True)
+            exp_eps = expansion
+            if criterion == "learned":
+                eps = alpha
+                tree.pack_learned(model, eps, quadrupole, bf16, scale=scale)
     main.wait_stream(side)
     if walk_events is not None:
         walk_events[0].record()
@@ -407,7 +428,7 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
     out = walk(tree, tree.coordinates[:n], tree.softenings[:n], perm, mask, quadrupole=quadrupole,
                group_size=group_size, theta=theta, G=G, precision="float32", counters=counters,
                groups=(offsets, count), criterion=criterion, eps=eps, model=model, bf16=bf16, eps_scale=scale,
-               sorted_targets=True,
+               sorted_targets=True, expansion=exp_eps,
                _events_end=walk_events[1] if walk_events is not None else None)
     if scale is not None:
         check_walk(1)  # the field-scale sample walk was not checked when it ran
@@ -494,7 +515,7 @@ def autotune_group_size(tree, pos, soft, perm, mask, quadrupole=False, theta=0.7
         return _tuned[key]
     times, inter = {}, {}
     for gs in sizes:
-        ctr = torch.zeros(3, dtype=torch.int64, device=device())
+        ctr = torch.zeros(4, dtype=torch.int64, device=device())
         kw = dict(quadrupole=quadrupole, group_size=int(gs), theta=theta, precision=precision, grouping=grouping,
                   criterion=criterion, eps=eps, model=model, bf16=bf16)
         walk(tree, pos, soft, perm, mask, counters=ctr, **kw)  # warm-up (and interaction count)

@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import PTG_ACC, PTG_F32, PTG_F64, check, ptr
+from ._lib import PTG_ACC, PTG_F32, PTG_F64, PTG_TIDAL, check, ptr
 from .engine import (TUNE_SIZES, DeviceTree, device, fields_mask, field_width, stream, to_dev, unpack_fields,
                      workspace)
 
@@ -128,12 +128,20 @@ class Forest:
         check(L.ptg_rms_rows(ptr(out), nt, 3, 0, 3, self.nsys, ptr(rms), stream()), "ptg_rms_rows")
         return rms
 
-    def _prepare(self, gs, theta, alpha, eps, model, bf16, goff, gcnt, gws):
-        """Groups, and for the learned criterion the opening radii; returns (pack, criterion, theta)."""
+    def _prepare(self, gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask=PTG_ACC, expansion=None):
+        """Groups, and for the learned criterion the opening radii; returns (pack, criterion, theta).
+        expansion: group-expansion tolerance in units of each system's RMS field (None: off)."""
         self._groups(gs, goff, gcnt, gws)
+        if expansion is not None:
+            if self.prec != PTG_F32 or self.quadrupole or (mask & PTG_TIDAL):
+                raise ValueError("expansion runs on the float32 walk with monopole interactions, for the potential "
+                                 "and/or the acceleration")
+            if not expansion > 0:
+                raise ValueError("expansion must be a positive tolerance (relative to each system's RMS field)")
         if self.criterion != "learned":
-            return self.tree.pack(self.prec, self.quadrupole), _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0), \
-                (0.7 if theta is None else theta)
+            crit = _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
+            self._expand(crit, expansion, None, goff, gcnt)
+            return self.tree.pack(self.prec, self.quadrupole), crit, (0.7 if theta is None else theta)
         if model is None:
             from .learned import OpeningMLP
 
@@ -147,20 +155,35 @@ class Forest:
             scale, base = self.system_field_scales(goff, gcnt), float(alpha)
         p = self.tree.pack_learned(model, base, self.quadrupole, bf16, scale=scale)
         self._scale = scale  # keeps the scale alive while the pack is cached under its address
-        return p, _lib.Criterion(_lib.PTG_CRIT_RADIUS, 0, 0.0), (2.0 if theta is None else theta)
+        crit = _lib.Criterion(_lib.PTG_CRIT_RADIUS, 0, 0.0)
+        self._expand(crit, expansion, scale if eps is None else None, goff, gcnt)
+        return p, crit, (2.0 if theta is None else theta)
+
+    def _expand(self, crit, expansion, rms, goff, gcnt):
+        """Switch the group expansion on in crit at expansion x each system's RMS field (device)."""
+        if expansion is None:
+            return
+        if rms is None:
+            rms = self.system_field_scales(goff, gcnt)
+        self._exp_scale = rms  # the walk (and any captured graph) reads it through crit
+        crit.group_expansion = 1
+        crit.expansion_eps = float(expansion)
+        crit.eps_scale = rms.data_ptr()
 
     def evaluate(self, potential=True, accel=True, tidal=False, theta=None, G=1.0, counters=None, alpha=None,
-                 eps=None, model=None, bf16=False):
+                 eps=None, model=None, bf16=False, expansion=None):
         """Fields of every particle from its own system, in input order: dict of device views
         ("potential" (n,), "accel" (n,3), "tidal" (n,3,3)).  theta defaults to 0.7 (geometric
         criterion) or 2.0 (the guard of the learned criterion, whose per-system tolerance is
-        alpha x the system's RMS field, or the given per-system eps)."""
+        alpha x the system's RMS field, or the given per-system eps).  expansion: group-expansion
+        tolerance relative to each system's RMS field (float32 monopole walks; None: off)."""
         if self.auto and self.tuning is None:
-            self.autotune(potential, accel, tidal, theta, G, alpha=alpha, eps=eps, model=model, bf16=bf16)
+            self.autotune(potential, accel, tidal, theta, G, alpha=alpha, eps=eps, model=model, bf16=bf16,
+                          expansion=expansion)
         gs = self.group_size
         mask = fields_mask(potential, accel, tidal)
         out, ws, goff, gcnt, gws = self._buffers(mask, gs)
-        p, crit, th = self._prepare(gs, theta, alpha, eps, model, bf16, goff, gcnt, gws)
+        p, crit, th = self._prepare(gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask, expansion)
         self._walk(p, mask, th, G, out, ws, gs, goff, gcnt, crit, counters)
         check(_lib.load().ptg_walk_check(ptr(ws), stream()), "ptg_walk")
         return unpack_fields(out, mask)
@@ -173,7 +196,7 @@ class Forest:
         for gs in sizes:
             out, ws, goff, gcnt, gws = self._buffers(mask, gs)
             p, crit, th = self._prepare(gs, theta, kw.get("alpha"), kw.get("eps"), kw.get("model"),
-                                        kw.get("bf16", False), goff, gcnt, gws)
+                                        kw.get("bf16", False), goff, gcnt, gws, mask, kw.get("expansion"))
             best = float("inf")
             for _ in range(2):
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -190,7 +213,7 @@ class Forest:
         return self.tuning
 
     def capture(self, potential=True, accel=True, tidal=False, theta=None, G=1.0, alpha=None, eps=None, model=None,
-                bf16=False):
+                bf16=False, expansion=None):
         """Capture the walk stage (groups + walk over all systems) into a HIP graph.  The pack (and,
         for the learned criterion, the opening radii) are made once before the capture.  Returns
         (replay, out): replay() re-runs the stage and checks the traversal overflow flag; out is
@@ -198,7 +221,7 @@ class Forest:
         gs = self.group_size
         mask = fields_mask(potential, accel, tidal)
         out, ws, goff, gcnt, gws = self._buffers(mask, gs)
-        p, crit, th = self._prepare(gs, theta, alpha, eps, model, bf16, goff, gcnt, gws)
+        p, crit, th = self._prepare(gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask, expansion)
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):  # warm-up on the capture stream
@@ -210,7 +233,7 @@ class Forest:
         with torch.cuda.graph(g, stream=s):
             self._groups(gs, goff, gcnt, gws)
             self._walk(p, mask, th, G, out, ws, gs, goff, gcnt, crit)
-        self._graphs[(mask, th, G)] = (g, out, ws, goff, gcnt, gws, p)
+        self._graphs[(mask, th, G)] = (g, out, ws, goff, gcnt, gws, p, crit, getattr(self, "_exp_scale", None))
 
         def replay():
             g.replay()
@@ -225,12 +248,12 @@ class Forest:
 
 
 def BatchedAccelPotential(pos, m, h, offsets, theta=None, G=1.0, quadrupole=False, precision="float32",
-                          group_size=None, return_device=False, criterion="theta", alpha=None):
+                          group_size=None, return_device=False, criterion="theta", alpha=None, expansion=None):
     """Accel and potential of many independent systems in one batched build and walk: returns
     (accel (n,3), potential (n,)) in input order, each particle feeling only its own system."""
     f = Forest(pos, m, h, offsets, quadrupole=quadrupole, precision=precision, group_size=group_size,
                criterion=criterion)
-    r = f.evaluate(potential=True, accel=True, theta=theta, G=G, alpha=alpha)
+    r = f.evaluate(potential=True, accel=True, theta=theta, G=G, alpha=alpha, expansion=expansion)
     if return_device:
         return r["accel"], r["potential"]
     return r["accel"].cpu().numpy(), r["potential"].cpu().numpy()

@@ -272,13 +272,23 @@ def TidalTensorTarget_grouped(pos, soft, tree, group_size=8, G=1.0, theta=0.7, q
 # shared drivers
 # ---------------------------------------------------------------------------------------------
 def _self_fields(pos, m, softening, mask, G, theta, tree, return_tree, parallel, method, quadrupole, group_size,
-                 precision, adaptive_n, criterion="theta", alpha=None):
+                 precision, adaptive_n, criterion="theta", alpha=None, expansion=None):
     """Potential/Accel/TidalTensor (frontend.py:145-293, 421-560, 688-789).
 
     criterion "error"/"learned" (float32 tree walks only, no reference counterpart): accept a node
     only when its estimated (analytic or learned) monopole error is below alpha x the RMS field;
-    theta then defaults to 2.0 and acts as the reference's geometric guard."""
+    theta then defaults to 2.0 and acts as the reference's geometric guard.  expansion (float32
+    monopole walks of the potential and/or acceleration, no reference counterpart): fold accepted
+    elements far enough from a target group into a third-order expansion about the group's centre
+    when its truncation bound is below expansion x the RMS field."""
     valueTestMethod(method)
+    if expansion is not None:
+        if precision != "float32" or quadrupole or (mask & PTG_TIDAL):
+            raise ValueError("expansion runs on the float32 walk with monopole interactions, for the potential "
+                             "and/or the acceleration")
+        if not expansion > 0:
+            raise ValueError("expansion must be a positive tolerance (relative to the RMS field)")
+        method = "tree"
     if criterion not in ("theta", "error", "learned"):
         raise ValueError(f"criterion must be 'theta', 'error' or 'learned', got {criterion!r}")
     if criterion != "theta":
@@ -307,13 +317,22 @@ def _self_fields(pos, m, softening, mask, G, theta, tree, return_tree, parallel,
             tree = ConstructTree(pos, m, softening, quadrupole=True)  # |Q| drives the criterion
             out = engine.self_fields(tree.dev, pos, softening, mask, theta=theta, G=G, quadrupole=quadrupole,
                                      group_size=64 if group_size == "auto" else group_size, criterion=criterion,
-                                     alpha=alpha)
+                                     alpha=alpha, expansion=expansion)
         else:
             checkTreeQuadrupoles(tree, True)
-            eps = alpha * engine.field_scale(tree.dev)
+            fs = engine.field_scale(tree.dev)
             out = engine.walk(tree.dev, pos, softening, _morton_order(pos), mask, quadrupole=quadrupole,
                               group_size=group_size, theta=theta, G=G, precision=precision, criterion=criterion,
-                              eps=eps, model=_learned_model() if criterion == "learned" else None)
+                              eps=alpha * fs, model=_learned_model() if criterion == "learned" else None,
+                              expansion=None if expansion is None else expansion * fs)
+    elif expansion is not None:
+        if tree is None:
+            tree = ConstructTree(pos, m, softening)
+            out = engine.self_fields(tree.dev, pos, softening, mask, theta=theta, G=G,
+                                     group_size=64 if group_size == "auto" else group_size, expansion=expansion)
+        else:
+            out = engine.walk(tree.dev, pos, softening, _morton_order(pos), mask, group_size=group_size, theta=theta,
+                              G=G, precision=precision, expansion=expansion * engine.field_scale(tree.dev))
     else:
         if tree is None:
             tree = ConstructTree(pos, m, softening, quadrupole=quadrupole)
@@ -383,32 +402,36 @@ def _learned_model():
 # ---------------------------------------------------------------------------------------------
 def Potential(pos, m, softening=None, G=1.0, theta=0.7, tree=None, return_tree=False, parallel=False,
               method="adaptive", quadrupole=False, group_size=None, device="cpu", precision=None, criterion="theta",
-              alpha=None):
-    """Gravitational potential at the particle positions, shape (N,) (frontend.py:145-293)."""
+              alpha=None, expansion=None):
+    """Gravitational potential at the particle positions, shape (N,) (frontend.py:145-293).
+
+    criterion/alpha/expansion (float32 only): see Accel."""
     valueTestMethod(method)
     _check_device(device)
     prec = _precision(precision, device)
     if criterion != "theta" and theta == 0.7:
         theta = 2.0
     out, tree, on_dev = _self_fields(pos, m, softening, PTG_POT, G, theta, tree, return_tree, parallel, method,
-                                     quadrupole, _group(group_size, prec), prec, None, criterion, alpha)
+                                     quadrupole, _group(group_size, prec), prec, None, criterion, alpha, expansion)
     return _ret(_finish(out[:, 0], on_dev), tree, return_tree)
 
 
 def Accel(pos, m, softening=None, G=1.0, theta=0.7, tree=None, return_tree=False, parallel=False,
           method="adaptive", quadrupole=False, group_size=None, device="cpu", precision=None, criterion="theta",
-          alpha=None):
+          alpha=None, expansion=None):
     """Gravitational acceleration at the particle positions, shape (N,3) (frontend.py:421-560).
 
     criterion="learned" / "error" with alpha (float32 only): the learned or analytic per-node error
-    bound instead of the geometric criterion (theta then defaults to 2.0, the guard)."""
+    bound instead of the geometric criterion (theta then defaults to 2.0, the guard).
+    expansion (float32, monopole): fold far accepted elements into a third-order expansion about
+    each target group's centre when its truncation bound is below expansion x the RMS field."""
     valueTestMethod(method)
     _check_device(device)
     prec = _precision(precision, device)
     if criterion != "theta" and theta == 0.7:
         theta = 2.0
     out, tree, on_dev = _self_fields(pos, m, softening, PTG_ACC, G, theta, tree, return_tree, parallel, method,
-                                     quadrupole, _group(group_size, prec), prec, None, criterion, alpha)
+                                     quadrupole, _group(group_size, prec), prec, None, criterion, alpha, expansion)
     return _ret(_finish(out, on_dev), tree, return_tree)
 
 

@@ -162,6 +162,8 @@ def open_radii(tree, pack, model, eps, bf16=False, scale=None):
     """Write the learned opening radius of every node into the FP32 pack (ptg_open_radii).  The
     tolerance is eps, or eps * scale for a device float64 `scale` (no host round trip): one value,
     or one per system of a forest tree."""
+    if scale is not None and scale.numel() < (tree.nsys if tree.forest_ref is not None else 1):
+        raise ValueError("scale needs one value per system of the forest")
     check(_lib.load().ptg_open_radii(_lib.ctypes.byref(tree.arrays), tree.n, tree.nnodes, tree.forest_ref,
                                      _lib.ctypes.byref(pack["wt"]), ptr(model.params), float(model.margin),
                                      float(eps), ptr(scale), int(bool(bf16)), stream()), "ptg_open_radii")

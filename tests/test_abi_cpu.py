@@ -4,6 +4,8 @@ declares, and the ctypes signatures cover exactly those functions (no compute ca
 import ctypes
 import os
 import re
+import shutil
+import subprocess
 
 import pytest
 
@@ -44,3 +46,27 @@ def test_struct_layouts():
     assert ctypes.sizeof(_lib.BuildInfo) == 8 + 8 + 4 * 4 + 3 * 8 + 8
     assert ctypes.sizeof(_lib.TreeArrays) == 13 * 8
     assert ctypes.sizeof(_lib.WalkTree) == 4 * 8 + 8 + 4 + 4 + 3 * 8
+
+
+@pytest.mark.skipif(not shutil.which("gcc"), reason="gcc not available")
+def test_struct_field_offsets_match_the_header(tmp_path):
+    """The C compiler's sizeof/offsetof of every header struct equal the ctypes mirror's."""
+    from pytreegrav_amd import _lib
+
+    mirror = {"ptg_build_info": _lib.BuildInfo, "ptg_tree_arrays": _lib.TreeArrays, "ptg_forest": _lib.Forest,
+              "ptg_walk_tree": _lib.WalkTree, "ptg_criterion": _lib.Criterion}
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "ptg.h"', "int main(void) {"]
+    expect = []
+    for cname, cls in mirror.items():
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        expect.append(ctypes.sizeof(cls))
+        for field, _ in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {field}));')
+            expect.append(getattr(cls, field).offset)
+    lines += ["return 0; }"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == expect

@@ -207,7 +207,7 @@ def test_trained_network_beats_geometric_criterion():
     d = engine.direct(x[idx], h[idx], x, m, h, PTG_ACC).cpu().numpy()
 
     def run(**kw):
-        ctr = torch.zeros(3, dtype=torch.int64, device="cuda")
+        ctr = torch.zeros(4, dtype=torch.int64, device="cuda")
         a = engine.walk(t, x, h, t.order, PTG_ACC, group_size=64, precision="float32", counters=ctr, **kw)
         return _rms(a.cpu().numpy()[idx], d), int(ctr[0].item())
 

@@ -21,7 +21,7 @@ def _walk(x, m, h, mask, quadrupole, gs, theta, precision, G=1.0, grouping="fixe
     from pytreegrav_amd import ConstructTree, engine
 
     t = ConstructTree(x, m, h, quadrupole=quadrupole)
-    ctr = torch.zeros(3, dtype=torch.int64, device=engine.device())
+    ctr = torch.zeros(4, dtype=torch.int64, device=engine.device())
     out = engine.walk(t.dev, x, h, t.dev.order, mask, quadrupole=quadrupole, group_size=gs, theta=theta, G=G,
                       precision=precision, counters=ctr, grouping=grouping)
     res = {k: v.cpu().numpy() for k, v in engine.unpack_fields(out, mask).items()}
