@@ -192,7 +192,12 @@ ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
  * is below expansion_eps is not evaluated per target but folded into a third-order Taylor expansion
  * of its field about the group centre, evaluated once per target after the traversal.
  * expansion_eps is an absolute acceleration (G = 1), times eps_scale[s] (device, nullable; s = 0 for
- * one tree, the group's system for a forest) when given. */
+ * one tree, the group's system for a forest) when given.  group_expansion == 2 adds a shared pass
+ * (needs group offsets and ptg_walk_shared_workspace_bytes more workspace): for each supergroup (8
+ * consecutive groups of one system, counted from its first group) one wavefront folds the elements every group of it accepts and whose expansion
+ * about the supergroup's centre is within the tolerance, opens the nodes every group opens, and lists
+ * the rest as a frontier from which each of its groups then walks; the per-group accepted-element
+ * sets are those of the one-level walk. */
 typedef struct {
     int32_t criterion;
     int32_t group_expansion;
@@ -202,6 +207,9 @@ typedef struct {
 } ptg_criterion;
 
 size_t ptg_walk_workspace_bytes(int64_t nt, int group_size);
+/* Extra workspace (appended after ptg_walk_workspace_bytes) for group_expansion == 2; nsys = the
+ * forest's system count (1 for one tree). */
+size_t ptg_walk_shared_workspace_bytes(int64_t nt, int64_t nsys, int group_size);
 /* targets_sorted != 0: pos and soft are already in sorted target order (pos[t], soft[t], e.g. the
  * tree's own Morton-ordered coordinates and softenings for self gravity); perm then only places
  * the outputs. */

@@ -240,7 +240,8 @@ def target_groups(pos, perm, group_size, tree=None, mode="cells", keys=None):
 
 def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7, G=1.0, precision="float64",
          counters=None, grouping=None, groups=None, criterion="theta", eps=None, model=None, bf16=False,
-         eps_scale=None, sorted_targets=False, expansion=None, _events_end=None, _check=True, _ws_slot=0):
+         eps_scale=None, sorted_targets=False, expansion=None, supergroups=False, _events_end=None, _check=True,
+         _ws_slot=0):
     """Grouped walk of `tree` at targets pos[perm[t]]; returns an (nt, W) float64 device tensor in the
     original target order (FieldsTarget_grouped, grouped_treewalk.py:350-397).
 
@@ -257,7 +258,9 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     expansion: tolerance (absolute acceleration, G=1, times eps_scale when given) of the group
     expansion: accepted elements far enough from a group's centre are folded into a third-order
     Taylor expansion about it instead of being evaluated per target (FP32, monopole interactions,
-    potential and/or acceleration)."""
+    potential and/or acceleration).  supergroups (with expansion): a shared pass per 8 consecutive
+    groups folds what all of them accept into one expansion and hands each group the frontier it
+    walks from (the same accepted elements per group, less traversal)."""
     L = _lib.load()
     pos = to_dev(pos, (-1, 3))
     nt = pos.shape[0]
@@ -278,7 +281,9 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     forest = tree.nsys > 1 or tree.forest is not None
     if tree.nsys > 1 and (nt != tree.n or perm.data_ptr() != tree.order.data_ptr()):
         raise ValueError("a forest is walked at its own particles (perm = tree.order)")
-    if groups is None and (grouping == "cells" or tree.nsys > 1):
+    if supergroups and expansion is None:
+        raise ValueError("supergroups needs the group expansion (expansion=...)")
+    if groups is None and (grouping == "cells" or tree.nsys > 1 or supergroups):
         own = nt == tree.n and perm.data_ptr() == tree.order.data_ptr()
         groups = target_groups(pos, perm, group_size, tree=tree if forest else None, mode=grouping,
                                keys=tree.keys if own else None)
@@ -312,14 +317,17 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
             raise ValueError("expansion must be a positive tolerance")
         if eps_scale is not None and eps_scale.numel() < (tree.nsys if forest else 1):
             raise ValueError("eps_scale needs one value per system of the forest")
-        crit.group_expansion = 1
+        crit.group_expansion = 2 if supergroups else 1
         crit.expansion_eps = float(expansion)
         crit.eps_scale = eps_scale.data_ptr() if eps_scale is not None else None
     if criterion == "learned":
         p = tree.pack_learned(model, eps, quadrupole, bf16, scale=eps_scale)
     else:
         p = tree.pack(prec, quadrupole)
-    ws = _walk_workspace(L.ptg_walk_workspace_bytes(nt, int(group_size)), _ws_slot)
+    wsb = L.ptg_walk_workspace_bytes(nt, int(group_size))
+    if supergroups:
+        wsb += L.ptg_walk_shared_workspace_bytes(nt, int(tree.nsys if forest else 1), int(group_size))
+    ws = _walk_workspace(wsb, _ws_slot)
     check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), tree.forest_ref if forest else None, ptr(pos), ptr(soft), ptr(perm),
                      int(bool(sorted_targets)), nt, int(mask), int(bool(quadrupole)),
                      int(group_size), ptr(goff), ptr(gcnt), float(theta), _lib.ctypes.byref(crit), float(G),
@@ -361,7 +369,8 @@ def _side_stream():
 
 
 def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, group_size=64, criterion="theta",
-                alpha=None, model=None, bf16=False, counters=None, walk_events=None, expansion=None):
+                alpha=None, model=None, bf16=False, counters=None, walk_events=None, expansion=None,
+                supergroups=False):
     """Self-gravity FP32 fields of the particles a tree was built from, in their input order.
 
     The cell-aligned target groups are formed on a side stream while the main stream packs the
@@ -428,7 +437,7 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
     out = walk(tree, tree.coordinates[:n], tree.softenings[:n], perm, mask, quadrupole=quadrupole,
                group_size=group_size, theta=theta, G=G, precision="float32", counters=counters,
                groups=(offsets, count), criterion=criterion, eps=eps, model=model, bf16=bf16, eps_scale=scale,
-               sorted_targets=True, expansion=exp_eps,
+               sorted_targets=True, expansion=exp_eps, supergroups=supergroups,
                _events_end=walk_events[1] if walk_events is not None else None)
     if scale is not None:
         check_walk(1)  # the field-scale sample walk was not checked when it ran

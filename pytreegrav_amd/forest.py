@@ -76,11 +76,14 @@ class Forest:
                                    int(self.prec == PTG_F64), ptr(out), ptr(counters), ptr(ws), ws.numel(),
                                    stream()), "ptg_walk")
 
-    def _buffers(self, mask, gs):
+    def _buffers(self, mask, gs, supergroups=False):
         L = _lib.load()
         dev = device()
         gws = workspace(L.ptg_groups_workspace_bytes(self.n, self.nsys))
-        ws = workspace(L.ptg_walk_workspace_bytes(self.n, gs))
+        wsb = L.ptg_walk_workspace_bytes(self.n, gs)
+        if supergroups:
+            wsb += L.ptg_walk_shared_workspace_bytes(self.n, self.nsys, gs)
+        ws = workspace(wsb)
         out = torch.empty((self.n, field_width(mask)), dtype=torch.float64, device=dev)
         goff = torch.empty(self.n + 1, dtype=torch.int32, device=dev)
         gcnt = torch.empty(1, dtype=torch.int32, device=dev)
@@ -128,9 +131,13 @@ class Forest:
         check(L.ptg_rms_rows(ptr(out), nt, 3, 0, 3, self.nsys, ptr(rms), stream()), "ptg_rms_rows")
         return rms
 
-    def _prepare(self, gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask=PTG_ACC, expansion=None):
+    def _prepare(self, gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask=PTG_ACC, expansion=None,
+                 supergroups=False):
         """Groups, and for the learned criterion the opening radii; returns (pack, criterion, theta).
-        expansion: group-expansion tolerance in units of each system's RMS field (None: off)."""
+        expansion: group-expansion tolerance in units of each system's RMS field (None: off);
+        supergroups: with it, the shared pass per 8 groups of a system."""
+        if supergroups and expansion is None:
+            raise ValueError("supergroups needs the group expansion (expansion=...)")
         self._groups(gs, goff, gcnt, gws)
         if expansion is not None:
             if self.prec != PTG_F32 or self.quadrupole or (mask & PTG_TIDAL):
@@ -140,7 +147,7 @@ class Forest:
                 raise ValueError("expansion must be a positive tolerance (relative to each system's RMS field)")
         if self.criterion != "learned":
             crit = _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
-            self._expand(crit, expansion, None, goff, gcnt)
+            self._expand(crit, expansion, None, goff, gcnt, supergroups)
             return self.tree.pack(self.prec, self.quadrupole), crit, (0.7 if theta is None else theta)
         if model is None:
             from .learned import OpeningMLP
@@ -156,22 +163,22 @@ class Forest:
         p = self.tree.pack_learned(model, base, self.quadrupole, bf16, scale=scale)
         self._scale = scale  # keeps the scale alive while the pack is cached under its address
         crit = _lib.Criterion(_lib.PTG_CRIT_RADIUS, 0, 0.0)
-        self._expand(crit, expansion, scale if eps is None else None, goff, gcnt)
+        self._expand(crit, expansion, scale if eps is None else None, goff, gcnt, supergroups)
         return p, crit, (2.0 if theta is None else theta)
 
-    def _expand(self, crit, expansion, rms, goff, gcnt):
+    def _expand(self, crit, expansion, rms, goff, gcnt, supergroups=False):
         """Switch the group expansion on in crit at expansion x each system's RMS field (device)."""
         if expansion is None:
             return
         if rms is None:
             rms = self.system_field_scales(goff, gcnt)
         self._exp_scale = rms  # the walk (and any captured graph) reads it through crit
-        crit.group_expansion = 1
+        crit.group_expansion = 2 if supergroups else 1
         crit.expansion_eps = float(expansion)
         crit.eps_scale = rms.data_ptr()
 
     def evaluate(self, potential=True, accel=True, tidal=False, theta=None, G=1.0, counters=None, alpha=None,
-                 eps=None, model=None, bf16=False, expansion=None):
+                 eps=None, model=None, bf16=False, expansion=None, supergroups=False):
         """Fields of every particle from its own system, in input order: dict of device views
         ("potential" (n,), "accel" (n,3), "tidal" (n,3,3)).  theta defaults to 0.7 (geometric
         criterion) or 2.0 (the guard of the learned criterion, whose per-system tolerance is
@@ -179,11 +186,11 @@ class Forest:
         tolerance relative to each system's RMS field (float32 monopole walks; None: off)."""
         if self.auto and self.tuning is None:
             self.autotune(potential, accel, tidal, theta, G, alpha=alpha, eps=eps, model=model, bf16=bf16,
-                          expansion=expansion)
+                          expansion=expansion, supergroups=supergroups)
         gs = self.group_size
         mask = fields_mask(potential, accel, tidal)
-        out, ws, goff, gcnt, gws = self._buffers(mask, gs)
-        p, crit, th = self._prepare(gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask, expansion)
+        out, ws, goff, gcnt, gws = self._buffers(mask, gs, supergroups)
+        p, crit, th = self._prepare(gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask, expansion, supergroups)
         self._walk(p, mask, th, G, out, ws, gs, goff, gcnt, crit, counters)
         check(_lib.load().ptg_walk_check(ptr(ws), stream()), "ptg_walk")
         return unpack_fields(out, mask)
@@ -194,9 +201,10 @@ class Forest:
         mask = fields_mask(potential, accel, tidal)
         times = {}
         for gs in sizes:
-            out, ws, goff, gcnt, gws = self._buffers(mask, gs)
+            out, ws, goff, gcnt, gws = self._buffers(mask, gs, kw.get("supergroups", False))
             p, crit, th = self._prepare(gs, theta, kw.get("alpha"), kw.get("eps"), kw.get("model"),
-                                        kw.get("bf16", False), goff, gcnt, gws, mask, kw.get("expansion"))
+                                        kw.get("bf16", False), goff, gcnt, gws, mask, kw.get("expansion"),
+                                        kw.get("supergroups", False))
             best = float("inf")
             for _ in range(2):
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -213,15 +221,15 @@ class Forest:
         return self.tuning
 
     def capture(self, potential=True, accel=True, tidal=False, theta=None, G=1.0, alpha=None, eps=None, model=None,
-                bf16=False, expansion=None):
+                bf16=False, expansion=None, supergroups=False):
         """Capture the walk stage (groups + walk over all systems) into a HIP graph.  The pack (and,
         for the learned criterion, the opening radii) are made once before the capture.  Returns
         (replay, out): replay() re-runs the stage and checks the traversal overflow flag; out is
         the (n, W) result buffer every replay rewrites."""
         gs = self.group_size
         mask = fields_mask(potential, accel, tidal)
-        out, ws, goff, gcnt, gws = self._buffers(mask, gs)
-        p, crit, th = self._prepare(gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask, expansion)
+        out, ws, goff, gcnt, gws = self._buffers(mask, gs, supergroups)
+        p, crit, th = self._prepare(gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask, expansion, supergroups)
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):  # warm-up on the capture stream
