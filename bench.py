@@ -80,6 +80,8 @@ def parse():
                          "from a group's centre are folded into a third-order expansion about it (default 4e-3 "
                          "with the learned criterion, off otherwise)")
     ap.add_argument("--no-expansion", action="store_true", help="walk without the group expansion")
+    ap.add_argument("--bf16", action="store_true", help="learned criterion: bfloat16 MLP inference for the radii")
+    ap.add_argument("--supergroups", action="store_true", help="with the expansion: the shared pass per 8 groups")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain theta=0.7 comparison line")
     ap.add_argument("--group-size", type=int, default=64)
     ap.add_argument("--softening", type=float, default=0.01)
@@ -300,7 +302,8 @@ def main():
             return engine.self_fields(tree.dev, pos, soft, mask, theta=theta, quadrupole=quad_walk,
                                       group_size=args.group_size, criterion=criterion,
                                       alpha=alpha if criterion != "theta" else None, model=model,
-                                      counters=ctr if record else None, walk_events=ev, expansion=expansion)
+                                      counters=ctr if record else None, walk_events=ev, expansion=expansion,
+                                      bf16=args.bf16, supergroups=args.supergroups and expansion is not None)
 
         for _ in range(args.warmup):
             step(False)
@@ -393,6 +396,8 @@ def main():
             "criterion": args.criterion,
             "theta": theta,
             "group_expansion": args.expansion,
+            "supergroups": bool(args.supergroups and args.expansion is not None),
+            "radius_inference": ("bf16" if args.bf16 else "fp32") if args.criterion == "learned" else None,
             "alpha": args.alpha if args.criterion in ("error", "learned") else None,
             "model": (model.meta.get("data", "") + f"; margin {model.margin:.3f}") if model is not None else None,
             "group_size": args.group_size,

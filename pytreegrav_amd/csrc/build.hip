@@ -1581,19 +1581,24 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
 // ============================================================================================
 // walk pack
 // ============================================================================================
+template <typename T> struct alignas(4 * sizeof(T)) PackV4 {
+    T x, y, z, w;
+};
+
+// one slot of the walk pack; xyzm, aux and kids go out as one vector store each (the slots of a
+// thread block are scattered, so partial-line scalar stores would multiply the write traffic)
 template <typename T>
 __device__ void pack_slot(int64_t e, int64_t slot, int64_t n, const ptg_tree_arrays &t, const ptg_walk_tree &wt,
                           const double *origin)
 {
-    T *xyzm = (T *)wt.xyzm + 4 * slot;
-    T *aux = (T *)wt.aux + 4 * slot;
-    xyzm[0] = (T)(t.coordinates[3 * e] - origin[0]);
-    xyzm[1] = (T)(t.coordinates[3 * e + 1] - origin[1]);
-    xyzm[2] = (T)(t.coordinates[3 * e + 2] - origin[2]);
-    xyzm[3] = (T)t.masses[e];
-    aux[0] = (T)t.softenings[e];
-    aux[1] = (T)t.sizes[e];
-    aux[2] = (T)t.deltas[e];
+    PackV4<T> X, X2;
+    X.x = (T)(t.coordinates[3 * e] - origin[0]);
+    X.y = (T)(t.coordinates[3 * e + 1] - origin[1]);
+    X.z = (T)(t.coordinates[3 * e + 2] - origin[2]);
+    X.w = (T)t.masses[e];
+    X2.x = (T)t.softenings[e];
+    X2.y = (T)t.sizes[e];
+    X2.z = (T)t.deltas[e];
     // Frobenius norm of the node's traceless quadrupole (the monopole truncation error scale used by
     // the error-bounded criterion); -1 when the tree carries no quadrupoles
     double qn = -1.0;
@@ -1605,10 +1610,11 @@ __device__ void pack_slot(int64_t e, int64_t slot, int64_t n, const ptg_tree_arr
     } else if (e < n) {
         qn = 0.0;
     }
-    aux[3] = (T)qn;
+    X2.w = (T)qn;
+    ((PackV4<T> *)wt.xyzm)[slot] = X;
+    ((PackV4<T> *)wt.aux)[slot] = X2;
     bool node = e >= n;
-    wt.kids[2 * slot] = node ? t.child_base[e] : (int32_t)e;
-    wt.kids[2 * slot + 1] = node ? t.nchild[e] : 0;
+    ((int2 *)wt.kids)[slot] = make_int2(node ? t.child_base[e] : (int32_t)e, node ? t.nchild[e] : 0);
     if (wt.quad) {
         T *q = (T *)wt.quad + 8 * slot;
         if (node && t.quadrupoles) {
@@ -1757,6 +1763,23 @@ __global__ __launch_bounds__(256) void k_cell_cuts(const uint64_t *keys, int64_t
         return;
     }
     int L = ptg_common_levels(keys[t - 1], keys[t]);
+    // quick rejection (most targets sit in small cells): with h = cap / 2, if neither position
+    // t - 1 - h nor t + h is in the cell, each side extends less than h and the cell holds at most
+    // cap targets
+    {
+        const int64_t h = cap / 2;
+        const uint64_t kt = keys[t];
+        auto in_cell = [&](int64_t j) {
+            if (j < lo_b || j >= hi_b) return false;
+            if (L == 0) return true;
+            if (L >= 21) return keys[j] == kt;
+            return (keys[j] >> (63 - 3 * L)) == (kt >> (63 - 3 * L));
+        };
+        if (!in_cell(t - 1 - h) && !in_cell(t + h)) {
+            pflag[t] = 0;
+            return;
+        }
+    }
     int64_t left = same_cell_extent(keys, lo_b, hi_b, t - 1, L, -1, cap);
     int64_t right = left >= cap ? 0 : same_cell_extent(keys, lo_b, hi_b, t, L, +1, cap);
     pflag[t] = (left + right + 2 > cap) ? 1 : 0;
