@@ -24,10 +24,11 @@
 #include "ptg_common.hpp"
 
 #define WALK_WAVES 4
-#define SCAP 1024      // LDS traversal stack entries per wave (>= SPILL + the 512 children one step can push)
-#define SPILL 512      // entries moved to / from global memory at a time (>= 512, the most one step pushes)
+#define SCAP 768       // LDS traversal stack entries per wave (>= 512, the most children one step pushes)
+#define SPILL 256      // entries refilled from global memory at a time (<= SCAP - 512)
 #define GCAP 16384     // global spill entries per wave
-#define ICAP 128       // interaction-list entries per list and wave
+#define LFLUSH 64      // a list is evaluated in multiples of LFLUSH entries once it holds LFLUSH (power of 2)
+#define ICAP (LFLUSH + 64)  // interaction-list entries per list and wave (LFLUSH - 1 + one step's 64)
 #define NXCD 8
 #define CTR_STRIDE 32  // int32 words between per-XCD counters (128 B)
 #define SUPER 8        // groups per supergroup of the shared pass
@@ -489,6 +490,34 @@ __device__ __forceinline__ void exp_eval(Acc<T> &a, const GroupExp<T> &E, T ux, 
     }
 }
 
+// gap between x and the interval [lo, hi] (0 inside): bmin - x and x - bmax are exact differences,
+// so this equals the two-sided branch form bit for bit
+__device__ __forceinline__ float box_gap(float x, float lo, float hi) { return fmaxf(fmaxf(lo - x, x - hi), 0.f); }
+__device__ __forceinline__ double box_gap(double x, double lo, double hi) { return fmax(fmax(lo - x, x - hi), 0.0); }
+
+// Stack spill: when sp entries plus the tot children about to be pushed exceed SCAP, the bottom
+// ks entries (the excess rounded up to 64, at most sp; tot <= 512 <= SCAP so ks >= the excess)
+// go to the wave's global spill stack and the rest move down.  Any order of visits gives the same
+// accepted sets, so the global stack is refilled from its top when the LDS stack runs empty.
+static_assert(SCAP >= 512 && SPILL >= 64 && SPILL <= SCAP - 512, "stack geometry");
+__device__ __forceinline__ int spill_count(int sp, int tot)
+{
+    const int ex = sp + tot - SCAP;
+    return std::min(sp, (ex + 63) & ~63);
+}
+__device__ __forceinline__ void spill_bottom(int *stack, int *dst, int sp, int ks, int lane)
+{
+    for (int q = lane; q < ks; q += 64) dst[q] = stack[q];
+    __builtin_amdgcn_wave_barrier();
+    // read [ks + q, ks + q + 64) before writing [q, q + 64); later reads lie above earlier writes
+    for (int q = 0; q < sp - ks; q += 64) {
+        const int v = (q + lane < sp - ks) ? stack[ks + q + lane] : 0;
+        __builtin_amdgcn_wave_barrier();
+        if (q + lane < sp - ks) stack[q + lane] = v;
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 // lanes below this one whose bit is set in m
 __device__ __forceinline__ int lanes_below(uint64_t m)
 {
@@ -550,8 +579,17 @@ __device__ __forceinline__ void flush_parts(const WaveLds<T, QUAD> &L, int cnt, 
     }
 }
 
+// waves per SIMD the FP32 one-target-per-lane walk is compiled for: its traversal is latency-bound
+// (3 -> 4 waves per SIMD: 4.5 -> 3.7 ms on the 1e7-particle walk), so registers are capped to fit
+// WALK_FP32_WAVES waves; the LDS per wave (7.7 KB) fits five 4-wave blocks per CU
+#define WALK_FP32_WAVES 5
+template <typename T, int TPL> struct WalkOcc {
+    static constexpr int value = (sizeof(T) == 4 && TPL == 1) ? WALK_FP32_WAVES : 1;
+};
+
 template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT, bool EXP>
-__global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
+__global__ __launch_bounds__(64 * WALK_WAVES) __attribute__((amdgpu_waves_per_eu(WalkOcc<T, TPL>::value)))
+void k_walk(WalkArgs<T> A)
 {
     static_assert(!EXP || (sizeof(T) == 4 && !QUAD && !EXACT && !NW<FIELDS>::TID),
                   "the group expansion is an FP32 monopole potential/acceleration path");
@@ -708,9 +746,9 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                 const V4<T> X2 = A.aux[e];
                 const int2 K = A.kids[e];
                 const bool ispart = K.y == 0;
-                T dxm = X.x < bmin0 ? bmin0 - X.x : (X.x > bmax0 ? X.x - bmax0 : T(0));
-                T dym = X.y < bmin1 ? bmin1 - X.y : (X.y > bmax1 ? X.y - bmax1 : T(0));
-                T dzm = X.z < bmin2 ? bmin2 - X.z : (X.z > bmax2 ? X.z - bmax2 : T(0));
+                // distance to the box per axis: max(bmin - x, x - bmax, 0), the same value as the
+                // two-sided test (one max3 per axis)
+                const T dxm = box_gap(X.x, bmin0, bmax0), dym = box_gap(X.y, bmin1, bmax1), dzm = box_gap(X.z, bmin2, bmax2);
                 const T r2m = dxm * dxm + dym * dym + dzm * dzm;
                 const T hh = X2.x > hmax ? X2.x : hmax;
                 bool acc_node, far_part = false;
@@ -784,21 +822,15 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                 int tot;
                 const int below = small_excl_scan(cnt, tot);
                 if (sp + tot > SCAP) {
-                    // move the bottom SPILL entries of the LDS stack to this wave's global region
-                    if (gsp + SPILL > GCAP) {
+                    // move the bottom ks entries of the LDS stack to this wave's global region
+                    const int ks = spill_count(sp, tot);
+                    if (gsp + ks > GCAP) {
                         overflow = true;
                         break;
                     }
-                    for (int q = lane; q < SPILL; q += 64) spill[gsp + q] = L.stack[q];
-                    __builtin_amdgcn_wave_barrier();
-                    for (int q = 0; q < sp - SPILL; q += 64) {
-                        int v = (q + lane < sp - SPILL) ? L.stack[SPILL + q + lane] : 0;
-                        __builtin_amdgcn_wave_barrier();
-                        if (q + lane < sp - SPILL) L.stack[q + lane] = v;
-                        __builtin_amdgcn_wave_barrier();
-                    }
-                    gsp += SPILL;
-                    sp -= SPILL;
+                    spill_bottom(L.stack, spill + gsp, sp, ks, lane);
+                    gsp += ks;
+                    sp -= ks;
                 }
                 {
                     // unrolled: eight masked LDS stores with immediate offsets, no per-lane loop
@@ -810,27 +842,29 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk(WalkArgs<T> A)
                 sp += tot;
                 __builtin_amdgcn_wave_barrier();
 
-                if (nn >= 64) {
-                    flush_nodes<T, TPL, FIELDS, QUAD, EXACT>(L, 64, acc, tx, ty, tz, th);
+                if (nn >= LFLUSH) {
+                    const int c = nn & ~(LFLUSH - 1);  // c >= LFLUSH > nn - c
+                    flush_nodes<T, TPL, FIELDS, QUAD, EXACT>(L, c, acc, tx, ty, tz, th);
                     __builtin_amdgcn_wave_barrier();
-                    if (lane < nn - 64) {  // reads [64, nn) and writes [0, nn-64): disjoint
-                        L.nodes[lane] = L.nodes[64 + lane];
-                        if (EXACT) L.nh[lane] = L.nh[64 + lane];
+                    if (lane < nn - c) {  // reads [c, nn) and writes [0, nn - c): disjoint
+                        L.nodes[lane] = L.nodes[c + lane];
+                        if (EXACT) L.nh[lane] = L.nh[c + lane];
                         if (QUAD)
-                            for (int c = 0; c < 6; c++) L.nq[6 * lane + c] = L.nq[6 * (64 + lane) + c];
+                            for (int q = 0; q < 6; q++) L.nq[6 * lane + q] = L.nq[6 * (c + lane) + q];
                     }
                     __builtin_amdgcn_wave_barrier();
-                    nn -= 64;
+                    nn -= c;
                 }
-                if (np >= 64) {
-                    flush_parts<T, TPL, FIELDS, QUAD, EXACT>(L, 64, acc, tx, ty, tz, th);
+                if (np >= LFLUSH) {
+                    const int c = np & ~(LFLUSH - 1);
+                    flush_parts<T, TPL, FIELDS, QUAD, EXACT>(L, c, acc, tx, ty, tz, th);
                     __builtin_amdgcn_wave_barrier();
-                    if (lane < np - 64) {  // disjoint ranges, as above
-                        L.parts[lane] = L.parts[64 + lane];
-                        L.ph[lane] = L.ph[64 + lane];
+                    if (lane < np - c) {  // disjoint ranges, as above
+                        L.parts[lane] = L.parts[c + lane];
+                        L.ph[lane] = L.ph[c + lane];
                     }
                     __builtin_amdgcn_wave_barrier();
-                    np -= 64;
+                    np -= c;
                 }
             }
             if (overflow) {
@@ -1001,9 +1035,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk_super(WalkArgs<float> 
                 const int2 K = A.kids[e];
                 const bool ispart = K.y == 0;
                 // nearest and farthest points of the box from the element
-                const T dxm = X.x < bmin0 ? bmin0 - X.x : (X.x > bmax0 ? X.x - bmax0 : T(0));
-                const T dym = X.y < bmin1 ? bmin1 - X.y : (X.y > bmax1 ? X.y - bmax1 : T(0));
-                const T dzm = X.z < bmin2 ? bmin2 - X.z : (X.z > bmax2 ? X.z - bmax2 : T(0));
+                const T dxm = box_gap(X.x, bmin0, bmax0), dym = box_gap(X.y, bmin1, bmax1), dzm = box_gap(X.z, bmin2, bmax2);
                 const T r2m = dxm * dxm + dym * dym + dzm * dzm;
                 const T fx = std::max(std::fabs(X.x - bmin0), std::fabs(X.x - bmax0));
                 const T fy = std::max(std::fabs(X.y - bmin1), std::fabs(X.y - bmax1));
@@ -1055,20 +1087,14 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk_super(WalkArgs<float> 
                 int tot;
                 const int below = small_excl_scan(cnt, tot);
                 if (sp + tot > SCAP) {
-                    if (gsp + SPILL > GCAP) {
+                    const int ks = spill_count(sp, tot);
+                    if (gsp + ks > GCAP) {
                         fail = true;
                         break;
                     }
-                    for (int q = lane; q < SPILL; q += 64) spill[gsp + q] = stack[q];
-                    __builtin_amdgcn_wave_barrier();
-                    for (int q = 0; q < sp - SPILL; q += 64) {
-                        int v = (q + lane < sp - SPILL) ? stack[SPILL + q + lane] : 0;
-                        __builtin_amdgcn_wave_barrier();
-                        if (q + lane < sp - SPILL) stack[q + lane] = v;
-                        __builtin_amdgcn_wave_barrier();
-                    }
-                    gsp += SPILL;
-                    sp -= SPILL;
+                    spill_bottom(stack, spill + gsp, sp, ks, lane);
+                    gsp += ks;
+                    sp -= ks;
                 }
                 int *dst = &stack[sp + below];
 #pragma unroll
@@ -1131,7 +1157,7 @@ static int num_cus()
     return g_num_cus;
 }
 
-#define WALK_MAX_BLOCKS_PER_CU 4
+#define WALK_MAX_BLOCKS_PER_CU 5
 
 // persistent grid: resident blocks per CU from the occupancy API, capped by the spill allocation
 static int64_t walk_grid_blocks() { return (int64_t)num_cus() * WALK_MAX_BLOCKS_PER_CU; }
