@@ -273,7 +273,8 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     if group_size == "auto":
         group_size = autotune_group_size(tree, pos, soft, perm, mask, quadrupole=quadrupole, theta=theta,
                                          precision=precision, grouping=grouping, criterion=criterion, eps=eps,
-                                         model=model, bf16=bf16)["choice"]
+                                         model=model, bf16=bf16, expansion=expansion, eps_scale=eps_scale,
+                                         supergroups=supergroups)["choice"]
     if grouping is None:
         grouping = "cells" if prec == PTG_F32 else "fixed"
     if grouping not in ("fixed", "cells"):
@@ -508,25 +509,29 @@ def tuning_log():
 
 
 def autotune_group_size(tree, pos, soft, perm, mask, quadrupole=False, theta=0.7, precision="float32", grouping=None,
-                        criterion="theta", eps=None, model=None, bf16=False, sizes=TUNE_SIZES, reps=2, force=False):
+                        criterion="theta", eps=None, model=None, bf16=False, sizes=TUNE_SIZES, reps=2, force=False,
+                        expansion=None, eps_scale=None, supergroups=False):
     """Pick the walk's target-group size by timing the walk (groups included) at each candidate.
 
     The choice depends on the target count, the tree and the opening criterion, not on the values
     the walk returns: every candidate evaluates the same sums to the same truncation order (a
     bigger group opens nodes for its whole bounding box, so it trades more interactions per target
     for fewer, longer traversals).  Decisions are cached per (tree size, target count, fields,
-    moment order, theta, precision, grouping, criterion) and listed by tuning_log()."""
+    moment order, theta, precision, grouping, criterion, tolerance, expansion) and listed by
+    tuning_log()."""
     pos = to_dev(pos, (-1, 3))
     nt = pos.shape[0]
     key = (_bucket(tree.n), _bucket(nt), int(mask), bool(quadrupole), float(theta), str(precision), grouping,
-           criterion, None if eps is None else float(eps), None if model is None else model.key(), bool(bf16))
+           criterion, None if eps is None else float(eps), None if model is None else model.key(), bool(bf16),
+           None if expansion is None else float(expansion), bool(supergroups))
     if key in _tuned and not force:
         return _tuned[key]
     times, inter = {}, {}
     for gs in sizes:
         ctr = torch.zeros(4, dtype=torch.int64, device=device())
         kw = dict(quadrupole=quadrupole, group_size=int(gs), theta=theta, precision=precision, grouping=grouping,
-                  criterion=criterion, eps=eps, model=model, bf16=bf16)
+                  criterion=criterion, eps=eps, model=model, bf16=bf16, expansion=expansion, eps_scale=eps_scale,
+                  supergroups=supergroups)
         walk(tree, pos, soft, perm, mask, counters=ctr, **kw)  # warm-up (and interaction count)
         torch.cuda.synchronize()
         inter[int(gs)] = int(ctr[0].item())

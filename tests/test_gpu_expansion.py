@@ -251,3 +251,14 @@ def test_forest_supergroups_equal_single_walks():
         replay()
         torch.cuda.synchronize()
         assert torch.equal(out, ref)
+
+
+def test_autotune_with_expansion():
+    x, m, h = case("plummer_1e5_h")
+    t = ConstructTree(x, m, h)
+    out = engine.walk(t.dev, x, h, t.dev.order, PTG_POT | PTG_ACC, group_size="auto", theta=0.7, precision="float32",
+                      expansion=1e-4)
+    log = [v for k, v in engine.tuning_log().items() if k[-2] == 1e-4]
+    assert log and log[-1]["choice"] in engine.TUNE_SIZES
+    d = engine.direct(x[:2000], h[:2000], x, m, h, PTG_ACC).cpu().numpy()
+    assert _rms(out.cpu().numpy()[:2000, 1:4], d) < 3e-3
