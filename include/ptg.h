@@ -275,6 +275,24 @@ ptg_status ptg_rms_rows(const double *x, int64_t rows, int stride, int col, int 
                         void *stream);
 
 /* ------------------------------------------------------------------------------------------
+ * Column density (treewalk.py:821-1276; frontend.ColumnDensity :1321-1457).  Particles are uniform
+ * spheres of radius equal to their softening; the tree is walked through its FirstSubnode /
+ * NextBranch links in the reference's depth-first order.  targets (nt, 3) float64 device array;
+ * err (device int32) is set to 1 if a link chain is corrupt.
+ * ------------------------------------------------------------------------------------------ */
+/* Ray-traced: out[j * nrays + i] = column from target j to infinity along unit ray i, exact
+ * (ColumnDensityWalk_singleray :904-973 per pair; the grouped and multi-ray walks of :822-901 and
+ * :1087-1194 give the same values).  rays: (nrays, 3) shared, or (nt, nrays, 3) when per_target
+ * != 0 (randomized ray grids, :1061-1078). */
+ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const double *targets, int64_t nt,
+                           const double *rays, int64_t nrays, int per_target, double *out, int32_t *err, void *stream);
+/* Angular-binned (rays=None): out (nt, 6), the mean column per equal-area sky bin, with groups of
+ * group_size consecutive targets sharing one traversal opened at the node's distance to the group's
+ * box (ColumnDensityBinned_grouped :1197-1276; group_size 1 is ColumnDensityWalk_binned :976-1028). */
+ptg_status ptg_column_binned(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const double *targets,
+                             int64_t nt, int64_t group_size, double theta, double *out, int32_t *err, void *stream);
+
+/* ------------------------------------------------------------------------------------------
  * Direct summation, LDS-tiled: replaces bruteforce.py.  self_mode != 0: targets are the sources
  * (nt == ns), pairs with equal index are excluded (Potential_bruteforce_parallel :86-116,
  * Accel_bruteforce_parallel :164-206, TidalTensor_bruteforce_parallel :319-350); self_mode == 0:

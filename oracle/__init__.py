@@ -48,6 +48,8 @@ def lib():
         L.ptgo_radix_argsort.argtypes = [_i64, ctypes.c_int64, _i64]
         L.ptgo_bounding_cube.argtypes = [_d, ctypes.c_int64, _d, _d]
         L.ptgo_morton_order.argtypes = [_d, ctypes.c_int64, _i64]
+        L.ptgo_column_rays.argtypes = [_vp, _d, ctypes.c_int64, _d, ctypes.c_int64, ctypes.c_int, _d, ctypes.c_int]
+        L.ptgo_column_binned.argtypes = [_vp, _d, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, _d, ctypes.c_int]
         L.ptgo_build.restype = _vp
         L.ptgo_build.argtypes = [_d, _d, _d, ctypes.c_int64, ctypes.c_int, ctypes.c_int]
         L.ptgo_tree_free.argtypes = [_vp]
@@ -257,3 +259,28 @@ def self_fields_tree(pos, m, h, potential=False, accel=False, tidal=False, quadr
         o[idx] = v
         out[k] = o
     return (out, ctr) if return_counters else out
+
+
+# ---- column density (treewalk.py:821-1276) ----
+def column_rays(tree, pos, rays, per_target=False, nthreads=0):
+    """Ray-traced column density of every target along every ray: (nt, nr); rays (nr, 3) shared or
+    (nt, nr, 3) per target (ColumnDensityWalk_singleray per pair, treewalk.py:904-973)."""
+    pos = _f64(pos, (-1, 3))
+    nt = len(pos)
+    rays = np.ascontiguousarray(rays, dtype=np.float64)
+    nr = rays.shape[-2] if rays.ndim == 3 else rays.shape[0]
+    out = np.zeros((nt, nr))
+    lib().ptgo_column_rays(tree.ptr, _p(pos), nt, _p(rays), nr, int(bool(per_target)), _p(out),
+                           nthreads or os.cpu_count() or 1)
+    return out
+
+
+def column_binned(tree, pos, group_size=16, theta=0.5, nthreads=0):
+    """Angular-binned column density, groups of group_size consecutive targets
+    (ColumnDensityBinned_grouped, treewalk.py:1197-1276): (nt, 6)."""
+    pos = _f64(pos, (-1, 3))
+    nt = len(pos)
+    out = np.zeros((nt, 6))
+    lib().ptgo_column_binned(tree.ptr, _p(pos), nt, int(group_size), float(theta), _p(out),
+                             nthreads or os.cpu_count() or 1)
+    return out

@@ -802,3 +802,140 @@ void ptgo_fields_self_bruteforce(const double *x, const double *m, const double 
         if (want_tidal) for (int q = 0; q < 9; q++) o[k + q] = G * T[q];
     }
 }
+
+/* ------------------------------------------------------------------------------------------ */
+/* Column density (treewalk.py:821-1276): particles are uniform spheres of radius Softenings  */
+/* ------------------------------------------------------------------------------------------ */
+
+#define PTGO_PI 3.14159265358979323846
+
+/* ColumnDensityWalk_singleray (treewalk.py:904-973), one target and one unit ray; the ray-traced
+ * walk is exact (a node opens whenever its inflated bounding sphere could meet the ray). */
+static double column_singleray(const ptgo_tree *t, const double *pos, const double *ray)
+{
+    const double fac_density = 3.0 / (4.0 * PTGO_PI);
+    double column = 0.0;
+    int64_t no = t->NumParticles;
+    const int64_t stop = t->NextBranch[no];
+    while (no > -1 && no != stop) {
+        double dx0 = t->Coordinates[3 * no] - pos[0];
+        double dx1 = t->Coordinates[3 * no + 1] - pos[1];
+        double dx2 = t->Coordinates[3 * no + 2] - pos[2];
+        double r2 = dx0 * dx0 + dx1 * dx1 + dx2 * dx2;
+        double r = sqrt(r2);
+        double z = ray[0] * dx0 + ray[1] * dx1 + ray[2] * dx2;
+        if (r2 - z * z < 0) {
+            no = t->NextBranch[no];
+            continue;
+        }
+        double h_no = t->Softenings[no];
+        if (no < t->NumParticles) {
+            if (h_no > 0) {
+                double hinv = 1.0 / h_no;
+                double fac = fac_density * t->Masses[no] * hinv * hinv;
+                double r_proj = sqrt(r2 - z * z);
+                double q = r_proj * hinv;
+                if (r_proj < h_no) {
+                    if (r > h_no) {
+                        if (z > 0) column += fac * 2 * sqrt(1 - q * q);
+                    } else {
+                        column += fac * (z * hinv + sqrt(1 - q * q));
+                    }
+                }
+            }
+            no = t->NextBranch[no];
+        } else {
+            double R_eff = t->Sizes[no] * 0.8660254037844386 + t->Deltas[no];
+            int hit = r < h_no + R_eff;
+            if (!hit && z > 0) hit = (r2 - z * z) < (h_no + R_eff) * (h_no + R_eff);
+            no = hit ? t->FirstSubnode[no] : t->NextBranch[no];
+        }
+    }
+    return column;
+}
+
+/* ColumnDensity_tree's ray-traced branches (treewalk.py:1031-1084): out[t][i] for nt targets and
+ * nr rays, shared (per_target = 0, rays (nr,3)) or per target (per_target = 1, rays (nt,nr,3)). */
+void ptgo_column_rays(void *vt, const double *pos, int64_t nt, const double *rays, int64_t nr, int per_target,
+                      double *out, int nthreads)
+{
+    const ptgo_tree *t = vt;
+    if (nthreads <= 0) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+    for (int64_t j = 0; j < nt; j++)
+        for (int64_t i = 0; i < nr; i++)
+            out[j * nr + i] = column_singleray(t, pos + 3 * j, per_target ? rays + 3 * (j * nr + i) : rays + 3 * i);
+}
+
+/* angular_bin_scalar (treewalk.py:38-45) */
+static int angular_bin(double d0, double d1, double d2)
+{
+    if (fabs(d0) > fabs(d1) && fabs(d0) > fabs(d2)) return d0 > 0 ? 0 : 1;
+    if (fabs(d1) > fabs(d2)) return d1 > 0 ? 2 : 3;
+    return d2 > 0 ? 4 : 5;
+}
+
+/* ColumnDensityBinned_grouped (treewalk.py:1197-1276): groups of gs consecutive targets share one
+ * traversal opened by the node's distance to the group box (gs = 1: ColumnDensityWalk_binned,
+ * treewalk.py:976-1028); out (nt, 6) mean column per equal-area sky bin. */
+void ptgo_column_binned(void *vt, const double *pos, int64_t nt, int64_t gs, double theta, double *out, int nthreads)
+{
+    const ptgo_tree *t = vt;
+    const double bin_size = (4.0 * PTGO_PI) / 6.0;
+    if (nthreads <= 0) nthreads = 1;
+    memset(out, 0, sizeof(double) * (size_t)nt * 6);
+    const int64_t ngroups = (nt + gs - 1) / gs;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+    for (int64_t gi = 0; gi < ngroups; gi++) {
+        const int64_t a = gi * gs, b = a + gs < nt ? a + gs : nt;
+        double bmin[3], bmax[3];
+        for (int k = 0; k < 3; k++) bmin[k] = bmax[k] = pos[3 * a + k];
+        for (int64_t j = a + 1; j < b; j++)
+            for (int k = 0; k < 3; k++) {
+                if (pos[3 * j + k] < bmin[k]) bmin[k] = pos[3 * j + k];
+                else if (pos[3 * j + k] > bmax[k]) bmax[k] = pos[3 * j + k];
+            }
+        int64_t no = t->NumParticles;
+        const int64_t stop = t->NextBranch[no];
+        while (no > -1 && no != stop) {
+            const double c0 = t->Coordinates[3 * no], c1 = t->Coordinates[3 * no + 1], c2 = t->Coordinates[3 * no + 2];
+            const double h_no = t->Softenings[no], M = t->Masses[no];
+            if (no < t->NumParticles) {
+                const double hh = h_no * h_no;
+                for (int64_t j = a; j < b; j++) {
+                    double d0 = c0 - pos[3 * j], d1 = c1 - pos[3 * j + 1], d2 = c2 - pos[3 * j + 2];
+                    double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                    int bin = angular_bin(d0, d1, d2);
+                    double *o = out + 6 * j;
+                    if (r2 > hh) {
+                        o[bin] += M / r2 / bin_size;
+                    } else if (h_no > 0) {
+                        double col0 = M * (3 / (4 * PTGO_PI * hh));
+                        double fac = sqrt(r2) / h_no;
+                        double col_iso = (1 - fac) * col0;
+                        for (int k = 0; k < 6; k++) o[k] += col_iso;
+                        o[bin] += col0 * fac * 2;
+                    }
+                }
+                no = t->NextBranch[no];
+            } else {
+                double dm[3];
+                const double c[3] = {c0, c1, c2};
+                for (int k = 0; k < 3; k++) dm[k] = c[k] < bmin[k] ? bmin[k] - c[k] : (c[k] > bmax[k] ? c[k] - bmax[k] : 0.0);
+                double r_min = sqrt(dm[0] * dm[0] + dm[1] * dm[1] + dm[2] * dm[2]);
+                double size = t->Sizes[no], delta = t->Deltas[no];
+                double c_a = size / theta + delta, c_b = h_no + size * 0.6 + delta;
+                if (r_min > (c_a > c_b ? c_a : c_b)) {
+                    for (int64_t j = a; j < b; j++) {
+                        double d0 = c0 - pos[3 * j], d1 = c1 - pos[3 * j + 1], d2 = c2 - pos[3 * j + 2];
+                        double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+                        out[6 * j + angular_bin(d0, d1, d2)] += M / r2 / bin_size;
+                    }
+                    no = t->NextBranch[no];
+                } else {
+                    no = t->FirstSubnode[no];
+                }
+            }
+        }
+    }
+}

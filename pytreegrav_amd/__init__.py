@@ -2,13 +2,14 @@
 
 The public names mirror pytreegrav's (src/pytreegrav/__init__.py re-exports octree and frontend):
 ConstructTree, Potential, Accel, PotentialTarget, AccelTarget, TidalTensor, TidalTensorTarget,
-Field, and the grouped-walk functions.  All compute runs in libptg_hip.so (hand-written HIP for
+Field, ColumnDensity, and the grouped-walk functions.  All compute runs in libptg_hip.so (hand-written HIP for
 gfx950); there is no CPU fallback.
 """
 
 from .frontend import (  # noqa: F401
     Accel,
     AccelTarget,
+    ColumnDensity,
     ConstructTree,
     Field,
     FieldsTarget_grouped,

@@ -35,6 +35,7 @@ UNITS = {
     "walk.hip": [],
     "direct.hip": [],
     "learned.hip": [],
+    "colden.hip": [],
 }
 
 

@@ -445,6 +445,39 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
     return out
 
 
+def column_rays(tree, targets, rays, per_target=False):
+    """Ray-traced column density (ptg_column_rays): (nt, nr) float64 device tensor for targets (nt, 3)
+    and unit rays (nr, 3), or per-target rays (nt, nr, 3) when per_target."""
+    L = _lib.load()
+    tp = to_dev(targets, (-1, 3))
+    nt = tp.shape[0]
+    r = to_dev(rays)
+    nr = r.shape[-2] if per_target else r.shape[0]
+    if (per_target and tuple(r.shape) != (nt, nr, 3)) or (not per_target and (r.dim() != 2 or r.shape[1] != 3)):
+        raise ValueError("rays must be (nr, 3), or (nt, nr, 3) with per_target")
+    out = torch.empty((nt, nr), dtype=torch.float64, device=device())
+    err = torch.zeros(1, dtype=torch.int32, device=device())
+    check(L.ptg_column_rays(_lib.ctypes.byref(tree.arrays), tree.n, tree.nnodes, ptr(tp), nt, ptr(r), nr,
+                            int(bool(per_target)), ptr(out), ptr(err), stream()), "ptg_column_rays")
+    if int(err.item()):
+        raise RuntimeError("ptg_column_rays: corrupt tree links")
+    return out
+
+
+def column_binned(tree, targets, group_size=16, theta=0.5):
+    """Angular-binned column density (ptg_column_binned): (nt, 6) float64 device tensor."""
+    L = _lib.load()
+    tp = to_dev(targets, (-1, 3))
+    nt = tp.shape[0]
+    out = torch.empty((nt, 6), dtype=torch.float64, device=device())
+    err = torch.zeros(1, dtype=torch.int32, device=device())
+    check(L.ptg_column_binned(_lib.ctypes.byref(tree.arrays), tree.n, tree.nnodes, ptr(tp), nt, int(group_size),
+                              float(theta), ptr(out), ptr(err), stream()), "ptg_column_binned")
+    if int(err.item()):
+        raise RuntimeError("ptg_column_binned: corrupt tree links")
+    return out
+
+
 def direct(pos_t, soft_t, pos_s, m_s, h_s, mask, G=1.0, self_mode=False, precision="float64"):
     """LDS-tiled direct sum (bruteforce.py); (nt, W) float64 device tensor."""
     L = _lib.load()

@@ -531,3 +531,89 @@ class Field:
 
     def tidal(self, pos_target=None, **kwargs):
         return self.evaluate(tidal=True, pos_target=pos_target, **kwargs)["tidal"]
+
+
+# ---------------------------------------------------------------------------------------------
+# column density (frontend.py:1321-1457)
+# ---------------------------------------------------------------------------------------------
+# below this many targets the reference walks each target alone; at or above it, groups of
+# group_size targets share a traversal (treewalk.py:19); only the binned estimator depends on it
+COLUMN_GROUP_MIN_TARGETS = 1000
+
+
+def ColumnDensity(pos, m, radii, rays=None, randomize_rays=False, healpix=False, tree=None, theta=0.5,
+                  return_tree=False, parallel=False, group_size=16, device="cpu"):
+    """Column density from each particle's position to infinity, the particles being uniform
+    spheres of the given radii (frontend.py:1321-1457).
+
+    rays: None (angle-binned estimator, 6 equal-area sky bins, opening angle theta), an int (6 = the
+    six axis directions, otherwise that many random directions from numpy's global generator), or an
+    (N_rays, 3) array of directions (normalised here).  randomize_rays rotates the ray grid of each
+    target by misc.random_rotation(its Morton rank).  healpix=NSIDE needs healpy.  Returns (N, N_rays)
+    (or (N, 6)) float64 in input order.  The ray-traced columns are exact (no opening-angle
+    truncation); the binned ones follow the reference's grouped estimator (group_size consecutive
+    Morton targets per traversal when there are at least COLUMN_GROUP_MIN_TARGETS of them).
+    ``parallel`` is accepted and ignored; ``device`` is validated as the reference does, and both
+    values run the FP64 walks on the GPU."""
+    from .misc import random_rotation
+
+    if np.any(np.asarray(radii.cpu() if isinstance(radii, torch.Tensor) else radii) <= 0):
+        warnings.warn(
+            "Some particle radii are <= 0; these particles are point-like and contribute no column "
+            "density. Supply a nonzero radius if their mass should be included."
+        )
+    on_dev = _is_dev(pos)
+    pos = _atleast_2d(_f64(pos))
+    if tree is None:
+        tree = ConstructTree(pos, _f64(m), _f64(radii))
+        idx = tree.dev.order  # the tree was built from pos: its walk order is pos's Morton order
+    else:
+        idx = _morton_order(pos)  # a supplied tree may hold other particles: sort the targets alone
+    pos_dev = engine.to_dev(pos, (-1, 3))
+    pos_sorted = pos_dev.index_select(0, idx)
+
+    if type(rays) == int:
+        if rays == 6:
+            rays = np.vstack([np.eye(3), -np.eye(3)])
+        else:
+            rays = np.random.normal(size=(rays, 3))
+    elif type(rays) == np.ndarray:
+        if not len(rays.shape) == 2:
+            raise Exception("rays array argument must be 2D.")
+        elif rays.shape[1] != 3:
+            raise Exception("rays array argument is not an array of 3D vectors.")
+        rays = np.array(rays, dtype=np.float64)
+    elif rays is not None:
+        raise Exception("rays argument type is not supported")
+    if healpix:
+        try:
+            import healpy as hp
+        except ImportError as e:
+            raise ImportError("healpix=... needs healpy: pip install healpy") from e
+        rays = np.array(hp.pix2vec(healpix, np.arange(hp.nside2npix(healpix)))).T
+    if rays is not None:
+        rays = rays / np.sqrt((rays * rays).sum(1))[:, None]
+
+    if device not in ("cpu", "cuda"):
+        raise ValueError(f"device must be 'cpu' or 'cuda', got {device!r}")
+    if device == "cuda" and (rays is None or randomize_rays):
+        raise ValueError("device='cuda' supports only the ray-traced path: pass rays, and not randomize_rays")
+
+    nt = pos_sorted.shape[0]
+    if rays is None:
+        gs = group_size if nt >= COLUMN_GROUP_MIN_TARGETS else 1
+        columns = engine.column_binned(tree.dev, pos_sorted, gs, theta)
+    elif randomize_rays:
+        # target k (in Morton order) sees the grid rotated by random_rotation(k) (treewalk.py:1061-1078)
+        rot = np.stack([rays @ random_rotation(k) for k in range(nt)])
+        columns = engine.column_rays(tree.dev, pos_sorted, rot, per_target=True)
+    else:
+        columns = engine.column_rays(tree.dev, pos_sorted, rays)
+    if bool(torch.isnan(columns).any()):
+        print("WARNING some column densities are NaN!")
+    unsorted = torch.empty_like(columns)
+    unsorted[idx] = columns  # undo the permutation
+    columns = _finish(unsorted, on_dev)
+    if return_tree:
+        return columns, tree
+    return columns
