@@ -288,9 +288,14 @@ ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, int64_t nnode
                            const double *rays, int64_t nrays, int per_target, double *out, int32_t *err, void *stream);
 /* Angular-binned (rays=None): out (nt, 6), the mean column per equal-area sky bin, with groups of
  * group_size consecutive targets sharing one traversal opened at the node's distance to the group's
- * box (ColumnDensityBinned_grouped :1197-1276; group_size 1 is ColumnDensityWalk_binned :976-1028). */
-ptg_status ptg_column_binned(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const double *targets,
-                             int64_t nt, int64_t group_size, double theta, double *out, int32_t *err, void *stream);
+ * box (ColumnDensityBinned_grouped :1197-1276; group_size 1 is ColumnDensityWalk_binned :976-1028).
+ * group_size <= 64 walks the FP64 walk pack wt of the same tree (ptg_pack_walk, PTG_F64) with
+ * ptg_column_workspace_bytes() of workspace; larger groups use the tree's links (wt, ws may be NULL).
+ * err: 1 corrupt links, 2 a traversal stack overflow. */
+size_t ptg_column_workspace_bytes(void);
+ptg_status ptg_column_binned(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_walk_tree *wt,
+                             const double *targets, int64_t nt, int64_t group_size, double theta, double *out,
+                             int32_t *err, void *ws, size_t ws_bytes, void *stream);
 
 /* ------------------------------------------------------------------------------------------
  * Direct summation, LDS-tiled: replaces bruteforce.py.  self_mode != 0: targets are the sources

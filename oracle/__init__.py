@@ -142,6 +142,7 @@ class OracleTree:
         m = _f64(m, (n,))
         h = _f64(h, (n,))
         L = lib()
+        self._free = L.ptgo_tree_free  # kept so the destructor still works at interpreter exit
         self.ptr = L.ptgo_build(_p(pos), _p(m), _p(h), n, int(quadrupole), int(compute_moments))
         info = np.zeros(5, np.int64)
         L.ptgo_tree_info(self.ptr, _p(info, _i64))
@@ -169,8 +170,8 @@ class OracleTree:
         self.TreewalkIndices = get(8, (n,), np.int64)
 
     def __del__(self):
-        if getattr(self, "ptr", None):
-            lib().ptgo_tree_free(self.ptr)
+        if getattr(self, "ptr", None) and getattr(self, "_free", None):
+            self._free(self.ptr)
             self.ptr = None
 
 

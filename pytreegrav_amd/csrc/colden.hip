@@ -8,9 +8,13 @@
 // exact: a node is opened whenever its bounding sphere, inflated by the largest particle radius in
 // it, could meet the ray; only particles contribute, by the closed-form chord integral.
 //
-// Angular-binned (ptg_column_binned): one thread per group of group_size consecutive targets, the
-// reference's grouped estimator: a node whose distance to the group's box passes the opening test
-// puts its mass in one of six equal-area sky bins per target; particles are summed per target.
+// Angular-binned (ptg_column_binned), the reference's grouped estimator: a node whose distance to
+// the box of group_size consecutive targets passes the opening test puts its mass in one of six
+// equal-area sky bins per target; particles are summed per target.  For group_size <= 64 one
+// wavefront walks a group over the FP64 walk pack (64 candidates per step, opened nodes pushed on an
+// LDS stack, accepted nodes and particles listed in LDS); every list entry is then evaluated for
+// the group's targets by 64 / group_size lane slices at once, whose partial bins are summed at the
+// end.  Larger groups take one thread per group along the depth-first links.
 #include <algorithm>
 
 #include "ptg_common.hpp"
@@ -156,7 +160,208 @@ __global__ __launch_bounds__(CD_THREADS) void k_column_binned(const ptg_tree_arr
     }
 }
 
+
+// ---- wavefront-per-group binned walk ----
+#define CB_WAVES 4
+#define CB_SCAP 1024   // LDS stack entries per wave (>= CB_SPILL + the 512 children one step can push)
+#define CB_SPILL 512
+#define CB_GCAP 16384  // global spill entries per wave
+#define CB_ICAP 128
+
+struct alignas(32) D4 {
+    double x, y, z, w;
+};
+
+struct CbLds {
+    int stack[CB_SCAP];
+    D4 nodes[CB_ICAP];
+    D4 parts[CB_ICAP];
+    double ph[CB_ICAP];
+};
+
+__device__ __forceinline__ void cb_add(double (&acc)[6], int bin, double v)
+{
+#pragma unroll
+    for (int k = 0; k < 6; k++) acc[k] += (k == bin) ? v : 0.0;
+}
+
+// list entries [0, cnt) on this lane's target, slice sl of `slices`
+__device__ __forceinline__ void cb_flush(const CbLds &L, int nn, int np, int sl, int slices, bool mine, double tx,
+                                         double ty, double tz, double (&acc)[6])
+{
+    const double bin_size = (4.0 * 3.14159265358979323846) / 6.0;
+    if (!mine) return;
+    for (int k = sl; k < nn; k += slices) {
+        const D4 c = L.nodes[k];
+        const double d0 = c.x - tx, d1 = c.y - ty, d2 = c.z - tz;
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        cb_add(acc, cd_bin(d0, d1, d2), c.w / r2 / bin_size);
+    }
+    for (int k = sl; k < np; k += slices) {
+        const D4 c = L.parts[k];
+        const double h = L.ph[k];
+        const double d0 = c.x - tx, d1 = c.y - ty, d2 = c.z - tz;
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        const int bin = cd_bin(d0, d1, d2);
+        const double hh = h * h;
+        if (r2 > hh) {
+            cb_add(acc, bin, c.w / r2 / bin_size);
+        } else if (h > 0) {
+            const double col0 = c.w * (3 / (4 * 3.14159265358979323846 * hh));
+            const double fac = sqrt(r2) / h;
+            const double iso = (1 - fac) * col0;
+#pragma unroll
+            for (int q = 0; q < 6; q++) acc[q] += iso;
+            cb_add(acc, bin, col0 * fac * 2);
+        }
+    }
+}
+
+__global__ __launch_bounds__(64 * CB_WAVES) void k_column_binned_wave(const D4 *xyzm, const D4 *aux, const int2 *kids,
+                                                                      int64_t nslots, const double *tp, int64_t nt,
+                                                                      int gs, double theta, double *out, int *spill_all,
+                                                                      int32_t *ctr, int32_t *err)
+{
+    __shared__ CbLds lds[CB_WAVES];
+    const int wave = threadIdx.x >> 6, lane = ptg_lane();
+    CbLds &L = lds[wave];
+    int *spill = spill_all + ((int64_t)blockIdx.x * CB_WAVES + wave) * CB_GCAP;
+    const int64_t ngroups = (nt + gs - 1) / gs;
+    const int slices = 64 / gs;
+    const int tl = lane % gs, sl = lane / gs;
+    for (;;) {
+        int gi = 0;
+        if (lane == 0) gi = atomicAdd(ctr, 1);
+        gi = __builtin_amdgcn_readfirstlane(gi);
+        if (gi >= ngroups) break;
+        const int64_t a = (int64_t)gi * gs;
+        const int mt = (int)std::min<int64_t>(gs, nt - a);
+        // the group's box (lanes 0..mt-1 hold its targets)
+        double bx0 = INFINITY, by0 = INFINITY, bz0 = INFINITY, bx1 = -INFINITY, by1 = -INFINITY, bz1 = -INFINITY;
+        if (lane < mt) {
+            bx0 = bx1 = tp[3 * (a + lane)];
+            by0 = by1 = tp[3 * (a + lane) + 1];
+            bz0 = bz1 = tp[3 * (a + lane) + 2];
+        }
+        bx0 = ptg_wave_min(bx0);
+        by0 = ptg_wave_min(by0);
+        bz0 = ptg_wave_min(bz0);
+        bx1 = ptg_wave_max(bx1);
+        by1 = ptg_wave_max(by1);
+        bz1 = ptg_wave_max(bz1);
+        const bool mine = sl < slices && tl < mt;
+        const double tx = mine ? tp[3 * (a + tl)] : 0.0, ty = mine ? tp[3 * (a + tl) + 1] : 0.0;
+        const double tz = mine ? tp[3 * (a + tl) + 2] : 0.0;
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        int sp = 1, gsp = 0, nn = 0, np = 0;
+        bool overflow = false;
+        if (lane == 0) L.stack[0] = 0;  // the root's walk slot
+        __builtin_amdgcn_wave_barrier();
+        for (;;) {
+            if (sp == 0) {
+                if (gsp == 0) break;
+                const int k = std::min(gsp, CB_SPILL);
+                for (int q = lane; q < k; q += 64) L.stack[q] = spill[gsp - k + q];
+                __builtin_amdgcn_wave_barrier();
+                gsp -= k;
+                sp = k;
+            }
+            const int k = std::min(sp, 64);
+            const int base = sp - k;
+            int e = lane < k ? L.stack[base + lane] : 0;
+            const bool ve = lane < k && e >= 0 && e < nslots;
+            if (!ve) e = 0;
+            __builtin_amdgcn_wave_barrier();
+            sp = base;
+            const D4 X = xyzm[e], X2 = aux[e];
+            const int2 K = kids[e];
+            const bool ispart = K.y == 0;
+            const double dxm = X.x < bx0 ? bx0 - X.x : (X.x > bx1 ? X.x - bx1 : 0.0);
+            const double dym = X.y < by0 ? by0 - X.y : (X.y > by1 ? X.y - by1 : 0.0);
+            const double dzm = X.z < bz0 ? bz0 - X.z : (X.z > bz1 ? X.z - bz1 : 0.0);
+            const double r_min = sqrt(dxm * dxm + dym * dym + dzm * dzm);
+            const double c1 = X2.y / theta + X2.z, c2 = X2.x + X2.y * 0.6 + X2.z;
+            const bool acc_node = ve && !ispart && r_min > (c1 > c2 ? c1 : c2);
+            const bool acc_part = ve && ispart;
+            const bool open = ve && !ispart && !acc_node;
+            const uint64_t bn = ptg_ballot(acc_node), bp = ptg_ballot(acc_part);
+            if (acc_node) L.nodes[nn + (int)__popcll(bn & ptg_lanemask_lt())] = X;
+            if (acc_part) {
+                const int at = np + (int)__popcll(bp & ptg_lanemask_lt());
+                L.parts[at] = X;
+                L.ph[at] = X2.x;
+            }
+            nn += (int)__popcll(bn);
+            np += (int)__popcll(bp);
+            const int cnt = open ? std::min(std::max(K.y, 0), 8) : 0;
+            const int incl = ptg_wave_incl_scan(cnt);
+            const int tot = __shfl(incl, 63, 64);
+            if (sp + tot > CB_SCAP) {
+                if (gsp + CB_SPILL > CB_GCAP) {
+                    overflow = true;
+                    break;
+                }
+                for (int q = lane; q < CB_SPILL; q += 64) spill[gsp + q] = L.stack[q];
+                __builtin_amdgcn_wave_barrier();
+                for (int q = 0; q < sp - CB_SPILL; q += 64) {
+                    const int v = (q + lane < sp - CB_SPILL) ? L.stack[CB_SPILL + q + lane] : 0;
+                    __builtin_amdgcn_wave_barrier();
+                    if (q + lane < sp - CB_SPILL) L.stack[q + lane] = v;
+                    __builtin_amdgcn_wave_barrier();
+                }
+                gsp += CB_SPILL;
+                sp -= CB_SPILL;
+            }
+            {
+                const int at = sp + incl - cnt;
+                for (int c = 0; c < cnt; c++) L.stack[at + c] = K.x + c;
+            }
+            sp += tot;
+            __builtin_amdgcn_wave_barrier();
+            if (nn >= 64 || np >= 64) {
+                cb_flush(L, nn, np, sl, slices, mine, tx, ty, tz, acc);
+                __builtin_amdgcn_wave_barrier();
+                nn = np = 0;
+            }
+        }
+        if (overflow) {
+            if (lane == 0) atomicOr(err, 2);
+            continue;
+        }
+        cb_flush(L, nn, np, sl, slices, mine, tx, ty, tz, acc);
+        __builtin_amdgcn_wave_barrier();
+        // sum the slices' partial bins into lanes 0 .. gs-1
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            double v = acc[q];
+            for (int s2 = 1; s2 < slices; s2++) {
+                const double o = __shfl(acc[q], tl + s2 * gs, 64);
+                if (sl == 0) v += o;
+            }
+            acc[q] = v;
+        }
+        if (sl == 0 && tl < mt)
+            for (int q = 0; q < 6; q++) out[6 * (a + tl) + q] = acc[q];
+    }
+}
+
+static int cb_cus()
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        cus = 256;
+    return cus;
+}
+
+extern "C" size_t ptg_column_workspace_bytes(void)
+{
+    const int cus = cb_cus();
+    return ptg_align256(sizeof(int32_t) * 64) + ptg_align256((size_t)cus * 2 * CB_WAVES * CB_GCAP * sizeof(int));
+}
+
 __global__ void k_column_err_init(int32_t *err) { *err = 0; }
+__global__ void k_column_ctr_init(int32_t *ctr) { *ctr = 0; }
 
 static ptg_status column_common(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const double *targets,
                                 int64_t nt, const char *who)
@@ -190,9 +395,10 @@ extern "C" ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, in
     return PTG_OK;
 }
 
-extern "C" ptg_status ptg_column_binned(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const double *targets,
-                                        int64_t nt, int64_t group_size, double theta, double *out, int32_t *err,
-                                        void *stream)
+extern "C" ptg_status ptg_column_binned(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
+                                        const ptg_walk_tree *wt, const double *targets, int64_t nt,
+                                        int64_t group_size, double theta, double *out, int32_t *err, void *ws,
+                                        size_t ws_bytes, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     ptg_status st = column_common(tree, n, nnodes, targets, nt, "ptg_column_binned");
@@ -204,6 +410,26 @@ extern "C" ptg_status ptg_column_binned(const ptg_tree_arrays *tree, int64_t n, 
     k_column_err_init<<<1, 1, 0, s>>>(err);
     PTG_LAUNCH_CHECK();
     if (nt == 0) return PTG_OK;
+    if (group_size <= 64) {
+        if (!wt || wt->prec != PTG_F64 || !wt->xyzm || !wt->aux || !wt->kids || wt->nslots != n + nnodes || !ws ||
+            ws_bytes < ptg_column_workspace_bytes()) {
+            ptg_set_error("ptg_column_binned: group_size <= 64 needs the FP64 walk pack of the tree and "
+                          "ptg_column_workspace_bytes() of workspace");
+            return PTG_ERR_ARG;
+        }
+        int32_t *ctr = (int32_t *)ws;
+        int *spill = (int *)((char *)ws + ptg_align256(sizeof(int32_t) * 64));
+        const int cus = cb_cus();
+        k_column_ctr_init<<<1, 1, 0, s>>>(ctr);
+        PTG_LAUNCH_CHECK();
+        const int64_t ngroups = ptg_cdiv(nt, group_size);
+        const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((int64_t)cus * 2, ptg_cdiv(ngroups, CB_WAVES)));
+        k_column_binned_wave<<<(unsigned)blocks, 64 * CB_WAVES, 0, s>>>(
+            (const D4 *)wt->xyzm, (const D4 *)wt->aux, (const int2 *)wt->kids, wt->nslots, targets, nt,
+            (int)group_size, theta, out, spill, ctr, err);
+        PTG_LAUNCH_CHECK();
+        return PTG_OK;
+    }
     const int64_t ngroups = ptg_cdiv(nt, group_size);
     k_column_binned<<<(unsigned)ptg_cdiv(ngroups, CD_THREADS), CD_THREADS, 0, s>>>(*tree, n, n + nnodes, targets, nt,
                                                                                    group_size, theta, out, err);

@@ -471,10 +471,16 @@ def column_binned(tree, targets, group_size=16, theta=0.5):
     nt = tp.shape[0]
     out = torch.empty((nt, 6), dtype=torch.float64, device=device())
     err = torch.zeros(1, dtype=torch.int32, device=device())
-    check(L.ptg_column_binned(_lib.ctypes.byref(tree.arrays), tree.n, tree.nnodes, ptr(tp), nt, int(group_size),
-                              float(theta), ptr(out), ptr(err), stream()), "ptg_column_binned")
-    if int(err.item()):
-        raise RuntimeError("ptg_column_binned: corrupt tree links")
+    wt, ws = None, None
+    if group_size <= 64:  # one wavefront per group over the FP64 walk pack
+        wt = _lib.ctypes.byref(tree.pack(PTG_F64, False)["wt"])
+        ws = workspace(L.ptg_column_workspace_bytes())
+    check(L.ptg_column_binned(_lib.ctypes.byref(tree.arrays), tree.n, tree.nnodes, wt, ptr(tp), nt, int(group_size),
+                              float(theta), ptr(out), ptr(err), ptr(ws), 0 if ws is None else ws.numel(), stream()),
+          "ptg_column_binned")
+    e = int(err.item())
+    if e:
+        raise RuntimeError("ptg_column_binned: " + ("corrupt tree links" if e & 1 else "traversal stack overflow"))
     return out
 
 

@@ -53,7 +53,7 @@ def test_per_target_rays_match_oracle():
     assert rel(got, want) < 1e-12
 
 
-@pytest.mark.parametrize("gs,theta", [(16, 0.5), (1, 0.5), (16, 0.8), (64, 0.3)])
+@pytest.mark.parametrize("gs,theta", [(16, 0.5), (1, 0.5), (16, 0.8), (64, 0.3), (48, 0.5), (100, 0.5)])
 def test_binned_matches_oracle(gs, theta):
     x, m, h = cloud(4000, 4)
     t = ConstructTree(x, m, h)
