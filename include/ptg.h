@@ -98,7 +98,9 @@ typedef struct {
 
 size_t ptg_build_workspace_bytes(int64_t n, int64_t nsys);
 /* A non-finite position, mass or softening fails with PTG_ERR_ARG and info->nonfinite = 1
- * (ConstructTree's check, frontend.py:123-125).  Pass ptg_build_tree the same pos, m and h.
+ * (ConstructTree's check, frontend.py:123-125).  Pass ptg_build_tree the same pos, m and h and the
+ * same workspace: the sort also packs the inputs into the workspace, and the tree's particle slots
+ * are gathered from that copy.
  * For a forest, info->center/size describe system 0 (all cubes are in forest->cubes). */
 ptg_status ptg_build_sort(const double *pos, const double *m, const double *h, int64_t n, const ptg_forest *forest,
                           void *ws, size_t ws_bytes, ptg_build_info *info, void *stream);

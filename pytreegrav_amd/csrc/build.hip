@@ -370,9 +370,25 @@ struct Counters {
     int32_t pad[9];
 };
 
-// Morton keys in input order; with m and h (and c), also the finiteness flag of every input
+// one particle's inputs in one 48-byte record, so that the Morton-order gather reads about one
+// cache line per particle instead of one from each of the three input arrays
+struct alignas(16) PartRec {
+    double x, y, z, m, h, pad;
+};
+
+__device__ __forceinline__ void put_rec(PartRec *rec, int64_t i, double x, double y, double z, double m, double h)
+{
+    double2 *r = (double2 *)(rec + i);
+    r[0] = make_double2(x, y);
+    r[1] = make_double2(z, m);
+    r[2] = make_double2(h, 0.0);
+}
+
+// Morton keys in input order; with m and h (and c), also the finiteness flag of every input and,
+// with rec, the particle records
 __global__ __launch_bounds__(256) void k_keys(const double *pos, const double *m, const double *h, int64_t n,
-                                              const Geom *g, uint64_t *keys, uint32_t *vals, Counters *c)
+                                              const Geom *g, uint64_t *keys, uint32_t *vals, Counters *c,
+                                              PartRec *rec)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool bad = false;
@@ -380,7 +396,11 @@ __global__ __launch_bounds__(256) void k_keys(const double *pos, const double *m
         const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
         keys[i] = morton_key(x, y, z, g->cmin, g->inv);
         vals[i] = (uint32_t)i;
-        if (c) bad = !isfinite(x) || !isfinite(y) || !isfinite(z) || !isfinite(m[i]) || !isfinite(h[i]);
+        if (c) {
+            const double mi = m[i], hi = h[i];
+            bad = !isfinite(x) || !isfinite(y) || !isfinite(z) || !isfinite(mi) || !isfinite(hi);
+            if (rec) put_rec(rec, i, x, y, z, mi, hi);
+        }
     }
     if (c && ptg_ballot(bad) && ptg_lane() == 0) c->nonfinite = 1;
 }
@@ -441,16 +461,19 @@ __global__ __launch_bounds__(256) void k_bbox_sys(const double *pos, const int64
     for (int k = 0; k < 3; k++) g->cmin[k] = g->center[k] - 0.5 * sz;
 }
 
-__global__ __launch_bounds__(256) void k_keys_sys(const double *pos, int64_t n, const Geom *geoms,
-                                                  const uint32_t *sys, uint64_t *keys, uint64_t *kU, uint32_t *vals)
+__global__ __launch_bounds__(256) void k_keys_sys(const double *pos, const double *m, const double *h, int64_t n,
+                                                  const Geom *geoms, const uint32_t *sys, uint64_t *keys,
+                                                  uint64_t *kU, uint32_t *vals, PartRec *rec)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Geom *g = geoms + sys[i];
-    uint64_t k = morton_key(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], g->cmin, g->inv);
+    const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    uint64_t k = morton_key(x, y, z, g->cmin, g->inv);
     keys[i] = k;
     kU[i] = k;
     vals[i] = (uint32_t)i;
+    if (rec) put_rec(rec, i, x, y, z, m[i], h[i]);
 }
 
 __global__ __launch_bounds__(256) void k_gather_sys_key(const uint32_t *sys, const uint32_t *vals, int64_t n,
@@ -515,6 +538,7 @@ struct BuildWS {
     uint64_t *kU;
     const int64_t *soff;
     int64_t nsys;
+    PartRec *rec;  // particle records in input order (written by the key pass when m and h are given)
 };
 
 static bool carve_build(PtgArena &A, int64_t n, BuildWS &w, int64_t nsys = 1)
@@ -548,6 +572,7 @@ static bool carve_build(PtgArena &A, int64_t n, BuildWS &w, int64_t nsys = 1)
     w.nsys = nsys;
     w.soff = nullptr;
     w.geoms = A.take<Geom>(nsys);
+    w.rec = A.take<PartRec>(n);
     if (nsys > 1) {
         w.sys = A.take<uint32_t>(n);
         w.kU = A.take<uint64_t>(n);
@@ -794,7 +819,8 @@ static ptg_status sort_by_keys(const double *pos, const double *m, const double 
     int nb = (int)std::min<int64_t>(BB_BLOCKS, std::max<int64_t>(1, ptg_cdiv(n, 1024)));
     k_bbox_partial<<<nb, 256, 0, s>>>(pos, n, w.part);
     k_bbox_final<<<1, 256, 0, s>>>(w.part, nb, w.geom);
-    k_keys<<<grid1(n), 256, 0, s>>>(pos, m, h, n, w.geom, w.kA, w.vA, m && h ? w.ctr : nullptr);
+    k_keys<<<grid1(n), 256, 0, s>>>(pos, m, h, n, w.geom, w.kA, w.vA, m && h ? w.ctr : nullptr,
+                                    m && h ? w.rec : nullptr);
     PTG_LAUNCH_CHECK();
     bool in_first;
     ptg_status st = radix_sort(w.kA, w.vA, w.kB, w.vB, n, 64, w.hist, w.bsum, &in_first, s);
@@ -820,7 +846,7 @@ static ptg_status sort_by_keys_forest(const double *pos, const double *m, const 
 {
     k_sys_of<<<grid1(n), 256, 0, s>>>(w.soff, w.nsys, n, w.sys);
     k_bbox_sys<<<(unsigned)w.nsys, 256, 0, s>>>(pos, w.soff, w.geoms);
-    k_keys_sys<<<grid1(n), 256, 0, s>>>(pos, n, w.geoms, w.sys, w.kA, w.kU, w.vA);
+    k_keys_sys<<<grid1(n), 256, 0, s>>>(pos, m, h, n, w.geoms, w.sys, w.kA, w.kU, w.vA, m && h ? w.rec : nullptr);
     PTG_LAUNCH_CHECK();
     bool in_first;
     ptg_status st = radix_sort(w.kA, w.vA, w.kB, w.vB, n, 64, w.hist, w.bsum, &in_first, s);
@@ -1108,20 +1134,22 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
     }
 }
 
-// particle slots: gather into Morton order (octree.py:577-603) and link to the next element
-__global__ __launch_bounds__(256) void k_particles(const double *pos, const double *m, const double *h,
-                                                   const uint32_t *order, const uint64_t *keys, const int32_t *lev,
-                                                   const int32_t *off, const uint32_t *sys, int64_t n,
-                                                   const ptg_tree_arrays t)
+// particle slots: gather the input records into Morton order (octree.py:577-603) and link to the
+// next element
+__global__ __launch_bounds__(256) void k_particles(const PartRec *rec, const uint32_t *order, const uint64_t *keys,
+                                                   const int32_t *lev, const int32_t *off, const uint32_t *sys,
+                                                   int64_t n, const ptg_tree_arrays t)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int64_t o = order[i];
-    t.coordinates[3 * i] = pos[3 * o];
-    t.coordinates[3 * i + 1] = pos[3 * o + 1];
-    t.coordinates[3 * i + 2] = pos[3 * o + 2];
-    t.masses[i] = m[o];
-    t.softenings[i] = h[o];
+    const double2 *r = (const double2 *)(rec + o);
+    const double2 r0 = r[0], r1 = r[1], r2 = r[2];
+    t.coordinates[3 * i] = r0.x;
+    t.coordinates[3 * i + 1] = r0.y;
+    t.coordinates[3 * i + 2] = r1.x;
+    t.masses[i] = r1.y;
+    t.softenings[i] = r2.x;
     t.sizes[i] = 0.0;
     t.deltas[i] = 0.0;
     t.treewalk_indices[i] = o;
@@ -1147,10 +1175,6 @@ __global__ void k_single_root(const Geom *g, const ptg_tree_arrays t)
 // links and moments
 // ============================================================================================
 // a child's moments as its parent reads them: mass, sum of m*x (nodes), softening, centre of mass
-struct MomRec {
-    double m, a0, a1, a2, h, c0, c1, c2;
-};
-
 struct NodeWS {
     double *acc;        // (nnodes, 3) sum of m*x of each node, by node offset (element - n)
     int32_t *err;       // [0] corrupt-chain flag
@@ -1160,6 +1184,7 @@ struct NodeWS {
     int32_t *hist;
     int32_t *lvstart;      // (max_level + 2) first position of each level in the sorted list
     int32_t *elem;         // (n + nnodes) element stored at each walk slot
+    int32_t *qbase;        // (nnodes) first child slot of each node, by level-sorted position
 };
 
 static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, int64_t max_level, NodeWS &w)
@@ -1175,6 +1200,7 @@ static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, int64_t max_leve
     w.hist = A.take<int32_t>(nh);
     w.lvstart = A.take<int32_t>(max_level + 2);
     w.elem = A.take<int32_t>(n + nnodes);
+    w.qbase = A.take<int32_t>(nnodes);
     return A.ok();
 }
 
@@ -1268,39 +1294,70 @@ __global__ __launch_bounds__(256) void k_level_starts(const uint64_t *lv, int64_
 }
 
 
-// One node's moments, in the reference's summation order: particle children in digit order
-// (folded during the split, octree.py:956-966), then node children in descending index order,
-// i.e. descending digit order (RollupMoments' reverse scan, octree.py:212-231).  Zero-mass node
-// children contribute nothing (:215-216).  Quadrupoles follow RollupQuadrupoles (:252-292).
-// K[0..nk): the children's records in digit order (the records at slots child_base + k).
-__device__ void node_moments(int64_t idx, int nk, const MomRec *K, const int32_t *kn, const double (*KQ)[9], int64_t n,
-                             const ptg_tree_arrays &t, NodeWS &w, bool quad)
+// One node's moments, one node per lane, in the reference's summation order: particle children in
+// digit order (folded during the split, octree.py:956-966), then node children in descending index
+// order, i.e. descending digit order (RollupMoments' reverse scan, octree.py:212-231).  Zero-mass
+// node children contribute nothing (:215-216).  Quadrupoles follow RollupQuadrupoles (:252-292).
+// Every child record is loaded unconditionally (absent children read a valid stand-in element) so
+// that all of a lane's gathers are in flight together; absent children are then masked out of the
+// sums, which therefore see exactly the reference's sequence of operations.
+template <bool QUAD>
+__device__ __forceinline__ void lane_moments(int64_t idx, int64_t n, int64_t ne, const ptg_tree_arrays &t, NodeWS &w)
 {
-    double m = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, hs = 0.0;
-    for (int k = 0; k < nk; k++) {
-        if (kn[k] >= 0) continue;
-        double mi = K[k].m;
-        m += mi;
-        a0 += mi * K[k].c0;
-        a1 += mi * K[k].c1;
-        a2 += mi * K[k].c2;
-        double hc = K[k].h;
-        if (!(hs >= hc)) hs = hc;
+    int nk = t.nchild[idx];
+    const int64_t base = t.child_base[idx];
+    if (nk < 0 || nk > 8 || base < 1 || base + nk > ne) {
+        w.err[0] = 1;
+        nk = 0;
     }
-    for (int k = nk - 1; k >= 0; k--) {
-        if (kn[k] < 0) continue;
-        double mc = K[k].m;
-        if (mc == 0.0) continue;
-        a0 += K[k].a0;
-        a1 += K[k].a1;
-        a2 += K[k].a2;
-        m += mc;
-        double hc = K[k].h;
-        if (!(hs >= hc)) hs = hc;
+    int32_t c[8];
+    bool part[8], node[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        int64_t e = k < nk ? (int64_t)w.elem[base + k] : 0;
+        if (e < 0 || e >= ne) {  // a malformed slot map is reported, never followed
+            w.err[0] = 1;
+            e = 0;
+        }
+        c[k] = (int32_t)e;
+        part[k] = k < nk && e < n;
+        node[k] = k < nk && e >= n;
+    }
+    // per child: mass, softening and one 3-vector (a particle's position, a node's sum of m*x)
+    double km[8], kh[8], kv0[8], kv1[8], kv2[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int64_t e = c[k];
+        const double *v = node[k] ? w.acc + 3 * (e - n) : t.coordinates + 3 * e;
+        km[k] = t.masses[e];
+        kh[k] = t.softenings[e];
+        kv0[k] = v[0];
+        kv1[k] = v[1];
+        kv2[k] = v[2];
+    }
+    double com0 = t.coordinates[3 * idx], com1 = t.coordinates[3 * idx + 1], com2 = t.coordinates[3 * idx + 2];
+    double m = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, hs = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        if (!part[k]) continue;
+        const double mi = km[k];
+        m += mi;
+        a0 += mi * kv0[k];
+        a1 += mi * kv1[k];
+        a2 += mi * kv2[k];
+        if (!(hs >= kh[k])) hs = kh[k];
+    }
+#pragma unroll
+    for (int k = 7; k >= 0; k--) {
+        if (!node[k] || km[k] == 0.0) continue;
+        a0 += kv0[k];
+        a1 += kv1[k];
+        a2 += kv2[k];
+        m += km[k];
+        if (!(hs >= kh[k])) hs = kh[k];
     }
     t.masses[idx] = m;
     t.softenings[idx] = hs;
-    double com0 = t.coordinates[3 * idx], com1 = t.coordinates[3 * idx + 1], com2 = t.coordinates[3 * idx + 2];
     if (m != 0.0) {
         double inv = 1.0 / m;
         double c0 = a0 * inv, c1 = a1 * inv, c2 = a2 * inv;
@@ -1323,7 +1380,7 @@ __device__ void node_moments(int64_t idx, int nk, const MomRec *K, const int32_t
     w.acc[3 * ia] = a0;
     w.acc[3 * ia + 1] = a1;
     w.acc[3 * ia + 2] = a2;
-    if (!quad) return;
+    if constexpr (!QUAD) return;
     double Q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     auto shift = [&](double mm, double r0, double r1, double r2) {
         double rsq = r0 * r0 + r1 * r1 + r2 * r2;
@@ -1337,102 +1394,51 @@ __device__ void node_moments(int64_t idx, int nk, const MomRec *K, const int32_t
         Q[7] += mm * 3.0 * r2 * r1;
         Q[8] += mm * (3.0 * r2 * r2 - rsq);
     };
-    for (int k = 0; k < nk; k++) {
-        if (kn[k] >= 0) continue;
-        shift(K[k].m, K[k].c0 - com0, K[k].c1 - com1, K[k].c2 - com2);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        if (!part[k]) continue;
+        shift(km[k], kv0[k] - com0, kv1[k] - com1, kv2[k] - com2);
     }
-    for (int k = nk - 1; k >= 0; k--) {
-        if (kn[k] < 0) continue;
-        double mc = K[k].m;
-        if (mc == 0.0) continue;
-        for (int c = 0; c < 9; c++) Q[c] += KQ[k][c];
-        shift(mc, K[k].c0 - com0, K[k].c1 - com1, K[k].c2 - com2);
+#pragma unroll
+    for (int k = 7; k >= 0; k--) {
+        if (!node[k] || km[k] == 0.0) continue;
+        const int64_t e = c[k];
+        const double *KQ = t.quadrupoles + 9 * e;
+#pragma unroll
+        for (int j = 0; j < 9; j++) Q[j] += KQ[j];
+        shift(km[k], t.coordinates[3 * e] - com0, t.coordinates[3 * e + 1] - com1, t.coordinates[3 * e + 2] - com2);
     }
-    for (int c = 0; c < 9; c++) t.quadrupoles[9 * idx + c] = Q[c];
+#pragma unroll
+    for (int j = 0; j < 9; j++) t.quadrupoles[9 * idx + j] = Q[j];
 }
 
-// Eight lanes per node: lane q loads the moments of child q into LDS, so a node's children are
-// fetched in parallel; the node's first lane then sums them.  A wave does eight nodes.
-struct CoopLds {
-    MomRec K[64];
-    double Q[64][9];
-    int32_t kn[64];
-};
-#define MC_WAVES 4
+#define MOM_THREADS 256
 
-__device__ void coop_moments(int64_t qi, bool valid, int64_t n, int64_t nnodes, const uint32_t *byid,
-                             const ptg_tree_arrays &t, NodeWS &w, bool quad, CoopLds &S)
+// one level, one node per lane
+template <bool QUAD>
+__global__ __launch_bounds__(MOM_THREADS) void k_moments_level(int64_t n, int64_t nnodes, int L,
+                                                               const int32_t *lvstart, const uint32_t *byid,
+                                                               const ptg_tree_arrays t, NodeWS w)
 {
-    const int lane = ptg_lane(), q = lane & 7, lead = lane & ~7;
-    const int64_t ne = n + nnodes;
-    const int64_t idx = valid ? n + (int64_t)byid[qi] : 0;
-    int nk = valid ? t.nchild[idx] : 0;
-    const int64_t base = valid ? t.child_base[idx] : 0;
-    const bool bad = valid && (nk < 0 || nk > 8 || base < 1 || base + nk > ne);
-    if (bad) {
-        if (q == 0) w.err[0] = 1;
-        nk = 0;
-    }
-    if (q < nk) {
-        int64_t c = w.elem[base + q];
-        if (c < 0 || c >= ne) {  // a malformed slot map is reported, never followed
-            w.err[0] = 1;
-            c = 0;
-        }
-        MomRec r;
-        r.m = t.masses[c];
-        r.h = t.softenings[c];
-        r.c0 = t.coordinates[3 * c];
-        r.c1 = t.coordinates[3 * c + 1];
-        r.c2 = t.coordinates[3 * c + 2];
-        r.a0 = r.a1 = r.a2 = 0.0;
-        if (c >= n) {
-            const int64_t ca = c - n;
-            r.a0 = w.acc[3 * ca];
-            r.a1 = w.acc[3 * ca + 1];
-            r.a2 = w.acc[3 * ca + 2];
-            if (quad)
-                for (int k = 0; k < 9; k++) S.Q[lane][k] = t.quadrupoles[9 * c + k];
-        }
-        S.K[lane] = r;
-        S.kn[lane] = c >= n ? (int32_t)(c - n) : -1;
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (q == 0 && valid && !bad) node_moments(idx, nk, S.K + lead, S.kn + lead, S.Q + lead, n, t, w, quad);
-    __builtin_amdgcn_wave_barrier();
-}
-
-// one level, eight nodes per wave
-__global__ __launch_bounds__(64 * MC_WAVES) void k_moments_level(int64_t n, int64_t nnodes, int L,
-                                                                 const int32_t *lvstart, const uint32_t *byid,
-                                                                 const ptg_tree_arrays t, NodeWS w, int quad)
-{
-    __shared__ CoopLds S[MC_WAVES];
-    const int wave = threadIdx.x >> 6, j = ptg_lane() >> 3;
     const int64_t lo = lvstart[L], hi = lvstart[L + 1];
-    const int64_t stride = (int64_t)gridDim.x * MC_WAVES * 8;
-    for (int64_t b = lo + ((int64_t)blockIdx.x * MC_WAVES + wave) * 8; b < hi; b += stride) {
-        const int64_t qi = b + j;
-        coop_moments(qi, qi < hi, n, nnodes, byid, t, w, quad != 0, S[wave]);
-    }
+    const int64_t stride = (int64_t)gridDim.x * MOM_THREADS;
+    for (int64_t qi = lo + (int64_t)blockIdx.x * MOM_THREADS + threadIdx.x; qi < hi; qi += stride)
+        lane_moments<QUAD>(n + (int64_t)byid[qi], n, n + nnodes, t, w);
 }
 
 // levels Lhi down to Llo (each with few nodes) in one block, a barrier between levels: all waves
 // of a workgroup share one CU's L1, so the barrier orders one level's writes before the next
 // level's reads
-#define MS_WAVES 16
-__global__ __launch_bounds__(64 * MS_WAVES) void k_moments_small(int64_t n, int64_t nnodes, int Lhi, int Llo,
-                                                                 const int32_t *lvstart, const uint32_t *byid,
-                                                                 const ptg_tree_arrays t, NodeWS w, int quad)
+#define MS_THREADS 1024
+template <bool QUAD>
+__global__ __launch_bounds__(MS_THREADS) void k_moments_small(int64_t n, int64_t nnodes, int Lhi, int Llo,
+                                                              const int32_t *lvstart, const uint32_t *byid,
+                                                              const ptg_tree_arrays t, NodeWS w)
 {
-    __shared__ CoopLds S[MS_WAVES];
-    const int wave = threadIdx.x >> 6, j = ptg_lane() >> 3;
     for (int L = Lhi; L >= Llo; L--) {
         const int64_t lo = lvstart[L], hi = lvstart[L + 1];
-        for (int64_t b = lo + (int64_t)wave * 8; b < hi; b += MS_WAVES * 8) {
-            const int64_t qi = b + j;
-            coop_moments(qi, qi < hi, n, nnodes, byid, t, w, quad != 0, S[wave]);
-        }
+        for (int64_t qi = lo + threadIdx.x; qi < hi; qi += MS_THREADS)
+            lane_moments<QUAD>(n + (int64_t)byid[qi], n, n + nnodes, t, w);
         __syncthreads();
     }
 }
@@ -1460,16 +1466,19 @@ __global__ __launch_bounds__(256) void k_fold_particles(int64_t n, int64_t nnode
         for (int k = 0; k < 9; k++) t.quadrupoles[9 * idx + k] = 0.0;
 }
 
-struct LoadNChild {
+struct LoadNChildByLevel {
     const int32_t *nchild;
     int64_t n;
-    __device__ int32_t operator()(int64_t a) const { return nchild[n + a]; }
+    const uint32_t *byid;
+    __device__ int32_t operator()(int64_t q) const { return nchild[n + byid[q]]; }
 };
 
-__global__ __launch_bounds__(256) void k_child_base_fix(int64_t n, int64_t nnodes, int64_t nsys, int32_t *child_base)
+__global__ __launch_bounds__(256) void k_child_base_scatter(int64_t n, int64_t nnodes, int64_t nsys,
+                                                            const uint32_t *byid, const int32_t *qbase,
+                                                            int32_t *child_base)
 {
-    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < nnodes) child_base[n + a] += (int32_t)nsys;  // slots [0, nsys) hold the roots
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nnodes) child_base[n + byid[q]] = qbase[q] + (int32_t)nsys;  // slots [0, nsys) hold the roots
 }
 
 // forest roots: pair offsets[s] (the first pair of system s) opens its root
@@ -1518,7 +1527,7 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     const bool F = nsys > 1;
     if (F) w.soff = forest->offsets;
     KeyView kv{w.kA, info->has_collisions ? w.ext : nullptr, w.sys, n};
-    k_particles<<<grid1(n), 256, 0, s>>>(pos, m, h, w.vA, w.kA, w.lev, w.off, w.sys, n, t);
+    k_particles<<<grid1(n), 256, 0, s>>>(w.rec, w.vA, w.kA, w.lev, w.off, w.sys, n, t);
     if (particles_ready) PTG_HIP_CHECK(hipEventRecord((hipEvent_t)particles_ready, s));
     if (n == 1) {
         k_single_root<<<1, 1, 0, s>>>(w.geoms, t);
@@ -1532,42 +1541,50 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     k_node_init<<<1, 64, 0, s>>>(nw);
     k_link<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t, nw);
     PTG_LAUNCH_CHECK();
-    // sibling-contiguous walk slots: exclusive scan of child counts in node order
-    st = excl_scan(LoadNChild{t.nchild, n}, nnodes, t.child_base + n, nw.bsum, nullptr, s);
+    // level-sorted node list (stable LSD radix sort of node offsets on the level): node index order
+    // within each level
+    k_iota_u32<<<grid1(nnodes), 256, 0, s>>>(nw.idA, nnodes);
+    int nbits = 1;
+    while ((1ll << nbits) <= (int64_t)max_level) nbits++;
+    nbits = ((nbits + RS_BITS - 1) / RS_BITS) * RS_BITS;
+    bool in_first;
+    st = radix_sort(nw.lvA, nw.idA, nw.lvB, nw.idB, nnodes, nbits, nw.hist, nw.bsum, &in_first, s);
     if (st) return st;
-    k_child_base_fix<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, t.child_base);
+    const uint64_t *lv = in_first ? nw.lvA : nw.lvB;
+    const uint32_t *byid = in_first ? nw.idA : nw.idB;
+    k_level_starts<<<grid1(nnodes + 1), 256, 0, s>>>(lv, nnodes, max_level, nw.lvstart);
+    PTG_LAUNCH_CHECK();
+    // sibling-contiguous walk slots, level-major: an exclusive scan of child counts over the level
+    // list puts the children of each level's nodes in one slot range, in node order
+    st = excl_scan(LoadNChildByLevel{t.nchild, n, byid}, nnodes, nw.qbase, nw.bsum, nullptr, s);
+    if (st) return st;
+    k_child_base_scatter<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, byid, nw.qbase, t.child_base);
     k_slots<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, F ? forest->roots : nullptr, t, nw);
     PTG_LAUNCH_CHECK();
     if (compute_moments) {
-        // counting sort of node offsets by level (stable LSD radix sort on the level)
-        k_iota_u32<<<grid1(nnodes), 256, 0, s>>>(nw.idA, nnodes);
-        int nbits = 1;
-        while ((1ll << nbits) <= (int64_t)max_level) nbits++;
-        nbits = ((nbits + RS_BITS - 1) / RS_BITS) * RS_BITS;
-        bool in_first;
-        ptg_status st0 = radix_sort(nw.lvA, nw.idA, nw.lvB, nw.idB, nnodes, nbits, nw.hist, nw.bsum, &in_first, s);
-        if (st0) return st0;
-        const uint64_t *lv = in_first ? nw.lvA : nw.lvB;
-        const uint32_t *byid = in_first ? nw.idA : nw.idB;
-        k_level_starts<<<grid1(nnodes + 1), 256, 0, s>>>(lv, nnodes, max_level, nw.lvstart);
-        PTG_LAUNCH_CHECK();
         // per-level node counts decide the launches: runs of small levels go to one block, the
         // others get a grid each
         std::vector<int32_t> ls(max_level + 2);
         PTG_HIP_CHECK(hipMemcpyAsync(ls.data(), nw.lvstart, (max_level + 2) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
         PTG_HIP_CHECK(hipStreamSynchronize(s));
-        const int64_t small = 512;
+        const int64_t small = MS_THREADS;
         int L = max_level;
         while (L >= 0) {
             if (ls[L + 1] - ls[L] <= small) {
                 int Llo = L;
                 while (Llo - 1 >= 0 && ls[Llo] - ls[Llo - 1] <= small) Llo--;
-                k_moments_small<<<1, 64 * MS_WAVES, 0, s>>>(n, nnodes, L, Llo, nw.lvstart, byid, t, nw, quadrupole);
+                if (quadrupole)
+                    k_moments_small<true><<<1, MS_THREADS, 0, s>>>(n, nnodes, L, Llo, nw.lvstart, byid, t, nw);
+                else
+                    k_moments_small<false><<<1, MS_THREADS, 0, s>>>(n, nnodes, L, Llo, nw.lvstart, byid, t, nw);
                 L = Llo - 1;
             } else {
                 const int64_t cnt = ls[L + 1] - ls[L];
-                const unsigned blocks = (unsigned)std::min<int64_t>(4096, ptg_cdiv(cnt, MC_WAVES * 8));
-                k_moments_level<<<blocks, 64 * MC_WAVES, 0, s>>>(n, nnodes, L, nw.lvstart, byid, t, nw, quadrupole);
+                const unsigned blocks = (unsigned)std::min<int64_t>(8192, ptg_cdiv(cnt, MOM_THREADS));
+                if (quadrupole)
+                    k_moments_level<true><<<blocks, MOM_THREADS, 0, s>>>(n, nnodes, L, nw.lvstart, byid, t, nw);
+                else
+                    k_moments_level<false><<<blocks, MOM_THREADS, 0, s>>>(n, nnodes, L, nw.lvstart, byid, t, nw);
                 L--;
             }
         }
