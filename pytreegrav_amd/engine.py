@@ -172,18 +172,23 @@ class DeviceTree:
         if key not in self._packs:
             # take over the plain FP32 pack if there is one (its aux[3] is overwritten, so it leaves
             # the plain cache and is rebuilt there when needed again)
-            p = self._packs.pop((PTG_F32, bool(quadrupole)), None) or self.pack(PTG_F32, quadrupole, cache=False)
+            p = self._packs.pop((PTG_F32, bool(quadrupole)), None) or self.pack(PTG_F32, quadrupole, cache=False,
+                                                                                 norms=False)
             open_radii(self, p, model, eps, bf16=bf16, scale=scale)
             for k in [k for k in self._packs if k[0] == "learned"]:  # one learned pack per tree
                 del self._packs[k]
             self._packs[key] = p
         return self._packs[key]
 
-    def pack(self, prec, quadrupole, cache=True):
-        """Sibling-contiguous walk copy (cached per precision and moment order)."""
+    def pack(self, prec, quadrupole, cache=True, norms=None):
+        """Sibling-contiguous walk copy (cached per precision and moment order).  aux[3] holds each
+        node's quadrupole norm, which only the error-bounded criterion reads: norms=True demands it,
+        norms=False skips it (aux[3] = -1, and the quadrupoles are not read) for a pack whose aux[3]
+        will carry learned opening radii, None takes either."""
         key = (prec, bool(quadrupole))
-        if cache and key in self._packs:
+        if cache and key in self._packs and (norms is not True or self._packs[key]["norms"]):
             return self._packs[key]
+        norms = norms is not False or bool(quadrupole)
         dev = device()
         dt = torch.float32 if prec == PTG_F32 else torch.float64
         ne = self.ne
@@ -194,9 +199,13 @@ class DeviceTree:
         origin = self.center if prec == PTG_F32 else np.zeros(3)
         wt = WalkTree(ptr(xyzm).value, ptr(aux).value, ptr(kids).value, ptr(quad).value if quad is not None else None,
                       ne, prec, int(bool(quadrupole)), (_lib._dbl * 3)(*origin))
-        check(_lib.load().ptg_pack_walk(_lib.ctypes.byref(self.arrays), self.n, self.nnodes, self.forest_ref,
+        arrays = self.arrays
+        if not norms and arrays.quadrupoles:
+            arrays = TreeArrays.from_buffer_copy(arrays)
+            arrays.quadrupoles = None
+        check(_lib.load().ptg_pack_walk(_lib.ctypes.byref(arrays), self.n, self.nnodes, self.forest_ref,
                                         _lib.ctypes.byref(wt), stream()), "ptg_pack_walk")
-        p = {"wt": wt, "tensors": (xyzm, aux, kids, quad)}
+        p = {"wt": wt, "tensors": (xyzm, aux, kids, quad), "norms": norms or not self.arrays.quadrupoles}
         if cache:
             self._packs[key] = p
         return p
@@ -324,7 +333,7 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     if criterion == "learned":
         p = tree.pack_learned(model, eps, quadrupole, bf16, scale=eps_scale)
     else:
-        p = tree.pack(prec, quadrupole)
+        p = tree.pack(prec, quadrupole, norms=True if criterion == "error" else None)
     wsb = L.ptg_walk_workspace_bytes(nt, int(group_size))
     if supergroups:
         wsb += L.ptg_walk_shared_workspace_bytes(nt, int(tree.nsys if forest else 1), int(group_size))
@@ -417,7 +426,7 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
     offsets.record_stream(main)
     count.record_stream(main)
     eps, scale, exp_eps = None, None, None
-    tree.pack(PTG_F32, quadrupole)
+    tree.pack(PTG_F32, quadrupole, norms=criterion != "learned")
     if criterion != "theta" or expansion is not None:
         main.wait_stream(side)  # the sample groups are taken from the cell groups
         if criterion == "error":

@@ -120,7 +120,7 @@ class Forest:
         out = torch.empty((nt, 3), dtype=torch.float64, device=dev)
         L = _lib.load()
         ws = workspace(L.ptg_walk_workspace_bytes(nt, gs))
-        p = t.pack(PTG_F32, False)
+        p = t.pack(PTG_F32, False, norms=False if self.criterion == "learned" else None)
         crit = _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
         perm = torch.arange(nt, device=dev, dtype=torch.int64)
         check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), _lib.ctypes.byref(fs), ptr(pos), ptr(h), ptr(perm), 1, nt,
