@@ -1087,9 +1087,12 @@ __device__ __forceinline__ int32_t head_of(int64_t q, int64_t n, const int32_t *
     return (int32_t)q;
 }
 
+// node geometry, levels and first links of the nodes opened by pair i (one thread per pair); each
+// node's NextBranch needs the end of its cell, searched by k_node_ends with one thread per node so
+// that the long searches of the large cells run in parallel
 __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *lev, const int32_t *off, int64_t n,
                                                     const Geom *geoms, double *coord, double *sizes,
-                                                    int32_t *first, int32_t *next, uint64_t *nlevel)
+                                                    int32_t *first, uint64_t *nlevel, int32_t *npair)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
@@ -1099,7 +1102,6 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
     const Geom *g = geoms + (kv.sys ? kv.sys[i] : 0);
     double c[3] = {g->center[0], g->center[1], g->center[2]};
     double s = g->size;
-    int64_t e = i;  // running end of the level-L cell containing slot i
     for (int L = 0; L <= li; L++) {
         if (L > lp) {
             int64_t idx = n + off[i] + (L - lp - 1);
@@ -1108,22 +1110,8 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
             coord[3 * idx + 2] = c[2];
             sizes[idx] = s;
             nlevel[idx - n] = (uint64_t)L;
+            npair[idx - n] = (int32_t)i;
             first[idx] = (L + 1 <= li) ? (int32_t)(idx + 1) : (int32_t)i;
-            // last slot of this cell: gallop then bisect on "same level-L cell as slot i"
-            int64_t lo = i + 1;  // slot i+1 shares level li >= L with slot i
-            int64_t step = 1;
-            while (lo + step < n && kv.same_cell(i, lo + step, L)) {
-                lo += step;
-                step <<= 1;
-            }
-            int64_t hi = std::min<int64_t>(lo + step, n);  // first candidate known not to match (or n)
-            while (hi - lo > 1) {
-                int64_t mid = (lo + hi) >> 1;
-                if (kv.same_cell(i, mid, L)) lo = mid;
-                else hi = mid;
-            }
-            e = lo;
-            next[idx] = head_of(e + 1, n, lev, off, kv.sys);
         }
         if (L < li) {
             int d = kv.digit(i, L + 1);
@@ -1132,6 +1120,35 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
             s = s * 0.5;
         }
     }
+}
+
+// NextBranch of node a (opened by pair i at level L): the element after the last slot of its cell,
+// found by galloping then bisecting on "same level-L cell as slot i"
+__global__ __launch_bounds__(256) void k_node_ends(KeyView kv, const int32_t *lev, const int32_t *off, int64_t n,
+                                                   int64_t nnodes, const uint64_t *nlevel, const int32_t *npair,
+                                                   int32_t *next)
+{
+    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nnodes) return;
+    const int64_t i = npair[a];
+    const int L = (int)nlevel[a];
+    if (i < 0 || i >= n - 1 || L < 0) {  // every node is opened by one pair: never taken
+        next[n + a] = -1;
+        return;
+    }
+    int64_t lo = i + 1;  // slot i+1 shares the pair's level >= L with slot i
+    int64_t step = 1;
+    while (lo + step < n && kv.same_cell(i, lo + step, L)) {
+        lo += step;
+        step <<= 1;
+    }
+    int64_t hi = std::min<int64_t>(lo + step, n);  // first candidate known not to match (or n)
+    while (hi - lo > 1) {
+        int64_t mid = (lo + hi) >> 1;
+        if (kv.same_cell(i, mid, L)) lo = mid;
+        else hi = mid;
+    }
+    next[n + a] = head_of(lo + 1, n, lev, off, kv.sys);
 }
 
 // particle slots: gather the input records into Morton order (octree.py:577-603) and link to the
@@ -1185,6 +1202,7 @@ struct NodeWS {
     int32_t *lvstart;      // (max_level + 2) first position of each level in the sorted list
     int32_t *elem;         // (n + nnodes) element stored at each walk slot
     int32_t *qbase;        // (nnodes) first child slot of each node, by level-sorted position
+    int32_t *npair;        // (nnodes) the Morton pair that opened each node
 };
 
 static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, int64_t max_level, NodeWS &w)
@@ -1201,6 +1219,7 @@ static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, int64_t max_leve
     w.lvstart = A.take<int32_t>(max_level + 2);
     w.elem = A.take<int32_t>(n + nnodes);
     w.qbase = A.take<int32_t>(nnodes);
+    w.npair = A.take<int32_t>(nnodes);
     return A.ok();
 }
 
@@ -1466,12 +1485,12 @@ __global__ __launch_bounds__(256) void k_fold_particles(int64_t n, int64_t nnode
         for (int k = 0; k < 9; k++) t.quadrupoles[9 * idx + k] = 0.0;
 }
 
-struct LoadNChildByLevel {
-    const int32_t *nchild;
-    int64_t n;
-    const uint32_t *byid;
-    __device__ int32_t operator()(int64_t q) const { return nchild[n + byid[q]]; }
-};
+__global__ __launch_bounds__(256) void k_nchild_by_level(int64_t n, int64_t nnodes, const uint32_t *byid,
+                                                         const int32_t *nchild, int32_t *out)
+{
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nnodes) out[q] = nchild[n + byid[q]];
+}
 
 __global__ __launch_bounds__(256) void k_child_base_scatter(int64_t n, int64_t nnodes, int64_t nsys,
                                                             const uint32_t *byid, const int32_t *qbase,
@@ -1534,7 +1553,8 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
         PTG_HIP_CHECK(hipMemsetAsync(nw.lvA, 0, sizeof(uint64_t), s));
     } else {
         k_emit_nodes<<<grid1(n - 1), 256, 0, s>>>(kv, w.lev, w.off, n, w.geoms, t.coordinates, t.sizes,
-                                                  t.first_subnode, t.next_branch, nw.lvA);
+                                                  t.first_subnode, nw.lvA, nw.npair);
+        k_node_ends<<<grid1(nnodes), 256, 0, s>>>(kv, w.lev, w.off, n, nnodes, nw.lvA, nw.npair, t.next_branch);
     }
     if (F) k_forest_roots<<<grid1(nsys), 256, 0, s>>>(w.soff, nsys, w.off, n, forest->roots);
     else if (forest) k_fill_i32<<<1, 64, 0, s>>>(forest->roots, 1, (int32_t)n);
@@ -1556,7 +1576,8 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     PTG_LAUNCH_CHECK();
     // sibling-contiguous walk slots, level-major: an exclusive scan of child counts over the level
     // list puts the children of each level's nodes in one slot range, in node order
-    st = excl_scan(LoadNChildByLevel{t.nchild, n, byid}, nnodes, nw.qbase, nw.bsum, nullptr, s);
+    k_nchild_by_level<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, byid, t.nchild, nw.qbase);
+    st = excl_scan(LoadArr{nw.qbase}, nnodes, nw.qbase, nw.bsum, nullptr, s);  // in place
     if (st) return st;
     k_child_base_scatter<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, byid, nw.qbase, t.child_base);
     k_slots<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, F ? forest->roots : nullptr, t, nw);
