@@ -46,6 +46,9 @@ HBM_PEAK_GBS = 8000.0
 FLOPS_PER_INTERACTION = 20  # monopole potential + acceleration
 FLOPS_PER_FOLD = 60  # one element added to a group's third-order expansion (walk.hip exp_add)
 FLOPS_PER_EXPANSION_EVAL = 100  # potential + acceleration of the expansion at one target (exp_eval)
+FP64_PEAK_TFLOPS = 78.6
+FLOPS_PER_COLUMN_PAIR = 10  # binned column: 3 differences, 5 for r^2, 2 divisions (M / r^2 / bin area)
+FLOPS_PER_RAY_VISIT = 20  # ray walk: 3 differences, 5 for r^2, sqrt, 5 for the projection, opening test
 # tree Potential + Accel at N=1e7 Plummer, theta=0.7, 32 threads, two calls:
 # 1.005e-6 + 1.124e-6 s/particle (examples/benchmark_scaling.json:51,131)
 REF_PUBLISHED = 1.0 / (1.005e-6 + 1.124e-6)
@@ -62,10 +65,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("-N", type=int, default=10_000_000)
-    ap.add_argument("--workload", choices=("plummer", "batched"), default="plummer",
+    ap.add_argument("-N", type=int, default=None, help="particles (default 1e7; 1e6 for --workload column)")
+    ap.add_argument("--workload", choices=("plummer", "batched", "column"), default="plummer",
                     help="plummer: one N-particle Plummer sphere (the headline); batched: 512 independent "
-                         "Plummer clusters (N log-uniform in [1e3, 1e5], scale radii 0.1-10) in one forest")
+                         "Plummer clusters (N log-uniform in [1e3, 1e5], scale radii 0.1-10) in one forest; "
+                         "column: ColumnDensity of an N=1e6 Gaussian cloud (angle-binned, 6-ray walk beside it)")
     ap.add_argument("--criterion", choices=("error", "theta", "learned"), default="learned",
                     help="opening criterion of the timed walk: 'error' bounds each accepted node's dropped "
                          "quadrupole term by alpha x (RMS field); 'learned' accepts a node beyond the opening "
@@ -252,6 +256,107 @@ def run_batched(args):
     print(json.dumps(line), flush=True)
 
 
+def run_column(args):
+    """ColumnDensity with rays=None (the reference's default, angle-binned grouped estimator) on a
+    Gaussian cloud of N particles with radii 3 N^(-1/3): each step is one whole call (tree build,
+    Morton order, binned walk over all N targets).  The exact 6-ray walk is timed beside it, and the C
+    oracle on the host cores gives the CPU line."""
+    import torch
+
+    import oracle
+    from pytreegrav_amd import ColumnDensity, ConstructTree, engine
+
+    dev = engine.device()
+    N = args.N or 1_000_000
+    rng = np.random.default_rng(args.seed)
+    x = rng.normal(size=(N, 3))
+    m = np.full(N, 1.0 / N)
+    h = np.full(N, 3.0 * N ** (-1 / 3))
+    xt, mt, ht = (torch.from_numpy(a).to(dev) for a in (x, m, h))
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    dt = timed(lambda: ColumnDensity(xt, mt, ht), args.steps, args.warmup)
+    dt_rays = timed(lambda: ColumnDensity(xt, mt, ht, rays=6), args.steps, args.warmup)
+    # the walks alone on a resident tree, HIP events on the stream the engine launches on
+    tree = ConstructTree(xt, mt, ht)
+    tgt = tree.dev.coordinates[:N]  # the Morton-ordered positions the frontend walks
+    rays = np.vstack([np.eye(3), -np.eye(3)])
+    cb = torch.zeros(2, dtype=torch.int64, device=dev)
+    cr = torch.zeros(1, dtype=torch.int64, device=dev)
+    engine.column_binned(tree.dev, tgt, 16, 0.5)
+    engine.column_rays(tree.dev, tgt, rays)
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    e[0].record()
+    engine.column_binned(tree.dev, tgt, 16, 0.5, counters=cb)
+    e[1].record()
+    e[2].record()
+    engine.column_rays(tree.dev, tgt, rays, counters=cr)
+    e[3].record()
+    torch.cuda.synchronize()
+    binned_ms, rays_ms = e[0].elapsed_time(e[1]), e[2].elapsed_time(e[3])
+    pairs, visits, ray_visits = int(cb[0].item()), int(cb[1].item()), int(cr[0].item())
+    achieved = pairs * FLOPS_PER_COLUMN_PAIR / (binned_ms * 1e-3) / 1e12
+    rays_achieved = ray_visits * FLOPS_PER_RAY_VISIT / (rays_ms * 1e-3) / 1e12
+    # CPU: the C oracle on the host cores, whole job for the binned estimator (build + all targets),
+    # a 20000-target sample for the rays
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    t0 = time.perf_counter()
+    o = oracle.build_tree(x, m, h)
+    t_build = time.perf_counter() - t0
+    xs = np.ascontiguousarray(x[o.TreewalkIndices])
+    t0 = time.perf_counter()
+    oracle.column_binned(o, xs, 16, 0.5, nthreads=cores)
+    t_binned = time.perf_counter() - t0
+    k = min(N, 20000)
+    t0 = time.perf_counter()
+    oracle.column_rays(o, x[:k], rays, nthreads=cores)
+    t_rays = time.perf_counter() - t0
+    line = {
+        "metric": "ColumnDensity (rays=None, angle-binned) targets/sec, N=1e6 Gaussian cloud, all particles as targets",
+        "value": N / dt,
+        "unit": "targets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "column_binned_gauss1e6", "N": N, "distribution": f"unit Gaussian, seed {args.seed}",
+                   "radii": "3 N^(-1/3)", "theta": 0.5, "group_size": 16, "rays": None},
+        "breakdown": {"binned_walk_ms": binned_ms, "pairs_per_target": pairs / N,
+                      "visits_per_group": visits / -(-N // 16),
+                      "rays6_ms_per_call": dt_rays * 1e3, "rays6_targets_per_s": N / dt_rays,
+                      "rays6_walk_ms": rays_ms, "ray_visits_per_walk": ray_visits / (6 * N),
+                      "rays6_walk_roofline": {"kernel": "k_column_rays", "bound": "latency (dependent link loads)",
+                                              "achieved": rays_achieved, "peak": FP64_PEAK_TFLOPS,
+                                              "unit": "TFLOP/s", "frac": rays_achieved / FP64_PEAK_TFLOPS,
+                                              "convention": f"{FLOPS_PER_RAY_VISIT} FP64 flops per element visited"}},
+        "roofline": {"kernel": "k_column_binned_wave (FP64 grouped binned walk)", "bound": "valu",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "convention": f"{FLOPS_PER_COLUMN_PAIR} FP64 flops per element-target pair (counted by the kernel)"},
+        "cpu_baseline": {"value": N / (t_build + t_binned), "unit": "targets/s", "cores": cores, "kind": "port",
+                         "sample": f"C oracle: serial build of all {N} particles ({t_build:.2f} s) + binned grouped walk "
+                                   f"(group_size 16, theta 0.5) over all {N} targets ({t_binned:.2f} s on {cores} threads); "
+                                   f"6-ray walk on {k} targets: {k / t_rays:.3g} targets/s",
+                         "rays6_targets_per_s": k / t_rays, "cpu_model": _cpu_model()},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -276,8 +381,10 @@ def main():
 
     if args.workload == "batched":
         return run_batched(args)
+    if args.workload == "column":
+        return run_column(args)
     dev = engine.device()
-    N = args.N
+    N = args.N or 10_000_000
     x, m = plummer_truncated(N, seed=args.seed, rmax=100.0)
     h = np.full(N, args.softening)
     pos = torch.from_numpy(x).to(dev)

@@ -285,19 +285,24 @@ ptg_status ptg_rms_rows(const double *x, int64_t rows, int stride, int col, int 
 /* Ray-traced: out[j * nrays + i] = column from target j to infinity along unit ray i, exact
  * (ColumnDensityWalk_singleray :904-973 per pair; the grouped and multi-ray walks of :822-901 and
  * :1087-1194 give the same values).  rays: (nrays, 3) shared, or (nt, nrays, 3) when per_target
- * != 0 (randomized ray grids, :1061-1078). */
+ * != 0 (randomized ray grids, :1061-1078).  counters (nullable, device int64[1], accumulated):
+ * [0] elements visited over all (target, ray) walks. */
 ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const double *targets, int64_t nt,
-                           const double *rays, int64_t nrays, int per_target, double *out, int32_t *err, void *stream);
+                           const double *rays, int64_t nrays, int per_target, double *out, int32_t *err,
+                           int64_t *counters, void *stream);
 /* Angular-binned (rays=None): out (nt, 6), the mean column per equal-area sky bin, with groups of
  * group_size consecutive targets sharing one traversal opened at the node's distance to the group's
  * box (ColumnDensityBinned_grouped :1197-1276; group_size 1 is ColumnDensityWalk_binned :976-1028).
  * group_size <= 64 walks the FP64 walk pack wt of the same tree (ptg_pack_walk, PTG_F64) with
- * ptg_column_workspace_bytes() of workspace; larger groups use the tree's links (wt, ws may be NULL).
- * err: 1 corrupt links, 2 a traversal stack overflow. */
-size_t ptg_column_workspace_bytes(void);
+ * ptg_column_workspace_bytes(group_size) of workspace (groups whose traversal passes 32768 visits are
+ * finished by a second pass that deals their subtrees out to 64 wavefronts); larger groups use the
+ * tree's links (wt, ws may be NULL).
+ * err: 1 corrupt links, 2 a traversal stack overflow.  counters (nullable, device int64[2],
+ * accumulated): [0] element-target pairs evaluated, [1] elements visited by the group traversals. */
+size_t ptg_column_workspace_bytes(int64_t group_size);
 ptg_status ptg_column_binned(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_walk_tree *wt,
                              const double *targets, int64_t nt, int64_t group_size, double theta, double *out,
-                             int32_t *err, void *ws, size_t ws_bytes, void *stream);
+                             int32_t *err, void *ws, size_t ws_bytes, int64_t *counters, void *stream);
 
 /* ------------------------------------------------------------------------------------------
  * Direct summation, LDS-tiled: replaces bruteforce.py.  self_mode != 0: targets are the sources

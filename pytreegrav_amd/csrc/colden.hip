@@ -28,19 +28,25 @@ __device__ __forceinline__ bool cd_bad(int64_t no, int64_t ne, int64_t &visits)
     return no >= ne || ++visits > ne;
 }
 
-__global__ __launch_bounds__(CD_THREADS) void k_column_rays(const ptg_tree_arrays t, int64_t n, int64_t ne,
-                                                            const double *tp, int64_t nt, const double *rays,
-                                                            int64_t nr, int per_target, double *out, int32_t *err)
+// adds one wave's counts to ctr[k] with one atomic (every lane of the wave takes part)
+__device__ __forceinline__ void cd_count(unsigned long long *ctr, int k, int64_t v)
 {
-    const int64_t id = (int64_t)blockIdx.x * CD_THREADS + threadIdx.x;
-    if (id >= nt * nr) return;
-    const int64_t i = id / nt, j = id - (id / nt) * nt;  // ray i, target j
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    if (ptg_lane() == 0 && v) atomicAdd(ctr + k, (unsigned long long)v);
+}
+
+// column along ray i from target j; visits counts the elements the walk touched
+__device__ __forceinline__ double ray_column(const ptg_tree_arrays &t, int64_t n, int64_t ne, const double *tp,
+                                             int64_t nt, const double *rays, int64_t nr, int per_target,
+                                             int64_t i, int64_t j, int32_t *err, int64_t &visits)
+{
     const double *ray = per_target ? rays + 3 * (j * nr + i) : rays + 3 * i;
     const double rx = ray[0], ry = ray[1], rz = ray[2];
     const double px = tp[3 * j], py = tp[3 * j + 1], pz = tp[3 * j + 2];
     const double fac_density = 3.0 / (4.0 * 3.14159265358979323846);
     double column = 0.0;
-    int64_t no = n, visits = 0;
+    int64_t no = n;
     const int64_t stop = t.next_branch[n];
     while (no > -1 && no != stop) {
         if (cd_bad(no, ne, visits)) {
@@ -79,7 +85,21 @@ __global__ __launch_bounds__(CD_THREADS) void k_column_rays(const ptg_tree_array
             no = open ? t.first_subnode[no] : t.next_branch[no];
         }
     }
-    out[j * nr + i] = column;
+    return column;
+}
+
+__global__ __launch_bounds__(CD_THREADS) void k_column_rays(const ptg_tree_arrays t, int64_t n, int64_t ne,
+                                                            const double *tp, int64_t nt, const double *rays,
+                                                            int64_t nr, int per_target, double *out, int32_t *err,
+                                                            unsigned long long *ctr)
+{
+    const int64_t id = (int64_t)blockIdx.x * CD_THREADS + threadIdx.x;
+    int64_t visits = 0;
+    if (id < nt * nr) {
+        const int64_t i = id / nt, j = id - (id / nt) * nt;  // ray i, target j
+        out[j * nr + i] = ray_column(t, n, ne, tp, nt, rays, nr, per_target, i, j, err, visits);
+    }
+    if (ctr) cd_count(ctr, 0, visits);
 }
 
 // angular_bin_scalar (treewalk.py:38-45)
@@ -92,7 +112,7 @@ __device__ __forceinline__ int cd_bin(double d0, double d1, double d2)
 
 __global__ __launch_bounds__(CD_THREADS) void k_column_binned(const ptg_tree_arrays t, int64_t n, int64_t ne,
                                                               const double *tp, int64_t nt, int64_t gs, double theta,
-                                                              double *out, int32_t *err)
+                                                              double *out, int32_t *err, unsigned long long *ctr)
 {
     const int64_t gi = (int64_t)blockIdx.x * CD_THREADS + threadIdx.x;
     const int64_t a = gi * gs;
@@ -111,7 +131,7 @@ __global__ __launch_bounds__(CD_THREADS) void k_column_binned(const ptg_tree_arr
     }
     for (int64_t j = a; j < b; j++)
         for (int k = 0; k < 6; k++) out[6 * j + k] = 0.0;
-    int64_t no = n, visits = 0;
+    int64_t no = n, visits = 0, pairs = 0;
     const int64_t stop = t.next_branch[n];
     while (no > -1 && no != stop) {
         if (cd_bad(no, ne, visits)) {
@@ -121,6 +141,7 @@ __global__ __launch_bounds__(CD_THREADS) void k_column_binned(const ptg_tree_arr
         const double cx = t.coordinates[3 * no], cy = t.coordinates[3 * no + 1], cz = t.coordinates[3 * no + 2];
         const double h_no = t.softenings[no], M = t.masses[no];
         if (no < n) {  // a particle contributes to every target of the group, no criterion
+            pairs += b - a;
             const double hh = h_no * h_no;
             for (int64_t j = a; j < b; j++) {
                 const double d0 = cx - tp[3 * j], d1 = cy - tp[3 * j + 1], d2 = cz - tp[3 * j + 2];
@@ -147,6 +168,7 @@ __global__ __launch_bounds__(CD_THREADS) void k_column_binned(const ptg_tree_arr
             const double size = t.sizes[no], delta = t.deltas[no];
             const double c1 = size / theta + delta, c2 = h_no + size * 0.6 + delta;
             if (r_min > (c1 > c2 ? c1 : c2)) {
+                pairs += b - a;
                 for (int64_t j = a; j < b; j++) {
                     const double d0 = cx - tp[3 * j], d1 = cy - tp[3 * j + 1], d2 = cz - tp[3 * j + 2];
                     const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
@@ -158,6 +180,10 @@ __global__ __launch_bounds__(CD_THREADS) void k_column_binned(const ptg_tree_arr
             }
         }
     }
+    if (ctr) {
+        atomicAdd(ctr, (unsigned long long)pairs);
+        atomicAdd(ctr + 1, (unsigned long long)visits);
+    }
 }
 
 
@@ -167,6 +193,10 @@ __global__ __launch_bounds__(CD_THREADS) void k_column_binned(const ptg_tree_arr
 #define CB_SPILL 512
 #define CB_GCAP 16384  // global spill entries per wave
 #define CB_ICAP 128
+#define CB_BUDGET 32768  // main-pass visits after which a group is queued for the heavy pass
+#define CB_HCAP 1024     // queued groups at most (further heavy groups finish in the main pass)
+#define CB_W 64          // shares of a queued group
+#define CB_D 4           // depth of the nodes whose children are dealt out to the shares
 
 struct alignas(32) D4 {
     double x, y, z, w;
@@ -187,10 +217,11 @@ __device__ __forceinline__ void cb_add(double (&acc)[6], int bin, double v)
 
 // list entries [0, cnt) on this lane's target, slice sl of `slices`
 __device__ __forceinline__ void cb_flush(const CbLds &L, int nn, int np, int sl, int slices, bool mine, double tx,
-                                         double ty, double tz, double (&acc)[6])
+                                         double ty, double tz, double (&acc)[6], int64_t &pairs)
 {
     const double bin_size = (4.0 * 3.14159265358979323846) / 6.0;
     if (!mine) return;
+    pairs += (nn > sl ? (nn - sl + slices - 1) / slices : 0) + (np > sl ? (np - sl + slices - 1) / slices : 0);
     for (int k = sl; k < nn; k += slices) {
         const D4 c = L.nodes[k];
         const double d0 = c.x - tx, d1 = c.y - ty, d2 = c.z - tz;
@@ -217,131 +248,207 @@ __device__ __forceinline__ void cb_flush(const CbLds &L, int nn, int np, int sl,
     }
 }
 
+// One task: the traversal of group gi's targets, whole (main pass) or the share w of CB_W (heavy
+// pass).  Stack entries carry the walk slot and the depth (slot * 8 + min(depth, 7)).  In the heavy
+// pass every wave walks the levels above CB_D alike, wave 0 alone lists what they accept, and below
+// an opened node of depth CB_D - 1 wave w descends only into the children whose slot is w modulo
+// CB_W, so the CB_W shares cover the group's traversal exactly once.  Returns 0 done, 1 deferred (a
+// main-pass group past CB_BUDGET visits, queued for the heavy pass), 2 stack overflow.
+template <bool HEAVY>
+__device__ int cb_task(CbLds &L, int *spill, const D4 *xyzm, const D4 *aux, const int2 *kids, int64_t nslots,
+                       const double *tp, int64_t nt, int gs, double theta, int64_t gi, int w, int32_t *ctl,
+                       int32_t *heavy, double (&acc)[6], int64_t &gpairs, int64_t &gvis)
+{
+    const int lane = ptg_lane();
+    const int slices = 64 / gs;
+    const int tl = lane % gs, sl = lane / gs;
+    const int64_t a = gi * gs;
+    const int mt = (int)std::min<int64_t>(gs, nt - a);
+    // the group's box (lanes 0..mt-1 hold its targets)
+    double bx0 = INFINITY, by0 = INFINITY, bz0 = INFINITY, bx1 = -INFINITY, by1 = -INFINITY, bz1 = -INFINITY;
+    if (lane < mt) {
+        bx0 = bx1 = tp[3 * (a + lane)];
+        by0 = by1 = tp[3 * (a + lane) + 1];
+        bz0 = bz1 = tp[3 * (a + lane) + 2];
+    }
+    bx0 = ptg_wave_min(bx0);
+    by0 = ptg_wave_min(by0);
+    bz0 = ptg_wave_min(bz0);
+    bx1 = ptg_wave_max(bx1);
+    by1 = ptg_wave_max(by1);
+    bz1 = ptg_wave_max(bz1);
+    const bool mine = sl < slices && tl < mt;
+    const double tx = mine ? tp[3 * (a + tl)] : 0.0, ty = mine ? tp[3 * (a + tl) + 1] : 0.0;
+    const double tz = mine ? tp[3 * (a + tl) + 2] : 0.0;
+    for (int q = 0; q < 6; q++) acc[q] = 0.0;
+    gpairs = gvis = 0;
+    bool budget = !HEAVY;
+    int sp = 1, gsp = 0, nn = 0, np = 0;
+    if (lane == 0) L.stack[0] = 0;  // the root: slot 0, depth 0
+    __builtin_amdgcn_wave_barrier();
+    for (;;) {
+        if (sp == 0) {
+            if (gsp == 0) break;
+            const int k = std::min(gsp, CB_SPILL);
+            for (int q = lane; q < k; q += 64) L.stack[q] = spill[gsp - k + q];
+            __builtin_amdgcn_wave_barrier();
+            gsp -= k;
+            sp = k;
+        }
+        const int k = std::min(sp, 64);
+        const int base = sp - k;
+        const int enc = lane < k ? L.stack[base + lane] : 0;
+        int e = enc >> 3;
+        const int d = enc & 7;
+        const bool ve = lane < k && enc >= 0 && e < nslots;
+        if (!ve) e = 0;
+        __builtin_amdgcn_wave_barrier();
+        sp = base;
+        // above the split depth only wave 0 lists and counts; the other waves only open
+        const bool lists = !HEAVY || w == 0 || d >= CB_D;
+        gvis += (int)__popcll(ptg_ballot(ve && lists));
+        const D4 X = xyzm[e], X2 = aux[e];
+        const int2 K = kids[e];
+        const bool ispart = K.y == 0;
+        const double dxm = X.x < bx0 ? bx0 - X.x : (X.x > bx1 ? X.x - bx1 : 0.0);
+        const double dym = X.y < by0 ? by0 - X.y : (X.y > by1 ? X.y - by1 : 0.0);
+        const double dzm = X.z < bz0 ? bz0 - X.z : (X.z > bz1 ? X.z - bz1 : 0.0);
+        const double r_min = sqrt(dxm * dxm + dym * dym + dzm * dzm);
+        const double c1 = X2.y / theta + X2.z, c2 = X2.x + X2.y * 0.6 + X2.z;
+        const bool acc_node = ve && !ispart && r_min > (c1 > c2 ? c1 : c2);
+        const bool open = ve && !ispart && !acc_node;
+        const bool list_node = acc_node && lists, list_part = ve && ispart && lists;
+        const uint64_t bn = ptg_ballot(list_node), bp = ptg_ballot(list_part);
+        if (list_node) L.nodes[nn + (int)__popcll(bn & ptg_lanemask_lt())] = X;
+        if (list_part) {
+            const int at = np + (int)__popcll(bp & ptg_lanemask_lt());
+            L.parts[at] = X;
+            L.ph[at] = X2.x;
+        }
+        nn += (int)__popcll(bn);
+        np += (int)__popcll(bp);
+        const int nk = open ? std::min(std::max(K.y, 0), 8) : 0;
+        // at depth CB_D - 1 (heavy pass) the one child, if any, whose slot is w modulo CB_W
+        const bool split = HEAVY && d == CB_D - 1;
+        const int cw = split ? (int)(((int64_t)w - K.x) % CB_W + CB_W) % CB_W : 0;
+        const int cnt = split ? (cw < nk ? 1 : 0) : nk;
+        const int incl = ptg_wave_incl_scan(cnt);
+        const int tot = __shfl(incl, 63, 64);
+        if (sp + tot > CB_SCAP) {
+            if (gsp + CB_SPILL > CB_GCAP) return 2;
+            for (int q = lane; q < CB_SPILL; q += 64) spill[gsp + q] = L.stack[q];
+            __builtin_amdgcn_wave_barrier();
+            for (int q = 0; q < sp - CB_SPILL; q += 64) {
+                const int v = (q + lane < sp - CB_SPILL) ? L.stack[CB_SPILL + q + lane] : 0;
+                __builtin_amdgcn_wave_barrier();
+                if (q + lane < sp - CB_SPILL) L.stack[q + lane] = v;
+                __builtin_amdgcn_wave_barrier();
+            }
+            gsp += CB_SPILL;
+            sp -= CB_SPILL;
+        }
+        {
+            const int at = sp + incl - cnt;
+            const int cd = d < 7 ? d + 1 : 7;
+            if (split) {
+                if (cnt) L.stack[at] = (K.x + cw) * 8 + cd;
+            } else {
+                for (int c = 0; c < cnt; c++) L.stack[at + c] = (K.x + c) * 8 + cd;
+            }
+        }
+        sp += tot;
+        __builtin_amdgcn_wave_barrier();
+        if (nn >= 64 || np >= 64) {
+            cb_flush(L, nn, np, sl, slices, mine, tx, ty, tz, acc, gpairs);
+            __builtin_amdgcn_wave_barrier();
+            nn = np = 0;
+        }
+        if (budget && gvis > CB_BUDGET) {
+            int h = 0;
+            if (lane == 0) h = atomicAdd(ctl + 1, 1);
+            h = __builtin_amdgcn_readfirstlane(h);
+            if (h < CB_HCAP) {
+                if (lane == 0) heavy[h] = (int32_t)gi;
+                return 1;
+            }
+            budget = false;  // the heavy queue is full: finish the group here
+        }
+    }
+    cb_flush(L, nn, np, sl, slices, mine, tx, ty, tz, acc, gpairs);
+    __builtin_amdgcn_wave_barrier();
+    // sum the slices' partial bins into lanes 0 .. gs-1
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        double v = acc[q];
+        for (int s2 = 1; s2 < slices; s2++) {
+            const double o = __shfl(acc[q], tl + s2 * gs, 64);
+            if (sl == 0) v += o;
+        }
+        acc[q] = v;
+    }
+    return 0;
+}
+
+// Main pass (HEAVY = false): one task per group, results into out; groups past the visit budget are
+// queued.  Heavy pass: CB_W tasks per queued group, partial bins into partial[task][target][6].
+template <bool HEAVY>
 __global__ __launch_bounds__(64 * CB_WAVES) void k_column_binned_wave(const D4 *xyzm, const D4 *aux, const int2 *kids,
                                                                       int64_t nslots, const double *tp, int64_t nt,
                                                                       int gs, double theta, double *out, int *spill_all,
-                                                                      int32_t *ctr, int32_t *err)
+                                                                      int32_t *ctl, int32_t *heavy, double *partial,
+                                                                      int32_t *err, unsigned long long *counts)
 {
     __shared__ CbLds lds[CB_WAVES];
     const int wave = threadIdx.x >> 6, lane = ptg_lane();
     CbLds &L = lds[wave];
     int *spill = spill_all + ((int64_t)blockIdx.x * CB_WAVES + wave) * CB_GCAP;
-    const int64_t ngroups = (nt + gs - 1) / gs;
-    const int slices = 64 / gs;
+    const int64_t ntasks = HEAVY ? (int64_t)std::min(ctl[1], CB_HCAP) * CB_W : (nt + gs - 1) / gs;
     const int tl = lane % gs, sl = lane / gs;
+    int64_t pairs = 0, vis = 0;
     for (;;) {
-        int gi = 0;
-        if (lane == 0) gi = atomicAdd(ctr, 1);
-        gi = __builtin_amdgcn_readfirstlane(gi);
-        if (gi >= ngroups) break;
-        const int64_t a = (int64_t)gi * gs;
-        const int mt = (int)std::min<int64_t>(gs, nt - a);
-        // the group's box (lanes 0..mt-1 hold its targets)
-        double bx0 = INFINITY, by0 = INFINITY, bz0 = INFINITY, bx1 = -INFINITY, by1 = -INFINITY, bz1 = -INFINITY;
-        if (lane < mt) {
-            bx0 = bx1 = tp[3 * (a + lane)];
-            by0 = by1 = tp[3 * (a + lane) + 1];
-            bz0 = bz1 = tp[3 * (a + lane) + 2];
-        }
-        bx0 = ptg_wave_min(bx0);
-        by0 = ptg_wave_min(by0);
-        bz0 = ptg_wave_min(bz0);
-        bx1 = ptg_wave_max(bx1);
-        by1 = ptg_wave_max(by1);
-        bz1 = ptg_wave_max(bz1);
-        const bool mine = sl < slices && tl < mt;
-        const double tx = mine ? tp[3 * (a + tl)] : 0.0, ty = mine ? tp[3 * (a + tl) + 1] : 0.0;
-        const double tz = mine ? tp[3 * (a + tl) + 2] : 0.0;
-        double acc[6] = {0, 0, 0, 0, 0, 0};
-        int sp = 1, gsp = 0, nn = 0, np = 0;
-        bool overflow = false;
-        if (lane == 0) L.stack[0] = 0;  // the root's walk slot
-        __builtin_amdgcn_wave_barrier();
-        for (;;) {
-            if (sp == 0) {
-                if (gsp == 0) break;
-                const int k = std::min(gsp, CB_SPILL);
-                for (int q = lane; q < k; q += 64) L.stack[q] = spill[gsp - k + q];
-                __builtin_amdgcn_wave_barrier();
-                gsp -= k;
-                sp = k;
-            }
-            const int k = std::min(sp, 64);
-            const int base = sp - k;
-            int e = lane < k ? L.stack[base + lane] : 0;
-            const bool ve = lane < k && e >= 0 && e < nslots;
-            if (!ve) e = 0;
-            __builtin_amdgcn_wave_barrier();
-            sp = base;
-            const D4 X = xyzm[e], X2 = aux[e];
-            const int2 K = kids[e];
-            const bool ispart = K.y == 0;
-            const double dxm = X.x < bx0 ? bx0 - X.x : (X.x > bx1 ? X.x - bx1 : 0.0);
-            const double dym = X.y < by0 ? by0 - X.y : (X.y > by1 ? X.y - by1 : 0.0);
-            const double dzm = X.z < bz0 ? bz0 - X.z : (X.z > bz1 ? X.z - bz1 : 0.0);
-            const double r_min = sqrt(dxm * dxm + dym * dym + dzm * dzm);
-            const double c1 = X2.y / theta + X2.z, c2 = X2.x + X2.y * 0.6 + X2.z;
-            const bool acc_node = ve && !ispart && r_min > (c1 > c2 ? c1 : c2);
-            const bool acc_part = ve && ispart;
-            const bool open = ve && !ispart && !acc_node;
-            const uint64_t bn = ptg_ballot(acc_node), bp = ptg_ballot(acc_part);
-            if (acc_node) L.nodes[nn + (int)__popcll(bn & ptg_lanemask_lt())] = X;
-            if (acc_part) {
-                const int at = np + (int)__popcll(bp & ptg_lanemask_lt());
-                L.parts[at] = X;
-                L.ph[at] = X2.x;
-            }
-            nn += (int)__popcll(bn);
-            np += (int)__popcll(bp);
-            const int cnt = open ? std::min(std::max(K.y, 0), 8) : 0;
-            const int incl = ptg_wave_incl_scan(cnt);
-            const int tot = __shfl(incl, 63, 64);
-            if (sp + tot > CB_SCAP) {
-                if (gsp + CB_SPILL > CB_GCAP) {
-                    overflow = true;
-                    break;
-                }
-                for (int q = lane; q < CB_SPILL; q += 64) spill[gsp + q] = L.stack[q];
-                __builtin_amdgcn_wave_barrier();
-                for (int q = 0; q < sp - CB_SPILL; q += 64) {
-                    const int v = (q + lane < sp - CB_SPILL) ? L.stack[CB_SPILL + q + lane] : 0;
-                    __builtin_amdgcn_wave_barrier();
-                    if (q + lane < sp - CB_SPILL) L.stack[q + lane] = v;
-                    __builtin_amdgcn_wave_barrier();
-                }
-                gsp += CB_SPILL;
-                sp -= CB_SPILL;
-            }
-            {
-                const int at = sp + incl - cnt;
-                for (int c = 0; c < cnt; c++) L.stack[at + c] = K.x + c;
-            }
-            sp += tot;
-            __builtin_amdgcn_wave_barrier();
-            if (nn >= 64 || np >= 64) {
-                cb_flush(L, nn, np, sl, slices, mine, tx, ty, tz, acc);
-                __builtin_amdgcn_wave_barrier();
-                nn = np = 0;
-            }
-        }
-        if (overflow) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(ctl + (HEAVY ? 2 : 0), 1);
+        t = __builtin_amdgcn_readfirstlane(t);
+        if (t >= ntasks) break;
+        const int64_t gi = HEAVY ? (int64_t)heavy[t / CB_W] : (int64_t)t;
+        const int w = HEAVY ? t % CB_W : 0;
+        double acc[6];
+        int64_t gpairs, gvis;
+        const int r = cb_task<HEAVY>(L, spill, xyzm, aux, kids, nslots, tp, nt, gs, theta, gi, w, ctl, heavy, acc,
+                                     gpairs, gvis);
+        if (r == 2) {
             if (lane == 0) atomicOr(err, 2);
             continue;
         }
-        cb_flush(L, nn, np, sl, slices, mine, tx, ty, tz, acc);
-        __builtin_amdgcn_wave_barrier();
-        // sum the slices' partial bins into lanes 0 .. gs-1
-#pragma unroll
-        for (int q = 0; q < 6; q++) {
-            double v = acc[q];
-            for (int s2 = 1; s2 < slices; s2++) {
-                const double o = __shfl(acc[q], tl + s2 * gs, 64);
-                if (sl == 0) v += o;
-            }
-            acc[q] = v;
+        if (r == 1) continue;
+        pairs += gpairs;
+        vis += gvis;
+        const int64_t a = gi * gs;
+        if (sl == 0 && a + tl < nt) {
+            double *o = HEAVY ? partial + ((int64_t)t * gs + tl) * 6 : out + 6 * (a + tl);
+            for (int q = 0; q < 6; q++) o[q] = acc[q];
         }
-        if (sl == 0 && tl < mt)
-            for (int q = 0; q < 6; q++) out[6 * (a + tl) + q] = acc[q];
+    }
+    if (counts) {
+        cd_count(counts, 0, pairs);
+        cd_count(counts, 1, lane == 0 ? vis : 0);
+    }
+}
+
+// out rows of the queued groups: their CB_W partial rows summed in share order
+__global__ __launch_bounds__(256) void k_column_heavy_sum(const int32_t *ctl, const int32_t *heavy,
+                                                          const double *partial, int64_t nt, int gs, double *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t h = i / gs;
+    const int tl = (int)(i % gs);
+    if (h >= std::min(ctl[1], CB_HCAP)) return;
+    const int64_t a = (int64_t)heavy[h] * gs;
+    if (a + tl >= nt) return;
+    for (int q = 0; q < 6; q++) {
+        double v = 0.0;
+        for (int w = 0; w < CB_W; w++) v += partial[((h * CB_W + w) * gs + tl) * 6 + q];
+        out[6 * (a + tl) + q] = v;
     }
 }
 
@@ -354,14 +461,41 @@ static int cb_cus()
     return cus;
 }
 
-extern "C" size_t ptg_column_workspace_bytes(void)
+// workspace: control words, the heavy-group queue, per-wave spill stacks, the heavy partial rows
+struct CbWs {
+    int32_t *ctl, *heavy;
+    int *spill;
+    double *partial;
+};
+
+static size_t cb_carve(void *ws, int64_t gs, CbWs *w)
 {
     const int cus = cb_cus();
-    return ptg_align256(sizeof(int32_t) * 64) + ptg_align256((size_t)cus * 2 * CB_WAVES * CB_GCAP * sizeof(int));
+    char *p = (char *)ws;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *q = p ? p + off : nullptr;
+        off += ptg_align256(bytes);
+        return q;
+    };
+    w->ctl = (int32_t *)take(sizeof(int32_t) * 64);
+    w->heavy = (int32_t *)take(sizeof(int32_t) * CB_HCAP);
+    w->spill = (int *)take((size_t)cus * 2 * CB_WAVES * CB_GCAP * sizeof(int));
+    w->partial = (double *)take((size_t)CB_HCAP * CB_W * std::max<int64_t>(gs, 1) * 6 * sizeof(double));
+    return off;
+}
+
+extern "C" size_t ptg_column_workspace_bytes(int64_t group_size)
+{
+    CbWs w;
+    return cb_carve(nullptr, group_size, &w);
 }
 
 __global__ void k_column_err_init(int32_t *err) { *err = 0; }
-__global__ void k_column_ctr_init(int32_t *ctr) { *ctr = 0; }
+__global__ void k_column_ctl_init(int32_t *ctl)
+{
+    if (threadIdx.x < 64) ctl[threadIdx.x] = 0;
+}
 
 static ptg_status column_common(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const double *targets,
                                 int64_t nt, const char *who)
@@ -376,7 +510,7 @@ static ptg_status column_common(const ptg_tree_arrays *tree, int64_t n, int64_t 
 
 extern "C" ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const double *targets,
                                       int64_t nt, const double *rays, int64_t nrays, int per_target, double *out,
-                                      int32_t *err, void *stream)
+                                      int32_t *err, int64_t *counters, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     ptg_status st = column_common(tree, n, nnodes, targets, nt, "ptg_column_rays");
@@ -390,7 +524,8 @@ extern "C" ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, in
     const int64_t work = nt * nrays;
     if (work == 0) return PTG_OK;
     k_column_rays<<<(unsigned)ptg_cdiv(work, CD_THREADS), CD_THREADS, 0, s>>>(*tree, n, n + nnodes, targets, nt, rays,
-                                                                              nrays, per_target != 0, out, err);
+                                                                              nrays, per_target != 0, out, err,
+                                                                              (unsigned long long *)counters);
     PTG_LAUNCH_CHECK();
     return PTG_OK;
 }
@@ -398,7 +533,7 @@ extern "C" ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, in
 extern "C" ptg_status ptg_column_binned(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
                                         const ptg_walk_tree *wt, const double *targets, int64_t nt,
                                         int64_t group_size, double theta, double *out, int32_t *err, void *ws,
-                                        size_t ws_bytes, void *stream)
+                                        size_t ws_bytes, int64_t *counters, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     ptg_status st = column_common(tree, n, nnodes, targets, nt, "ptg_column_binned");
@@ -412,27 +547,35 @@ extern "C" ptg_status ptg_column_binned(const ptg_tree_arrays *tree, int64_t n, 
     if (nt == 0) return PTG_OK;
     if (group_size <= 64) {
         if (!wt || wt->prec != PTG_F64 || !wt->xyzm || !wt->aux || !wt->kids || wt->nslots != n + nnodes || !ws ||
-            ws_bytes < ptg_column_workspace_bytes()) {
-            ptg_set_error("ptg_column_binned: group_size <= 64 needs the FP64 walk pack of the tree and "
-                          "ptg_column_workspace_bytes() of workspace");
+            ws_bytes < ptg_column_workspace_bytes(group_size) || n + nnodes >= (1ll << 28)) {
+            ptg_set_error("ptg_column_binned: group_size <= 64 needs the FP64 walk pack of the tree (fewer than "
+                          "2^28 elements) and ptg_column_workspace_bytes(group_size) of workspace");
             return PTG_ERR_ARG;
         }
-        int32_t *ctr = (int32_t *)ws;
-        int *spill = (int *)((char *)ws + ptg_align256(sizeof(int32_t) * 64));
+        CbWs w;
+        cb_carve(ws, group_size, &w);
         const int cus = cb_cus();
-        k_column_ctr_init<<<1, 1, 0, s>>>(ctr);
+        k_column_ctl_init<<<1, 64, 0, s>>>(w.ctl);
         PTG_LAUNCH_CHECK();
         const int64_t ngroups = ptg_cdiv(nt, group_size);
         const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((int64_t)cus * 2, ptg_cdiv(ngroups, CB_WAVES)));
-        k_column_binned_wave<<<(unsigned)blocks, 64 * CB_WAVES, 0, s>>>(
-            (const D4 *)wt->xyzm, (const D4 *)wt->aux, (const int2 *)wt->kids, wt->nslots, targets, nt,
-            (int)group_size, theta, out, spill, ctr, err);
+        const D4 *X = (const D4 *)wt->xyzm, *X2 = (const D4 *)wt->aux;
+        const int2 *K = (const int2 *)wt->kids;
+        k_column_binned_wave<false><<<(unsigned)blocks, 64 * CB_WAVES, 0, s>>>(
+            X, X2, K, wt->nslots, targets, nt, (int)group_size, theta, out, w.spill, w.ctl, w.heavy, w.partial, err,
+            (unsigned long long *)counters);
+        k_column_binned_wave<true><<<(unsigned)cus * 2, 64 * CB_WAVES, 0, s>>>(
+            X, X2, K, wt->nslots, targets, nt, (int)group_size, theta, out, w.spill, w.ctl, w.heavy, w.partial, err,
+            (unsigned long long *)counters);
+        k_column_heavy_sum<<<(unsigned)ptg_cdiv((int64_t)CB_HCAP * group_size, 256), 256, 0, s>>>(
+            w.ctl, w.heavy, w.partial, nt, (int)group_size, out);
         PTG_LAUNCH_CHECK();
         return PTG_OK;
     }
     const int64_t ngroups = ptg_cdiv(nt, group_size);
     k_column_binned<<<(unsigned)ptg_cdiv(ngroups, CD_THREADS), CD_THREADS, 0, s>>>(*tree, n, n + nnodes, targets, nt,
-                                                                                   group_size, theta, out, err);
+                                                                                   group_size, theta, out, err,
+                                                                                   (unsigned long long *)counters);
     PTG_LAUNCH_CHECK();
     return PTG_OK;
 }

@@ -454,9 +454,16 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
     return out
 
 
-def column_rays(tree, targets, rays, per_target=False):
+def _check_counters(counters, k):
+    if counters is not None and (not isinstance(counters, torch.Tensor) or counters.dtype != torch.int64
+                                 or not counters.is_cuda or not counters.is_contiguous() or counters.numel() < k):
+        raise ValueError(f"counters must be a contiguous int64 device tensor of at least {k} entries")
+
+
+def column_rays(tree, targets, rays, per_target=False, counters=None):
     """Ray-traced column density (ptg_column_rays): (nt, nr) float64 device tensor for targets (nt, 3)
-    and unit rays (nr, 3), or per-target rays (nt, nr, 3) when per_target."""
+    and unit rays (nr, 3), or per-target rays (nt, nr, 3) when per_target.  counters: optional device
+    int64 tensor; [0] accumulates the elements visited."""
     L = _lib.load()
     tp = to_dev(targets, (-1, 3))
     nt = tp.shape[0]
@@ -464,28 +471,32 @@ def column_rays(tree, targets, rays, per_target=False):
     nr = r.shape[-2] if per_target else r.shape[0]
     if (per_target and tuple(r.shape) != (nt, nr, 3)) or (not per_target and (r.dim() != 2 or r.shape[1] != 3)):
         raise ValueError("rays must be (nr, 3), or (nt, nr, 3) with per_target")
+    _check_counters(counters, 1)
     out = torch.empty((nt, nr), dtype=torch.float64, device=device())
     err = torch.zeros(1, dtype=torch.int32, device=device())
     check(L.ptg_column_rays(_lib.ctypes.byref(tree.arrays), tree.n, tree.nnodes, ptr(tp), nt, ptr(r), nr,
-                            int(bool(per_target)), ptr(out), ptr(err), stream()), "ptg_column_rays")
+                            int(bool(per_target)), ptr(out), ptr(err), ptr(counters), stream()), "ptg_column_rays")
     if int(err.item()):
         raise RuntimeError("ptg_column_rays: corrupt tree links")
     return out
 
 
-def column_binned(tree, targets, group_size=16, theta=0.5):
-    """Angular-binned column density (ptg_column_binned): (nt, 6) float64 device tensor."""
+def column_binned(tree, targets, group_size=16, theta=0.5, counters=None):
+    """Angular-binned column density (ptg_column_binned): (nt, 6) float64 device tensor.  counters:
+    optional device int64 tensor of 2; accumulates [0] element-target pairs, [1] elements visited."""
     L = _lib.load()
     tp = to_dev(targets, (-1, 3))
     nt = tp.shape[0]
     out = torch.empty((nt, 6), dtype=torch.float64, device=device())
     err = torch.zeros(1, dtype=torch.int32, device=device())
+    _check_counters(counters, 2)
     wt, ws = None, None
     if group_size <= 64:  # one wavefront per group over the FP64 walk pack
         wt = _lib.ctypes.byref(tree.pack(PTG_F64, False)["wt"])
-        ws = workspace(L.ptg_column_workspace_bytes())
+        ws = workspace(L.ptg_column_workspace_bytes(int(group_size)))
     check(L.ptg_column_binned(_lib.ctypes.byref(tree.arrays), tree.n, tree.nnodes, wt, ptr(tp), nt, int(group_size),
-                              float(theta), ptr(out), ptr(err), ptr(ws), 0 if ws is None else ws.numel(), stream()),
+                              float(theta), ptr(out), ptr(err), ptr(ws), 0 if ws is None else ws.numel(), ptr(counters),
+                              stream()),
           "ptg_column_binned")
     e = int(err.item())
     if e:

@@ -130,3 +130,50 @@ def test_frontend_checks():
     except ImportError:
         with pytest.raises(ImportError, match="healpy"):
             ColumnDensity(x, m, h, healpix=1)
+
+
+@pytest.mark.parametrize("gs", [16, 48, 100])
+def test_binned_counters_count_every_pair_when_nothing_is_accepted(gs):
+    """theta -> 0 accepts no node, so every group visits every element of the tree once and evaluates
+    every particle for every one of its targets: pairs = n * nt, visits = groups * (n + nnodes)."""
+    x, m, h = cloud(3000, 6)
+    t = ConstructTree(x, m, h)
+    targets = x[t.TreewalkIndices][:1000]
+    ctr = torch.zeros(2, dtype=torch.int64, device="cuda")
+    got = engine.column_binned(t.dev, targets, gs, 1e-9, counters=ctr).cpu().numpy()
+    assert ctr[0].item() == 3000 * 1000
+    assert ctr[1].item() == -(-1000 // gs) * t.dev.ne
+    o = oracle.build_tree(x, m, h)
+    assert rel(got, oracle.column_binned(o, targets, gs, 1e-9)) < 1e-12
+
+
+def test_ray_counters_accumulate_and_checks():
+    x, m, h = cloud(2000, 7)
+    t = ConstructTree(x, m, h)
+    rays = np.vstack([np.eye(3), -np.eye(3)])
+    ctr = torch.zeros(1, dtype=torch.int64, device="cuda")
+    engine.column_rays(t.dev, x[:500], rays, counters=ctr)
+    one = ctr[0].item()
+    assert one >= 500 * 6  # every walk visits at least the root
+    engine.column_rays(t.dev, x[:500], rays, counters=ctr)
+    assert ctr[0].item() == 2 * one
+    with pytest.raises(ValueError):
+        engine.column_rays(t.dev, x[:500], rays, counters=torch.zeros(1, dtype=torch.int32, device="cuda"))
+    with pytest.raises(ValueError):
+        engine.column_binned(t.dev, x[:500], 16, 0.5, counters=torch.zeros(1, dtype=torch.int64, device="cuda"))
+
+
+@pytest.mark.parametrize("gs", [16, 64])
+def test_binned_heavy_groups_split_across_wavefronts_match_oracle(gs):
+    """At N=1e5 some groups of consecutive Morton targets straddle the whole cloud and visit the whole
+    tree (far past the 32768-visit budget): they are finished by the pass that deals their subtrees out
+    to 64 wavefronts and summed share by share; every group must still equal the oracle."""
+    x, m, h = cloud(100_000, 8)
+    t = ConstructTree(x, m, h)
+    targets = x[t.TreewalkIndices]
+    got = engine.column_binned(t.dev, targets, gs, 0.5).cpu().numpy()
+    o = oracle.build_tree(x, m, h)
+    want = oracle.column_binned(o, targets, gs, 0.5, nthreads=16)
+    assert rel(got, want) < 1e-12
+    again = engine.column_binned(t.dev, targets, gs, 0.5).cpu().numpy()
+    assert np.array_equal(got, again)  # deterministic: shares are summed in a fixed order
