@@ -21,6 +21,11 @@
 
 #define CD_THREADS 256
 
+// one 32-byte record of the FP64 walk pack
+struct alignas(32) D4 {
+    double x, y, z, w;
+};
+
 // a corrupt link (index out of range, or a chain longer than the element count) stops the walk and
 // sets err[0]; the depth-first chain visits each element at most once
 __device__ __forceinline__ bool cd_bad(int64_t no, int64_t ne, int64_t &visits)
@@ -99,6 +104,124 @@ __global__ __launch_bounds__(CD_THREADS) void k_column_rays(const ptg_tree_array
         const int64_t i = id / nt, j = id - (id / nt) * nt;  // ray i, target j
         out[j * nr + i] = ray_column(t, n, ne, tp, nt, rays, nr, per_target, i, j, err, visits);
     }
+    if (ctr) cd_count(ctr, 0, visits);
+}
+
+// ---- ray-traced walk, one wavefront per 64 targets along one ray index ----
+// The wavefront walks the union of its lanes' walks in depth-first preorder over the FP64 walk pack:
+// a stack of (first slot, count, lane mask) sibling ranges, one element per iteration, its record
+// read once for all lanes.  Each lane applies exactly the tests of the per-thread walk to the
+// elements its own walk would reach (those whose ancestors it opened), in the same order, so every
+// column gets the same terms summed in the same sequence.
+#define CR_WAVES 4
+#define CR_DEPTH 256  // sibling ranges on the stack (one per tree level on the current path)
+
+struct CrLds {
+    int s[CR_DEPTH], c[CR_DEPTH];
+    uint64_t m[CR_DEPTH];
+};
+
+__global__ __launch_bounds__(64 * CR_WAVES) void k_column_rays_wave(const D4 *xyzm, const D4 *aux, const int2 *kids,
+                                                                    int64_t nslots, const double *tp, int64_t nt,
+                                                                    const double *rays, int64_t nr, int per_target,
+                                                                    double *out, int32_t *err,
+                                                                    unsigned long long *ctr)
+{
+    __shared__ CrLds lds[CR_WAVES];
+    const int wave = threadIdx.x >> 6, lane = ptg_lane();
+    CrLds &S = lds[wave];
+    const int64_t ntb = (nt + 63) / 64;
+    const int64_t task = (int64_t)blockIdx.x * CR_WAVES + wave;
+    if (task >= ntb * nr) return;  // whole wavefronts only: task is uniform
+    const int64_t i = task / ntb, j = (task % ntb) * 64 + lane;  // ray i, target j
+    const bool live = j < nt;
+    double rx = 0, ry = 0, rz = 0, px = 0, py = 0, pz = 0;
+    if (live) {
+        const double *ray = per_target ? rays + 3 * (j * nr + i) : rays + 3 * i;
+        rx = ray[0];
+        ry = ray[1];
+        rz = ray[2];
+        px = tp[3 * j];
+        py = tp[3 * j + 1];
+        pz = tp[3 * j + 2];
+    }
+    const double fac_density = 3.0 / (4.0 * 3.14159265358979323846);
+    const uint64_t me = 1ull << lane;
+    double column = 0.0;
+    int64_t visits = 0;
+    int top = 0;
+    const uint64_t all = ptg_ballot(live);  // outside the lane-0 branch: a ballot sees the active lanes only
+    if (lane == 0) {
+        S.s[0] = 0;  // the root's walk slot
+        S.c[0] = 1;
+        S.m[0] = all;
+    }
+    __builtin_amdgcn_wave_barrier();
+    while (top >= 0) {
+        const int sl = S.s[top], cnt = S.c[top];
+        const uint64_t m = S.m[top];
+        __builtin_amdgcn_wave_barrier();
+        if (cnt > 1) {
+            if (lane == 0) {
+                S.s[top] = sl + 1;
+                S.c[top] = cnt - 1;
+            }
+        } else {
+            top--;
+        }
+        if (sl < 0 || sl >= nslots) {  // a corrupt range is reported, never followed
+            if (lane == 0) atomicOr(err, 1);
+            __builtin_amdgcn_wave_barrier();
+            continue;
+        }
+        const D4 X = xyzm[sl], X2 = aux[sl];
+        const int2 K = kids[sl];
+        bool open = false;
+        if (m & me) {
+            visits++;
+            const double dx = X.x - px, dy = X.y - py, dz = X.z - pz;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double r = sqrt(r2);
+            const double z = rx * dx + ry * dy + rz * dz;
+            if (!(r2 - z * z < 0)) {  // rounding below zero: skipped, as ColumnDensityWalk_singleray does
+                const double h_no = X2.x;
+                if (K.y == 0) {
+                    if (h_no > 0) {  // a zero radius has no cross-section
+                        const double hinv = 1.0 / h_no;
+                        const double fac = fac_density * X.w * hinv * hinv;
+                        const double r_proj = sqrt(r2 - z * z);
+                        const double q = r_proj * hinv;
+                        if (r_proj < h_no) {
+                            if (r > h_no) {
+                                if (z > 0) column += fac * 2 * sqrt(1 - q * q);
+                            } else {
+                                column += fac * (z * hinv + sqrt(1 - q * q));
+                            }
+                        }
+                    }
+                } else {
+                    const double reach = h_no + X2.y * 0.8660254037844386 + X2.z;
+                    open = r < reach;
+                    if (!open && z > 0) open = (r2 - z * z) < reach * reach;
+                }
+            }
+        }
+        const uint64_t mo = ptg_ballot(open);
+        if (mo && K.y > 0) {
+            if (top + 1 >= CR_DEPTH) {  // deeper than the stack: the caller reruns the stackless walk
+                if (lane == 0) atomicOr(err, 2);
+                break;
+            }
+            top++;
+            if (lane == 0) {
+                S.s[top] = K.x;
+                S.c[top] = std::min(K.y, 8);
+                S.m[top] = mo;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (live) out[j * nr + i] = column;
     if (ctr) cd_count(ctr, 0, visits);
 }
 
@@ -197,10 +320,6 @@ __global__ __launch_bounds__(CD_THREADS) void k_column_binned(const ptg_tree_arr
 #define CB_HCAP 1024     // queued groups at most (further heavy groups finish in the main pass)
 #define CB_W 64          // shares of a queued group
 #define CB_D 4           // depth of the nodes whose children are dealt out to the shares
-
-struct alignas(32) D4 {
-    double x, y, z, w;
-};
 
 struct CbLds {
     int stack[CB_SCAP];
@@ -508,9 +627,10 @@ static ptg_status column_common(const ptg_tree_arrays *tree, int64_t n, int64_t 
     return PTG_OK;
 }
 
-extern "C" ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const double *targets,
-                                      int64_t nt, const double *rays, int64_t nrays, int per_target, double *out,
-                                      int32_t *err, int64_t *counters, void *stream)
+extern "C" ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
+                                      const ptg_walk_tree *wt, const double *targets, int64_t nt, const double *rays,
+                                      int64_t nrays, int per_target, double *out, int32_t *err, int64_t *counters,
+                                      void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     ptg_status st = column_common(tree, n, nnodes, targets, nt, "ptg_column_rays");
@@ -523,6 +643,18 @@ extern "C" ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, in
     PTG_LAUNCH_CHECK();
     const int64_t work = nt * nrays;
     if (work == 0) return PTG_OK;
+    if (wt) {
+        if (wt->prec != PTG_F64 || !wt->xyzm || !wt->aux || !wt->kids || wt->nslots != n + nnodes) {
+            ptg_set_error("ptg_column_rays: wt must be the FP64 walk pack of the tree (or NULL)");
+            return PTG_ERR_ARG;
+        }
+        const int64_t tasks = ptg_cdiv(nt, 64) * nrays;
+        k_column_rays_wave<<<(unsigned)ptg_cdiv(tasks, CR_WAVES), 64 * CR_WAVES, 0, s>>>(
+            (const D4 *)wt->xyzm, (const D4 *)wt->aux, (const int2 *)wt->kids, wt->nslots, targets, nt, rays, nrays,
+            per_target != 0, out, err, (unsigned long long *)counters);
+        PTG_LAUNCH_CHECK();
+        return PTG_OK;
+    }
     k_column_rays<<<(unsigned)ptg_cdiv(work, CD_THREADS), CD_THREADS, 0, s>>>(*tree, n, n + nnodes, targets, nt, rays,
                                                                               nrays, per_target != 0, out, err,
                                                                               (unsigned long long *)counters);

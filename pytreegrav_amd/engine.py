@@ -474,10 +474,18 @@ def column_rays(tree, targets, rays, per_target=False, counters=None):
     _check_counters(counters, 1)
     out = torch.empty((nt, nr), dtype=torch.float64, device=device())
     err = torch.zeros(1, dtype=torch.int32, device=device())
-    check(L.ptg_column_rays(_lib.ctypes.byref(tree.arrays), tree.n, tree.nnodes, ptr(tp), nt, ptr(r), nr,
-                            int(bool(per_target)), ptr(out), ptr(err), ptr(counters), stream()), "ptg_column_rays")
-    if int(err.item()):
-        raise RuntimeError("ptg_column_rays: corrupt tree links")
+    snap = None if counters is None else counters.clone()
+    # the wavefront walk over the FP64 pack; a tree deeper than its stack reruns the stackless walk
+    for wt in (_lib.ctypes.byref(tree.pack(PTG_F64, False)["wt"]), None):
+        check(L.ptg_column_rays(_lib.ctypes.byref(tree.arrays), tree.n, tree.nnodes, wt, ptr(tp), nt, ptr(r), nr,
+                                int(bool(per_target)), ptr(out), ptr(err), ptr(counters), stream()), "ptg_column_rays")
+        e = int(err.item())
+        if e & 1:
+            raise RuntimeError("ptg_column_rays: corrupt tree links")
+        if not e:
+            break
+        if snap is not None:
+            counters.copy_(snap)
     return out
 
 

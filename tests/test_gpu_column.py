@@ -177,3 +177,33 @@ def test_binned_heavy_groups_split_across_wavefronts_match_oracle(gs):
     assert rel(got, want) < 1e-12
     again = engine.column_binned(t.dev, targets, gs, 0.5).cpu().numpy()
     assert np.array_equal(got, again)  # deterministic: shares are summed in a fixed order
+
+
+def test_wavefront_ray_walk_equals_stackless_walk():
+    """The wavefront walk (64 targets along one ray, the union of their paths in depth-first order)
+    applies each lane's own tests in its own order: the per-thread stackless walk's visit counts
+    exactly and its columns to rounding, for shared and per-target rays."""
+    x, m, h = cloud(20_000, 9)
+    t = ConstructTree(x, m, h)
+    L = engine._lib.load()
+    rays = np.vstack([np.eye(3), -np.eye(3), [[0.6, 0.0, 0.8]]])
+    rot = np.stack([rays @ random_rotation(k) for k in range(777)])
+    for tg, ry, pt in ((x[:5000], rays, 0), (x[:777], rot, 1)):
+        nt, nr = len(tg), ry.shape[-2]
+        tp = engine.to_dev(tg, (-1, 3))
+        rr = engine.to_dev(ry)
+        outs, visits = [], []
+        for wt in (engine._lib.ctypes.byref(t.dev.pack(engine.PTG_F64, False)["wt"]), None):
+            out = torch.empty((nt, nr), dtype=torch.float64, device="cuda")
+            err = torch.zeros(1, dtype=torch.int32, device="cuda")
+            ctr = torch.zeros(1, dtype=torch.int64, device="cuda")
+            engine.check(L.ptg_column_rays(engine._lib.ctypes.byref(t.dev.arrays), t.dev.n, t.dev.nnodes, wt,
+                                           engine.ptr(tp), nt, engine.ptr(rr), nr, pt, engine.ptr(out),
+                                           engine.ptr(err), engine.ptr(ctr), engine.stream()), "ptg_column_rays")
+            assert int(err.item()) == 0
+            outs.append(out.cpu().numpy())
+            visits.append(int(ctr.item()))
+        assert visits[0] == visits[1]
+        # same terms in the same order; the two kernels may contract multiply-adds differently, which
+        # grazing chords (sqrt(1 - q^2) with q near 1) amplify to about 1e-13 of the largest column
+        assert rel(outs[0], outs[1]) < 1e-12
