@@ -1203,6 +1203,7 @@ struct NodeWS {
     int32_t *elem;         // (n + nnodes) element stored at each walk slot
     int32_t *qbase;        // (nnodes) first child slot of each node, by level-sorted position
     int32_t *npair;        // (nnodes) the Morton pair that opened each node
+    uint32_t *qperm;       // (nnodes) node order of the slot layout when it is not the level order
 };
 
 static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, int64_t max_level, NodeWS &w)
@@ -1220,6 +1221,7 @@ static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, int64_t max_leve
     w.elem = A.take<int32_t>(n + nnodes);
     w.qbase = A.take<int32_t>(nnodes);
     w.npair = A.take<int32_t>(nnodes);
+    w.qperm = A.take<uint32_t>(nnodes);
     return A.ok();
 }
 
@@ -1574,12 +1576,19 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     const uint32_t *byid = in_first ? nw.idA : nw.idB;
     k_level_starts<<<grid1(nnodes + 1), 256, 0, s>>>(lv, nnodes, max_level, nw.lvstart);
     PTG_LAUNCH_CHECK();
-    // sibling-contiguous walk slots, level-major: an exclusive scan of child counts over the level
-    // list puts the children of each level's nodes in one slot range, in node order
-    k_nchild_by_level<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, byid, t.nchild, nw.qbase);
+    // sibling-contiguous walk slots.  One tree: level-major (an exclusive scan of child counts over
+    // the level list puts the children of each level's nodes in one slot range, in node order,
+    // which the walk's 64-candidate steps read with fewer cache lines).  A forest keeps node order
+    // (depth-first, system after system), so each system's slots stay together.
+    if (F) {
+        k_iota_u32<<<grid1(nnodes), 256, 0, s>>>(nw.qperm, nnodes);
+        k_nchild_by_level<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nw.qperm, t.nchild, nw.qbase);
+    } else {
+        k_nchild_by_level<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, byid, t.nchild, nw.qbase);
+    }
     st = excl_scan(LoadArr{nw.qbase}, nnodes, nw.qbase, nw.bsum, nullptr, s);  // in place
     if (st) return st;
-    k_child_base_scatter<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, byid, nw.qbase, t.child_base);
+    k_child_base_scatter<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, F ? nw.qperm : byid, nw.qbase, t.child_base);
     k_slots<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, F ? forest->roots : nullptr, t, nw);
     PTG_LAUNCH_CHECK();
     if (compute_moments) {
