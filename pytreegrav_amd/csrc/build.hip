@@ -1087,12 +1087,47 @@ __device__ __forceinline__ int32_t head_of(int64_t q, int64_t n, const int32_t *
     return (int32_t)q;
 }
 
-// node geometry, levels and first links of the nodes opened by pair i (one thread per pair); each
-// node's NextBranch needs the end of its cell, searched by k_node_ends with one thread per node so
-// that the long searches of the large cells run in parallel
+// Nodes are numbered breadth-first: level by level (system by system in a forest), each level in
+// depth-first preorder, so that the nodes of one level, which the moment rollup visits together,
+// and their children are contiguous.  Pair i opens the preorder offsets off[i] .. off[i] + li - lp - 1
+// (levels lp+1 .. li); pi maps a preorder offset to its node offset.
+
+// pass 1, one thread per pair: level, opening pair and ordering key (system, level) of every node, by
+// preorder offset
+__global__ __launch_bounds__(256) void k_emit_levels(const int32_t *lev, const int32_t *off, int64_t n,
+                                                     const uint32_t *sys, int lbits, uint64_t *nkey, int32_t *nlev,
+                                                     int32_t *npair)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int lp = i == 0 ? -1 : lev[i - 1];
+    const int li = lev[i];
+    if (li <= lp) return;
+    const uint64_t hi = sys ? (uint64_t)sys[i] << lbits : 0;
+    for (int L = lp + 1; L <= li; L++) {
+        const int64_t a = off[i] + (L - lp - 1);
+        nkey[a] = hi | (uint64_t)L;
+        nlev[a] = L;
+        npair[a] = (int32_t)i;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_invert(const uint32_t *order, int64_t m, uint32_t *pi)
+{
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < m) pi[order[q]] = (uint32_t)q;
+}
+
+// an element as head_of returns it (particle, preorder node n + offset, or -1) in the final numbering
+__device__ __forceinline__ int32_t bfs_elem(int32_t h, int64_t n, const uint32_t *pi)
+{
+    return h < n ? h : (int32_t)(n + pi[h - n]);
+}
+
+// pass 2, one thread per pair: node geometry by the reference's centre recurrence and FirstSubnode
 __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *lev, const int32_t *off, int64_t n,
-                                                    const Geom *geoms, double *coord, double *sizes,
-                                                    int32_t *first, uint64_t *nlevel, int32_t *npair)
+                                                    const Geom *geoms, const uint32_t *pi, double *coord,
+                                                    double *sizes, int32_t *first)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
@@ -1104,14 +1139,13 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
     double s = g->size;
     for (int L = 0; L <= li; L++) {
         if (L > lp) {
-            int64_t idx = n + off[i] + (L - lp - 1);
+            const int64_t a = off[i] + (L - lp - 1);
+            const int64_t idx = n + pi[a];
             coord[3 * idx] = c[0];
             coord[3 * idx + 1] = c[1];
             coord[3 * idx + 2] = c[2];
             sizes[idx] = s;
-            nlevel[idx - n] = (uint64_t)L;
-            npair[idx - n] = (int32_t)i;
-            first[idx] = (L + 1 <= li) ? (int32_t)(idx + 1) : (int32_t)i;
+            first[idx] = (L + 1 <= li) ? (int32_t)(n + pi[a + 1]) : (int32_t)i;
         }
         if (L < li) {
             int d = kv.digit(i, L + 1);
@@ -1122,18 +1156,20 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
     }
 }
 
-// NextBranch of node a (opened by pair i at level L): the element after the last slot of its cell,
-// found by galloping then bisecting on "same level-L cell as slot i"
+// NextBranch of the node at preorder offset a (opened by pair i at level L): the element after the
+// last slot of its cell, found by galloping then bisecting on "same level-L cell as slot i"; one
+// thread per node, so the long searches of the large cells run side by side
 __global__ __launch_bounds__(256) void k_node_ends(KeyView kv, const int32_t *lev, const int32_t *off, int64_t n,
-                                                   int64_t nnodes, const uint64_t *nlevel, const int32_t *npair,
-                                                   int32_t *next)
+                                                   int64_t nnodes, const int32_t *nlev, const int32_t *npair,
+                                                   const uint32_t *pi, int32_t *next)
 {
     int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nnodes) return;
     const int64_t i = npair[a];
-    const int L = (int)nlevel[a];
+    const int L = nlev[a];
+    const int64_t idx = n + pi[a];
     if (i < 0 || i >= n - 1 || L < 0) {  // every node is opened by one pair: never taken
-        next[n + a] = -1;
+        next[idx] = -1;
         return;
     }
     int64_t lo = i + 1;  // slot i+1 shares the pair's level >= L with slot i
@@ -1148,13 +1184,30 @@ __global__ __launch_bounds__(256) void k_node_ends(KeyView kv, const int32_t *le
         if (kv.same_cell(i, mid, L)) lo = mid;
         else hi = mid;
     }
-    next[n + a] = head_of(lo + 1, n, lev, off, kv.sys);
+    next[idx] = bfs_elem(head_of(lo + 1, n, lev, off, kv.sys), n, pi);
 }
 
-// particle slots: gather the input records into Morton order (octree.py:577-603) and link to the
-// next element
+// NextBranch of every particle
+__global__ __launch_bounds__(256) void k_particle_next(const int32_t *lev, const int32_t *off, const uint32_t *sys,
+                                                       int64_t n, const uint32_t *pi, int32_t *next)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) next[i] = n == 1 ? -1 : bfs_elem(head_of(i + 1, n, lev, off, sys), n, pi);
+}
+
+// levels of the nodes in their final (system, level) order, for the level-sorted list of a forest
+__global__ __launch_bounds__(256) void k_key_levels(const uint64_t *skey, int64_t m, int lbits, uint64_t *lv,
+                                                    uint32_t *ids)
+{
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    lv[q] = skey[q] & ((1ull << lbits) - 1);
+    ids[q] = (uint32_t)q;
+}
+
+// particle slots: gather the input records into Morton order (octree.py:577-603); their NextBranch
+// follows once the nodes are numbered (k_particle_next)
 __global__ __launch_bounds__(256) void k_particles(const PartRec *rec, const uint32_t *order, const uint64_t *keys,
-                                                   const int32_t *lev, const int32_t *off, const uint32_t *sys,
                                                    int64_t n, const ptg_tree_arrays t)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1172,7 +1225,6 @@ __global__ __launch_bounds__(256) void k_particles(const PartRec *rec, const uin
     t.treewalk_indices[i] = o;
     if (t.keys) t.keys[i] = keys[i];
     t.first_subnode[i] = -1;
-    t.next_branch[i] = n == 1 ? -1 : head_of(i + 1, n, lev, off, sys);
     t.nchild[i] = 0;
     t.child_base[i] = -1;
     if (t.quadrupoles)
@@ -1196,14 +1248,14 @@ struct NodeWS {
     double *acc;        // (nnodes, 3) sum of m*x of each node, by node offset (element - n)
     int32_t *err;       // [0] corrupt-chain flag
     int32_t *bsum;
-    uint64_t *lvA, *lvB;   // node levels (sort keys)
-    uint32_t *idA, *idB;   // node offsets (sort values): level-sorted node list
+    uint64_t *lvA, *lvB;   // node (system, level) keys, then levels (sort keys)
+    uint32_t *idA, *idB;   // node offsets (sort values)
     int32_t *hist;
     int32_t *lvstart;      // (max_level + 2) first position of each level in the sorted list
     int32_t *elem;         // (n + nnodes) element stored at each walk slot
-    int32_t *qbase;        // (nnodes) first child slot of each node, by level-sorted position
-    int32_t *npair;        // (nnodes) the Morton pair that opened each node
-    uint32_t *qperm;       // (nnodes) node order of the slot layout when it is not the level order
+    int32_t *npair;        // (nnodes) the Morton pair that opened each node, by preorder offset
+    int32_t *nlev;         // (nnodes) its level, by preorder offset
+    uint32_t *pi;          // (nnodes) node offset of each preorder offset
 };
 
 static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, int64_t max_level, NodeWS &w)
@@ -1219,9 +1271,9 @@ static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, int64_t max_leve
     w.hist = A.take<int32_t>(nh);
     w.lvstart = A.take<int32_t>(max_level + 2);
     w.elem = A.take<int32_t>(n + nnodes);
-    w.qbase = A.take<int32_t>(nnodes);
     w.npair = A.take<int32_t>(nnodes);
-    w.qperm = A.take<uint32_t>(nnodes);
+    w.nlev = A.take<int32_t>(nnodes);
+    w.pi = A.take<uint32_t>(nnodes);
     return A.ok();
 }
 
@@ -1444,7 +1496,7 @@ __global__ __launch_bounds__(MOM_THREADS) void k_moments_level(int64_t n, int64_
     const int64_t lo = lvstart[L], hi = lvstart[L + 1];
     const int64_t stride = (int64_t)gridDim.x * MOM_THREADS;
     for (int64_t qi = lo + (int64_t)blockIdx.x * MOM_THREADS + threadIdx.x; qi < hi; qi += stride)
-        lane_moments<QUAD>(n + (int64_t)byid[qi], n, n + nnodes, t, w);
+        lane_moments<QUAD>(n + (byid ? (int64_t)byid[qi] : qi), n, n + nnodes, t, w);
 }
 
 // levels Lhi down to Llo (each with few nodes) in one block, a barrier between levels: all waves
@@ -1459,7 +1511,7 @@ __global__ __launch_bounds__(MS_THREADS) void k_moments_small(int64_t n, int64_t
     for (int L = Lhi; L >= Llo; L--) {
         const int64_t lo = lvstart[L], hi = lvstart[L + 1];
         for (int64_t qi = lo + threadIdx.x; qi < hi; qi += MS_THREADS)
-            lane_moments<QUAD>(n + (int64_t)byid[qi], n, n + nnodes, t, w);
+            lane_moments<QUAD>(n + (byid ? (int64_t)byid[qi] : qi), n, n + nnodes, t, w);
         __syncthreads();
     }
 }
@@ -1487,27 +1539,18 @@ __global__ __launch_bounds__(256) void k_fold_particles(int64_t n, int64_t nnode
         for (int k = 0; k < 9; k++) t.quadrupoles[9 * idx + k] = 0.0;
 }
 
-__global__ __launch_bounds__(256) void k_nchild_by_level(int64_t n, int64_t nnodes, const uint32_t *byid,
-                                                         const int32_t *nchild, int32_t *out)
+__global__ __launch_bounds__(256) void k_child_base_fix(int64_t n, int64_t nnodes, int64_t nsys, int32_t *child_base)
 {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < nnodes) out[q] = nchild[n + byid[q]];
-}
-
-__global__ __launch_bounds__(256) void k_child_base_scatter(int64_t n, int64_t nnodes, int64_t nsys,
-                                                            const uint32_t *byid, const int32_t *qbase,
-                                                            int32_t *child_base)
-{
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < nnodes) child_base[n + byid[q]] = qbase[q] + (int32_t)nsys;  // slots [0, nsys) hold the roots
+    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < nnodes) child_base[n + a] += (int32_t)nsys;  // slots [0, nsys) hold the roots
 }
 
 // forest roots: pair offsets[s] (the first pair of system s) opens its root
 __global__ __launch_bounds__(256) void k_forest_roots(const int64_t *soff, int64_t nsys, const int32_t *off, int64_t n,
-                                                      int32_t *roots)
+                                                      const uint32_t *pi, int32_t *roots)
 {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nsys) roots[s] = (int32_t)(n + off[soff[s]]);
+    if (s < nsys) roots[s] = (int32_t)(n + pi[off[soff[s]]]);
 }
 
 extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const double *h, const ptg_build_info *info,
@@ -1548,47 +1591,60 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     const bool F = nsys > 1;
     if (F) w.soff = forest->offsets;
     KeyView kv{w.kA, info->has_collisions ? w.ext : nullptr, w.sys, n};
-    k_particles<<<grid1(n), 256, 0, s>>>(w.rec, w.vA, w.kA, w.lev, w.off, w.sys, n, t);
+    k_particles<<<grid1(n), 256, 0, s>>>(w.rec, w.vA, w.kA, n, t);
     if (particles_ready) PTG_HIP_CHECK(hipEventRecord((hipEvent_t)particles_ready, s));
+    // node order: (system, level), depth-first preorder within a level (stable LSD radix sort of the
+    // preorder offsets)
+    int lbits = 1;
+    while ((1ll << lbits) <= (int64_t)max_level) lbits++;
+    lbits = ((lbits + RS_BITS - 1) / RS_BITS) * RS_BITS;
+    const int kbits = F ? ((lbits + sys_bits(nsys) + RS_BITS - 1) / RS_BITS) * RS_BITS : lbits;
     if (n == 1) {
-        k_single_root<<<1, 1, 0, s>>>(w.geoms, t);
         PTG_HIP_CHECK(hipMemsetAsync(nw.lvA, 0, sizeof(uint64_t), s));
     } else {
-        k_emit_nodes<<<grid1(n - 1), 256, 0, s>>>(kv, w.lev, w.off, n, w.geoms, t.coordinates, t.sizes,
-                                                  t.first_subnode, nw.lvA, nw.npair);
-        k_node_ends<<<grid1(nnodes), 256, 0, s>>>(kv, w.lev, w.off, n, nnodes, nw.lvA, nw.npair, t.next_branch);
+        k_emit_levels<<<grid1(n - 1), 256, 0, s>>>(w.lev, w.off, n, F ? w.sys : nullptr, lbits, nw.lvA, nw.nlev,
+                                                   nw.npair);
     }
-    if (F) k_forest_roots<<<grid1(nsys), 256, 0, s>>>(w.soff, nsys, w.off, n, forest->roots);
+    k_iota_u32<<<grid1(nnodes), 256, 0, s>>>(nw.idA, nnodes);
+    bool in_first;
+    st = radix_sort(nw.lvA, nw.idA, nw.lvB, nw.idB, nnodes, kbits, nw.hist, nw.bsum, &in_first, s);
+    if (st) return st;
+    const uint64_t *skey = in_first ? nw.lvA : nw.lvB;
+    k_invert<<<grid1(nnodes), 256, 0, s>>>(in_first ? nw.idA : nw.idB, nnodes, nw.pi);
+    if (n == 1) {
+        k_single_root<<<1, 1, 0, s>>>(w.geoms, t);
+    } else {
+        k_emit_nodes<<<grid1(n - 1), 256, 0, s>>>(kv, w.lev, w.off, n, w.geoms, nw.pi, t.coordinates, t.sizes,
+                                                  t.first_subnode);
+        k_node_ends<<<grid1(nnodes), 256, 0, s>>>(kv, w.lev, w.off, n, nnodes, nw.nlev, nw.npair, nw.pi, t.next_branch);
+    }
+    k_particle_next<<<grid1(n), 256, 0, s>>>(w.lev, w.off, w.sys, n, nw.pi, t.next_branch);
+    if (F) k_forest_roots<<<grid1(nsys), 256, 0, s>>>(w.soff, nsys, w.off, n, nw.pi, forest->roots);
     else if (forest) k_fill_i32<<<1, 64, 0, s>>>(forest->roots, 1, (int32_t)n);
+    PTG_LAUNCH_CHECK();
+    // the level-sorted node list of the rollup: the node order itself for one tree; a forest sorts
+    // its (system, level) order by level
+    const uint32_t *byid = nullptr;
+    const uint64_t *lv = skey;
+    if (F) {
+        uint64_t *lk = in_first ? nw.lvB : nw.lvA, *lk2 = in_first ? nw.lvA : nw.lvB;
+        uint32_t *id = in_first ? nw.idB : nw.idA, *id2 = in_first ? nw.idA : nw.idB;
+        k_key_levels<<<grid1(nnodes), 256, 0, s>>>(skey, nnodes, lbits, lk, id);
+        bool first2;
+        st = radix_sort(lk, id, lk2, id2, nnodes, lbits, nw.hist, nw.bsum, &first2, s);
+        if (st) return st;
+        lv = first2 ? lk : lk2;
+        byid = first2 ? id : id2;
+    }
+    k_level_starts<<<grid1(nnodes + 1), 256, 0, s>>>(lv, nnodes, max_level, nw.lvstart);
     k_node_init<<<1, 64, 0, s>>>(nw);
     k_link<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t, nw);
     PTG_LAUNCH_CHECK();
-    // level-sorted node list (stable LSD radix sort of node offsets on the level): node index order
-    // within each level
-    k_iota_u32<<<grid1(nnodes), 256, 0, s>>>(nw.idA, nnodes);
-    int nbits = 1;
-    while ((1ll << nbits) <= (int64_t)max_level) nbits++;
-    nbits = ((nbits + RS_BITS - 1) / RS_BITS) * RS_BITS;
-    bool in_first;
-    st = radix_sort(nw.lvA, nw.idA, nw.lvB, nw.idB, nnodes, nbits, nw.hist, nw.bsum, &in_first, s);
+    // sibling-contiguous walk slots: an exclusive scan of child counts in node order, which puts the
+    // children of each level's nodes (of each system) in one slot range
+    st = excl_scan(LoadArr{t.nchild + n}, nnodes, t.child_base + n, nw.bsum, nullptr, s);
     if (st) return st;
-    const uint64_t *lv = in_first ? nw.lvA : nw.lvB;
-    const uint32_t *byid = in_first ? nw.idA : nw.idB;
-    k_level_starts<<<grid1(nnodes + 1), 256, 0, s>>>(lv, nnodes, max_level, nw.lvstart);
-    PTG_LAUNCH_CHECK();
-    // sibling-contiguous walk slots.  One tree: level-major (an exclusive scan of child counts over
-    // the level list puts the children of each level's nodes in one slot range, in node order,
-    // which the walk's 64-candidate steps read with fewer cache lines).  A forest keeps node order
-    // (depth-first, system after system), so each system's slots stay together.
-    if (F) {
-        k_iota_u32<<<grid1(nnodes), 256, 0, s>>>(nw.qperm, nnodes);
-        k_nchild_by_level<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nw.qperm, t.nchild, nw.qbase);
-    } else {
-        k_nchild_by_level<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, byid, t.nchild, nw.qbase);
-    }
-    st = excl_scan(LoadArr{nw.qbase}, nnodes, nw.qbase, nw.bsum, nullptr, s);  // in place
-    if (st) return st;
-    k_child_base_scatter<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, F ? nw.qperm : byid, nw.qbase, t.child_base);
+    k_child_base_fix<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, t.child_base);
     k_slots<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, F ? forest->roots : nullptr, t, nw);
     PTG_LAUNCH_CHECK();
     if (compute_moments) {
