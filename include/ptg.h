@@ -80,6 +80,10 @@ typedef struct {
     int32_t *child_base;      /* (n+nnodes): walk slot of the first child (sibling-contiguous layout) */
     int32_t *slot;            /* (n+nnodes): walk slot of each element (the root is slot 0) */
     uint64_t *keys;           /* (n) or NULL: 63-bit Morton key of each particle slot (of its system's cube) */
+    double *node_stats;       /* (nnodes, 5) or NULL (build with quadrupoles and moments): per node |Q|_F, the
+                               * largest child mass, sum m d^2, sum m d^3 and the largest d over its
+                               * children (d = child centre to the node's centre of mass); the learned
+                               * opening radii read these instead of walking each node's links */
 } ptg_tree_arrays;
 
 /* Forest: several independent systems (no reference counterpart; the reference builds one Octree

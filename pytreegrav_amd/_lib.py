@@ -49,6 +49,7 @@ class TreeArrays(_c.Structure):
         ("child_base", _vp),
         ("slot", _vp),
         ("keys", _vp),
+        ("node_stats", _vp),
     ]
 
 

@@ -1467,22 +1467,52 @@ __device__ __forceinline__ void lane_moments(int64_t idx, int64_t n, int64_t ne,
         Q[7] += mm * 3.0 * r2 * r1;
         Q[8] += mm * (3.0 * r2 * r2 - rsq);
     };
+    // child statistics of the learned criterion's features, gathered on the way: largest child mass,
+    // sums of m d^2 and m d^3, largest d (d = child centre to the centre of mass)
+    const bool stats = t.node_stats != nullptr;
+    double mmax = 0.0, m2 = 0.0, m3 = 0.0, dmax = 0.0;
+    auto stat = [&](double mc, double r0, double r1, double r2) {
+        const double d = sqrt(r0 * r0 + r1 * r1 + r2 * r2);
+        mmax = fmax(mmax, mc);
+        m2 += mc * d * d;
+        m3 += mc * d * d * d;
+        dmax = fmax(dmax, d);
+    };
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         if (!part[k]) continue;
-        shift(km[k], kv0[k] - com0, kv1[k] - com1, kv2[k] - com2);
+        const double r0 = kv0[k] - com0, r1 = kv1[k] - com1, r2 = kv2[k] - com2;
+        shift(km[k], r0, r1, r2);
+        if (stats) stat(km[k], r0, r1, r2);
     }
 #pragma unroll
     for (int k = 7; k >= 0; k--) {
-        if (!node[k] || km[k] == 0.0) continue;
+        if (!node[k]) continue;
         const int64_t e = c[k];
+        const double r0 = t.coordinates[3 * e] - com0, r1 = t.coordinates[3 * e + 1] - com1,
+                     r2 = t.coordinates[3 * e + 2] - com2;
+        if (km[k] == 0.0) {  // a massless child adds nothing but its distance
+            if (stats) stat(0.0, r0, r1, r2);
+            continue;
+        }
         const double *KQ = t.quadrupoles + 9 * e;
 #pragma unroll
         for (int j = 0; j < 9; j++) Q[j] += KQ[j];
-        shift(km[k], t.coordinates[3 * e] - com0, t.coordinates[3 * e + 1] - com1, t.coordinates[3 * e + 2] - com2);
+        shift(km[k], r0, r1, r2);
+        if (stats) stat(km[k], r0, r1, r2);
     }
 #pragma unroll
     for (int j = 0; j < 9; j++) t.quadrupoles[9 * idx + j] = Q[j];
+    if (stats) {
+        double q2 = 0.0;
+        for (int j = 0; j < 9; j++) q2 += Q[j] * Q[j];
+        double *st = t.node_stats + 5 * (idx - n);
+        st[0] = sqrt(q2);
+        st[1] = mmax;
+        st[2] = m2;
+        st[3] = m3;
+        st[4] = dmax;
+    }
 }
 
 #define MOM_THREADS 256
@@ -1571,6 +1601,10 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
         !t.next_branch || !t.treewalk_indices || !t.nchild || !t.child_base || !t.slot ||
         (quadrupole && !t.quadrupoles)) {
         ptg_set_error("ptg_build_tree: a required tree array is NULL");
+        return PTG_ERR_ARG;
+    }
+    if (t.node_stats && !(quadrupole && compute_moments)) {
+        ptg_set_error("ptg_build_tree: node_stats needs a build with quadrupoles and moments");
         return PTG_ERR_ARG;
     }
     int64_t n = info->n, nnodes = info->nnodes;

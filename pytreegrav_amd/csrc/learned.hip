@@ -45,6 +45,28 @@ __device__ __forceinline__ int node_kids(const ptg_tree_arrays &t, int64_t e, in
     return nk;
 }
 
+// Features of node e from its quadrupole norm qn, child count nk and child statistics: the largest
+// child mass mmax, sum m d^2, sum m d^3 and the largest d over its children (d = child centre to the
+// node's centre of mass).
+__device__ void features_from(const ptg_tree_arrays &t, int64_t e, double eps, double s, double M, double qn, int nk,
+                              double mmax, double m2, double m3, double dmax, float *f)
+{
+    // log10(r_Q / s) = log10(3.5 |Q| / (eps s^4)) / 4, r_Q = (3.5 |Q| / eps)^(1/4); the ratios are
+    // formed in FP64 and clamped before the FP32 logarithms
+    const double s2 = s * s;
+    const float lq = log10f((float)clampd(3.5 * qn / (eps * s2 * s2), 1e-12, 1e12));
+    f[0] = fminf(fmaxf(0.25f * lq, -2.f), 3.f) * 0.5f;
+    f[1] = (float)clampd(t.deltas[e] / s, 0.0, 2.0);
+    f[2] = (float)clampd(qn / (M * s * s), 0.0, 2.0);
+    f[3] = (float)clampd(t.softenings[e] / s, 0.0, 4.0);
+    f[4] = (float)nk * 0.125f;
+    f[5] = (float)(mmax / M);
+    f[6] = (float)clampd(m2 / (M * s * s), 0.0, 2.0);
+    f[7] = (float)clampd(m3 / (M * s * s * s), 0.0, 2.0);
+    f[8] = (float)clampd(dmax / s, 0.0, 2.0);
+    f[9] = log10f((float)clampd(M / (eps * s2), 1e-4, 1e12)) * 0.25f;
+}
+
 // Returns false for a massless or zero-size node (its radius is then 0: the geometric test decides).
 __device__ bool node_features(const ptg_tree_arrays &t, int64_t ne, int64_t e, double eps, float *f)
 {
@@ -75,20 +97,20 @@ __device__ bool node_features(const ptg_tree_arrays &t, int64_t ne, int64_t e, d
         m3 += mc * d * d * d;
         dmax = fmax(dmax, d);
     }
-    // log10(r_Q / s) = log10(3.5 |Q| / (eps s^4)) / 4, r_Q = (3.5 |Q| / eps)^(1/4); the ratios are
-    // formed in FP64 and clamped before the FP32 logarithms
-    const double s2 = s * s;
-    const float lq = log10f((float)clampd(3.5 * qn / (eps * s2 * s2), 1e-12, 1e12));
-    f[0] = fminf(fmaxf(0.25f * lq, -2.f), 3.f) * 0.5f;
-    f[1] = (float)clampd(t.deltas[e] / s, 0.0, 2.0);
-    f[2] = (float)clampd(qn / (M * s * s), 0.0, 2.0);
-    f[3] = (float)clampd(t.softenings[e] / s, 0.0, 4.0);
-    f[4] = (float)nk * 0.125f;
-    f[5] = (float)(mmax / M);
-    f[6] = (float)clampd(m2 / (M * s * s), 0.0, 2.0);
-    f[7] = (float)clampd(m3 / (M * s * s * s), 0.0, 2.0);
-    f[8] = (float)clampd(dmax / s, 0.0, 2.0);
-    f[9] = log10f((float)clampd(M / (eps * s2), 1e-4, 1e12)) * 0.25f;
+    features_from(t, e, eps, s, M, qn, nk, mmax, m2, m3, dmax, f);
+    return true;
+}
+
+// The same features from the statistics the build stored per node (no walk over the links).
+__device__ bool node_features_stats(const ptg_tree_arrays &t, int64_t n, int64_t e, double eps, float *f)
+{
+    const double s = t.sizes[e], M = t.masses[e];
+    if (!(s > 0.0) || !(M > 0.0)) {
+        for (int k = 0; k < MLP_IN; k++) f[k] = 0.f;
+        return false;
+    }
+    const double *st = t.node_stats + 5 * (e - n);
+    features_from(t, e, eps, s, M, st[0], t.nchild[e], st[1], st[2], st[3], st[4], f);
     return true;
 }
 
@@ -601,7 +623,7 @@ __global__ __launch_bounds__(256) void k_open_radii(const ptg_tree_arrays t, int
     }
     float f[MLP_IN];
     float radius = 0.f;
-    if (node_features(t, n + nnodes, e, eps, f)) {
+    if (t.node_stats ? node_features_stats(t, n, e, eps, f) : node_features(t, n + nnodes, e, eps, f)) {
         const float yhat = (bf16 ? mlp_forward_bf16(Q, f) : mlp_forward_f32(P, f)) + margin;
         const float rho = fminf(exp10f(yhat), rho_max);
         radius = (float)(t.sizes[e] * (double)rho);

@@ -126,11 +126,15 @@ class DeviceTree:
         self.child_base = torch.empty(ne, **i32)
         self.slot = torch.empty(ne, **i32)
         self.keys = torch.empty(n, dtype=torch.int64, device=dev)  # 63-bit Morton keys of the particle slots
+        # per-node child statistics for the learned criterion's features (trees with quadrupoles)
+        stats = bool(quadrupole) and bool(compute_moments)
+        self.node_stats = torch.empty((nn, 5), **f64) if stats else None
         self.arrays = TreeArrays(
             ptr(self.coordinates).value, ptr(self.masses).value, ptr(self.softenings).value, ptr(self.sizes).value,
             ptr(self.deltas).value, ptr(self.quadrupoles).value if quadrupole else None,
             ptr(self.first_subnode).value, ptr(self.next_branch).value, ptr(self.order).value,
-            ptr(self.nchild).value, ptr(self.child_base).value, ptr(self.slot).value, ptr(self.keys).value)
+            ptr(self.nchild).value, ptr(self.child_base).value, ptr(self.slot).value, ptr(self.keys).value,
+            ptr(self.node_stats).value if stats else None)
         nws = workspace(L.ptg_node_workspace_bytes(n, nn, int(info.max_level)))
         # recorded by the build once the particle slots exist (order, keys): target groups can start
         # on a side stream while the nodes and moments are still being built

@@ -44,7 +44,7 @@ def test_struct_layouts():
     from pytreegrav_amd import _lib
 
     assert ctypes.sizeof(_lib.BuildInfo) == 8 + 8 + 4 * 4 + 3 * 8 + 8
-    assert ctypes.sizeof(_lib.TreeArrays) == 13 * 8
+    assert ctypes.sizeof(_lib.TreeArrays) == 14 * 8
     assert ctypes.sizeof(_lib.WalkTree) == 4 * 8 + 8 + 4 + 4 + 3 * 8
 
 

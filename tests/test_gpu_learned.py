@@ -252,3 +252,30 @@ def test_frontend_criteria_and_self_fields_pipeline():
         Accel(x, m, h, device="cpu", criterion="learned", alpha=1e-3)  # float64 walk has no learned form
     with pytest.raises(ValueError):
         Accel(x, m, h, device="cuda", criterion="learned")  # alpha required
+
+
+def test_radii_from_build_statistics_equal_the_link_walk():
+    """The build stores each node's child statistics (quadrupole norm, largest child mass, sums of
+    m d^2 and m d^3, largest d); the radius pass reading them must give the radii of the pass that
+    walks every node's links (the FP64 statistics may differ in the last bit, rarely visible in the
+    FP32 features)."""
+    x, m, h = case("plummer_3e4")
+    t = engine.DeviceTree(x, m, np.full(len(m), 0.01), quadrupole=True)
+    assert t.node_stats is not None
+    model = learned.OpeningMLP.load()
+    radii = []
+    for use_stats in (True, False):
+        p = t.pack(engine.PTG_F32, False, cache=False, norms=False)
+        saved = t.arrays
+        if not use_stats:
+            t.arrays = engine.TreeArrays.from_buffer_copy(saved)
+            t.arrays.node_stats = None
+        try:
+            learned.open_radii(t, p, model, 1e-3)
+        finally:
+            t.arrays = saved
+        radii.append(p["tensors"][1][:, 3].cpu().numpy().copy())
+    a, b = radii
+    assert np.all(np.isfinite(a)) and (a > 0).sum() > 1000
+    assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-30)) < 1e-4
+    assert np.mean(a == b) > 0.99
