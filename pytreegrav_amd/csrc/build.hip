@@ -1124,50 +1124,52 @@ __device__ __forceinline__ int32_t bfs_elem(int32_t h, int64_t n, const uint32_t
     return h < n ? h : (int32_t)(n + pi[h - n]);
 }
 
-// pass 2, one thread per pair: node geometry by the reference's centre recurrence and FirstSubnode
-__global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *lev, const int32_t *off, int64_t n,
-                                                    const Geom *geoms, const uint32_t *pi, double *coord,
+// pass 2, one thread per node in node order (coalesced writes): geometry by the reference's centre
+// recurrence from the root down the opening pair's key digits, and FirstSubnode.  order[q] is the
+// preorder offset of node offset q.
+__global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *lev, int64_t n, int64_t nnodes,
+                                                    const Geom *geoms, const uint32_t *order, const int32_t *nlev,
+                                                    const int32_t *npair, const uint32_t *pi, double *coord,
                                                     double *sizes, int32_t *first)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
-    int lp = i == 0 ? -1 : lev[i - 1];
-    int li = lev[i];
-    if (li <= lp) return;
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nnodes) return;
+    const int64_t a = order[q], idx = n + q;
+    const int64_t i = npair[a];
+    const int L = nlev[a];
+    if (i < 0 || i >= n - 1 || L < 0) {  // every node is opened by one pair: never taken
+        first[idx] = -1;
+        return;
+    }
     const Geom *g = geoms + (kv.sys ? kv.sys[i] : 0);
     double c[3] = {g->center[0], g->center[1], g->center[2]};
     double s = g->size;
-    for (int L = 0; L <= li; L++) {
-        if (L > lp) {
-            const int64_t a = off[i] + (L - lp - 1);
-            const int64_t idx = n + pi[a];
-            coord[3 * idx] = c[0];
-            coord[3 * idx + 1] = c[1];
-            coord[3 * idx + 2] = c[2];
-            sizes[idx] = s;
-            first[idx] = (L + 1 <= li) ? (int32_t)(n + pi[a + 1]) : (int32_t)i;
-        }
-        if (L < li) {
-            int d = kv.digit(i, L + 1);
+    for (int l = 0; l < L; l++) {
+        const int d = kv.digit(i, l + 1);
 #pragma unroll
-            for (int k = 0; k < 3; k++) c[k] = c[k] + s * ptg_octant_offset(d, k);
-            s = s * 0.5;
-        }
+        for (int k = 0; k < 3; k++) c[k] = c[k] + s * ptg_octant_offset(d, k);
+        s = s * 0.5;
     }
+    coord[3 * idx] = c[0];
+    coord[3 * idx + 1] = c[1];
+    coord[3 * idx + 2] = c[2];
+    sizes[idx] = s;
+    first[idx] = (L + 1 <= lev[i]) ? (int32_t)(n + pi[a + 1]) : (int32_t)i;
 }
 
-// NextBranch of the node at preorder offset a (opened by pair i at level L): the element after the
+// NextBranch of node offset q (preorder offset a, opened by pair i at level L): the element after the
 // last slot of its cell, found by galloping then bisecting on "same level-L cell as slot i"; one
 // thread per node, so the long searches of the large cells run side by side
 __global__ __launch_bounds__(256) void k_node_ends(KeyView kv, const int32_t *lev, const int32_t *off, int64_t n,
-                                                   int64_t nnodes, const int32_t *nlev, const int32_t *npair,
-                                                   const uint32_t *pi, int32_t *next)
+                                                   int64_t nnodes, const uint32_t *order, const int32_t *nlev,
+                                                   const int32_t *npair, const uint32_t *pi, int32_t *next)
 {
-    int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nnodes) return;
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nnodes) return;
+    const int64_t a = order[q];
     const int64_t i = npair[a];
     const int L = nlev[a];
-    const int64_t idx = n + pi[a];
+    const int64_t idx = n + q;
     if (i < 0 || i >= n - 1 || L < 0) {  // every node is opened by one pair: never taken
         next[idx] = -1;
         return;
@@ -1648,9 +1650,10 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     if (n == 1) {
         k_single_root<<<1, 1, 0, s>>>(w.geoms, t);
     } else {
-        k_emit_nodes<<<grid1(n - 1), 256, 0, s>>>(kv, w.lev, w.off, n, w.geoms, nw.pi, t.coordinates, t.sizes,
-                                                  t.first_subnode);
-        k_node_ends<<<grid1(nnodes), 256, 0, s>>>(kv, w.lev, w.off, n, nnodes, nw.nlev, nw.npair, nw.pi, t.next_branch);
+        k_emit_nodes<<<grid1(nnodes), 256, 0, s>>>(kv, w.lev, n, nnodes, w.geoms, in_first ? nw.idA : nw.idB,
+                                                   nw.nlev, nw.npair, nw.pi, t.coordinates, t.sizes, t.first_subnode);
+        k_node_ends<<<grid1(nnodes), 256, 0, s>>>(kv, w.lev, w.off, n, nnodes, in_first ? nw.idA : nw.idB, nw.nlev,
+                                                  nw.npair, nw.pi, t.next_branch);
     }
     k_particle_next<<<grid1(n), 256, 0, s>>>(w.lev, w.off, w.sys, n, nw.pi, t.next_branch);
     if (F) k_forest_roots<<<grid1(nsys), 256, 0, s>>>(w.soff, nsys, w.off, n, nw.pi, forest->roots);
