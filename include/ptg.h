@@ -292,11 +292,13 @@ ptg_status ptg_rms_rows(const double *x, int64_t rows, int stride, int col, int 
  * != 0 (randomized ray grids, :1061-1078).  wt: the FP64 walk pack of the same tree (ptg_pack_walk,
  * PTG_F64) for the wavefront walk (64 targets per wavefront walk the union of their paths, one
  * record load per element), or NULL for the stackless per-thread walk on the links; err 2 means
- * the tree was deeper than the wavefront walk's stack: rerun with wt = NULL.  counters (nullable,
- * device int64[1], accumulated): [0] elements visited over all (target, ray) walks. */
+ * the tree was deeper than the wavefront walk's stack: rerun with wt = NULL.  order (nullable, used
+ * with wt): (nrays, nt) permutations of the targets, one per ray, grouping 64 targets per wavefront
+ * in that order (e.g. by position across the ray, so their paths overlap); results do not depend on
+ * it.  counters (nullable, device int64[1], accumulated): [0] elements visited over all walks. */
 ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_walk_tree *wt,
                            const double *targets, int64_t nt, const double *rays, int64_t nrays, int per_target,
-                           double *out, int32_t *err, int64_t *counters, void *stream);
+                           const int64_t *order, double *out, int32_t *err, int64_t *counters, void *stream);
 /* Angular-binned (rays=None): out (nt, 6), the mean column per equal-area sky bin, with groups of
  * group_size consecutive targets sharing one traversal opened at the node's distance to the group's
  * box (ColumnDensityBinned_grouped :1197-1276; group_size 1 is ColumnDensityWalk_binned :976-1028).

@@ -96,7 +96,7 @@ SIGNATURES = {
     "ptg_pack_walk": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _F, _c.POINTER(WalkTree), _vp]),
     "ptg_walk_workspace_bytes": (_c.c_size_t, [_i64, _c.c_int]),
     "ptg_column_rays": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _c.POINTER(WalkTree), _vp, _i64, _vp, _i64,
-                                   _c.c_int, _vp, _vp, _vp, _vp]),
+                                   _c.c_int, _vp, _vp, _vp, _vp, _vp]),
     "ptg_column_binned": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _c.POINTER(WalkTree), _vp, _i64, _i64, _dbl,
                                      _vp, _vp, _vp, _c.c_size_t, _vp, _vp]),
     "ptg_column_workspace_bytes": (_c.c_size_t, [_i64]),

@@ -124,7 +124,7 @@ struct CrLds {
 __global__ __launch_bounds__(64 * CR_WAVES) void k_column_rays_wave(const D4 *xyzm, const D4 *aux, const int2 *kids,
                                                                     int64_t nslots, const double *tp, int64_t nt,
                                                                     const double *rays, int64_t nr, int per_target,
-                                                                    double *out, int32_t *err,
+                                                                    const int64_t *order, double *out, int32_t *err,
                                                                     unsigned long long *ctr)
 {
     __shared__ CrLds lds[CR_WAVES];
@@ -133,8 +133,10 @@ __global__ __launch_bounds__(64 * CR_WAVES) void k_column_rays_wave(const D4 *xy
     const int64_t ntb = (nt + 63) / 64;
     const int64_t task = (int64_t)blockIdx.x * CR_WAVES + wave;
     if (task >= ntb * nr) return;  // whole wavefronts only: task is uniform
-    const int64_t i = task / ntb, j = (task % ntb) * 64 + lane;  // ray i, target j
-    const bool live = j < nt;
+    const int64_t i = task / ntb, jr = (task % ntb) * 64 + lane;  // ray i, rank jr of the target
+    const bool live = jr < nt;
+    // the target order of ray i (if given) puts targets whose rays run side by side in one wavefront
+    const int64_t j = live && order ? order[i * nt + jr] : jr;
     double rx = 0, ry = 0, rz = 0, px = 0, py = 0, pz = 0;
     if (live) {
         const double *ray = per_target ? rays + 3 * (j * nr + i) : rays + 3 * i;
@@ -629,8 +631,8 @@ static ptg_status column_common(const ptg_tree_arrays *tree, int64_t n, int64_t 
 
 extern "C" ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
                                       const ptg_walk_tree *wt, const double *targets, int64_t nt, const double *rays,
-                                      int64_t nrays, int per_target, double *out, int32_t *err, int64_t *counters,
-                                      void *stream)
+                                      int64_t nrays, int per_target, const int64_t *order, double *out, int32_t *err,
+                                      int64_t *counters, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     ptg_status st = column_common(tree, n, nnodes, targets, nt, "ptg_column_rays");
@@ -651,7 +653,7 @@ extern "C" ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, in
         const int64_t tasks = ptg_cdiv(nt, 64) * nrays;
         k_column_rays_wave<<<(unsigned)ptg_cdiv(tasks, CR_WAVES), 64 * CR_WAVES, 0, s>>>(
             (const D4 *)wt->xyzm, (const D4 *)wt->aux, (const int2 *)wt->kids, wt->nslots, targets, nt, rays, nrays,
-            per_target != 0, out, err, (unsigned long long *)counters);
+            per_target != 0, order, out, err, (unsigned long long *)counters);
         PTG_LAUNCH_CHECK();
         return PTG_OK;
     }

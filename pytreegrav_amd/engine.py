@@ -464,6 +464,21 @@ def _check_counters(counters, k):
         raise ValueError(f"counters must be a contiguous int64 device tensor of at least {k} entries")
 
 
+def _ray_orders(tp, rays):
+    """(nr, nt) int64: for each ray u, the targets sorted by the 2D Morton order of their coordinates
+    in a plane perpendicular to u (ptg_morton_order with the third coordinate zero)."""
+    orders = []
+    for u in rays.cpu().numpy():
+        u = u / np.linalg.norm(u)
+        a = np.array([1.0, 0.0, 0.0]) if abs(u[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+        e1 = np.cross(u, a)
+        e1 /= np.linalg.norm(e1)
+        e2 = np.cross(u, e1)
+        basis = torch.tensor(np.stack([e1, e2, np.zeros(3)], 1), dtype=torch.float64, device=tp.device)
+        orders.append(morton_order(tp @ basis))
+    return torch.stack(orders).contiguous()
+
+
 def column_rays(tree, targets, rays, per_target=False, counters=None):
     """Ray-traced column density (ptg_column_rays): (nt, nr) float64 device tensor for targets (nt, 3)
     and unit rays (nr, 3), or per-target rays (nt, nr, 3) when per_target.  counters: optional device
@@ -479,10 +494,14 @@ def column_rays(tree, targets, rays, per_target=False, counters=None):
     out = torch.empty((nt, nr), dtype=torch.float64, device=device())
     err = torch.zeros(1, dtype=torch.int32, device=device())
     snap = None if counters is None else counters.clone()
+    # shared rays: per ray, the targets in 2D Morton order of their position across the ray, so the
+    # 64 targets of a wavefront walk nearly the same path
+    order = _ray_orders(tp, r) if not per_target and nt > 64 else None
     # the wavefront walk over the FP64 pack; a tree deeper than its stack reruns the stackless walk
     for wt in (_lib.ctypes.byref(tree.pack(PTG_F64, False)["wt"]), None):
         check(L.ptg_column_rays(_lib.ctypes.byref(tree.arrays), tree.n, tree.nnodes, wt, ptr(tp), nt, ptr(r), nr,
-                                int(bool(per_target)), ptr(out), ptr(err), ptr(counters), stream()), "ptg_column_rays")
+                                int(bool(per_target)), ptr(order) if wt is not None else None, ptr(out), ptr(err),
+                                ptr(counters), stream()), "ptg_column_rays")
         e = int(err.item())
         if e & 1:
             raise RuntimeError("ptg_column_rays: corrupt tree links")

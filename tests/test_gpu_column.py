@@ -193,17 +193,27 @@ def test_wavefront_ray_walk_equals_stackless_walk():
         tp = engine.to_dev(tg, (-1, 3))
         rr = engine.to_dev(ry)
         outs, visits = [], []
-        for wt in (engine._lib.ctypes.byref(t.dev.pack(engine.PTG_F64, False)["wt"]), None):
+        pack = engine._lib.ctypes.byref(t.dev.pack(engine.PTG_F64, False)["wt"])
+        # wavefront walk in target order, in per-ray orders across the ray (shared rays), stackless walk
+        order = engine._ray_orders(tp, rr) if not pt else None
+        for wt, od in ((pack, None), (pack, order), (None, None)):
+            if wt is not None and od is None and len(outs) == 1:
+                continue
             out = torch.empty((nt, nr), dtype=torch.float64, device="cuda")
             err = torch.zeros(1, dtype=torch.int32, device="cuda")
             ctr = torch.zeros(1, dtype=torch.int64, device="cuda")
             engine.check(L.ptg_column_rays(engine._lib.ctypes.byref(t.dev.arrays), t.dev.n, t.dev.nnodes, wt,
-                                           engine.ptr(tp), nt, engine.ptr(rr), nr, pt, engine.ptr(out),
-                                           engine.ptr(err), engine.ptr(ctr), engine.stream()), "ptg_column_rays")
+                                           engine.ptr(tp), nt, engine.ptr(rr), nr, pt, engine.ptr(od),
+                                           engine.ptr(out), engine.ptr(err), engine.ptr(ctr), engine.stream()),
+                         "ptg_column_rays")
             assert int(err.item()) == 0
             outs.append(out.cpu().numpy())
             visits.append(int(ctr.item()))
-        assert visits[0] == visits[1]
-        # same terms in the same order; the two kernels may contract multiply-adds differently, which
-        # grazing chords (sqrt(1 - q^2) with q near 1) amplify to about 1e-13 of the largest column
-        assert rel(outs[0], outs[1]) < 1e-12
+        assert len(set(visits)) == 1
+        for o in outs[1:]:
+            # same terms in the same order; the kernels may contract multiply-adds differently, which
+            # grazing chords (sqrt(1 - q^2) with q near 1) amplify to about 1e-13 of the largest column
+            assert rel(outs[0], o) < 1e-12
+        if order is not None:
+            assert np.array_equal(outs[0], outs[1])  # the packet composition does not change a column
+            assert np.array_equal(np.sort(order.cpu().numpy(), 1), np.tile(np.arange(nt), (nr, 1)))
