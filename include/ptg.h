@@ -100,6 +100,27 @@ typedef struct {
     double *cubes;          /* device (nsys, 4): root cube centre x, y, z and side; written by ptg_build_sort */
 } ptg_forest;
 
+/* ------------------------------------------------------------------------------------------
+ * Walk tree: the sibling-contiguous packed copy of the tree the walk reads (one slot per
+ * element; the children of a node occupy nchild consecutive slots from child_base).  Coordinates
+ * are stored relative to `origin` (node-relative precision for FP32 packs; pass 0 for FP64).  For
+ * a forest, `origin` is ignored: FP32 packs are relative to each system's cube centre and FP64
+ * packs absolute.
+ * Per slot: xyzm (4 reals: x, y, z, mass), aux (4 reals: softening, size, delta, 0),
+ * kids (2 int32: first child slot, child count; count 0 marks a particle),
+ * quad (8 reals: xx, xy, xz, yy, yz, zz, 0, 0) when quadrupole != 0.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct {
+    void *xyzm;
+    void *aux;
+    int32_t *kids;
+    void *quad; /* NULL for a monopole pack */
+    int64_t nslots;
+    int32_t prec; /* PTG_F32 / PTG_F64 */
+    int32_t has_quad;
+    double origin[3];
+} ptg_walk_tree;
+
 size_t ptg_build_workspace_bytes(int64_t n, int64_t nsys);
 /* A non-finite position, mass or softening fails with PTG_ERR_ARG and info->nonfinite = 1
  * (ConstructTree's check, frontend.py:123-125).  Pass ptg_build_tree the same pos, m and h and the
@@ -111,11 +132,14 @@ ptg_status ptg_build_sort(const double *pos, const double *m, const double *h, i
 size_t ptg_node_workspace_bytes(int64_t n, int64_t nnodes, int max_level);
 /* particles_ready (nullable hipEvent_t): recorded on the stream as soon as the particle slots
  * (coordinates, masses, softenings, treewalk_indices, keys) are written, so a caller can start
- * work that needs only those (target groups) on another stream while nodes and moments are built. */
+ * work that needs only those (target groups) on another stream while nodes and moments are built.
+ * pack (nullable): a walk copy to fill at the end of the build, the same as ptg_pack_walk's with the
+ * quadrupole norms in aux[3], but written in slot order through the build's slot map (coalesced
+ * stores); its ptg_walk_tree is as for ptg_pack_walk (nslots = n + nnodes). */
 ptg_status ptg_build_tree(const double *pos, const double *m, const double *h, const ptg_build_info *info,
                           const ptg_forest *forest, int quadrupole, int compute_moments, void *ws, size_t ws_bytes,
-                          void *node_ws, size_t node_ws_bytes, const ptg_tree_arrays *tree, void *particles_ready,
-                          void *stream);
+                          void *node_ws, size_t node_ws_bytes, const ptg_tree_arrays *tree,
+                          const ptg_walk_tree *pack, void *particles_ready, void *stream);
 
 /* Morton permutation of arbitrary points: replaces grouped_treewalk._morton_order (:269-275).
  * Uses a ptg_build_workspace_bytes(n, 1) workspace; asynchronous. */
@@ -138,26 +162,6 @@ ptg_status ptg_target_groups(const double *pos, const int64_t *perm, const uint6
                              const ptg_forest *forest, int group_size, int mode, int32_t *offsets, int32_t *ngroups,
                              void *ws, size_t ws_bytes, void *stream);
 
-/* ------------------------------------------------------------------------------------------
- * Walk tree: the sibling-contiguous packed copy of the tree the walk reads (one slot per
- * element; the children of a node occupy nchild consecutive slots from child_base).  Coordinates
- * are stored relative to `origin` (node-relative precision for FP32 packs; pass 0 for FP64).  For
- * a forest, `origin` is ignored: FP32 packs are relative to each system's cube centre and FP64
- * packs absolute.
- * Per slot: xyzm (4 reals: x, y, z, mass), aux (4 reals: softening, size, delta, 0),
- * kids (2 int32: first child slot, child count; count 0 marks a particle),
- * quad (8 reals: xx, xy, xz, yy, yz, zz, 0, 0) when quadrupole != 0.
- * ------------------------------------------------------------------------------------------ */
-typedef struct {
-    void *xyzm;
-    void *aux;
-    int32_t *kids;
-    void *quad; /* NULL for a monopole pack */
-    int64_t nslots;
-    int32_t prec; /* PTG_F32 / PTG_F64 */
-    int32_t has_quad;
-    double origin[3];
-} ptg_walk_tree;
 
 ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_forest *forest,
                          const ptg_walk_tree *wt, void *stream);

@@ -91,7 +91,7 @@ SIGNATURES = {
     "ptg_target_groups": (_c.c_int, [_vp, _vp, _vp, _i64, _F, _c.c_int, _c.c_int, _vp, _vp, _vp, _c.c_size_t,
                                      _vp]),
     "ptg_build_tree": (_c.c_int, [_vp, _vp, _vp, _c.POINTER(BuildInfo), _F, _c.c_int, _c.c_int, _vp, _c.c_size_t,
-                                  _vp, _c.c_size_t, _c.POINTER(TreeArrays), _vp, _vp]),
+                                  _vp, _c.c_size_t, _c.POINTER(TreeArrays), _c.POINTER(WalkTree), _vp, _vp]),
     "ptg_morton_order": (_c.c_int, [_vp, _i64, _vp, _vp, _c.c_size_t, _vp]),
     "ptg_pack_walk": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _F, _c.POINTER(WalkTree), _vp]),
     "ptg_walk_workspace_bytes": (_c.c_size_t, [_i64, _c.c_int]),

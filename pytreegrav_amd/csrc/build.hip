@@ -1603,10 +1603,15 @@ __global__ __launch_bounds__(256) void k_forest_roots(const int64_t *soff, int64
     if (s < nsys) roots[s] = (int32_t)(n + pi[off[soff[s]]]);
 }
 
+static ptg_status pack_in_slot_order(const ptg_tree_arrays &t, int64_t n, int64_t nnodes, int64_t nsys,
+                                     const ptg_forest *forest, const ptg_walk_tree *pack, const int32_t *elem,
+                                     hipStream_t s);
+
 extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const double *h, const ptg_build_info *info,
                                      const ptg_forest *forest, int quadrupole, int compute_moments, void *ws,
                                      size_t ws_bytes, void *node_ws, size_t node_ws_bytes,
-                                     const ptg_tree_arrays *tree, void *particles_ready, void *stream)
+                                     const ptg_tree_arrays *tree, const ptg_walk_tree *pack, void *particles_ready,
+                                     void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!pos || !m || !h || !info || !tree || info->n < 1) {
@@ -1625,6 +1630,11 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     }
     if (t.node_stats && !(quadrupole && compute_moments)) {
         ptg_set_error("ptg_build_tree: node_stats needs a build with quadrupoles and moments");
+        return PTG_ERR_ARG;
+    }
+    if (pack && (!pack->xyzm || !pack->aux || !pack->kids || pack->nslots != info->n + info->nnodes ||
+                 (pack->prec != PTG_F32 && pack->prec != PTG_F64) || (pack->quad && !quadrupole))) {
+        ptg_set_error("ptg_build_tree: pack must describe a walk copy of this tree (see ptg_pack_walk)");
         return PTG_ERR_ARG;
     }
     int64_t n = info->n, nnodes = info->nnodes;
@@ -1741,6 +1751,10 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
         k_fold_particles<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t);
     }
     PTG_LAUNCH_CHECK();
+    if (pack) {  // the walk copy, in slot order through the build's slot map
+        st = pack_in_slot_order(t, n, nnodes, nsys, forest, pack, nw.elem, s);
+        if (st) return st;
+    }
     return PTG_OK;
 }
 
@@ -1831,6 +1845,40 @@ __global__ __launch_bounds__(256) void k_pack(int64_t n, int64_t nnodes, const p
         for (int k = 0; k < 3; k++) origin[k] = rel ? f.cubes[4 * sys + k] : 0.0;
     }
     pack_slot<T>(e, sl, n, t, wt, origin);
+}
+
+// The same copy written in slot order (coalesced stores), for the build: elem maps each slot to its
+// element (the build's slot map), so the reads are the gathered side.
+template <typename T>
+__global__ __launch_bounds__(256) void k_pack_slots(int64_t n, int64_t ne, const ptg_tree_arrays t,
+                                                    const ptg_walk_tree wt, PackForest f, const int32_t *elem)
+{
+    int64_t sl = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sl >= ne) return;
+    const int64_t e = elem[sl];
+    if (e < 0 || e >= ne) return;
+    double origin[3] = {wt.origin[0], wt.origin[1], wt.origin[2]};
+    if (f.nsys > 0) {
+        const int64_t sys = f.nsys == 1 ? 0 : e < n ? upper_index(f.soff, f.nsys, e) : upper_index(f.roots, f.nsys, e);
+        const bool rel = sizeof(T) == 4;
+        for (int k = 0; k < 3; k++) origin[k] = rel ? f.cubes[4 * sys + k] : 0.0;
+    }
+    pack_slot<T>(e, sl, n, t, wt, origin);
+}
+
+static ptg_status pack_in_slot_order(const ptg_tree_arrays &t, int64_t n, int64_t nnodes, int64_t nsys,
+                                     const ptg_forest *forest, const ptg_walk_tree *pack, const int32_t *elem,
+                                     hipStream_t s)
+{
+    PackForest f{forest ? nsys : 0, forest ? forest->offsets : nullptr, forest ? forest->roots : nullptr,
+                 forest ? forest->cubes : nullptr};
+    const int64_t ne = n + nnodes;
+    if (pack->prec == PTG_F32)
+        k_pack_slots<float><<<grid1(ne), 256, 0, s>>>(n, ne, t, *pack, f, elem);
+    else
+        k_pack_slots<double><<<grid1(ne), 256, 0, s>>>(n, ne, t, *pack, f, elem);
+    PTG_LAUNCH_CHECK();
+    return PTG_OK;
 }
 
 extern "C" ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_forest *forest,

@@ -76,7 +76,7 @@ class DeviceTree:
     system gets its own cube, keys and tree, exactly as if it were built alone; particles stay
     system-major in Morton order and the root of system s sits at walk slot s."""
 
-    def __init__(self, pos, m, h, quadrupole=False, compute_moments=True, offsets=None):
+    def __init__(self, pos, m, h, quadrupole=False, compute_moments=True, offsets=None, prepack=True):
         L = _lib.load()
         dev = device()
         pos = to_dev(pos, (-1, 3))
@@ -140,13 +140,19 @@ class DeviceTree:
         # on a side stream while the nodes and moments are still being built
         self.particles_ready = torch.cuda.Event()
         self.particles_ready.record()  # creates the event; the build records it again
+        self._packs = {}
+        # prepack: the build also writes the FP32 monopole walk copy, in slot order (cheaper than a
+        # separate ptg_pack_walk, whose stores are scattered)
+        pk = self._new_pack(PTG_F32, False) if prepack else None
         check(L.ptg_build_tree(ptr(pos), ptr(m), ptr(h), _lib.ctypes.byref(info), fref, int(quadrupole),
                                int(compute_moments), ptr(ws), ws.numel(), ptr(nws), nws.numel(),
-                               _lib.ctypes.byref(self.arrays), _lib._vp(self.particles_ready.cuda_event), s),
+                               _lib.ctypes.byref(self.arrays), _lib.ctypes.byref(pk["wt"]) if pk else None,
+                               _lib._vp(self.particles_ready.cuda_event), s),
               "ptg_build_tree")
+        if pk:
+            self._packs[(PTG_F32, False)] = pk
         # ws / nws go back to torch's stream-ordered caching allocator here; later allocations on
         # this stream are ordered after the build kernels that use them
-        self._packs = {}
 
     @property
     def forest_ref(self):
@@ -184,15 +190,8 @@ class DeviceTree:
             self._packs[key] = p
         return self._packs[key]
 
-    def pack(self, prec, quadrupole, cache=True, norms=None):
-        """Sibling-contiguous walk copy (cached per precision and moment order).  aux[3] holds each
-        node's quadrupole norm, which only the error-bounded criterion reads: norms=True demands it,
-        norms=False skips it (aux[3] = -1, and the quadrupoles are not read) for a pack whose aux[3]
-        will carry learned opening radii, None takes either."""
-        key = (prec, bool(quadrupole))
-        if cache and key in self._packs and (norms is not True or self._packs[key]["norms"]):
-            return self._packs[key]
-        norms = norms is not False or bool(quadrupole)
+    def _new_pack(self, prec, quadrupole):
+        """Device buffers and descriptor of a walk copy (filled by ptg_pack_walk or by the build)."""
         dev = device()
         dt = torch.float32 if prec == PTG_F32 else torch.float64
         ne = self.ne
@@ -203,13 +202,26 @@ class DeviceTree:
         origin = self.center if prec == PTG_F32 else np.zeros(3)
         wt = WalkTree(ptr(xyzm).value, ptr(aux).value, ptr(kids).value, ptr(quad).value if quad is not None else None,
                       ne, prec, int(bool(quadrupole)), (_lib._dbl * 3)(*origin))
+        return {"wt": wt, "tensors": (xyzm, aux, kids, quad), "norms": True}
+
+    def pack(self, prec, quadrupole, cache=True, norms=None):
+        """Sibling-contiguous walk copy (cached per precision and moment order).  aux[3] holds each
+        node's quadrupole norm, which only the error-bounded criterion reads: norms=True demands it,
+        norms=False skips it (aux[3] = -1, and the quadrupoles are not read) for a pack whose aux[3]
+        will carry learned opening radii, None takes either."""
+        key = (prec, bool(quadrupole))
+        if cache and key in self._packs and (norms is not True or self._packs[key]["norms"]):
+            return self._packs[key]
+        norms = norms is not False or bool(quadrupole)
+        p = self._new_pack(prec, quadrupole)
+        wt = p["wt"]
         arrays = self.arrays
         if not norms and arrays.quadrupoles:
             arrays = TreeArrays.from_buffer_copy(arrays)
             arrays.quadrupoles = None
         check(_lib.load().ptg_pack_walk(_lib.ctypes.byref(arrays), self.n, self.nnodes, self.forest_ref,
                                         _lib.ctypes.byref(wt), stream()), "ptg_pack_walk")
-        p = {"wt": wt, "tensors": (xyzm, aux, kids, quad), "norms": norms or not self.arrays.quadrupoles}
+        p["norms"] = norms or not self.arrays.quadrupoles
         if cache:
             self._packs[key] = p
         return p

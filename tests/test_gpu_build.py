@@ -84,3 +84,20 @@ def test_large_build_invariants(n):
             break
         cur[isn] = First[cur[isn]]
     assert np.array_equal(cur, nodes + 1)
+
+
+@pytest.mark.parametrize("quadrupole", [False, True])
+def test_build_fills_the_walk_copy_like_ptg_pack_walk(quadrupole):
+    """The FP32 walk copy the build writes in slot order equals ptg_pack_walk's, bit for bit, for one
+    tree and for a forest."""
+    from pytreegrav_amd import engine
+    from pytreegrav_amd._lib import PTG_F32
+
+    x, m, h = case("plummer_3e4")
+    off = np.array([0, 7000, 19000, len(m)], dtype=np.int64)
+    for offsets in (None, off):
+        t = engine.DeviceTree(x, m, h, quadrupole=quadrupole, offsets=offsets)
+        built = [a.cpu().numpy() for a in t._packs[(PTG_F32, False)]["tensors"][:3]]
+        fresh = [a.cpu().numpy() for a in t.pack(PTG_F32, False, cache=False, norms=True)["tensors"][:3]]
+        for b, f in zip(built, fresh):
+            assert np.array_equal(b, f)
