@@ -43,8 +43,8 @@ __device__ __forceinline__ void cd_count(unsigned long long *ctr, int k, int64_t
 
 // column along ray i from target j; visits counts the elements the walk touched
 __device__ __forceinline__ double ray_column(const ptg_tree_arrays &t, int64_t n, int64_t ne, const double *tp,
-                                             int64_t nt, const double *rays, int64_t nr, int per_target,
-                                             int64_t i, int64_t j, int32_t *err, int64_t &visits)
+                                             const double *rays, int64_t nr, int per_target, int64_t i, int64_t j,
+                                             int32_t *err, int64_t &visits)
 {
     const double *ray = per_target ? rays + 3 * (j * nr + i) : rays + 3 * i;
     const double rx = ray[0], ry = ray[1], rz = ray[2];
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(CD_THREADS) void k_column_rays(const ptg_tree_array
     int64_t visits = 0;
     if (id < nt * nr) {
         const int64_t i = id / nt, j = id - (id / nt) * nt;  // ray i, target j
-        out[j * nr + i] = ray_column(t, n, ne, tp, nt, rays, nr, per_target, i, j, err, visits);
+        out[j * nr + i] = ray_column(t, n, ne, tp, rays, nr, per_target, i, j, err, visits);
     }
     if (ctr) cd_count(ctr, 0, visits);
 }

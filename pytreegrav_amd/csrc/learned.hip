@@ -284,7 +284,8 @@ __global__ __launch_bounds__(EC_THREADS) void k_errcurve(const ptg_tree_arrays t
     __shared__ double emax[EC_DIRS * EC_MAXR];
     __shared__ int64_t range[2];
     __shared__ int ok;
-    const int64_t row = blockIdx.x;
+    const int64_t row = blockIdx.x;  // one block per row: the grid is sized to count
+    if (row >= count) return;
     const int64_t e = nodes[row];
     if (threadIdx.x == 0) {
         ok = 0;

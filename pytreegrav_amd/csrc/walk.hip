@@ -1149,7 +1149,7 @@ static int num_cus()
 {
     if (g_num_cus == 0) {
         int dev = 0;
-        hipGetDevice(&dev);
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
         hipDeviceProp_t p;
         if (hipGetDeviceProperties(&p, dev) == hipSuccess) g_num_cus = p.multiProcessorCount;
         if (g_num_cus <= 0) g_num_cus = 256;
