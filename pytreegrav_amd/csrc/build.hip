@@ -784,11 +784,18 @@ struct LoadNodeCount {
 
 __global__ __launch_bounds__(256) void k_max_level(const int32_t *lev, int64_t m, Counters *c)
 {
+    __shared__ int wmax[4];
     int v = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
         v = max(v, lev[i]);
     v = ptg_wave_max(v);
-    if (ptg_lane() == 0 && v > 0) atomicMax(&c->max_level, v);
+    if (ptg_lane() == 0) wmax[threadIdx.x >> 6] = v;
+    __syncthreads();
+    // one atomic per block
+    if (threadIdx.x == 0) {
+        v = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+        if (v > 0) atomicMax(&c->max_level, v);
+    }
 }
 
 __global__ void k_fill_i32(int32_t *a, int64_t n, int32_t v)
@@ -1016,7 +1023,8 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
         }
         st = excl_scan(LoadNodeCount{w.lev}, n - 1, w.off, w.bsum, &w.ctr->nnodes, s);
         if (st) return st;
-        k_max_level<<<256, 256, 0, s>>>(w.lev, n - 1, w.ctr);
+        k_max_level<<<(unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, ptg_cdiv(n - 1, 256 * 8))), 256, 0, s>>>(
+            w.lev, n - 1, w.ctr);
         PTG_LAUNCH_CHECK();
     }
     Geom hg;
