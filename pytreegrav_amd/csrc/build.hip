@@ -139,7 +139,7 @@ struct LoadArr {
 // ============================================================================================
 #define RS_BITS 8
 #define RS_BINS 256
-#define RS_ITEMS 16
+#define RS_ITEMS 8
 #define RS_WAVES 4
 #define RS_THREADS (64 * RS_WAVES)
 #define RS_TILE (RS_THREADS * RS_ITEMS)
