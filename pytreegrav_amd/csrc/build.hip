@@ -1791,10 +1791,14 @@ __device__ void pack_slot(int64_t e, int64_t slot, int64_t n, const ptg_tree_arr
     // the error-bounded criterion); -1 when the tree carries no quadrupoles
     double qn = -1.0;
     if (e >= n && t.quadrupoles) {
-        const double *Q = t.quadrupoles + 9 * e;
-        double s2 = 0.0;
-        for (int k = 0; k < 9; k++) s2 += Q[k] * Q[k];
-        qn = sqrt(s2);
+        if (t.node_stats) {
+            qn = t.node_stats[5 * (e - n)];  // the same sum, stored by the moment pass
+        } else {
+            const double *Q = t.quadrupoles + 9 * e;
+            double s2 = 0.0;
+            for (int k = 0; k < 9; k++) s2 += Q[k] * Q[k];
+            qn = sqrt(s2);
+        }
     } else if (e < n) {
         qn = 0.0;
     }

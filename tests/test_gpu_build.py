@@ -101,3 +101,9 @@ def test_build_fills_the_walk_copy_like_ptg_pack_walk(quadrupole):
         fresh = [a.cpu().numpy() for a in t.pack(PTG_F32, False, cache=False, norms=True)["tensors"][:3]]
         for b, f in zip(built, fresh):
             assert np.array_equal(b, f)
+        if quadrupole:  # aux[3] of a node slot: its quadrupole's Frobenius norm
+            slot = t.slot.cpu().numpy()[t.n:]
+            q = t.quadrupoles.cpu().numpy()[t.n:]
+            want = np.sqrt((q * q).sum(1))
+            got = built[1][slot, 3].astype(np.float64)
+            assert np.allclose(got, want, rtol=1e-6, atol=1e-30)
