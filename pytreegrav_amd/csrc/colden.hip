@@ -337,17 +337,18 @@ __device__ __forceinline__ void cb_add(double (&acc)[6], int bin, double v)
 }
 
 // list entries [0, cnt) on this lane's target, slice sl of `slices`
+// far terms M / r^2 go to far[], divided by the bin's solid angle once at the end of the task;
+// overlapping particles' terms go to near[]
 __device__ __forceinline__ void cb_flush(const CbLds &L, int nn, int np, int sl, int slices, bool mine, double tx,
-                                         double ty, double tz, double (&acc)[6], int64_t &pairs)
+                                         double ty, double tz, double (&far)[6], double (&near)[6], int64_t &pairs)
 {
-    const double bin_size = (4.0 * 3.14159265358979323846) / 6.0;
     if (!mine) return;
     pairs += (nn > sl ? (nn - sl + slices - 1) / slices : 0) + (np > sl ? (np - sl + slices - 1) / slices : 0);
     for (int k = sl; k < nn; k += slices) {
         const D4 c = L.nodes[k];
         const double d0 = c.x - tx, d1 = c.y - ty, d2 = c.z - tz;
         const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        cb_add(acc, cd_bin(d0, d1, d2), c.w / r2 / bin_size);
+        cb_add(far, cd_bin(d0, d1, d2), c.w / r2);
     }
     for (int k = sl; k < np; k += slices) {
         const D4 c = L.parts[k];
@@ -357,14 +358,14 @@ __device__ __forceinline__ void cb_flush(const CbLds &L, int nn, int np, int sl,
         const int bin = cd_bin(d0, d1, d2);
         const double hh = h * h;
         if (r2 > hh) {
-            cb_add(acc, bin, c.w / r2 / bin_size);
+            cb_add(far, bin, c.w / r2);
         } else if (h > 0) {
             const double col0 = c.w * (3 / (4 * 3.14159265358979323846 * hh));
             const double fac = sqrt(r2) / h;
             const double iso = (1 - fac) * col0;
 #pragma unroll
-            for (int q = 0; q < 6; q++) acc[q] += iso;
-            cb_add(acc, bin, col0 * fac * 2);
+            for (int q = 0; q < 6; q++) near[q] += iso;
+            cb_add(near, bin, col0 * fac * 2);
         }
     }
 }
@@ -401,7 +402,8 @@ __device__ int cb_task(CbLds &L, int *spill, const D4 *xyzm, const D4 *aux, cons
     const bool mine = sl < slices && tl < mt;
     const double tx = mine ? tp[3 * (a + tl)] : 0.0, ty = mine ? tp[3 * (a + tl) + 1] : 0.0;
     const double tz = mine ? tp[3 * (a + tl) + 2] : 0.0;
-    for (int q = 0; q < 6; q++) acc[q] = 0.0;
+    double far[6], near[6];
+    for (int q = 0; q < 6; q++) far[q] = near[q] = 0.0;
     gpairs = gvis = 0;
     bool budget = !HEAVY;
     int sp = 1, gsp = 0, nn = 0, np = 0;
@@ -480,7 +482,7 @@ __device__ int cb_task(CbLds &L, int *spill, const D4 *xyzm, const D4 *aux, cons
         sp += tot;
         __builtin_amdgcn_wave_barrier();
         if (nn >= 64 || np >= 64) {
-            cb_flush(L, nn, np, sl, slices, mine, tx, ty, tz, acc, gpairs);
+            cb_flush(L, nn, np, sl, slices, mine, tx, ty, tz, far, near, gpairs);
             __builtin_amdgcn_wave_barrier();
             nn = np = 0;
         }
@@ -495,8 +497,10 @@ __device__ int cb_task(CbLds &L, int *spill, const D4 *xyzm, const D4 *aux, cons
             budget = false;  // the heavy queue is full: finish the group here
         }
     }
-    cb_flush(L, nn, np, sl, slices, mine, tx, ty, tz, acc, gpairs);
+    cb_flush(L, nn, np, sl, slices, mine, tx, ty, tz, far, near, gpairs);
     __builtin_amdgcn_wave_barrier();
+    const double bin_size = (4.0 * 3.14159265358979323846) / 6.0;
+    for (int q = 0; q < 6; q++) acc[q] = far[q] / bin_size + near[q];
     // sum the slices' partial bins into lanes 0 .. gs-1
 #pragma unroll
     for (int q = 0; q < 6; q++) {
