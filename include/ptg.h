@@ -72,7 +72,8 @@ typedef struct {
     double *softenings;       /* (n+nnodes): particle h, node max h */
     double *sizes;            /* (n+nnodes): node side, 0 for particles */
     double *deltas;           /* (n+nnodes): |COM - geometric centre| */
-    double *quadrupoles;      /* (n+nnodes, 9) traceless quadrupoles, or NULL for a monopole tree */
+    double *quadrupoles;      /* (n+nnodes, 9) traceless quadrupoles of the nodes, or NULL for a monopole
+                               * tree; particle rows are not written (a particle's quadrupole is zero) */
     int32_t *first_subnode;   /* (n+nnodes), -1 for particles */
     int32_t *next_branch;     /* (n+nnodes), -1 terminates the walk */
     int64_t *treewalk_indices;/* (n): input index of each Morton slot */

@@ -1237,8 +1237,7 @@ __global__ __launch_bounds__(256) void k_particles(const PartRec *rec, const uin
     t.first_subnode[i] = -1;
     t.nchild[i] = 0;
     t.child_base[i] = -1;
-    if (t.quadrupoles)
-        for (int k = 0; k < 9; k++) t.quadrupoles[9 * i + k] = 0.0;
+    // a particle's quadrupole row is not written: it is zero by definition and no kernel reads it
 }
 
 __global__ void k_single_root(const Geom *g, const ptg_tree_arrays t)

@@ -175,7 +175,12 @@ class Octree:
     def Quadrupoles(self):
         if not self.HasQuads:
             return np.zeros((0, 3, 3))
-        return self._get("Quadrupoles", lambda: self.dev.quadrupoles.cpu().numpy().reshape(-1, 3, 3))
+        def host():
+            q = self.dev.quadrupoles.cpu().numpy().reshape(-1, 3, 3)
+            q[: self.dev.n] = 0.0  # particle rows are not written on the device; a particle has none
+            return q
+
+        return self._get("Quadrupoles", host)
 
     @property
     def FirstSubnode(self):
