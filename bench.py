@@ -49,7 +49,7 @@ FLOPS_PER_EXPANSION_EVAL = 100  # potential + acceleration of the expansion at o
 FP64_PEAK_TFLOPS = 78.6
 FLOPS_PER_COLUMN_PAIR = 10  # binned column: 3 differences, 5 for r^2, 2 divisions (M / r^2 / bin area)
 FLOPS_PER_RAY_VISIT = 20  # ray walk: 3 differences, 5 for r^2, sqrt, 5 for the projection, opening test
-# tree Potential + Accel at N=1e7 Plummer, theta=0.7, 32 threads, two calls:
+# Published reference figure: tree Potential + Accel at N=1e7 Plummer, theta=0.7, 32 threads, two calls:
 # 1.005e-6 + 1.124e-6 s/particle (examples/benchmark_scaling.json:51,131)
 REF_PUBLISHED = 1.0 / (1.005e-6 + 1.124e-6)
 # Calibrated HBM bytes per walk launch from two rocprofv3 --pmc passes (FETCH_SIZE x2.000,
@@ -361,7 +361,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding of the walk is a "
-              f"design (targets split across GPUs with a replicated tree), not something this build launches.",
+              f"design (targets split across GPUs with a replicated tree), not something this program launches.",
               file=sys.stderr)
         sys.exit(2)
 

@@ -4,7 +4,7 @@
  * The reference (pytreegrav) has no C ABI: its operator interface is the Python API in
  * src/pytreegrav/frontend.py and the Octree jitclass in src/pytreegrav/octree.py.  Each entry point
  * below replaces the compute behind one of those Python functions; the Python mirror in
- * pytreegrav_amd/frontend.py binds them with ctypes (see INTEGRATION.md).
+ * pytreegrav_amd/frontend.py binds them with ctypes.
  *
  * Conventions
  *   - Every function returns a ptg_status: 0 on success, a negative code on failure; the message

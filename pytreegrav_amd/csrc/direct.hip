@@ -1,6 +1,6 @@
 // LDS-tiled direct summation (bruteforce.py): one thread per target, sources staged through LDS
 // in tiles of DS_TILE and read back by every thread of the block (broadcast).  FP64 is the
-// oracle-grade path used to judge tree errors; FP32 is the fast brute-force method.
+// oracle-grade path used to measure tree errors; FP32 is the fast brute-force method.
 #include <algorithm>
 
 #include "ptg_common.hpp"

@@ -206,7 +206,7 @@ def test_frontend_expansion():
 @pytest.mark.parametrize("criterion", ["theta", "learned"])
 def test_supergroup_pass_accuracy_and_work(criterion):
     """The shared supergroup pass keeps each group's accepted elements (only where they are folded
-    changes): the FP64-judged error stays that of the one-level expansion, the per-target work
+    changes): the error against FP64 stays that of the one-level expansion, the per-target work
     does not grow and the total traversal shrinks."""
     N = 200_000
     x, m = plummer_truncated(N, seed=6)

@@ -2,7 +2,7 @@
 gradient is pinned by finite differences in test_mlp_cpu.py), features against a host
 recomputation, labels against an independent float64 direct sum and against the quadrupole far
 field, determinism of training, the geometric guard, and the gain of a trained network over the
-geometric criterion at equal or lower error (FP64 direct summation as the judge)."""
+geometric criterion at equal or lower error (errors against FP64 direct summation)."""
 
 import numpy as np
 import pytest
