@@ -70,8 +70,8 @@ typedef struct {
     double *coordinates;      /* (n+nnodes, 3): particle positions, node COM (geometric centre if mass 0) */
     double *masses;           /* (n+nnodes) */
     double *softenings;       /* (n+nnodes): particle h, node max h */
-    double *sizes;            /* (n+nnodes): node side, 0 for particles */
-    double *deltas;           /* (n+nnodes): |COM - geometric centre| */
+    double *sizes;            /* (n+nnodes): node side; particle rows are not written (zero) */
+    double *deltas;           /* (n+nnodes): |COM - geometric centre|; particle rows not written */
     double *quadrupoles;      /* (n+nnodes, 9) traceless quadrupoles of the nodes, or NULL for a monopole
                                * tree; particle rows are not written (a particle's quadrupole is zero) */
     int32_t *first_subnode;   /* (n+nnodes), -1 for particles */

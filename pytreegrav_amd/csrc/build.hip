@@ -1230,14 +1230,13 @@ __global__ __launch_bounds__(256) void k_particles(const PartRec *rec, const uin
     t.coordinates[3 * i + 2] = r1.x;
     t.masses[i] = r1.y;
     t.softenings[i] = r2.x;
-    t.sizes[i] = 0.0;
-    t.deltas[i] = 0.0;
     t.treewalk_indices[i] = o;
     if (t.keys) t.keys[i] = keys[i];
     t.first_subnode[i] = -1;
     t.nchild[i] = 0;
     t.child_base[i] = -1;
-    // a particle's quadrupole row is not written: it is zero by definition and no kernel reads it
+    // a particle's size, delta and quadrupole rows are not written: all are zero by definition, the
+    // pack writes zeros for particle slots and no other kernel reads them
 }
 
 __global__ void k_single_root(const Geom *g, const ptg_tree_arrays t)
@@ -1784,8 +1783,8 @@ __device__ void pack_slot(int64_t e, int64_t slot, int64_t n, const ptg_tree_arr
     X.z = (T)(t.coordinates[3 * e + 2] - origin[2]);
     X.w = (T)t.masses[e];
     X2.x = (T)t.softenings[e];
-    X2.y = (T)t.sizes[e];
-    X2.z = (T)t.deltas[e];
+    X2.y = e >= n ? (T)t.sizes[e] : (T)0;  // particle rows of sizes and deltas are not written
+    X2.z = e >= n ? (T)t.deltas[e] : (T)0;
     // Frobenius norm of the node's traceless quadrupole (the monopole truncation error scale used by
     // the error-bounded criterion); -1 when the tree carries no quadrupoles
     double qn = -1.0;

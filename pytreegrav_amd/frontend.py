@@ -163,13 +163,18 @@ class Octree:
     def Softenings(self):
         return self._get("Softenings", lambda: self.dev.softenings.cpu().numpy())
 
+    def _node_rows(self, t):
+        a = t.cpu().numpy()
+        a[: self.dev.n] = 0.0  # particle rows are not written on the device; a particle has zero extent
+        return a
+
     @property
     def Sizes(self):
-        return self._get("Sizes", lambda: self.dev.sizes.cpu().numpy())
+        return self._get("Sizes", lambda: self._node_rows(self.dev.sizes))
 
     @property
     def Deltas(self):
-        return self._get("Deltas", lambda: self.dev.deltas.cpu().numpy())
+        return self._get("Deltas", lambda: self._node_rows(self.dev.deltas))
 
     @property
     def Quadrupoles(self):

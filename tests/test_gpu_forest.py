@@ -80,7 +80,7 @@ def test_forest_tree_equals_single_builds(quadrupole):
             if name in ("first_subnode", "next_branch"):
                 single = single.astype(np.int64)
             got = _sub(f, off, s, name)
-            if name == "quadrupoles":  # node rows only: particle rows are not written
+            if name in ("quadrupoles", "sizes", "deltas"):  # node rows only: particle rows are not written
                 got, single = got[off[s + 1] - off[s]:], single[u.n:]
             assert np.array_equal(got, single), (s, name)
     assert f.nnodes == nn_total
