@@ -107,3 +107,40 @@ def test_build_fills_the_walk_copy_like_ptg_pack_walk(quadrupole):
             want = np.sqrt((q * q).sum(1))
             got = built[1][slot, 3].astype(np.float64)
             assert np.allclose(got, want, rtol=1e-6, atol=1e-30)
+
+
+@pytest.mark.parametrize("quadrupole", [False, True])
+def test_particle_rows_left_unwritten_are_never_read(quadrupole):
+    """The build does not write the particle rows of sizes, deltas and quadrupoles (all zero by
+    definition).  With those rows poisoned with NaN, the FP32 and FP64 walk copies, the binned column
+    walk (which reads the tree arrays directly) and the host views are unchanged."""
+    import torch
+
+    from pytreegrav_amd import ConstructTree, engine
+    from pytreegrav_amd._lib import PTG_F32, PTG_F64
+
+    x, m, h = case("plummer_3e4")
+    g = ConstructTree(x, m, h, quadrupole=quadrupole)
+    t, n = g.dev, g.dev.n
+    order = g.TreewalkIndices
+    targets = x[order][:3000]
+    before = [[a.cpu().numpy() for a in t.pack(p, q, cache=False, norms=True)["tensors"][:4] if a is not None]
+              for p, q in ((PTG_F32, False), (PTG_F64, False), (PTG_F32, quadrupole))]
+    col = engine.column_binned(t, targets, 16, 0.5).cpu().numpy()
+    t.sizes[:n] = float("nan")
+    t.deltas[:n] = float("nan")
+    if quadrupole:
+        t.quadrupoles.view(-1, 9)[:n] = float("nan")
+    torch.cuda.synchronize()
+    after = [[a.cpu().numpy() for a in t.pack(p, q, cache=False, norms=True)["tensors"][:4] if a is not None]
+             for p, q in ((PTG_F32, False), (PTG_F64, False), (PTG_F32, quadrupole))]
+    for b, a in zip(before, after):
+        assert len(b) == len(a)
+        for u, v in zip(b, a):
+            assert np.array_equal(u, v)
+    assert np.array_equal(engine.column_binned(t, targets, 16, 0.5).cpu().numpy(), col)
+    # host views: particle rows zero, node rows as on the device
+    for view, dev in ((g.Sizes, t.sizes), (g.Deltas, t.deltas)):
+        assert np.all(view[:n] == 0.0) and np.array_equal(view[n:], dev.cpu().numpy()[n:])
+    if quadrupole:
+        assert np.all(g.Quadrupoles[:n] == 0.0)
