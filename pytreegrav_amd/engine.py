@@ -439,18 +439,24 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
         check(L.ptg_target_groups(ptr(pos), ptr(perm), ptr(tree.keys), n, tree.forest_ref, int(group_size),
                                   _lib.PTG_GROUPS_CELLS, ptr(offsets), ptr(count), ptr(gws), gws.numel(), stream()),
               "ptg_target_groups")
+        # the field-scale sample targets read only the particle slots, final once the particles are
+        # placed, so they are gathered here, off the main stream's path
+        sample = field_scale_sample(tree, cells=(offsets, count)) if criterion != "theta" or expansion is not None \
+            else None
     offsets.record_stream(main)
     count.record_stream(main)
+    for s_t in sample[:3] if sample is not None else ():
+        s_t.record_stream(main)
     eps, scale, exp_eps = None, None, None
     tree.pack(PTG_F32, quadrupole, norms=criterion != "learned")
     if criterion != "theta" or expansion is not None:
-        main.wait_stream(side)  # the sample groups are taken from the cell groups
+        main.wait_stream(side)  # the sample targets come from the side stream
         if criterion == "error":
-            rms = field_scale(tree, cells=(offsets, count))
+            rms = field_scale(tree, sample=sample)
             eps = alpha * rms
             exp_eps = None if expansion is None else expansion * rms
         else:  # the tolerances stay on the device: no host round trip
-            scale = field_scale(tree, cells=(offsets, count), on_device=True)
+            scale = field_scale(tree, sample=sample, on_device=True)
             exp_eps = expansion
             if criterion == "learned":
                 eps = alpha
@@ -564,15 +570,27 @@ def direct(pos_t, soft_t, pos_s, m_s, h_s, mask, G=1.0, self_mode=False, precisi
     return out
 
 
-def field_scale(tree, groups=64, group_size=64, theta=1.0, cells=None, on_device=False):
+def field_scale(tree, groups=64, group_size=64, theta=1.0, cells=None, on_device=False, sample=None):
     """RMS |acceleration| (G=1) of the tree's own particles, estimated by a theta=1 FP32 walk of
     `groups` sample groups of `group_size` Morton-consecutive particles spread evenly over the
     Morton order; turns a relative accuracy target into the absolute tolerance of the error-bounded
     and learned criteria.  cells: the (offsets, count) cell-aligned groups of the tree's own
     particles, if already formed: each sample is then one of those groups (its last target repeated
     to fill group_size), so no sample straddles two distant cells, whose bounding box would make
-    its walk open most of the tree.  on_device: return a device float64 scalar instead of a float
+    its walk open most of the tree.  sample: what field_scale_sample returned, gathered beforehand
+    (e.g. on another stream); groups, group_size and cells are then not used.  on_device: return a device float64 scalar instead of a float
     (no host synchronisation)."""
+    pos, h, perm, gs = field_scale_sample(tree, groups, group_size, cells) if sample is None else sample
+    a = walk(tree, pos, h, perm, PTG_ACC, group_size=gs,
+             theta=theta, precision="float32", grouping="fixed", _check=not on_device, _ws_slot=1)
+    rms = torch.empty(1, dtype=torch.float64, device=device())
+    check(_lib.load().ptg_rms_rows(ptr(a), a.shape[0], 3, 0, 3, 1, ptr(rms), stream()), "ptg_rms_rows")
+    return rms if on_device else float(rms.item())
+
+
+def field_scale_sample(tree, groups=64, group_size=64, cells=None):
+    """The field-scale sample targets (see field_scale): (positions, softenings, output rows, group
+    size), read from the particle slots [0, n) of the tree only, on the current stream."""
     n = tree.n
     k = min(groups, max(1, n // group_size))
     gs = min(group_size, n)
@@ -587,11 +605,7 @@ def field_scale(tree, groups=64, group_size=64, theta=1.0, cells=None, on_device
         idx = (starts[:, None] + j).reshape(-1)
     pos = tree.coordinates.index_select(0, idx)  # particle slots [0, n) are in Morton order
     h = tree.softenings.index_select(0, idx)
-    a = walk(tree, pos, h, torch.arange(idx.shape[0], device=device()), PTG_ACC, group_size=gs,
-             theta=theta, precision="float32", grouping="fixed", _check=not on_device, _ws_slot=1)
-    rms = torch.empty(1, dtype=torch.float64, device=device())
-    check(_lib.load().ptg_rms_rows(ptr(a), a.shape[0], 3, 0, 3, 1, ptr(rms), stream()), "ptg_rms_rows")
-    return rms if on_device else float(rms.item())
+    return pos, h, torch.arange(idx.shape[0], device=device()), gs
 
 
 # ---------------------------------------------------------------------------------------------
