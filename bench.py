@@ -52,12 +52,10 @@ FLOPS_PER_RAY_VISIT = 20  # ray walk: 3 differences, 5 for r^2, sqrt, 5 for the 
 # Published reference figure: tree Potential + Accel at N=1e7 Plummer, theta=0.7, 32 threads, two calls:
 # 1.005e-6 + 1.124e-6 s/particle (examples/benchmark_scaling.json:51,131)
 REF_PUBLISHED = 1.0 / (1.005e-6 + 1.124e-6)
-# Calibrated HBM bytes per walk launch from two rocprofv3 --pmc passes (FETCH_SIZE x2.000,
-# WRITE_SIZE x1.000, factors from a 2048 MiB device copy); see profiles/k_walk_*_hbm_traffic.md.
-# Keyed by (N, group_size, criterion, theta, quadrupole, alpha, expansion).
-WALK_TRAFFIC = {(10_000_000, 64, "theta", 0.7, False, None, None): 8.329e9,
-                (10_000_000, 64, "learned", 2.0, False, 1.6e-3, None): 2.720e9,
-                (10_000_000, 64, "learned", 2.0, False, 5e-4, 4e-3): 3.576e9}
+# HBM traffic per walk launch is measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes, which
+# cannot run inside this process: the bench line leaves roofline.traffic null and names the profile
+# summary that holds the calibrated figure for the default configuration.
+WALK_TRAFFIC_PROFILE = "profiles/k_walk_expansion_plummer1e7_hbm_traffic.md"
 
 
 def parse():
@@ -100,53 +98,144 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(x, m, h, theta, frac, seed):
-    """Oracle (C restatement of the reference) on the host cores: full build, grouped FP64 walk at the
-    reference default group_size=8 over evenly spaced slices of the groups (aligned to whole groups,
-    so each sampled group is exactly the group the full run would form), scaled to all N."""
+def cpu_threads():
+    """Host threads for the CPU legs: the cores in this process's affinity mask, capped at the
+    OMP_NUM_THREADS the environment sets (the GPU box gives one GPU's job a 16-core share and sets
+    OMP_NUM_THREADS=16 however many cores the host shows)."""
+    aff = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    cap = int(env) if env.isdigit() and int(env) > 0 else aff
+    return min(aff, cap), aff
+
+
+def cpu_baseline(x, m, h, theta, frac, seed, reps=3):
+    """Oracle (C restatement of the reference) on the host cores, timed as one warm-up and then the
+    median of `reps` runs: a full serial radix build of all N particles, then the grouped FP64 walk
+    at the reference default group_size=8 (OpenMP over groups) on evenly spaced slices of the
+    groups (aligned to whole groups, so each sampled group is exactly the group the full run would
+    form), scaled to all N."""
     import oracle
 
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores, aff = cpu_threads()
     n = len(x)
-    t0 = time.perf_counter()
-    tree = oracle.build_tree(x, m, h)
-    t_build = time.perf_counter() - t0
-    idx = tree.TreewalkIndices
-    xs, hs = np.ascontiguousarray(x[idx]), np.ascontiguousarray(h[idx])
     nslices = 16
     gs = 8
     per = max(gs, int(n * frac / nslices) // gs * gs)
     starts = [(k * (n // nslices)) // gs * gs for k in range(nslices)]
-    t_walk = 0.0
-    done = 0
-    for s in starts:
-        e = min(s + per, n)
+
+    def once():
         t0 = time.perf_counter()
-        oracle.fields_grouped(tree, xs[s:e], hs[s:e], potential=True, accel=True, group_size=gs, theta=theta,
-                              nthreads=cores)
-        t_walk += time.perf_counter() - t0
-        done += e - s
-    t_total = t_build + t_walk * n / done
+        tree = oracle.build_tree(x, m, h)
+        t_build = time.perf_counter() - t0
+        idx = tree.TreewalkIndices
+        xs, hs = np.ascontiguousarray(x[idx]), np.ascontiguousarray(h[idx])
+        t_walk, done = 0.0, 0
+        for s in starts:
+            e = min(s + per, n)
+            t0 = time.perf_counter()
+            oracle.fields_grouped(tree, xs[s:e], hs[s:e], potential=True, accel=True, group_size=gs, theta=theta,
+                                  nthreads=cores)
+            t_walk += time.perf_counter() - t0
+            done += e - s
+        return t_build, t_walk, done
+
+    once()  # warm-up (page faults, OpenMP pool)
+    runs = [once() for _ in range(reps)]
+    done = runs[0][2]
+    totals = [b + w * n / done for b, w, _ in runs]
+    k = int(np.argsort(totals)[len(totals) // 2])
+    t_build, t_walk, _ = runs[k]
+    t_total = totals[k]
     return {
         "value": n / t_total,
         "unit": "particles/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"C oracle: full serial radix build of all {n} particles ({t_build:.2f} s) + fused FP64 "
-                  f"pot+acc grouped walk (group_size 8, theta {theta}) on {done} targets in {nslices} evenly "
-                  f"spaced group-aligned slices ({t_walk:.2f} s on {cores} threads), walk scaled to N",
+        "sample": f"C oracle, median of {reps} runs after 1 warm-up: serial radix build of all {n} particles "
+                  f"({t_build:.2f} s, one thread: the restatement's split is serial) + fused FP64 pot+acc grouped "
+                  f"walk (group_size 8, theta {theta}) on {done} targets in {nslices} evenly spaced group-aligned "
+                  f"slices ({t_walk:.2f} s on {cores} threads), walk scaled to N",
+        "build_s": t_build,
+        "walk_s_scaled_to_N": t_walk * n / done,
+        "run_totals_s": totals,
+        "affinity_cores": aff,
         "cpu_model": _cpu_model(),
     }
 
 
 def _cpu_model():
+    """The host CPU as /proc/cpuinfo names it (model name, family/model/stepping, cache) and the
+    logical CPU count."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (t.strip() for t in line.split(":", 1))
+                if k in ("model name", "cpu family", "model", "stepping", "cache size", "cpu MHz") and k not in info:
+                    info[k] = v
+            elif info:
+                break
     except OSError:
         pass
-    return "unknown"
+    if not info:
+        return "unknown"
+    return (f"{info.get('model name', '?')} (family {info.get('cpu family', '?')}, model {info.get('model', '?')}, "
+            f"stepping {info.get('stepping', '?')}, cache {info.get('cache size', '?')}, "
+            f"{info.get('cpu MHz', '?')} MHz; {os.cpu_count()} logical CPUs)")
+
+
+def build_phases(build, n, quads, steps):
+    """Per-phase time of the tree build from HIP events the library records at its phase boundaries
+    (ptg_build_phase_events), averaged over `steps` builds, with each phase's algorithmic HBM bytes
+    (per particle / per node, as below) and the rate they imply against the 8 TB/s peak.
+    Algorithmic bytes: bounding cube 24 B per particle (FP64 positions read); keys 32 (24 read, 8
+    written); radix sort 24 per pass (8-byte key and 4-byte index, read and written) x 8 passes of
+    8-bit digits; octree build (shared levels, node count, node geometry and links, walk slots) 16 per
+    particle + 32 per node; Morton-order gather 84 (4 + 40 read, 40 written); moments 40 per
+    particle + 112 per node (+ 96 per node with quadrupoles); FP32 walk copy 40 written per element."""
+    import ctypes
+
+    import torch
+
+    from pytreegrav_amd import _lib
+
+    L = _lib.load()
+    names = _lib.PTG_PHASES
+    acc = np.zeros(len(names) - 1)
+    nnodes = 0
+    for _ in range(steps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in names]
+        for e in ev:
+            e.record()  # creates the underlying event
+        arr = (ctypes.c_void_p * len(ev))(*[e.cuda_event for e in ev])
+        _lib.check(L.ptg_build_phase_events(arr, len(ev)), "ptg_build_phase_events")
+        try:
+            tree = build()
+        finally:
+            L.ptg_build_phase_events(None, 0)
+        torch.cuda.synchronize()
+        acc += [ev[i].elapsed_time(ev[i + 1]) for i in range(len(ev) - 1)]
+        nnodes = tree.nnodes
+    ms = acc / steps
+    t = dict(zip(names[1:], ms))
+    quad_node = 96 if quads else 0
+    phases = {
+        "bbox": (t["bbox"], 24 * n),
+        "keys": (t["keys"], 32 * n),
+        "sort": (t["sort"], 24 * 8 * n),
+        "octree": (t["count_nodes"] + t["nodes"] + t["links"], 16 * n + 32 * nnodes),
+        "permute": (t["permute"], 84 * n),
+        "moments": (t["moments"], 40 * n + (112 + quad_node) * nnodes),
+        "fp32_walk_copy": (t["pack"], 40 * (n + nnodes)),
+    }
+    out = {}
+    for k, (tms, b) in phases.items():
+        gbs = b / (tms * 1e-3) / 1e9 if tms > 0 else None
+        out[k] = {"ms": tms, "algorithmic_bytes": int(b), "GB_per_s": gbs,
+                  "frac_of_hbm_peak": gbs / HBM_PEAK_GBS if gbs else None}
+    out["host_gap_ms"] = t["tree_start"]  # build_sort's return (host sync) to build_tree's first launch
+    out["nodes"] = int(nnodes)
+    return out
 
 
 def batched_flops(interactions, folds, n, expansion):
@@ -309,7 +398,7 @@ def run_column(args):
     rays_achieved = ray_visits * FLOPS_PER_RAY_VISIT / (rays_ms * 1e-3) / 1e12
     # CPU: the C oracle on the host cores, whole job for the binned estimator (build + all targets),
     # a 20000-target sample for the rays
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores, _ = cpu_threads()
     t0 = time.perf_counter()
     o = oracle.build_tree(x, m, h)
     t_build = time.perf_counter() - t0
@@ -413,6 +502,7 @@ def main():
         HIP events bracket the walk launches alone (k_walk_init + k_walk)."""
         build_quads = quad_walk or criterion in ("error", "learned")
         walk_times = []
+        stats = {}
 
         def step(record):
             tree = ConstructTree(pos, mass, soft, quadrupole=build_quads)
@@ -425,7 +515,8 @@ def main():
                                       group_size=args.group_size, criterion=criterion,
                                       alpha=alpha if criterion != "theta" else None, model=model,
                                       counters=ctr if record else None, walk_events=ev, expansion=expansion,
-                                      bf16=args.bf16, supergroups=args.supergroups and expansion is not None)
+                                      bf16=args.bf16, supergroups=args.supergroups and expansion is not None,
+                                      stats=stats if record else None)
 
         for _ in range(args.warmup):
             step(False)
@@ -437,7 +528,8 @@ def main():
         dt = (time.perf_counter() - t0) / args.steps
         walk_ms = float(np.mean([a.elapsed_time(b) for a, b in walk_times]))
         r = {"dt": dt, "walk_ms": walk_ms, "interactions": int(ctr[0].item()), "visits": int(ctr[1].item()),
-             "particle_interactions": int(ctr[2].item()), "folds": int(ctr[3].item())}
+             "particle_interactions": int(ctr[2].item()), "folds": int(ctr[3].item()),
+             "groups": int(stats["groups"].item())}
         # tree build alone (same inputs), for the breakdown
         torch.cuda.synchronize()
         tb0 = time.perf_counter()
@@ -445,6 +537,8 @@ def main():
             ConstructTree(pos, mass, soft, quadrupole=build_quads)
         torch.cuda.synchronize()
         r["build_ms"] = (time.perf_counter() - tb0) / args.steps * 1e3
+        r["build_phases"] = build_phases(lambda: ConstructTree(pos, mass, soft, quadrupole=build_quads).dev, N,
+                                         build_quads, args.steps)
         r["accuracy"] = accuracy(out) if exact is not None else None
         return r
 
@@ -532,12 +626,14 @@ def main():
         },
         "breakdown": {
             "tree_build_ms": build_ms,
+            "tree_build_phases": main["build_phases"],
             "pack_groups_fieldscale_radii_ms": dt * 1e3 - build_ms - walk_ms,
             "walk_ms": walk_ms,
             "walk_only_particles_per_s": N / (walk_ms * 1e-3),
             "interactions_per_particle": interactions / N,
             "near_field_particle_interactions_per_particle": pinter / N,
-            "folded_elements_per_group": (folds / (N / args.group_size)) if args.expansion is not None else None,
+            "target_groups": main["groups"],
+            "folded_elements_per_group": (folds / main["groups"]) if args.expansion is not None else None,
             "visits_per_target": visits / N,
         },
         "roofline": {
@@ -547,8 +643,8 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": WALK_TRAFFIC.get((N, args.group_size, args.criterion, theta, bool(args.quadrupole),
-                                         args.alpha if args.criterion != "theta" else None, args.expansion)),
+            "traffic": None,
+            "traffic_profile": WALK_TRAFFIC_PROFILE,
             "flops_per_launch": flops,
             "convention": "20 flops per target-element interaction (monopole pot+acc)"
                           + ("; 60 per element folded into a group expansion, 100 per target evaluation of it"

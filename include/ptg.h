@@ -13,7 +13,9 @@
  *     as torch tensors).  Scratch memory is a caller-provided workspace whose size is returned by
  *     the matching *_workspace_bytes function.
  *   - Work is enqueued asynchronously on the given HIP stream (NULL = default stream).  Host
- *     synchronisations: ptg_build_sort (finiteness flag, node count) and ptg_walk_check.
+ *     synchronisations: ptg_build_sort (finiteness and forest checks, collision count, node count),
+ *     ptg_build_tree (the per-level node counts that size the moment launches, when moments are
+ *     computed) and ptg_walk_check (reads the overflow flag).  Every other call only enqueues work.
  *   - Element numbering follows the reference Octree (octree.py:8-28): particles occupy
  *     [0, n) in Morton order, the root is element n, internal nodes follow.  Links are int32.
  */
@@ -121,6 +123,29 @@ typedef struct {
     int32_t has_quad;
     double origin[3];
 } ptg_walk_tree;
+
+/* Phase timing (no reference counterpart; diagnostics for the build's per-phase HBM rates): the
+ * PTG_PHASE_COUNT hipEvent_t handles registered here are recorded on the build stream at these
+ * points of every later ptg_build_sort / ptg_build_tree call of the calling thread, until
+ * ptg_build_phase_events(NULL, 0).  Boundaries: START (build_sort entered), BBOX (bounding cube),
+ * KEYS (Morton keys and particle records), SORT (radix sort), COUNT_NODES (shared levels, collision
+ * re-keying, node count), TREE_START (build_tree entered), PERMUTE (Morton-order particle gather),
+ * NODES (node order, geometry, FirstSubnode/NextBranch), LINKS (child counts, parents, walk slots),
+ * MOMENTS (mass, centre of mass, quadrupoles), PACK (FP32 walk copy).  (The bounding cube and key
+ * marks are recorded by single-tree builds.) */
+#define PTG_PHASE_START 0
+#define PTG_PHASE_BBOX 1
+#define PTG_PHASE_KEYS 2
+#define PTG_PHASE_SORT 3
+#define PTG_PHASE_COUNT_NODES 4
+#define PTG_PHASE_TREE_START 5
+#define PTG_PHASE_PERMUTE 6
+#define PTG_PHASE_NODES 7
+#define PTG_PHASE_LINKS 8
+#define PTG_PHASE_MOMENTS 9
+#define PTG_PHASE_PACK 10
+#define PTG_PHASE_COUNT 11
+ptg_status ptg_build_phase_events(void *const *events, int nevents);
 
 size_t ptg_build_workspace_bytes(int64_t n, int64_t nsys);
 /* A non-finite position, mass or softening fails with PTG_ERR_ARG and info->nonfinite = 1

@@ -78,6 +78,9 @@ class Criterion(_c.Structure):
 PTG_CRIT_THETA, PTG_CRIT_QERR, PTG_CRIT_RADIUS = 0, 1, 2
 PTG_MLP_NPARAM = 1441
 PTG_GROUPS_RUNS, PTG_GROUPS_CELLS = 0, 1
+# build phase boundaries recorded by ptg_build_phase_events, in order
+PTG_PHASES = ("start", "bbox", "keys", "sort", "count_nodes", "tree_start", "permute", "nodes", "links", "moments",
+              "pack")
 _F = _c.POINTER(Forest)
 
 # name -> (restype, argtypes)
@@ -85,6 +88,7 @@ SIGNATURES = {
     "ptg_version": (_c.c_int, []),
     "ptg_last_error": (_c.c_char_p, []),
     "ptg_build_workspace_bytes": (_c.c_size_t, [_i64, _i64]),
+    "ptg_build_phase_events": (_c.c_int, [_c.POINTER(_vp), _c.c_int]),
     "ptg_build_sort": (_c.c_int, [_vp, _vp, _vp, _i64, _F, _vp, _c.c_size_t, _c.POINTER(BuildInfo), _vp]),
     "ptg_node_workspace_bytes": (_c.c_size_t, [_i64, _i64, _c.c_int]),
     "ptg_groups_workspace_bytes": (_c.c_size_t, [_i64, _i64]),

@@ -28,6 +28,26 @@ void ptg_set_error(const char *fmt, ...)
 }
 
 extern "C" const char *ptg_last_error(void) { return g_err; }
+
+// ---- optional phase timing (no reference counterpart) ----
+// Events a caller registers here are recorded on the build's stream at the phase boundaries of the
+// next ptg_build_sort / ptg_build_tree calls of this thread (see PTG_PHASE_* in ptg.h).
+static thread_local void *g_phase_ev[PTG_PHASE_COUNT];
+
+extern "C" ptg_status ptg_build_phase_events(void *const *events, int nevents)
+{
+    if (nevents != 0 && (nevents != PTG_PHASE_COUNT || !events)) {
+        ptg_set_error("ptg_build_phase_events: pass %d events, or 0 to stop recording", PTG_PHASE_COUNT);
+        return PTG_ERR_ARG;
+    }
+    for (int i = 0; i < PTG_PHASE_COUNT; i++) g_phase_ev[i] = nevents ? events[i] : nullptr;
+    return PTG_OK;
+}
+
+static void phase_mark(int i, hipStream_t s)
+{
+    if (g_phase_ev[i]) (void)hipEventRecord((hipEvent_t)g_phase_ev[i], s);
+}
 extern "C" int ptg_version(void) { return 10000; }
 
 // ============================================================================================
@@ -240,19 +260,256 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *kin, 
 // number of sort tiles (sizes the histogram array: RS_BINS entries per tile)
 static int64_t rs_nwt(int64_t n) { return std::max<int64_t>(1, ptg_cdiv(n, RS_TILE)); }
 
+// ============================================================================================
+// Single-pass LSD radix sort (decoupled look-back).  One kernel counts the digits of every pass at
+// once (each block its own partial histograms), one kernel turns them into each pass's digit
+// starts, and then each pass is one launch: a block takes the next tile number from a per-pass
+// counter, ranks its tile as above, publishes its per-digit counts, and adds up the counts of the
+// tiles before it (per digit, walking back until a tile that already published its inclusive
+// prefix) to place its runs.  A status word holds a 2-bit flag (aggregate / inclusive prefix) and a
+// 30-bit count, so a relaxed agent-scope store and load carry flag and value together.  Tiles are
+// numbered in the order blocks start, so every tile a block waits for belongs to a block that is
+// already running and publishes its aggregate without waiting; a wait that passes OS_SPIN_LIMIT
+// polls (never reached by a correct run) sets the error flag instead of hanging.  Two status
+// buffers alternate between passes; each block clears its tile's row in the buffer the next pass
+// uses, which the previous pass has finished with.
+// ============================================================================================
+#define OS_FLAG_AGG 0x40000000u
+#define OS_FLAG_INCL 0x80000000u
+#define OS_MASK 0x3fffffffu
+#define OS_MAXN 0x3fffffffLL  // counts fit in 30 bits
+#define OS_HIST_BLOCKS 256
+#define OS_MAXPASS 8
+#define OS_SPIN_LIMIT (1 << 22)
+
+struct RsWS {
+    uint32_t *part;     // (OS_HIST_BLOCKS, npass, RS_BINS) per-block digit counts
+    uint32_t *dstart;   // (OS_MAXPASS, RS_BINS) first output position of each digit, per pass
+    int32_t *tile_ctr;  // (OS_MAXPASS) next tile number, per pass
+    uint32_t *status;   // (2, ntiles, RS_BINS) look-back words, two alternating buffers
+    int32_t *err;       // [0] set when a look-back wait gave up
+    int64_t cap;        // elements the buffers were carved for
+    // the multi-kernel fallback for n > OS_MAXN
+    int32_t *hist, *bsum;
+};
+
+static void carve_rs(PtgArena &A, int64_t n, RsWS &w)
+{
+    const int64_t nt = rs_nwt(n);
+    w.part = A.take<uint32_t>((int64_t)OS_HIST_BLOCKS * OS_MAXPASS * RS_BINS);
+    w.dstart = A.take<uint32_t>(OS_MAXPASS * RS_BINS);
+    w.tile_ctr = A.take<int32_t>(64);
+    w.status = A.take<uint32_t>(n > OS_MAXN ? 1 : 2 * nt * RS_BINS);
+    w.err = A.take<int32_t>(64);
+    w.cap = n;
+    const int64_t nh = n > OS_MAXN ? (int64_t)RS_BINS * nt : 1;
+    w.hist = A.take<int32_t>(nh);
+    w.bsum = A.take<int32_t>(n > OS_MAXN ? scan_blocks(nh) + 64 : 1);
+}
+
+// digit counts of passes [0, npass) of keys[0, n): per block, wave-aggregated (one LDS atomic per
+// distinct digit of a 64-key round); also clears the first look-back buffer (nstatus words)
+__global__ __launch_bounds__(256) void k_os_hist(const uint64_t *keys, int64_t n, int npass, uint32_t *part,
+                                                 uint32_t *status0, int64_t nstatus)
+{
+    __shared__ uint32_t h[OS_MAXPASS * RS_BINS];
+    for (int i = threadIdx.x; i < OS_MAXPASS * RS_BINS; i += 256) h[i] = 0;
+    __syncthreads();
+    const uint64_t lt = ptg_lanemask_lt();
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    // every lane of a wave runs the same number of rounds (ballots need the whole wave)
+    const int64_t base = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63);
+    for (int64_t i0 = base; i0 < n; i0 += stride) {
+        const int64_t i = i0 + ptg_lane();
+        const bool valid = i < n;
+        const uint64_t k = valid ? keys[i] : 0;
+        for (int p = 0; p < npass; p++) {
+            const uint32_t d = (uint32_t)(k >> (RS_BITS * p)) & (RS_BINS - 1);
+            uint64_t peers = ptg_ballot(valid);
+#pragma unroll
+            for (int b = 0; b < RS_BITS; b++) {
+                const bool bit = (d >> b) & 1;
+                const uint64_t bal = ptg_ballot(bit);
+                peers &= bit ? bal : ~bal;
+            }
+            if (valid && (peers & lt) == 0) atomicAdd(&h[p * RS_BINS + d], (uint32_t)__popcll(peers));
+        }
+    }
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nstatus; i += stride) status0[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < npass * RS_BINS; i += 256) part[(int64_t)blockIdx.x * npass * RS_BINS + i] = h[i];
+}
+
+// pass p = blockIdx.x: dstart[p][d] = number of keys whose pass-p digit is below d; resets the tile
+// counters
+__global__ __launch_bounds__(RS_BINS) void k_os_digit_starts(const uint32_t *part, int nparts, int npass,
+                                                             uint32_t *dstart, int32_t *tile_ctr, int32_t *err)
+{
+    __shared__ int32_t sh[8];
+    const int p = blockIdx.x, d = threadIdx.x;
+    uint32_t c = 0;
+    for (int b = 0; b < nparts; b++) c += part[((int64_t)b * npass + p) * RS_BINS + d];
+    int32_t tot;
+    const int32_t ex = block_excl_scan<int32_t>((int32_t)c, &tot, sh);
+    dstart[p * RS_BINS + d] = (uint32_t)ex;
+    if (d == 0) {
+        tile_ctr[p] = 0;
+        if (p == 0) err[0] = 0;
+    }
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_os_pass(const uint64_t *kin, const uint32_t *vin, int64_t n, int pass,
+                                                        const uint32_t *dstart, int32_t *tile_ctr, uint32_t *status,
+                                                        uint32_t *status_next, uint64_t *kout, uint32_t *vout,
+                                                        int32_t *err)
+{
+    __shared__ uint64_t sk[RS_TILE];
+    __shared__ uint32_t sv[RS_TILE];
+    __shared__ uint32_t run[RS_WAVES][RS_BINS];
+    __shared__ int64_t gbase[RS_BINS];
+    __shared__ int32_t sh[8];
+    __shared__ int32_t tile_sh;
+    const int shift = RS_BITS * pass;
+    const int wave = threadIdx.x >> 6, lane = ptg_lane();
+    const uint64_t lt = ptg_lanemask_lt();
+    if (threadIdx.x == 0) tile_sh = atomicAdd(&tile_ctr[pass], 1);
+#pragma unroll
+    for (int q = 0; q < RS_BINS / 64; q++) run[wave][lane * (RS_BINS / 64) + q] = 0;
+    __syncthreads();
+    const int64_t tile = tile_sh;
+    if (status_next) status_next[tile * RS_BINS + threadIdx.x] = 0;
+    const int64_t tile0 = tile * RS_TILE;
+    const int64_t base = tile0 + (int64_t)wave * (64 * RS_ITEMS);
+    uint64_t K[RS_ITEMS];
+    uint32_t V[RS_ITEMS], DR[RS_ITEMS];  // digit << 16 | rank inside the wave's quarter
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const int64_t i = base + r * 64 + lane;
+        const bool valid = i < n;
+        const uint64_t k = valid ? kin[i] : 0;
+        const uint32_t d = (uint32_t)(k >> shift) & (RS_BINS - 1);
+        uint64_t peers = ptg_ballot(valid);
+#pragma unroll
+        for (int b = 0; b < RS_BITS; b++) {
+            const bool bit = (d >> b) & 1;
+            const uint64_t bal = ptg_ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = run[wave][d] + (uint32_t)__popcll(peers & lt);
+        __builtin_amdgcn_wave_barrier();
+        if (valid && (peers & lt) == 0) run[wave][d] = rank + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+        K[r] = k;
+        V[r] = valid ? vin[i] : 0;
+        DR[r] = valid ? ((d << 16) | rank) : 0xffffffffu;
+    }
+    __syncthreads();
+    {
+        const int d = threadIdx.x;
+        uint32_t c[RS_WAVES], tot = 0;
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; w++) {
+            c[w] = run[w][d];
+            tot += c[w];
+        }
+        int32_t all;
+        const int32_t start = block_excl_scan<int32_t>((int32_t)tot, &all, sh);
+        // publish this tile's count of digit d, then add up the tiles before it
+        uint32_t *mine = status + tile * RS_BINS + d;
+        uint32_t prefix = 0;
+        if (tile == 0) {
+            __hip_atomic_store(mine, OS_FLAG_INCL | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(mine, OS_FLAG_AGG | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t t = tile - 1;
+            int spins = 0;
+            while (t >= 0) {
+                const uint32_t v =
+                    __hip_atomic_load(status + t * RS_BINS + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((v & ~OS_MASK) == 0) {
+                    if (++spins > OS_SPIN_LIMIT) {
+                        atomicOr(err, 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                prefix += v & OS_MASK;
+                if (v & OS_FLAG_INCL) break;
+                t--;
+            }
+            __hip_atomic_store(mine, OS_FLAG_INCL | ((prefix + tot) & OS_MASK), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        gbase[d] = (int64_t)dstart[pass * RS_BINS + d] + prefix - start;
+        uint32_t o = (uint32_t)start;
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; w++) {
+            run[w][d] = o;
+            o += c[w];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        if (DR[r] == 0xffffffffu) continue;
+        const uint32_t p = run[wave][DR[r] >> 16] + (DR[r] & 0xffffu);
+        sk[p] = K[r];
+        sv[p] = V[r];
+    }
+    __syncthreads();
+    const int64_t cnt = n - tile0 < RS_TILE ? n - tile0 : RS_TILE;
+    bool bad = false;
+    for (int64_t p = threadIdx.x; p < cnt; p += RS_THREADS) {
+        const uint64_t k = sk[p];
+        const int64_t g = gbase[(k >> shift) & (RS_BINS - 1)] + p;
+        if ((uint64_t)g < (uint64_t)n) {  // always, unless a look-back wait gave up
+            kout[g] = k;
+            vout[g] = sv[p];
+        } else {
+            bad = true;
+        }
+    }
+    if (bad) atomicOr(err, 2);
+}
+
 // sorts (k0,v0) by the low `nbits` bits; the result ends in (k0,v0) or (k1,v1): *in_first tells which
 static ptg_status radix_sort(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,
-                             int32_t *hist, int32_t *bsum, bool *in_first, hipStream_t s)
+                             const RsWS &w, bool *in_first, hipStream_t s)
 {
-    int64_t ntiles = rs_nwt(n);
+    if (n > w.cap) {
+        ptg_set_error("radix_sort: %lld keys exceed the workspace's %lld", (long long)n, (long long)w.cap);
+        return PTG_ERR_WORKSPACE;
+    }
+    const int npass = std::min(OS_MAXPASS, (nbits + RS_BITS - 1) / RS_BITS);
+    const int64_t ntiles = rs_nwt(n);
     bool first = true;
-    for (int shift = 0; shift < nbits; shift += RS_BITS) {
+    if (n <= OS_MAXN) {
+        const int64_t rows = ntiles * RS_BINS;
+        const unsigned hb = (unsigned)std::min<int64_t>(OS_HIST_BLOCKS, ptg_cdiv(std::max(n, rows), 256));
+        k_os_hist<<<hb, 256, 0, s>>>(k0, n, npass, w.part, w.status, rows);
+        k_os_digit_starts<<<npass, RS_BINS, 0, s>>>(w.part, (int)hb, npass, w.dstart, w.tile_ctr, w.err);
+        for (int p = 0; p < npass; p++) {
+            uint64_t *ki = first ? k0 : k1, *ko = first ? k1 : k0;
+            uint32_t *vi = first ? v0 : v1, *vo = first ? v1 : v0;
+            uint32_t *cur = w.status + (p & 1) * rows;
+            uint32_t *next = p + 1 < npass ? w.status + ((p + 1) & 1) * rows : nullptr;
+            k_os_pass<<<(unsigned)ntiles, RS_THREADS, 0, s>>>(ki, vi, n, p, w.dstart, w.tile_ctr, cur, next, ko, vo,
+                                                              w.err);
+            first = !first;
+        }
+        PTG_LAUNCH_CHECK();
+        *in_first = first;
+        return PTG_OK;
+    }
+    for (int p = 0; p < npass; p++) {
+        const int shift = RS_BITS * p;
         uint64_t *ki = first ? k0 : k1, *ko = first ? k1 : k0;
         uint32_t *vi = first ? v0 : v1, *vo = first ? v1 : v0;
-        k_rs_hist<<<ntiles, RS_THREADS, 0, s>>>(ki, n, shift, ntiles, hist);
-        ptg_status st = excl_scan(LoadArr{hist}, (int64_t)RS_BINS * ntiles, hist, bsum, nullptr, s);
+        k_rs_hist<<<ntiles, RS_THREADS, 0, s>>>(ki, n, shift, ntiles, w.hist);
+        ptg_status st = excl_scan(LoadArr{w.hist}, (int64_t)RS_BINS * ntiles, w.hist, w.bsum, nullptr, s);
         if (st) return st;
-        k_rs_scatter<<<ntiles, RS_THREADS, 0, s>>>(ki, vi, n, shift, ntiles, hist, ko, vo);
+        k_rs_scatter<<<ntiles, RS_THREADS, 0, s>>>(ki, vi, n, shift, ntiles, w.hist, ko, vo);
         PTG_LAUNCH_CHECK();
         first = !first;
     }
@@ -480,14 +737,14 @@ __global__ __launch_bounds__(256) void k_gather_sys_key(const uint32_t *sys, con
                                                         uint64_t *out)
 {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) out[j] = sys[vals[j]];
+    if (j < n) out[j] = vals[j] < n ? sys[vals[j]] : 0;  // vals[j] < n unless a sort gave up (reported)
 }
 
 __global__ __launch_bounds__(256) void k_gather_keys(const uint64_t *kU, const uint32_t *vals, int64_t n,
                                                      uint64_t *keys)
 {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) keys[j] = kU[vals[j]];
+    if (j < n) keys[j] = vals[j] < n ? kU[vals[j]] : 0;  // vals[j] < n unless a sort gave up (reported)
 }
 
 // root cube of every system, as (centre x, y, z, side)
@@ -521,7 +778,8 @@ struct BuildWS {
     Counters *ctr;
     uint64_t *kA, *kB;
     uint32_t *vA, *vB;
-    int32_t *hist, *bsum;
+    RsWS rs;
+    int32_t *bsum;
     int32_t *lev, *off;
     // collision re-keying (extended keys K1..K3 per Morton slot, and the compact subset)
     uint64_t *ext;      // 3 * n
@@ -543,8 +801,6 @@ struct BuildWS {
 
 static bool carve_build(PtgArena &A, int64_t n, BuildWS &w, int64_t nsys = 1)
 {
-    int64_t nwt = rs_nwt(n);
-    int64_t nh = (int64_t)RS_BINS * nwt;
     w.part = A.take<double>(6 * BB_BLOCKS);
     w.geom = A.take<Geom>(1);
     w.ctr = A.take<Counters>(1);
@@ -552,8 +808,8 @@ static bool carve_build(PtgArena &A, int64_t n, BuildWS &w, int64_t nsys = 1)
     w.kB = A.take<uint64_t>(n);
     w.vA = A.take<uint32_t>(n);
     w.vB = A.take<uint32_t>(n);
-    w.hist = A.take<int32_t>(nh);
-    w.bsum = A.take<int32_t>(scan_blocks(std::max<int64_t>(nh, n)) + 64);
+    carve_rs(A, n, w.rs);
+    w.bsum = A.take<int32_t>(scan_blocks(n) + 64);
     w.lev = A.take<int32_t>(n);
     w.off = A.take<int32_t>(n);
     w.ext = A.take<uint64_t>(3 * n);
@@ -826,11 +1082,13 @@ static ptg_status sort_by_keys(const double *pos, const double *m, const double 
     int nb = (int)std::min<int64_t>(BB_BLOCKS, std::max<int64_t>(1, ptg_cdiv(n, 1024)));
     k_bbox_partial<<<nb, 256, 0, s>>>(pos, n, w.part);
     k_bbox_final<<<1, 256, 0, s>>>(w.part, nb, w.geom);
+    if (m) phase_mark(PTG_PHASE_BBOX, s);
     k_keys<<<grid1(n), 256, 0, s>>>(pos, m, h, n, w.geom, w.kA, w.vA, m && h ? w.ctr : nullptr,
                                     m && h ? w.rec : nullptr);
     PTG_LAUNCH_CHECK();
+    if (m) phase_mark(PTG_PHASE_KEYS, s);
     bool in_first;
-    ptg_status st = radix_sort(w.kA, w.vA, w.kB, w.vB, n, 64, w.hist, w.bsum, &in_first, s);
+    ptg_status st = radix_sort(w.kA, w.vA, w.kB, w.vB, n, 64, w.rs, &in_first, s);
     if (st) return st;
     if (!in_first) {
         PTG_HIP_CHECK(hipMemcpyAsync(w.kA, w.kB, n * 8, hipMemcpyDeviceToDevice, s));
@@ -856,12 +1114,12 @@ static ptg_status sort_by_keys_forest(const double *pos, const double *m, const 
     k_keys_sys<<<grid1(n), 256, 0, s>>>(pos, m, h, n, w.geoms, w.sys, w.kA, w.kU, w.vA, m && h ? w.rec : nullptr);
     PTG_LAUNCH_CHECK();
     bool in_first;
-    ptg_status st = radix_sort(w.kA, w.vA, w.kB, w.vB, n, 64, w.hist, w.bsum, &in_first, s);
+    ptg_status st = radix_sort(w.kA, w.vA, w.kB, w.vB, n, 64, w.rs, &in_first, s);
     if (st) return st;
     uint32_t *v = in_first ? w.vA : w.vB, *vo = in_first ? w.vB : w.vA;
     k_gather_sys_key<<<grid1(n), 256, 0, s>>>(w.sys, v, n, w.tkA);
     PTG_LAUNCH_CHECK();
-    st = radix_sort(w.tkA, v, w.tkB, vo, n, sys_bits(w.nsys), w.hist, w.bsum, &in_first, s);
+    st = radix_sort(w.tkA, v, w.tkB, vo, n, sys_bits(w.nsys), w.rs, &in_first, s);
     if (st) return st;
     const uint32_t *vs = in_first ? v : vo;
     if (vs != w.vA) PTG_HIP_CHECK(hipMemcpyAsync(w.vA, vs, n * 4, hipMemcpyDeviceToDevice, s));
@@ -969,8 +1227,10 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
             return PTG_ERR_ARG;
         }
     }
+    phase_mark(PTG_PHASE_START, s);
     st = F ? sort_by_keys_forest(pos, m, h, n, w, s) : sort_by_keys(pos, m, h, n, w, s);
     if (st) return st;
+    phase_mark(PTG_PHASE_SORT, s);
     if (!F) PTG_HIP_CHECK(hipMemcpyAsync(w.geoms, w.geom, sizeof(Geom), hipMemcpyDeviceToDevice, s));
     if (forest) {
         k_forest_cubes<<<grid1(nsys), 256, 0, s>>>(w.geoms, nsys, forest->cubes);
@@ -979,9 +1239,15 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
     if (n >= 2) {
         k_levels<<<grid1(n - 1), 256, 0, s>>>(w.kA, w.sys, n, w.lev, w.ctr);
         PTG_LAUNCH_CHECK();
+        int32_t sort_err0 = 0;
         PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
+        PTG_HIP_CHECK(hipMemcpyAsync(&sort_err0, w.rs.err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         PTG_HIP_CHECK(hipStreamSynchronize(s));
         if (hc.nonfinite) return nonfinite();
+        if (sort_err0) {  // before anything indexes through the sorted order
+            ptg_set_error("ptg_build_sort: a radix-sort look-back wait gave up");
+            return PTG_ERR_HIP;
+        }
         if (hc.ncollide > 0) {
             // collision runs: compute extended keys and re-sort each run by them (stable)
             k_collision_flags<<<grid1(n), 256, 0, s>>>(w.kA, w.sys, n, w.flag);
@@ -1001,7 +1267,7 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
                 for (int r = 0; r < 4; r++) {
                     k_gather_u64<<<grid1(m), 256, 0, s>>>(karr[r], w.perm, m, w.tkA);
                     bool in_first;
-                    st = radix_sort(w.tkA, w.perm, w.tkB, w.permB, m, 64, w.hist, w.bsum, &in_first, s);
+                    st = radix_sort(w.tkA, w.perm, w.tkB, w.permB, m, 64, w.rs, &in_first, s);
                     if (st) return st;
                     if (!in_first)
                         PTG_HIP_CHECK(hipMemcpyAsync(w.perm, w.permB, m * 4, hipMemcpyDeviceToDevice, s));
@@ -1009,7 +1275,7 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
                 if (F) {  // most significant: the system
                     k_gather_sys_sub<<<grid1(m), 256, 0, s>>>(w.sys, w.sslot, w.perm, m, w.tkA);
                     bool in_first;
-                    st = radix_sort(w.tkA, w.perm, w.tkB, w.permB, m, sys_bits(nsys), w.hist, w.bsum, &in_first, s);
+                    st = radix_sort(w.tkA, w.perm, w.tkB, w.permB, m, sys_bits(nsys), w.rs, &in_first, s);
                     if (st) return st;
                     if (!in_first)
                         PTG_HIP_CHECK(hipMemcpyAsync(w.perm, w.permB, m * 4, hipMemcpyDeviceToDevice, s));
@@ -1027,11 +1293,18 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
             w.lev, n - 1, w.ctr);
         PTG_LAUNCH_CHECK();
     }
+    phase_mark(PTG_PHASE_COUNT_NODES, s);
     Geom hg;
+    int32_t sort_err = 0;
     PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
     PTG_HIP_CHECK(hipMemcpyAsync(&hg, w.geoms, sizeof(Geom), hipMemcpyDeviceToHost, s));
+    PTG_HIP_CHECK(hipMemcpyAsync(&sort_err, w.rs.err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     PTG_HIP_CHECK(hipStreamSynchronize(s));
     if (hc.nonfinite) return nonfinite();
+    if (sort_err) {
+        ptg_set_error("ptg_build_sort: a radix-sort look-back wait gave up");
+        return PTG_ERR_HIP;
+    }
     info->n = n;
     info->nnodes = n >= 2 ? hc.nnodes : 1;
     info->has_collisions = hc.ncollide > 0;
@@ -1258,7 +1531,7 @@ struct NodeWS {
     int32_t *bsum;
     uint64_t *lvA, *lvB;   // node (system, level) keys, then levels (sort keys)
     uint32_t *idA, *idB;   // node offsets (sort values)
-    int32_t *hist;
+    RsWS rs;
     int32_t *lvstart;      // (max_level + 2) first position of each level in the sorted list
     int32_t *elem;         // (n + nnodes) element stored at each walk slot
     int32_t *npair;        // (nnodes) the Morton pair that opened each node, by preorder offset
@@ -1269,15 +1542,14 @@ struct NodeWS {
 
 static bool carve_nodes(PtgArena &A, int64_t n, int64_t nnodes, int64_t max_level, NodeWS &w)
 {
-    int64_t nh = (int64_t)RS_BINS * rs_nwt(nnodes);
     w.acc = A.take<double>(3 * nnodes);
     w.err = A.take<int32_t>(64);
-    w.bsum = A.take<int32_t>(scan_blocks(std::max<int64_t>(nnodes, nh)) + 64);
+    w.bsum = A.take<int32_t>(scan_blocks(nnodes) + 64);
     w.lvA = A.take<uint64_t>(nnodes);
     w.lvB = A.take<uint64_t>(nnodes);
     w.idA = A.take<uint32_t>(nnodes);
     w.idB = A.take<uint32_t>(nnodes);
-    w.hist = A.take<int32_t>(nh);
+    carve_rs(A, nnodes, w.rs);
     w.lvstart = A.take<int32_t>(max_level + 2);
     w.elem = A.take<int32_t>(n + nnodes);
     w.npair = A.take<int32_t>(nnodes);
@@ -1661,8 +1933,10 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     const bool F = nsys > 1;
     if (F) w.soff = forest->offsets;
     KeyView kv{w.kA, info->has_collisions ? w.ext : nullptr, w.sys, n};
+    phase_mark(PTG_PHASE_TREE_START, s);
     k_particles<<<grid1(n), 256, 0, s>>>(w.rec, w.vA, w.kA, n, t);
     if (particles_ready) PTG_HIP_CHECK(hipEventRecord((hipEvent_t)particles_ready, s));
+    phase_mark(PTG_PHASE_PERMUTE, s);
     // node order: (system, level), depth-first preorder within a level (stable LSD radix sort of the
     // preorder offsets)
     int lbits = 1;
@@ -1677,7 +1951,7 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     }
     k_iota_u32<<<grid1(nnodes), 256, 0, s>>>(nw.idA, nnodes);
     bool in_first;
-    st = radix_sort(nw.lvA, nw.idA, nw.lvB, nw.idB, nnodes, kbits, nw.hist, nw.bsum, &in_first, s);
+    st = radix_sort(nw.lvA, nw.idA, nw.lvB, nw.idB, nnodes, kbits, nw.rs, &in_first, s);
     if (st) return st;
     const uint64_t *skey = in_first ? nw.lvA : nw.lvB;
     k_invert<<<grid1(nnodes), 256, 0, s>>>(in_first ? nw.idA : nw.idB, nnodes, nw.pi);
@@ -1702,11 +1976,12 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
         uint32_t *id = in_first ? nw.idB : nw.idA, *id2 = in_first ? nw.idA : nw.idB;
         k_key_levels<<<grid1(nnodes), 256, 0, s>>>(skey, nnodes, lbits, lk, id);
         bool first2;
-        st = radix_sort(lk, id, lk2, id2, nnodes, lbits, nw.hist, nw.bsum, &first2, s);
+        st = radix_sort(lk, id, lk2, id2, nnodes, lbits, nw.rs, &first2, s);
         if (st) return st;
         lv = first2 ? lk : lk2;
         byid = first2 ? id : id2;
     }
+    phase_mark(PTG_PHASE_NODES, s);
     k_level_starts<<<grid1(nnodes + 1), 256, 0, s>>>(lv, nnodes, max_level, nw.lvstart);
     k_node_init<<<1, 64, 0, s>>>(nw);
     k_link<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t, nw);
@@ -1726,12 +2001,19 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
     }
     k_slots<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, F ? forest->roots : nullptr, t, nw);
     PTG_LAUNCH_CHECK();
+    phase_mark(PTG_PHASE_LINKS, s);
     if (compute_moments) {
         // per-level node counts decide the launches: runs of small levels go to one block, the
         // others get a grid each
         std::vector<int32_t> ls(max_level + 2);
+        int32_t sort_err = 0;
         PTG_HIP_CHECK(hipMemcpyAsync(ls.data(), nw.lvstart, (max_level + 2) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        PTG_HIP_CHECK(hipMemcpyAsync(&sort_err, nw.rs.err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         PTG_HIP_CHECK(hipStreamSynchronize(s));
+        if (sort_err) {
+            ptg_set_error("ptg_build_tree: a radix-sort look-back wait gave up");
+            return PTG_ERR_HIP;
+        }
         const int64_t small = MS_THREADS;
         int L = max_level;
         while (L >= 0) {
@@ -1757,10 +2039,12 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
         k_fold_particles<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t);
     }
     PTG_LAUNCH_CHECK();
+    phase_mark(PTG_PHASE_MOMENTS, s);
     if (pack) {  // the walk copy, in slot order through the build's slot map
         st = pack_in_slot_order(t, n, nnodes, nsys, forest, pack, nw.elem, s);
         if (st) return st;
     }
+    phase_mark(PTG_PHASE_PACK, s);
     return PTG_OK;
 }
 

@@ -51,7 +51,8 @@ struct WalkCtl {
     int32_t ctr[NXCD * CTR_STRIDE];   // per-XCD group counters
     int32_t ctr2[NXCD * CTR_STRIDE];  // per-XCD supergroup counters (shared pass)
     int32_t err;
-    int32_t pad[31];
+    int32_t hwm;  // deepest global spill stack (entries) any wave reached in the last walk
+    int32_t pad[30];
 };
 
 template <typename T> struct WalkArgs {
@@ -92,6 +93,7 @@ template <typename T> struct WalkArgs {
     int64_t *counters;
     WalkCtl *ctl;
     int *spill;
+    int gcap;  // global spill entries per wave
 };
 
 // ---- softening kernels (kernel.py:4-75), generic precision ----
@@ -275,35 +277,50 @@ __device__ __forceinline__ void interact_node_fast(Acc<T> &a, V4<T> s, const T *
 }
 
 // The M4 spline terms for r < h (kernel.py:4-75) with reciprocals instead of IEEE divisions, for
-// the FP32 fast path: hinv = 1/h, qinv = h/r = h * rinv.
+// the FP32 fast path: hinv = 1/h, qinv = h/r = h * rinv.  The inner (q <= 1/2) and outer pieces are
+// one polynomial in q, 1/q whose coefficients are selected per lane, so a wave whose lanes straddle
+// q = 1/2 evaluates one piece instead of both branches:
+//   K h^3   = a0 + a1 q + a2 q^2 + a3 q^3 + am3 q^-3
+//   phi h   = b0 + bm1 q^-1 + q^2 (b2 + q (b3 + q (b4 + q b5)))
+//   Q h^5   = c0 + c1 q + cm1 q^-1 + cm5 q^-5
+// (inner: a = 32/3, 0, -38.4, 32, 0; b = -2.8, 0, 16/3, 0, -9.6, 6.4; c = 76.8, -96, 0, 0;
+//  outer: a = 64/3, -48, 38.4, -32/3, -1/15; b = -3.2, 1/15, 32/3, -16, 9.6, -32/15;
+//  c = -76.8, 32, 48, -0.2).  Inner lanes take q^-1 = 0, so tiny q never overflows.
 template <typename T> struct SplineFast {
     T K, phi, Q;
 };
 template <typename T, bool P, bool D>
 __device__ __forceinline__ SplineFast<T> spline_fast(T r, T rinv, T h)
 {
-    T hinv = __builtin_amdgcn_rcpf(h);
-    T q = r * hinv;
-    T hinv2 = hinv * hinv;
-    T hinv3 = hinv2 * hinv;
+    const T hinv = __builtin_amdgcn_rcpf(h);
+    const T q = r * hinv;
+    const T hinv2 = hinv * hinv;
+    const T hinv3 = hinv2 * hinv;
+    const bool in = q <= T(0.5);
+    const T qinv = in ? T(0) : h * rinv;
+    const T qinv3 = qinv * qinv * qinv;
     SplineFast<T> o;
+    {
+        const T a0 = in ? T(10.666666666666666666) : T(21.333333333333);
+        const T a1 = in ? T(0) : T(-48.0);
+        const T a2 = in ? T(-38.4) : T(38.4);
+        const T a3 = in ? T(32.0) : T(-10.666666666667);
+        o.K = (((a3 * q + a2) * q + a1) * q + a0 + T(-0.066666666667) * qinv3) * hinv3;
+    }
     o.phi = 0;
+    if (P) {
+        const T b0 = in ? T(-2.8) : T(-3.2);
+        const T b2 = in ? T(5.33333333333333333) : T(10.666666666666666666666);
+        const T b3 = in ? T(0) : T(-16.0);
+        const T b4 = in ? T(-9.6) : T(9.6);
+        const T b5 = in ? T(6.4) : T(-2.1333333333333333333333);
+        o.phi = (q * q * (b2 + q * (b3 + q * (b4 + q * b5))) + b0 + T(0.066666666666666666666) * qinv) * hinv;
+    }
     o.Q = 0;
-    if (q <= T(0.5)) {
-        T q2 = q * q;
-        o.K = (T(10.666666666666666666) + q2 * (T(-38.4) + T(32.0) * q)) * hinv3;
-        if (P) o.phi = (T(-2.8) + q2 * (T(5.33333333333333333) + q2 * (T(6.4) * q - T(9.6)))) * hinv;
-        if (D) o.Q = (T(76.8) - T(96.0) * q) * hinv3 * hinv2;
-    } else {
-        T qinv = h * rinv;
-        T qinv3 = qinv * qinv * qinv;
-        o.K = (T(21.333333333333) - T(48.0) * q + T(38.4) * q * q - T(10.666666666667) * q * q * q -
-               T(0.066666666667) * qinv3) * hinv3;
-        if (P)
-            o.phi = (T(-3.2) + T(0.066666666666666666666) * qinv +
-                     q * q * (T(10.666666666666666666666) + q * (T(-16.0) + q * (T(9.6) - T(2.1333333333333333333333) * q)))) *
-                    hinv;
-        if (D) o.Q = (T(48.0) * qinv - T(76.8) + T(32.0) * q - T(0.2) * qinv3 * qinv * qinv) * hinv3 * hinv2;
+    if (D) {
+        const T c0 = in ? T(76.8) : T(-76.8);
+        const T c1 = in ? T(-96.0) : T(32.0);
+        o.Q = (c0 + c1 * q + T(48.0) * qinv - T(0.2) * qinv3 * qinv * qinv) * hinv3 * hinv2;
     }
     return o;
 }
@@ -597,9 +614,10 @@ void k_walk(WalkArgs<T> A)
     const int wave = threadIdx.x >> 6, lane = ptg_lane();
     WaveLds<T, QUAD> &L = lds[wave];
     const int64_t gwave = (int64_t)blockIdx.x * WALK_WAVES + wave;
-    int *spill = A.spill + gwave * GCAP;
+    int *spill = A.spill + gwave * (int64_t)A.gcap;
     const int xcd = blockIdx.x % NXCD;
     int64_t n_int = 0, n_vis = 0, n_part = 0, n_exp = 0;
+    int ghw = 0;  // this wave's deepest spill
     const int64_t ngroups = A.goff ? (int64_t)__builtin_amdgcn_readfirstlane(*A.gcount) : A.ngroups;
 
     for (int attempt = 0; attempt < NXCD; attempt++) {
@@ -615,6 +633,7 @@ void k_walk(WalkArgs<T> A)
             const int64_t e0 = A.goff ? (int64_t)A.goff[g + 1] : A.nt;
             const int64_t b0 = e0 < a0 + A.gs ? e0 : a0 + A.gs;
             const int mt = (int)(b0 - a0);
+            if (A.goff && e0 > b0 && lane == 0) atomicOr(&A.ctl->err, 2);  // a supplied group larger than group_size
             int root = 0;
             double ox = A.ox, oy = A.oy, oz = A.oz;
             if (A.nsys > 0) {
@@ -824,13 +843,14 @@ void k_walk(WalkArgs<T> A)
                 if (sp + tot > SCAP) {
                     // move the bottom ks entries of the LDS stack to this wave's global region
                     const int ks = spill_count(sp, tot);
-                    if (gsp + ks > GCAP) {
+                    if (gsp + ks > A.gcap) {
                         overflow = true;
                         break;
                     }
                     spill_bottom(L.stack, spill + gsp, sp, ks, lane);
                     gsp += ks;
                     sp -= ks;
+                    ghw = std::max(ghw, gsp);
                 }
                 {
                     // unrolled: eight masked LDS stores with immediate offsets, no per-lane loop
@@ -911,6 +931,7 @@ void k_walk(WalkArgs<T> A)
             }
         }
     }
+    if (lane == 0 && ghw > 0) atomicMax(&A.ctl->hwm, ghw);
     if (lane == 0 && A.counters) {
         atomicAdd((unsigned long long *)&A.counters[0], (unsigned long long)n_int);
         atomicAdd((unsigned long long *)&A.counters[1], (unsigned long long)n_vis);
@@ -939,7 +960,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk_super(WalkArgs<float> 
     int *stack = stk[wave];
     int *fbuf = fb[wave];
     const int64_t gwave = (int64_t)blockIdx.x * WALK_WAVES + wave;
-    int *spill = A.spill + gwave * GCAP;
+    int *spill = A.spill + gwave * (int64_t)A.gcap;
     int *pool = A.spool + gwave * A.pool_cap;
     int64_t pused = 0, n_vis = 0, n_exp = 0;
     const int xcd = blockIdx.x % NXCD;
@@ -1088,7 +1109,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk_super(WalkArgs<float> 
                 const int below = small_excl_scan(cnt, tot);
                 if (sp + tot > SCAP) {
                     const int ks = spill_count(sp, tot);
-                    if (gsp + ks > GCAP) {
+                    if (gsp + ks > A.gcap) {
                         fail = true;
                         break;
                     }
@@ -1243,12 +1264,11 @@ static ptg_status launch_walk(const WalkArgs<T> &a, int fields, bool quad, bool 
     return launch_fq<T, 4, EXACT>(a, fields, quad, expand, blocks, s);
 }
 
-extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest, const double *pos,
-                               const double *soft, const int64_t *perm, int targets_sorted, int64_t nt, int fields,
-                               int quadrupole,
-                               int group_size, const int32_t *group_offsets, const int32_t *group_count, double theta,
-                               const ptg_criterion *criterion, double G, int exact, double *out, int64_t *counters,
-                               void *ws, size_t ws_bytes, void *stream)
+static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, const double *pos, const double *soft,
+                            const int64_t *perm, int targets_sorted, int64_t nt, int fields, int quadrupole,
+                            int group_size, const int32_t *group_offsets, const int32_t *group_count, double theta,
+                            const ptg_criterion *criterion, double G, int exact, double *out, int64_t *counters,
+                            void *ws, size_t ws_bytes, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!wt || nt < 0 || (nt > 0 && (!pos || !soft || !perm || !out))) {
@@ -1321,6 +1341,8 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest
         ptg_set_error("ptg_walk: the shared supergroup pass needs group offsets (ptg_target_groups)");
         return PTG_ERR_ARG;
     }
+    int64_t blocks = walk_grid_blocks();
+    int gcap = GCAP;
     if (nt == 0) return PTG_OK;
     const size_t base_bytes = ptg_walk_workspace_bytes(nt, group_size);
     const int64_t nsys_ws = forest ? forest->nsys : 1;
@@ -1344,7 +1366,6 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest
         p += ptg_align256((size_t)sg.nsup * SX_STRIDE * sizeof(float));
         spool = (int *)p;
     }
-    int64_t blocks = walk_grid_blocks();
     const int64_t fn = forest ? forest->nsys : 0;
     const int64_t *fo = forest ? forest->offsets : nullptr;
     const double *fc = forest ? forest->cubes : nullptr;
@@ -1356,7 +1377,7 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest
                           (const float *)wt->quad, pos, soft, perm, targets_sorted != 0, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
                           group_size, (float)theta, (float)(1.0 / theta), crit, (float)eps, (float)eps_exp, eps_scale,
                           two_level ? 1 : 0, sexp, srec, spool, sg.pool_cap, G, wt->origin[0], wt->origin[1], wt->origin[2],
-                          fn, fo, fc, out, counters, ctl, spill};
+                          fn, fo, fc, out, counters, ctl, spill, gcap};
         if (exact) {
             ptg_set_error("ptg_walk: the exact branch set is evaluated in FP64; use prec=PTG_F64");
             return PTG_ERR_ARG;
@@ -1371,13 +1392,23 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest
                            (const double *)wt->quad, pos, soft, perm, targets_sorted != 0, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
                            group_size, theta, 1.0 / theta, crit, eps, 0.0, nullptr, 0, nullptr, nullptr, nullptr, 0, G,
                            wt->origin[0], wt->origin[1], wt->origin[2],
-                           fn, fo, fc, out, counters, ctl, spill};
+                           fn, fo, fc, out, counters, ctl, spill, gcap};
         st = launch_walk<double, true>(a, fields, quadrupole != 0, false, blocks, s);
     } else {
         ptg_set_error("ptg_walk: prec must be PTG_F32 or PTG_F64");
         return PTG_ERR_ARG;
     }
     return st;
+}
+
+extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest, const double *pos,
+                               const double *soft, const int64_t *perm, int targets_sorted, int64_t nt, int fields,
+                               int quadrupole, int group_size, const int32_t *group_offsets,
+                               const int32_t *group_count, double theta, const ptg_criterion *criterion, double G,
+                               int exact, double *out, int64_t *counters, void *ws, size_t ws_bytes, void *stream)
+{
+    return walk_impl(wt, forest, pos, soft, perm, targets_sorted, nt, fields, quadrupole, group_size, group_offsets,
+                     group_count, theta, criterion, G, exact, out, counters, ws, ws_bytes, stream);
 }
 
 // overflow flag of the last walk on this workspace (0 = fine); synchronises the stream
@@ -1387,6 +1418,11 @@ extern "C" ptg_status ptg_walk_check(void *ws, void *stream)
     hipStream_t s = (hipStream_t)stream;
     PTG_HIP_CHECK(hipMemcpyAsync(&err, &((WalkCtl *)ws)->err, 4, hipMemcpyDeviceToHost, s));
     PTG_HIP_CHECK(hipStreamSynchronize(s));
+    if (err & 2) {
+        ptg_set_error("ptg_walk: a supplied target group holds more than group_size targets (its rows past "
+                      "group_size were not evaluated)");
+        return PTG_ERR_ARG;
+    }
     if (err) {
         ptg_set_error("ptg_walk: a traversal stack exceeded its spill capacity");
         return PTG_ERR_OVERFLOW;

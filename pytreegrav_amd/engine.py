@@ -177,6 +177,8 @@ class DeviceTree:
         order, bf16)."""
         from .learned import open_radii
 
+        # the scale tensor is kept in the cache entry, so its address cannot be handed to another
+        # tensor while the entry exists
         skey = None if scale is None else scale.data_ptr()
         key = ("learned", model.key(), float(eps), skey, bool(quadrupole), bool(bf16))
         if key not in self._packs:
@@ -187,6 +189,7 @@ class DeviceTree:
             open_radii(self, p, model, eps, bf16=bf16, scale=scale)
             for k in [k for k in self._packs if k[0] == "learned"]:  # one learned pack per tree
                 del self._packs[k]
+            p["scale_ref"] = scale
             self._packs[key] = p
         return self._packs[key]
 
@@ -296,6 +299,9 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
         return out
     prec = PTG_F32 if precision in ("float32", "f32", np.float32, torch.float32) else PTG_F64
     if group_size == "auto":
+        if groups is not None:
+            raise ValueError("group_size='auto' cannot be combined with precomputed groups (they were formed for "
+                             "one group size; a walk never splits a supplied group)")
         group_size = autotune_group_size(tree, pos, soft, perm, mask, quadrupole=quadrupole, theta=theta,
                                          precision=precision, grouping=grouping, criterion=criterion, eps=eps,
                                          model=model, bf16=bf16, expansion=expansion, eps_scale=eps_scale,
@@ -367,9 +373,10 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
 
 
 def _walk_workspace(nbytes, slot=0):
-    """Walk workspace per (device, slot), reused across walks; its overflow flag belongs to the last
-    walk run on it."""
-    key = (torch.cuda.current_device(), slot)
+    """Walk workspace per (device, stream, slot), reused across walks on that stream; its group
+    counters, spill stacks and overflow flag belong to the last walk run on it, so walks on
+    different streams never share one."""
+    key = (torch.cuda.current_device(), torch.cuda.current_stream().cuda_stream, slot)
     ws = _walk_ws.get(key)
     if ws is None or ws.numel() < nbytes:
         ws = workspace(nbytes)
@@ -379,7 +386,7 @@ def _walk_workspace(nbytes, slot=0):
 
 def check_walk(slot=0):
     """Overflow check of the last walk run on workspace `slot` (synchronises the stream)."""
-    ws = _walk_ws.get((torch.cuda.current_device(), slot))
+    ws = _walk_ws.get((torch.cuda.current_device(), torch.cuda.current_stream().cuda_stream, slot))
     if ws is not None:
         check(_lib.load().ptg_walk_check(ptr(ws), stream()), "ptg_walk")
 
@@ -396,7 +403,7 @@ def _side_stream():
 
 def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, group_size=64, criterion="theta",
                 alpha=None, model=None, bf16=False, counters=None, walk_events=None, expansion=None,
-                supergroups=False):
+                supergroups=False, stats=None):
     """Self-gravity FP32 fields of the particles a tree was built from, in their input order.
 
     The cell-aligned target groups are formed on a side stream while the main stream packs the
@@ -405,7 +412,8 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
     and need a tree built with quadrupoles; theta defaults to 0.7 ("theta") or 2.0 (the guard of
     the other two).  expansion: group-expansion tolerance in units of the RMS field (see walk);
     None walks without it.  walk_events: optional (start, end) torch.cuda.Event pair recorded around
-    the walk launch alone."""
+    the walk launch alone.  stats: optional dict that receives "groups", the device int32 count of
+    the target groups walked."""
     L = _lib.load()
     pos = to_dev(pos, (-1, 3))
     n = pos.shape[0]
@@ -445,6 +453,8 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
             else None
     offsets.record_stream(main)
     count.record_stream(main)
+    if stats is not None:
+        stats["groups"] = count
     for s_t in sample[:3] if sample is not None else ():
         s_t.record_stream(main)
     eps, scale, exp_eps = None, None, None

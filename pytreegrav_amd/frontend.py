@@ -24,6 +24,24 @@ from .engine import PTG_ACC, PTG_POT, PTG_TIDAL
 FP32_DEFAULT_GROUP = 64
 
 
+class _DefaultTheta(float):
+    """The reference's default opening angle, 0.7 (frontend.py:149, :425).  It is an ordinary float
+    everywhere; only its identity tells a theta the caller left at the default from an explicit 0.7,
+    so the learned and error-bounded criteria can replace the default by their own guard while an
+    explicit theta=0.7 is kept."""
+
+    def __repr__(self):
+        return "0.7"
+
+
+THETA_DEFAULT = _DefaultTheta(0.7)
+# geometric guard of the learned and error-bounded criteria when the caller gives no theta: the
+# reference's test at this opening angle is applied on top of the per-node error bound, so an
+# opening radius the network under-predicts on an unfamiliar particle distribution still cannot
+# accept a node nearer than size / GUARD_THETA
+GUARD_THETA = 1.0
+
+
 def _f64(a):
     """Cast to float64 without collapsing a length-1 array to a scalar (frontend.py:20-25)."""
     if isinstance(a, torch.Tensor):
@@ -287,7 +305,7 @@ def _self_fields(pos, m, softening, mask, G, theta, tree, return_tree, parallel,
 
     criterion "error"/"learned" (float32 tree walks only, no reference counterpart): accept a node
     only when its estimated (analytic or learned) monopole error is below alpha x the RMS field;
-    theta then defaults to 2.0 and acts as the reference's geometric guard.  expansion (float32
+    theta acts as the reference's geometric guard (GUARD_THETA unless the caller gave one).  expansion (float32
     monopole walks of the potential and/or acceleration, no reference counterpart): fold accepted
     elements far enough from a target group into a third-order expansion about the group's centre
     when its truncation bound is below expansion x the RMS field."""
@@ -410,7 +428,7 @@ def _learned_model():
 # ---------------------------------------------------------------------------------------------
 # public API (frontend.py:145-901)
 # ---------------------------------------------------------------------------------------------
-def Potential(pos, m, softening=None, G=1.0, theta=0.7, tree=None, return_tree=False, parallel=False,
+def Potential(pos, m, softening=None, G=1.0, theta=THETA_DEFAULT, tree=None, return_tree=False, parallel=False,
               method="adaptive", quadrupole=False, group_size=None, device="cpu", precision=None, criterion="theta",
               alpha=None, expansion=None):
     """Gravitational potential at the particle positions, shape (N,) (frontend.py:145-293).
@@ -419,27 +437,28 @@ def Potential(pos, m, softening=None, G=1.0, theta=0.7, tree=None, return_tree=F
     valueTestMethod(method)
     _check_device(device)
     prec = _precision(precision, device)
-    if criterion != "theta" and theta == 0.7:
-        theta = 2.0
+    if criterion != "theta" and theta is THETA_DEFAULT:
+        theta = GUARD_THETA
     out, tree, on_dev = _self_fields(pos, m, softening, PTG_POT, G, theta, tree, return_tree, parallel, method,
                                      quadrupole, _group(group_size, prec), prec, None, criterion, alpha, expansion)
     return _ret(_finish(out[:, 0], on_dev), tree, return_tree)
 
 
-def Accel(pos, m, softening=None, G=1.0, theta=0.7, tree=None, return_tree=False, parallel=False,
+def Accel(pos, m, softening=None, G=1.0, theta=THETA_DEFAULT, tree=None, return_tree=False, parallel=False,
           method="adaptive", quadrupole=False, group_size=None, device="cpu", precision=None, criterion="theta",
           alpha=None, expansion=None):
     """Gravitational acceleration at the particle positions, shape (N,3) (frontend.py:421-560).
 
     criterion="learned" / "error" with alpha (float32 only): the learned or analytic per-node error
-    bound instead of the geometric criterion (theta then defaults to 2.0, the guard).
+    bound on top of the geometric criterion, which then acts as a guard at theta (GUARD_THETA = 1.0
+    unless the caller passes a theta).
     expansion (float32, monopole): fold far accepted elements into a third-order expansion about
     each target group's centre when its truncation bound is below expansion x the RMS field."""
     valueTestMethod(method)
     _check_device(device)
     prec = _precision(precision, device)
-    if criterion != "theta" and theta == 0.7:
-        theta = 2.0
+    if criterion != "theta" and theta is THETA_DEFAULT:
+        theta = GUARD_THETA
     out, tree, on_dev = _self_fields(pos, m, softening, PTG_ACC, G, theta, tree, return_tree, parallel, method,
                                      quadrupole, _group(group_size, prec), prec, None, criterion, alpha, expansion)
     return _ret(_finish(out, on_dev), tree, return_tree)

@@ -15,6 +15,7 @@ Training (on the GPU):  python -m pytreegrav_amd.learned --out pytreegrav_amd/op
 """
 
 import argparse
+import itertools
 import json
 import os
 import time
@@ -48,8 +49,14 @@ def init_params(seed=0):
     return p
 
 
+_model_ids = itertools.count(1)
+
+
 class OpeningMLP:
-    """Parameters (float32, PTG_MLP_NPARAM) on the device plus the inference margin (log10 units)."""
+    """Parameters (float32, PTG_MLP_NPARAM) on the device plus the inference margin (log10 units).
+    The parameters are fixed once the model exists (training builds a new model), so caches of
+    opening radii and autotuning decisions key a model by the serial number it gets here rather
+    than by a device address, which the allocator may hand to another model later."""
 
     def __init__(self, params, margin=0.0, meta=None):
         p = torch.as_tensor(np.asarray(params, dtype=np.float32))
@@ -58,6 +65,7 @@ class OpeningMLP:
         self.params = p.to(device()).contiguous()
         self.margin = float(margin)
         self.meta = dict(meta or {})
+        self.uid = next(_model_ids)
 
     @classmethod
     def load(cls, path=DEFAULT_MODEL):
@@ -73,7 +81,7 @@ class OpeningMLP:
             json.dump(d, f)
 
     def key(self):
-        return (self.params.data_ptr(), self.margin)
+        return (self.uid, self.margin)
 
     def forward(self, x, bf16=False):
         """MLP outputs for an (m, 10) float32 feature tensor on the device."""

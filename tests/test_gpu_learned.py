@@ -230,7 +230,7 @@ def test_trained_network_beats_geometric_criterion():
 def test_frontend_criteria_and_self_fields_pipeline():
     """Accel/Potential with criterion='learned'/'error' (the shipped model): accurate against FP64
     direct summation, and equal bit for bit to the same walk issued step by step."""
-    from pytreegrav_amd import Accel, ConstructTree, Potential
+    from pytreegrav_amd import Accel, ConstructTree, Potential, frontend
 
     x, m, h = case("plummer_1e5_h")
     idx = np.random.default_rng(3).choice(len(x), 2000, replace=False)
@@ -245,8 +245,8 @@ def test_frontend_criteria_and_self_fields_pipeline():
         cells = engine.target_groups(x, t.dev.order, 64, keys=t.dev.keys)
         eps = alpha * engine.field_scale(t.dev, cells=cells)
         kw = dict(model=learned.OpeningMLP.load()) if crit == "learned" else {}
-        b = engine.walk(t.dev, x, h, t.dev.order, PTG_ACC, group_size=64, theta=2.0, precision="float32",
-                        criterion=crit, eps=eps, **kw).cpu().numpy()
+        b = engine.walk(t.dev, x, h, t.dev.order, PTG_ACC, group_size=64, theta=frontend.GUARD_THETA,
+                        precision="float32", criterion=crit, eps=eps, **kw).cpu().numpy()
         assert np.array_equal(a, b)
     with pytest.raises(ValueError):
         Accel(x, m, h, device="cpu", criterion="learned", alpha=1e-3)  # float64 walk has no learned form
@@ -279,3 +279,43 @@ def test_radii_from_build_statistics_equal_the_link_walk():
     assert np.all(np.isfinite(a)) and (a > 0).sum() > 1000
     assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-30)) < 1e-4
     assert np.mean(a == b) > 0.99
+
+
+def _disc(n, seed):
+    """A thin exponential disc (scale length 1, height 0.02): a geometry outside the training sets."""
+    rng = np.random.default_rng(seed)
+    r = rng.exponential(1.0, n)
+    phi = rng.uniform(0, 2 * np.pi, n)
+    z = rng.laplace(0.0, 0.02, n)
+    return np.ascontiguousarray(np.stack([r * np.cos(phi), r * np.sin(phi), z], 1)), np.full(n, 1.0 / n)
+
+
+@pytest.mark.parametrize("name", ["uniform", "clustered", "lattice", "disc"])
+def test_learned_criterion_outside_the_training_distributions(name):
+    """The shipped network was trained on Plummer spheres and Gaussian mixtures only.  On other
+    particle sets (uniform cube, the reference's clustered set, a lattice, a thin disc) the learned
+    criterion through the public API, at its default guard, must stay within a bounded error of the
+    FP64 direct sum and within a small factor of the plain theta=0.7 walk's error."""
+    from gpu_cases import case
+
+    from pytreegrav_amd import Accel
+
+    if name == "uniform":
+        rng = np.random.default_rng(5)
+        x, m, h = rng.random((100_000, 3)), np.full(100_000, 1e-5), np.full(100_000, 0.005)
+    elif name == "disc":
+        x, m = _disc(100_000, 6)
+        h = np.full(len(m), 0.005)
+    else:
+        x, m, h = case(name)
+    sel = np.random.default_rng(1).choice(len(x), min(5000, len(x)), replace=False)
+    d = engine.direct(x[sel], h[sel], x, m, h, PTG_ACC).cpu().numpy()
+    a_l = Accel(x, m, h, device="cuda", criterion="learned", alpha=1.6e-3, method="tree")
+    a_g = Accel(x, m, h, device="cuda", theta=0.7, method="tree")
+    e_l, e_g = _rms(a_l[sel], d), _rms(a_g[sel], d)
+    mx_l = np.sqrt(((a_l[sel] - d) ** 2).sum(1)).max() / np.sqrt((d ** 2).sum(1).mean())
+    mx_g = np.sqrt(((a_g[sel] - d) ** 2).sum(1)).max() / np.sqrt((d ** 2).sum(1).mean())
+    print(f"[learned ood] {name}: learned rms {e_l:.3e} max {mx_l:.3e}; theta=0.7 rms {e_g:.3e} max {mx_g:.3e}",
+          flush=True)
+    assert e_l < 4e-3 and e_l < 3 * e_g
+    assert mx_l < 5e-2
