@@ -258,6 +258,23 @@ ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest, const dou
  * PTG_ERR_OVERFLOW if a traversal stack exceeded its spill capacity. */
 ptg_status ptg_walk_check(void *ws, void *stream);
 
+/* ptg_walk without a workspace, for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  The same
+ * kernels and results as ptg_walk; the walk control block and the per-wave spill stacks live in a
+ * static device array of the library (80 MiB + 4 KiB per GPU, held until the process ends: 4096
+ * spill entries per wave for the 5120 waves of a 256-CU grid; measured walks need at most 1344).
+ * Nothing is allocated and nothing synchronises.  err (device int32, required) receives the
+ * walk's flags instead of ptg_walk_check: 0 fine, bit 0 a traversal stack exceeded the spill
+ * capacity, bit 1 a supplied group held more than group_size targets.  Rejected with PTG_ERR_ARG:
+ * group_expansion == 2 (its frontier pool grows with nt), and a device whose resident grid would
+ * leave fewer than 2048 spill entries per wave.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one device (different streams) race. */
+ptg_status ptg_walk_noalloc(const ptg_walk_tree *wt, const ptg_forest *forest, const double *pos, const double *soft,
+                            const int64_t *perm, int targets_sorted, int64_t nt, int fields, int quadrupole,
+                            int group_size, const int32_t *group_offsets, const int32_t *group_count, double theta,
+                            const ptg_criterion *criterion, double G, int exact, double *out, int64_t *counters,
+                            int32_t *err, void *stream);
+
 /* ------------------------------------------------------------------------------------------
  * Learned opening criterion (no reference counterpart; it replaces the geometric test of
  * treewalk.py:22-25 by a per-node opening radius, keeping that test as the guard).  A 10-32-32-1

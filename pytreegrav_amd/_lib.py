@@ -109,6 +109,9 @@ SIGNATURES = {
                             _vp, _vp,
                             _dbl, _c.POINTER(Criterion), _dbl, _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),
     "ptg_walk_check": (_c.c_int, [_vp, _vp]),
+    "ptg_walk_noalloc": (_c.c_int, [_c.POINTER(WalkTree), _F, _vp, _vp, _vp, _c.c_int, _i64, _c.c_int, _c.c_int,
+                                    _c.c_int, _vp, _vp, _dbl, _c.POINTER(Criterion), _dbl, _c.c_int, _vp, _vp, _vp,
+                                    _vp]),
     "ptg_node_features": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _vp, _i64, _dbl, _vp, _vp]),
     "ptg_node_errcurve": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _F, _vp, _i64, _i64, _c.c_int, _dbl, _dbl,
                                      _c.c_int, _vp, _vp]),

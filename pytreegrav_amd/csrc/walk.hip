@@ -20,6 +20,7 @@
 // S's frontier instead of the root, with its own criterion and expansion, and adds S's expansion
 // evaluated at its targets.  The accepted-element set per group is the one-level walk's.
 #include <algorithm>
+#include <mutex>
 
 #include "ptg_common.hpp"
 
@@ -93,7 +94,7 @@ template <typename T> struct WalkArgs {
     int64_t *counters;
     WalkCtl *ctl;
     int *spill;
-    int gcap;  // global spill entries per wave
+    int gcap;  // global spill entries per wave (GCAP, or what the static scratch of ptg_walk_noalloc gives)
 };
 
 // ---- softening kernels (kernel.py:4-75), generic precision ----
@@ -1216,6 +1217,59 @@ extern "C" size_t ptg_walk_shared_workspace_bytes(int64_t nt, int64_t nsys, int 
            ptg_align256((size_t)g.blocks * WALK_WAVES * g.pool_cap * sizeof(int));
 }
 
+// ---- static scratch of ptg_walk_noalloc ----
+// Callers that cannot allocate (code under HIP graph capture, serving loops, harnesses that forbid
+// hipMalloc) walk with this file-scope array as the workspace: the walk control block (per-XCD group
+// counters, overflow flag) and the per-wave global spill stacks.  Every GPU that runs the library
+// holds its own copy until the process ends; every call on a device shares it, so calls that
+// overlap on one device (different streams) race.  Host code never passes the array's name to a
+// kernel (that is a host-side stand-in): it uses the address hipGetSymbolAddress returns, looked up
+// once per device with that device current and kept per device.
+#define NOALLOC_BYTES ((80u << 20) + 4096u)  // 4096 spill entries per wave for 5120 resident waves
+#define NOALLOC_MIN_GCAP 2048  // spill entries per wave the static scratch must leave
+#define NOALLOC_MAX_DEVICES 64
+static __device__ __align__(256) char scratch[NOALLOC_BYTES];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+static ptg_status noalloc_scratch(void **p)
+{
+    int dev = 0;
+    PTG_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+        ptg_set_error("ptg_walk_noalloc: device %d beyond the %d the library tracks", dev, NOALLOC_MAX_DEVICES);
+        return PTG_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            ptg_set_error("ptg_walk_noalloc: hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                          hipGetErrorString(e));
+            return PTG_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = ptr;
+    }
+    *p = g_scratch_ptr[dev];
+    return PTG_OK;
+}
+
+// spill entries per wave the static scratch leaves for `blocks` resident blocks (a multiple of 64,
+// at most GCAP)
+static int noalloc_gcap(int64_t blocks)
+{
+    const int64_t waves = blocks * WALK_WAVES;
+    int64_t per = (int64_t)(NOALLOC_BYTES - ptg_align256(sizeof(WalkCtl))) / (waves * (int64_t)sizeof(int));
+    per = std::min<int64_t>(per, GCAP) & ~(int64_t)63;
+    return (int)per;
+}
+
+__global__ void k_walk_err_out(const WalkCtl *ctl, int32_t *err)
+{
+    if (threadIdx.x == 0) *err = ctl->err;
+}
+
 template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT, bool EXP>
 static ptg_status launch_t(const WalkArgs<T> &a, int64_t max_blocks, hipStream_t s)
 {
@@ -1264,11 +1318,13 @@ static ptg_status launch_walk(const WalkArgs<T> &a, int fields, bool quad, bool 
     return launch_fq<T, 4, EXACT>(a, fields, quad, expand, blocks, s);
 }
 
+// ptg_walk and ptg_walk_noalloc: ws/ws_bytes is the caller's workspace, or nullptr with noalloc set
+// (the static scratch below; err_out then receives the overflow flag)
 static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, const double *pos, const double *soft,
                             const int64_t *perm, int targets_sorted, int64_t nt, int fields, int quadrupole,
                             int group_size, const int32_t *group_offsets, const int32_t *group_count, double theta,
                             const ptg_criterion *criterion, double G, int exact, double *out, int64_t *counters,
-                            void *ws, size_t ws_bytes, void *stream)
+                            void *ws, size_t ws_bytes, bool noalloc, int32_t *err_out, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!wt || nt < 0 || (nt > 0 && (!pos || !soft || !perm || !out))) {
@@ -1343,8 +1399,29 @@ static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, c
     }
     int64_t blocks = walk_grid_blocks();
     int gcap = GCAP;
+    if (noalloc) {
+        if (two_level) {
+            ptg_set_error("ptg_walk_noalloc: the shared supergroup pass (group_expansion 2) needs a frontier pool "
+                          "that grows with the target count; use ptg_walk with a workspace");
+            return PTG_ERR_ARG;
+        }
+        if (!err_out) {
+            ptg_set_error("ptg_walk_noalloc: err (a device int32 that receives the overflow flag) is required");
+            return PTG_ERR_ARG;
+        }
+        ptg_status st = noalloc_scratch(&ws);
+        if (st) return st;
+        ws_bytes = NOALLOC_BYTES;
+        gcap = noalloc_gcap(blocks);
+        if (gcap < NOALLOC_MIN_GCAP) {
+            ptg_set_error("ptg_walk_noalloc: %lld resident waves leave %d spill entries each in the %u-byte static "
+                          "scratch (at least %d needed); use ptg_walk with a workspace",
+                          (long long)(blocks * WALK_WAVES), gcap, NOALLOC_BYTES, NOALLOC_MIN_GCAP);
+            return PTG_ERR_ARG;
+        }
+    }
     if (nt == 0) return PTG_OK;
-    const size_t base_bytes = ptg_walk_workspace_bytes(nt, group_size);
+    const size_t base_bytes = noalloc ? ws_bytes : ptg_walk_workspace_bytes(nt, group_size);
     const int64_t nsys_ws = forest ? forest->nsys : 1;
     const size_t need = base_bytes + (two_level ? ptg_walk_shared_workspace_bytes(nt, nsys_ws, group_size) : 0);
     if (ws_bytes < need || !ws) {
@@ -1398,6 +1475,10 @@ static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, c
         ptg_set_error("ptg_walk: prec must be PTG_F32 or PTG_F64");
         return PTG_ERR_ARG;
     }
+    if (st == PTG_OK && noalloc) {
+        k_walk_err_out<<<1, 64, 0, s>>>(ctl, err_out);
+        PTG_LAUNCH_CHECK();
+    }
     return st;
 }
 
@@ -1408,7 +1489,18 @@ extern "C" ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest
                                int exact, double *out, int64_t *counters, void *ws, size_t ws_bytes, void *stream)
 {
     return walk_impl(wt, forest, pos, soft, perm, targets_sorted, nt, fields, quadrupole, group_size, group_offsets,
-                     group_count, theta, criterion, G, exact, out, counters, ws, ws_bytes, stream);
+                     group_count, theta, criterion, G, exact, out, counters, ws, ws_bytes, false, nullptr, stream);
+}
+
+extern "C" ptg_status ptg_walk_noalloc(const ptg_walk_tree *wt, const ptg_forest *forest, const double *pos,
+                                       const double *soft, const int64_t *perm, int targets_sorted, int64_t nt,
+                                       int fields, int quadrupole, int group_size, const int32_t *group_offsets,
+                                       const int32_t *group_count, double theta, const ptg_criterion *criterion,
+                                       double G, int exact, double *out, int64_t *counters, int32_t *err,
+                                       void *stream)
+{
+    return walk_impl(wt, forest, pos, soft, perm, targets_sorted, nt, fields, quadrupole, group_size, group_offsets,
+                     group_count, theta, criterion, G, exact, out, counters, nullptr, 0, true, err, stream);
 }
 
 // overflow flag of the last walk on this workspace (0 = fine); synchronises the stream
