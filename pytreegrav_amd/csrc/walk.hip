@@ -28,8 +28,8 @@
 #define SCAP 768       // LDS traversal stack entries per wave (>= 512, the most children one step pushes)
 #define SPILL 256      // entries refilled from global memory at a time (<= SCAP - 512)
 #define GCAP 16384     // global spill entries per wave
-#define LFLUSH 64      // a list is evaluated in multiples of LFLUSH entries once it holds LFLUSH (power of 2)
-#define ICAP (LFLUSH + 64)  // interaction-list entries per list and wave (LFLUSH - 1 + one step's 64)
+// a list is evaluated in multiples of LF entries once it holds LF (a power of 2: 64, or 32 for the
+// six-wave variant); a list holds LF - 1 + one step's 64 entries (ICAP)
 #define NXCD 8
 #define CTR_STRIDE 32  // int32 words between per-XCD counters (128 B)
 #define SUPER 8        // groups per supergroup of the shared pass
@@ -39,7 +39,8 @@ template <typename T> struct alignas(4 * sizeof(T)) V4 {
     T x, y, z, w;
 };
 
-template <typename T, bool QUAD> struct WaveLds {
+template <typename T, bool QUAD, int LF> struct WaveLds {
+    static constexpr int ICAP = LF + 64;
     int stack[SCAP];
     V4<T> nodes[ICAP];  // accepted nodes: x y z m
     T nh[sizeof(T) == 8 ? ICAP : 1];  // node softening (FP64 exact path only)
@@ -76,6 +77,7 @@ template <typename T> struct WalkArgs {
     int crit;  // 0: geometric (treewalk.py:22-25); 1: quadrupole error bound; 2: learned
     T eps;     // criterion 1/2: per-node acceleration error tolerance (already divided by 3.5)
     T eps_exp;                 // group expansion: per-element tolerance of the expansion test
+    T exp_rel;                 // group expansion: cap of the truncation bound relative to M / D^2 (0: none)
     const double *eps_scale;   // nullable: eps_exp * eps_scale[system] (one scale per system)
     // shared supergroup pass (two_level != 0; needs goff)
     int two_level;
@@ -278,50 +280,35 @@ __device__ __forceinline__ void interact_node_fast(Acc<T> &a, V4<T> s, const T *
 }
 
 // The M4 spline terms for r < h (kernel.py:4-75) with reciprocals instead of IEEE divisions, for
-// the FP32 fast path: hinv = 1/h, qinv = h/r = h * rinv.  The inner (q <= 1/2) and outer pieces are
-// one polynomial in q, 1/q whose coefficients are selected per lane, so a wave whose lanes straddle
-// q = 1/2 evaluates one piece instead of both branches:
-//   K h^3   = a0 + a1 q + a2 q^2 + a3 q^3 + am3 q^-3
-//   phi h   = b0 + bm1 q^-1 + q^2 (b2 + q (b3 + q (b4 + q b5)))
-//   Q h^5   = c0 + c1 q + cm1 q^-1 + cm5 q^-5
-// (inner: a = 32/3, 0, -38.4, 32, 0; b = -2.8, 0, 16/3, 0, -9.6, 6.4; c = 76.8, -96, 0, 0;
-//  outer: a = 64/3, -48, 38.4, -32/3, -1/15; b = -3.2, 1/15, 32/3, -16, 9.6, -32/15;
-//  c = -76.8, 32, 48, -0.2).  Inner lanes take q^-1 = 0, so tiny q never overflows.
+// the FP32 fast path: hinv = 1/h, qinv = h/r = h * rinv.
 template <typename T> struct SplineFast {
     T K, phi, Q;
 };
 template <typename T, bool P, bool D>
 __device__ __forceinline__ SplineFast<T> spline_fast(T r, T rinv, T h)
 {
-    const T hinv = __builtin_amdgcn_rcpf(h);
-    const T q = r * hinv;
-    const T hinv2 = hinv * hinv;
-    const T hinv3 = hinv2 * hinv;
-    const bool in = q <= T(0.5);
-    const T qinv = in ? T(0) : h * rinv;
-    const T qinv3 = qinv * qinv * qinv;
+    T hinv = __builtin_amdgcn_rcpf(h);
+    T q = r * hinv;
+    T hinv2 = hinv * hinv;
+    T hinv3 = hinv2 * hinv;
     SplineFast<T> o;
-    {
-        const T a0 = in ? T(10.666666666666666666) : T(21.333333333333);
-        const T a1 = in ? T(0) : T(-48.0);
-        const T a2 = in ? T(-38.4) : T(38.4);
-        const T a3 = in ? T(32.0) : T(-10.666666666667);
-        o.K = (((a3 * q + a2) * q + a1) * q + a0 + T(-0.066666666667) * qinv3) * hinv3;
-    }
     o.phi = 0;
-    if (P) {
-        const T b0 = in ? T(-2.8) : T(-3.2);
-        const T b2 = in ? T(5.33333333333333333) : T(10.666666666666666666666);
-        const T b3 = in ? T(0) : T(-16.0);
-        const T b4 = in ? T(-9.6) : T(9.6);
-        const T b5 = in ? T(6.4) : T(-2.1333333333333333333333);
-        o.phi = (q * q * (b2 + q * (b3 + q * (b4 + q * b5))) + b0 + T(0.066666666666666666666) * qinv) * hinv;
-    }
     o.Q = 0;
-    if (D) {
-        const T c0 = in ? T(76.8) : T(-76.8);
-        const T c1 = in ? T(-96.0) : T(32.0);
-        o.Q = (c0 + c1 * q + T(48.0) * qinv - T(0.2) * qinv3 * qinv * qinv) * hinv3 * hinv2;
+    if (q <= T(0.5)) {
+        T q2 = q * q;
+        o.K = (T(10.666666666666666666) + q2 * (T(-38.4) + T(32.0) * q)) * hinv3;
+        if (P) o.phi = (T(-2.8) + q2 * (T(5.33333333333333333) + q2 * (T(6.4) * q - T(9.6)))) * hinv;
+        if (D) o.Q = (T(76.8) - T(96.0) * q) * hinv3 * hinv2;
+    } else {
+        T qinv = h * rinv;
+        T qinv3 = qinv * qinv * qinv;
+        o.K = (T(21.333333333333) - T(48.0) * q + T(38.4) * q * q - T(10.666666666667) * q * q * q -
+               T(0.066666666667) * qinv3) * hinv3;
+        if (P)
+            o.phi = (T(-3.2) + T(0.066666666666666666666) * qinv +
+                     q * q * (T(10.666666666666666666666) + q * (T(-16.0) + q * (T(9.6) - T(2.1333333333333333333333) * q)))) *
+                    hinv;
+        if (D) o.Q = (T(48.0) * qinv - T(76.8) + T(32.0) * q - T(0.2) * qinv3 * qinv * qinv) * hinv3 * hinv2;
     }
     return o;
 }
@@ -415,14 +402,18 @@ __device__ __forceinline__ void exp_add(GroupExp<T> &E, T M, T Dx, T Dy, T Dz, T
     E.yyz += (qyy + p) * Dz;
 }
 
-// the fold test: truncation bound of the expansion about a centre at offset -D, radius rg
+// the fold test: truncation bound of the expansion about a centre at offset -D, radius rg, below
+// the absolute tolerance eps and, when rel > 0, below rel times the element's own field M / D^2
+// (bound / (M / D^2) = x^3 (4 - 3x) / (1 - x)^2 depends on x alone, so rel caps x)
 template <typename T>
-__device__ __forceinline__ bool fold_test(T M, T Dx, T Dy, T Dz, T rg, T eps, T &rinv)
+__device__ __forceinline__ bool fold_test(T M, T Dx, T Dy, T Dz, T rg, T eps, T rel, T &rinv)
 {
     const T D2 = Dx * Dx + Dy * Dy + Dz * Dz;
     rinv = fast_rsqrt(D2);  // D2 == 0 gives x = inf or nan: no fold
     const T x = rg * rinv, omx = T(1) - x;
-    return x < T(1) && M * x * x * x * (T(4) - T(3) * x) < eps * D2 * omx * omx;
+    const T lhs = x * x * x * (T(4) - T(3) * x);
+    const T rhs = omx * omx;
+    return x < T(1) && M * lhs < eps * D2 * rhs && (rel <= T(0) || lhs < rel * rhs);
 }
 
 // first index i in [lo, hi) with a[i] >= v (a ascending)
@@ -557,8 +548,8 @@ __device__ __forceinline__ int small_excl_scan(int v, int &tot)
     return below;
 }
 
-template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT>
-__device__ __forceinline__ void flush_nodes(const WaveLds<T, QUAD> &L, int cnt, Acc<T> (&acc)[TPL],
+template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT, int LF>
+__device__ __forceinline__ void flush_nodes(const WaveLds<T, QUAD, LF> &L, int cnt, Acc<T> (&acc)[TPL],
                                             const T (&tx)[TPL], const T (&ty)[TPL], const T (&tz)[TPL],
                                             const T (&th)[TPL])
 {
@@ -579,8 +570,8 @@ __device__ __forceinline__ void flush_nodes(const WaveLds<T, QUAD> &L, int cnt, 
     }
 }
 
-template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT>
-__device__ __forceinline__ void flush_parts(const WaveLds<T, QUAD> &L, int cnt, Acc<T> (&acc)[TPL],
+template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT, int LF>
+__device__ __forceinline__ void flush_parts(const WaveLds<T, QUAD, LF> &L, int cnt, Acc<T> (&acc)[TPL],
                                             const T (&tx)[TPL], const T (&ty)[TPL], const T (&tz)[TPL],
                                             const T (&th)[TPL])
 {
@@ -597,23 +588,33 @@ __device__ __forceinline__ void flush_parts(const WaveLds<T, QUAD> &L, int cnt, 
     }
 }
 
-// waves per SIMD the FP32 one-target-per-lane walk is compiled for: its traversal is latency-bound
-// (3 -> 4 waves per SIMD: 4.5 -> 3.7 ms on the 1e7-particle walk), so registers are capped to fit
-// WALK_FP32_WAVES waves; the LDS per wave (7.7 KB) fits five 4-wave blocks per CU
-#define WALK_FP32_WAVES 5
-template <typename T, int TPL> struct WalkOcc {
-    static constexpr int value = (sizeof(T) == 4 && TPL == 1) ? WALK_FP32_WAVES : 1;
+// Kernel variants of the FP32 one-target-per-lane walk (ptg_criterion.variant; the autotuner picks
+// one per problem).  The traversal is latency-bound (3 -> 4 waves per SIMD: 4.5 -> 3.7 ms on the
+// 1e7-particle walk), so registers are capped to fit more waves:
+//   0: 5 waves per SIMD, lists evaluated at 64 entries (7.7 KB of LDS per wave, five 4-wave blocks per CU)
+//   1: 4 waves per SIMD, lists at 64 (more registers per lane)
+//   2: 6 waves per SIMD, lists at 32 (6.4 KB of LDS per wave, six blocks per CU)
+// Every variant evaluates the same terms; variants 0 and 1 in the same order (bit-identical results),
+// variant 2's shorter lists interleave the node and particle list evaluations differently, so its
+// FP32 sums differ at the rounding level.
+#define WALK_NVARIANTS 3
+template <typename T, int TPL, int VAR> struct WalkOcc {
+    static constexpr bool fast = sizeof(T) == 4 && TPL == 1;
+    static constexpr int waves = !fast ? 1 : VAR == 1 ? 4 : VAR == 2 ? 6 : 5;
+    static constexpr int lf = fast && VAR == 2 ? 32 : 64;
+    static constexpr int blocks_per_cu = fast && VAR == 2 ? 6 : 5;
 };
 
-template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT, bool EXP>
-__global__ __launch_bounds__(64 * WALK_WAVES) __attribute__((amdgpu_waves_per_eu(WalkOcc<T, TPL>::value)))
+template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT, bool EXP, int VAR>
+__global__ __launch_bounds__(64 * WALK_WAVES) __attribute__((amdgpu_waves_per_eu(WalkOcc<T, TPL, VAR>::waves)))
 void k_walk(WalkArgs<T> A)
 {
+    constexpr int LF = WalkOcc<T, TPL, VAR>::lf;
     static_assert(!EXP || (sizeof(T) == 4 && !QUAD && !EXACT && !NW<FIELDS>::TID),
                   "the group expansion is an FP32 monopole potential/acceleration path");
-    __shared__ WaveLds<T, QUAD> lds[WALK_WAVES];
+    __shared__ WaveLds<T, QUAD, LF> lds[WALK_WAVES];
     const int wave = threadIdx.x >> 6, lane = ptg_lane();
-    WaveLds<T, QUAD> &L = lds[wave];
+    WaveLds<T, QUAD, LF> &L = lds[wave];
     const int64_t gwave = (int64_t)blockIdx.x * WALK_WAVES + wave;
     int *spill = A.spill + gwave * (int64_t)A.gcap;
     const int xcd = blockIdx.x % NXCD;
@@ -807,7 +808,7 @@ void k_walk(WalkArgs<T> A)
                         // so D >= r_min > 0; fold if the truncation bound is within the tolerance
                         const T Dx = X.x - cx, Dy = X.y - cy, Dz = X.z - cz;
                         T rinv;
-                        if (fold_test(X.w, Dx, Dy, Dz, rg, eps_x, rinv)) {
+                        if (fold_test(X.w, Dx, Dy, Dz, rg, eps_x, A.exp_rel, rinv)) {
                             exp_add(E, X.w, Dx, Dy, Dz, rinv);
                             acc_node = false;
                             n_exp++;
@@ -863,9 +864,9 @@ void k_walk(WalkArgs<T> A)
                 sp += tot;
                 __builtin_amdgcn_wave_barrier();
 
-                if (nn >= LFLUSH) {
-                    const int c = nn & ~(LFLUSH - 1);  // c >= LFLUSH > nn - c
-                    flush_nodes<T, TPL, FIELDS, QUAD, EXACT>(L, c, acc, tx, ty, tz, th);
+                if (nn >= LF) {
+                    const int c = nn & ~(LF - 1);  // c >= LF > nn - c
+                    flush_nodes<T, TPL, FIELDS, QUAD, EXACT, LF>(L, c, acc, tx, ty, tz, th);
                     __builtin_amdgcn_wave_barrier();
                     if (lane < nn - c) {  // reads [c, nn) and writes [0, nn - c): disjoint
                         L.nodes[lane] = L.nodes[c + lane];
@@ -876,9 +877,9 @@ void k_walk(WalkArgs<T> A)
                     __builtin_amdgcn_wave_barrier();
                     nn -= c;
                 }
-                if (np >= LFLUSH) {
-                    const int c = np & ~(LFLUSH - 1);
-                    flush_parts<T, TPL, FIELDS, QUAD, EXACT>(L, c, acc, tx, ty, tz, th);
+                if (np >= LF) {
+                    const int c = np & ~(LF - 1);
+                    flush_parts<T, TPL, FIELDS, QUAD, EXACT, LF>(L, c, acc, tx, ty, tz, th);
                     __builtin_amdgcn_wave_barrier();
                     if (lane < np - c) {  // disjoint ranges, as above
                         L.parts[lane] = L.parts[c + lane];
@@ -892,8 +893,8 @@ void k_walk(WalkArgs<T> A)
                 if (lane == 0) atomicOr(&A.ctl->err, 1);
                 continue;
             }
-            flush_nodes<T, TPL, FIELDS, QUAD, EXACT>(L, nn, acc, tx, ty, tz, th);
-            flush_parts<T, TPL, FIELDS, QUAD, EXACT>(L, np, acc, tx, ty, tz, th);
+            flush_nodes<T, TPL, FIELDS, QUAD, EXACT, LF>(L, nn, acc, tx, ty, tz, th);
+            flush_parts<T, TPL, FIELDS, QUAD, EXACT, LF>(L, np, acc, tx, ty, tz, th);
             __builtin_amdgcn_wave_barrier();
             if constexpr (EXP) {
                 exp_wave_sum(E);
@@ -1082,7 +1083,7 @@ __global__ __launch_bounds__(64 * WALK_WAVES) void k_walk_super(WalkArgs<float> 
                 if (acc) {
                     const T Dx = X.x - cx, Dy = X.y - cy, Dz = X.z - cz;
                     T rinv;
-                    if (fold_test(X.w, Dx, Dy, Dz, rg, eps_x, rinv)) {
+                    if (fold_test(X.w, Dx, Dy, Dz, rg, eps_x, A.exp_rel, rinv)) {
                         exp_add(E, X.w, Dx, Dy, Dz, rinv);
                         fold = true;
                         n_exp++;
@@ -1179,10 +1180,15 @@ static int num_cus()
     return g_num_cus;
 }
 
-#define WALK_MAX_BLOCKS_PER_CU 5
+#define WALK_MAX_BLOCKS_PER_CU 6  // the most any variant runs (variant 2)
 
-// persistent grid: resident blocks per CU from the occupancy API, capped by the spill allocation
-static int64_t walk_grid_blocks() { return (int64_t)num_cus() * WALK_MAX_BLOCKS_PER_CU; }
+// persistent grid: resident blocks per CU from the occupancy API, capped by the variant's plan and
+// by the spill allocation
+static int64_t walk_grid_blocks(int variant = -1)
+{
+    const int per = variant == 2 ? 6 : variant < 0 ? WALK_MAX_BLOCKS_PER_CU : 5;
+    return (int64_t)num_cus() * per;
+}
 
 extern "C" size_t ptg_walk_workspace_bytes(int64_t nt, int group_size)
 {
@@ -1270,32 +1276,50 @@ __global__ void k_walk_err_out(const WalkCtl *ctl, int32_t *err)
     if (threadIdx.x == 0) *err = ctl->err;
 }
 
-template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT, bool EXP>
+template <typename T, int TPL, int FIELDS, bool QUAD, bool EXACT, bool EXP, int VAR = 0>
 static ptg_status launch_t(const WalkArgs<T> &a, int64_t max_blocks, hipStream_t s)
 {
     static int per_cu = 0;
     if (per_cu == 0) {
         int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_walk<T, TPL, FIELDS, QUAD, EXACT, EXP>, 64 * WALK_WAVES,
-                                                         0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_walk<T, TPL, FIELDS, QUAD, EXACT, EXP, VAR>,
+                                                         64 * WALK_WAVES, 0) != hipSuccess ||
             nb < 1)
             nb = 1;
-        per_cu = std::min(nb, WALK_MAX_BLOCKS_PER_CU);
+        per_cu = std::min(nb, WalkOcc<T, TPL, VAR>::blocks_per_cu);
     }
     int64_t blocks = std::min<int64_t>(max_blocks, (int64_t)num_cus() * per_cu);
-    k_walk<T, TPL, FIELDS, QUAD, EXACT, EXP><<<(unsigned)blocks, 64 * WALK_WAVES, 0, s>>>(a);
+    k_walk<T, TPL, FIELDS, QUAD, EXACT, EXP, VAR><<<(unsigned)blocks, 64 * WALK_WAVES, 0, s>>>(a);
     PTG_LAUNCH_CHECK();
     return PTG_OK;
 }
 
+// the monopole potential/acceleration kernels of the one-target-per-lane FP32 walk come in
+// WALK_NVARIANTS variants; every other kernel in variant 0
+template <typename T, int TPL, int FIELDS, bool EXP>
+static ptg_status launch_var(const WalkArgs<T> &a, int variant, int64_t blocks, hipStream_t s)
+{
+    if constexpr (sizeof(T) == 4 && TPL == 1) {
+        if (variant == 1) return launch_t<T, TPL, FIELDS, false, false, EXP, 1>(a, blocks, s);
+        if (variant == 2) return launch_t<T, TPL, FIELDS, false, false, EXP, 2>(a, blocks, s);
+    }
+    return launch_t<T, TPL, FIELDS, false, false, EXP, 0>(a, blocks, s);
+}
+
 template <typename T, int TPL, bool EXACT>
-static ptg_status launch_fq(const WalkArgs<T> &a, int fields, bool quad, bool expand, int64_t blocks, hipStream_t s)
+static ptg_status launch_fq(const WalkArgs<T> &a, int fields, bool quad, bool expand, int variant, int64_t blocks,
+                            hipStream_t s)
 {
     if constexpr (sizeof(T) == 4 && !EXACT) {
         if (expand) {  // checked by the caller: monopole, potential and/or acceleration
-            if (fields == PTG_POT) return launch_t<T, TPL, PTG_POT, false, false, true>(a, blocks, s);
-            if (fields == PTG_ACC) return launch_t<T, TPL, PTG_ACC, false, false, true>(a, blocks, s);
-            return launch_t<T, TPL, PTG_POT | PTG_ACC, false, false, true>(a, blocks, s);
+            if (fields == PTG_POT) return launch_var<T, TPL, PTG_POT, true>(a, variant, blocks, s);
+            if (fields == PTG_ACC) return launch_var<T, TPL, PTG_ACC, true>(a, variant, blocks, s);
+            return launch_var<T, TPL, PTG_POT | PTG_ACC, true>(a, variant, blocks, s);
+        }
+        if (!quad) {
+            if (fields == PTG_POT) return launch_var<T, TPL, PTG_POT, false>(a, variant, blocks, s);
+            if (fields == PTG_ACC) return launch_var<T, TPL, PTG_ACC, false>(a, variant, blocks, s);
+            if (fields == (PTG_POT | PTG_ACC)) return launch_var<T, TPL, PTG_POT | PTG_ACC, false>(a, variant, blocks, s);
         }
     }
 #define PTG_F(F)                                                              \
@@ -1311,11 +1335,12 @@ static ptg_status launch_fq(const WalkArgs<T> &a, int fields, bool quad, bool ex
 }
 
 template <typename T, bool EXACT>
-static ptg_status launch_walk(const WalkArgs<T> &a, int fields, bool quad, bool expand, int64_t blocks, hipStream_t s)
+static ptg_status launch_walk(const WalkArgs<T> &a, int fields, bool quad, bool expand, int variant, int64_t blocks,
+                              hipStream_t s)
 {
-    if (a.gs <= 64) return launch_fq<T, 1, EXACT>(a, fields, quad, expand, blocks, s);
-    if (a.gs <= 128) return launch_fq<T, 2, EXACT>(a, fields, quad, expand, blocks, s);
-    return launch_fq<T, 4, EXACT>(a, fields, quad, expand, blocks, s);
+    if (a.gs <= 64) return launch_fq<T, 1, EXACT>(a, fields, quad, expand, variant, blocks, s);
+    if (a.gs <= 128) return launch_fq<T, 2, EXACT>(a, fields, quad, expand, variant, blocks, s);
+    return launch_fq<T, 4, EXACT>(a, fields, quad, expand, variant, blocks, s);
 }
 
 // ptg_walk and ptg_walk_noalloc: ws/ws_bytes is the caller's workspace, or nullptr with noalloc set
@@ -1385,6 +1410,10 @@ static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, c
             ptg_set_error("ptg_walk: the group expansion needs expansion_eps > 0");
             return PTG_ERR_ARG;
         }
+        if (!(criterion->expansion_rel >= 0.0f)) {
+            ptg_set_error("ptg_walk: expansion_rel must be >= 0 (0 = no relative cap)");
+            return PTG_ERR_ARG;
+        }
     }
     const double eps_exp = expand ? criterion->expansion_eps : 0.0;
     const double *eps_scale = expand ? criterion->eps_scale : nullptr;
@@ -1397,7 +1426,12 @@ static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, c
         ptg_set_error("ptg_walk: the shared supergroup pass needs group offsets (ptg_target_groups)");
         return PTG_ERR_ARG;
     }
-    int64_t blocks = walk_grid_blocks();
+    const int variant = criterion ? criterion->variant : 0;
+    if (variant < 0 || variant >= WALK_NVARIANTS) {
+        ptg_set_error("ptg_walk: variant must be in [0, %d), got %d", WALK_NVARIANTS, variant);
+        return PTG_ERR_ARG;
+    }
+    int64_t blocks = walk_grid_blocks(variant);
     int gcap = GCAP;
     if (noalloc) {
         if (two_level) {
@@ -1452,7 +1486,8 @@ static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, c
     if (wt->prec == PTG_F32) {
         WalkArgs<float> a{(const V4<float> *)wt->xyzm, (const V4<float> *)wt->aux, (const int2 *)wt->kids,
                           (const float *)wt->quad, pos, soft, perm, targets_sorted != 0, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
-                          group_size, (float)theta, (float)(1.0 / theta), crit, (float)eps, (float)eps_exp, eps_scale,
+                          group_size, (float)theta, (float)(1.0 / theta), crit, (float)eps, (float)eps_exp,
+                          expand ? criterion->expansion_rel : 0.0f, eps_scale,
                           two_level ? 1 : 0, sexp, srec, spool, sg.pool_cap, G, wt->origin[0], wt->origin[1], wt->origin[2],
                           fn, fo, fc, out, counters, ctl, spill, gcap};
         if (exact) {
@@ -1463,14 +1498,14 @@ static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, c
             k_walk_super<<<(unsigned)sg.blocks, 64 * WALK_WAVES, 0, s>>>(a);
             PTG_LAUNCH_CHECK();
         }
-        st = launch_walk<float, false>(a, fields, quadrupole != 0, expand, blocks, s);
+        st = launch_walk<float, false>(a, fields, quadrupole != 0, expand, variant, blocks, s);
     } else if (wt->prec == PTG_F64) {
         WalkArgs<double> a{(const V4<double> *)wt->xyzm, (const V4<double> *)wt->aux, (const int2 *)wt->kids,
                            (const double *)wt->quad, pos, soft, perm, targets_sorted != 0, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
-                           group_size, theta, 1.0 / theta, crit, eps, 0.0, nullptr, 0, nullptr, nullptr, nullptr, 0, G,
+                           group_size, theta, 1.0 / theta, crit, eps, 0.0, 0.0, nullptr, 0, nullptr, nullptr, nullptr, 0, G,
                            wt->origin[0], wt->origin[1], wt->origin[2],
                            fn, fo, fc, out, counters, ctl, spill, gcap};
-        st = launch_walk<double, true>(a, fields, quadrupole != 0, false, blocks, s);
+        st = launch_walk<double, true>(a, fields, quadrupole != 0, false, 0, blocks, s);
     } else {
         ptg_set_error("ptg_walk: prec must be PTG_F32 or PTG_F64");
         return PTG_ERR_ARG;

@@ -15,7 +15,8 @@ C3  Plummer N=1e7 sources, 1e6 targets uniform in [-2, 2]^3: AccelTarget / Poten
     target walk equals the oracle on Morton slices.
 C4  512 Plummer clusters (N log-uniform in [1e3, 1e5]) in one forest: per-system invariants, 16
     systems against the FP64 direct sum, three systems bit-equal to their own single-system walk,
-    and HIP-graph replays equal to the eager result.
+    and HIP-graph replays (the walk stage, and the whole step from the tree build on) equal to the
+    eager result.
 C5  Learned criterion + group expansion against the plain theta=0.7 FP32 walk at N=1e7 Plummer,
     both judged by the FP64 direct sum on 1e4 targets.
 
@@ -225,8 +226,9 @@ def test_c3_targets_1e7_sources_1e6_targets_fp32_auto():
     log = engine.tuning_log()
     assert len(log) >= 1
     for rec in log.values():
-        assert rec["choice"] in engine.TUNE_SIZES and set(rec["ms"]) == set(engine.TUNE_SIZES)
-    print(f"[C3] autotune {[(r['choice'], r['ms']) for r in log.values()]}", flush=True)
+        assert rec["choice"] in engine.TUNE_SIZES and 0 <= rec["variant"] < len(engine.WALK_VARIANTS)
+        assert {int(k.split("/")[0]) for k in rec["ms"]} == set(engine.TUNE_SIZES)
+    print(f"[C3] autotune {[(r['choice'], r['variant_name'], r['ms']) for r in log.values()]}", flush=True)
     rng = np.random.default_rng(33)
     sel = np.sort(rng.choice(len(xt), 10_000, replace=False))
     d = _direct(xt[sel], np.zeros(len(sel)), x, m, h, PTG_POT | PTG_ACC)
@@ -319,14 +321,20 @@ def test_c4_512_clusters_forest():
                         precision="float32").cpu().numpy()
         assert np.array_equal(fp[off[s]:off[s + 1]], o[:, 0]), s
         assert np.array_equal(fa[off[s]:off[s + 1]], o[:, 1:4]), s
-    # graph replays equal the eager result, with eager work between them
+    # graph replays equal the eager result, with eager work between them: the walk stage alone, and
+    # the whole step (deferred build, groups, field scales, radii, walk) in one graph
     ref = torch.cat([r["potential"][:, None], r["accel"]], 1).clone()
     replay, out = F.capture(potential=True, accel=True, **kw)
+    replay_step, out_step = F.capture_step(potential=True, accel=True, **kw)
     for k in range(3):
         out.zero_()
         replay()
         torch.cuda.synchronize()
         assert torch.equal(out, ref), k
+        out_step.zero_()
+        replay_step()
+        torch.cuda.synchronize()
+        assert torch.equal(out_step, ref), k
         P.evaluate(potential=True, accel=True)
 
 

@@ -231,11 +231,21 @@ def test_torch_inputs_stay_on_device():
 
 
 def test_adaptive_dispatch_thresholds():
-    """frontend.py:210-220, 364-368: tree above 1000 (serial) particles or a supplied tree."""
-    from pytreegrav_amd import Potential, PotentialTarget
+    """frontend.py:210-220, 364-368: float64 walks the tree above 1000 (serial) / 4000 (parallel)
+    particles or with a supplied tree, as the reference does; float32 above the crossover measured on
+    the GPU (FP32_TREE_MIN_N)."""
+    from pytreegrav_amd import Accel, Potential, PotentialTarget, frontend
 
     x, m, h = cloud(800)
     assert np.array_equal(Potential(x, m, h), Potential(x, m, h, method="bruteforce"))
+    x, m, h = cloud(2000)
+    assert np.array_equal(Accel(x, m, h), Accel(x, m, h, method="tree"))  # float64: the reference's 1000
+    assert np.array_equal(Accel(x, m, h, parallel=True), Accel(x, m, h, method="bruteforce"))  # its 4000
+    assert 2000 < frontend.FP32_TREE_MIN_N
+    assert np.array_equal(Accel(x, m, h, device="cuda"), Accel(x, m, h, device="cuda", method="bruteforce"))
+    x, m, h = cloud(frontend.FP32_TREE_MIN_N + 1)
+    assert np.array_equal(Accel(x, m, h, device="cuda"), Accel(x, m, h, device="cuda", method="tree"))
+    x, m, h = cloud(800)
     xt = x[:100]
     assert np.array_equal(PotentialTarget(xt, x, m, h[:100], h), PotentialTarget(xt, x, m, h[:100], h,
                                                                                   method="bruteforce"))
