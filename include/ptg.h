@@ -87,6 +87,10 @@ typedef struct {
                                * largest child mass, sum m d^2, sum m d^3 and the largest d over its
                                * children (d = child centre to the node's centre of mass); the learned
                                * opening radii read these instead of walking each node's links */
+    int32_t *node_count;      /* device int32 or NULL.  Set for a tree from ptg_build_deferred: the node
+                               * arrays are sized for a capacity, the build writes the number of nodes
+                               * here (0 if the tree did not fit), and every call that takes nnodes with
+                               * this tree treats nnodes as the capacity and reads the count here */
 } ptg_tree_arrays;
 
 /* Forest: several independent systems (no reference counterpart; the reference builds one Octree
@@ -167,6 +171,42 @@ ptg_status ptg_build_tree(const double *pos, const double *m, const double *h, c
                           void *node_ws, size_t node_ws_bytes, const ptg_tree_arrays *tree,
                           const ptg_walk_tree *pack, void *particles_ready, void *stream);
 
+/* ------------------------------------------------------------------------------------------
+ * Deferred build (no reference counterpart): ptg_build_sort + ptg_build_tree in one call that
+ * never synchronises with the host, so a caller can capture it (with the groups and the walk) in a
+ * HIP graph and replay it on new positions of the same particle count.  The node arrays (tree,
+ * pack) are sized for n + nnodes_cap elements and the moments run for levels 0..level_cap; the
+ * build writes the actual node count to *tree->node_count and everything else to *counts (device).
+ * If the tree needs more than nnodes_cap nodes or level_cap levels, or more than
+ * PTG_DEFERRED_COLLISION_CAP particles share 63-bit keys, *tree->node_count is 0, counts->overflow
+ * says why and the node arrays are left unset (rebuild with larger capacities, or with
+ * ptg_build_sort + ptg_build_tree).  Forest offsets are not validated here (ptg_build_sort does).
+ * Workspaces: ptg_build_workspace_bytes(n, nsys) and ptg_node_workspace_bytes(n, nnodes_cap,
+ * level_cap).  Results equal those of the two-call build bit for bit.
+ * ------------------------------------------------------------------------------------------ */
+#define PTG_DEFERRED_COLLISION_CAP 8192
+#define PTG_OVERFLOW_NODES 1
+#define PTG_OVERFLOW_LEVELS 2
+#define PTG_OVERFLOW_COLLISIONS 4
+#define PTG_OVERFLOW_SORT 8
+typedef struct {
+    int32_t nnodes;         /* internal nodes including the roots (as computed, even when over capacity) */
+    int32_t max_level;      /* deepest node level */
+    int32_t has_collisions; /* some particles shared a 63-bit key */
+    int32_t has_coincident; /* bucket chains of bit-identical positions */
+    int32_t nonfinite;      /* a position, mass or softening was NaN or infinite (the tree is garbage) */
+    int32_t overflow;       /* PTG_OVERFLOW_* bits; 0 when the tree is complete */
+    int32_t pad[2];
+    double center[3];       /* root cube centre (system 0 of a forest) */
+    double size;            /* root cube side */
+} ptg_build_counts;
+
+ptg_status ptg_build_deferred(const double *pos, const double *m, const double *h, int64_t n,
+                              const ptg_forest *forest, int quadrupole, int compute_moments, int64_t nnodes_cap,
+                              int level_cap, void *ws, size_t ws_bytes, void *node_ws, size_t node_ws_bytes,
+                              const ptg_tree_arrays *tree, const ptg_walk_tree *pack, ptg_build_counts *counts,
+                              void *particles_ready, void *stream);
+
 /* Morton permutation of arbitrary points: replaces grouped_treewalk._morton_order (:269-275).
  * Uses a ptg_build_workspace_bytes(n, 1) workspace; asynchronous. */
 ptg_status ptg_morton_order(const double *pos, int64_t n, int64_t *order, void *ws, size_t ws_bytes, void *stream);
@@ -234,12 +274,22 @@ ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
  * about the supergroup's centre is within the tolerance, opens the nodes every group opens, and lists
  * the rest as a frontier from which each of its groups then walks; the per-group accepted-element
  * sets are those of the one-level walk. */
+/* expansion_rel (group expansion; 0 = off): an element is folded only if its truncation bound is
+ * also below expansion_rel x (M / D^2), its own field at the group centre, i.e. x = R / D is capped
+ * by x^3 (4 - 3x) / (1 - x)^2 < expansion_rel whatever the absolute tolerance allows.
+ * variant (no reference counterpart): the kernel variant of the FP32 one-target-per-lane walk
+ * (group_size <= 64) for monopole potential/acceleration, 0 = 5 waves per SIMD with interaction lists
+ * evaluated at 64 entries, 1 = 4 waves per SIMD, 2 = 6 waves per SIMD with lists at 32 (less LDS per
+ * wave); variant 1 gives results bit-identical to 0, variant 2 sums the same terms in another order
+ * (FP32 rounding-level differences); the autotuner picks one.  Other walks ignore it. */
 typedef struct {
     int32_t criterion;
     int32_t group_expansion;
     double eps;
     double expansion_eps;
     const double *eps_scale;
+    int32_t variant;
+    float expansion_rel;
 } ptg_criterion;
 
 size_t ptg_walk_workspace_bytes(int64_t nt, int group_size);
@@ -255,7 +305,8 @@ ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest, const dou
                     const int32_t *group_count, double theta, const ptg_criterion *criterion, double G,
                     int exact, double *out, int64_t *counters, void *ws, size_t ws_bytes, void *stream);
 /* Reads the overflow flag of the last walk on this workspace (synchronises the stream):
- * PTG_ERR_OVERFLOW if a traversal stack exceeded its spill capacity. */
+ * PTG_ERR_OVERFLOW if a traversal stack exceeded its spill capacity, PTG_ERR_ARG if a supplied
+ * group held more than group_size targets. */
 ptg_status ptg_walk_check(void *ws, void *stream);
 
 /* ptg_walk without a workspace, for callers that cannot allocate (code under HIP graph capture,
@@ -356,6 +407,12 @@ ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, int64_t nnode
  * err: 1 corrupt links, 2 a traversal stack overflow.  counters (nullable, device int64[2],
  * accumulated): [0] element-target pairs evaluated, [1] elements visited by the group traversals. */
 size_t ptg_column_workspace_bytes(int64_t group_size);
+/* Per-target ray grids for randomize_rays (treewalk.py:1061-1078, misc.random_rotation misc.py:6-54):
+ * out (nt, nrays, 3) = rays (nrays, 3) @ R(first_seed + k) for target k, R(s) the rotation drawn from
+ * the Mersenne-Twister stream seeded with integer s (x axis, then a second uniform direction made
+ * orthonormal to it, z their cross product), as numpy's seeded generator draws it. */
+ptg_status ptg_random_rotations(int64_t first_seed, int64_t nt, const double *rays, int64_t nrays, double *out,
+                                void *stream);
 ptg_status ptg_column_binned(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_walk_tree *wt,
                              const double *targets, int64_t nt, int64_t group_size, double theta, double *out,
                              int32_t *err, void *ws, size_t ws_bytes, int64_t *counters, void *stream);

@@ -50,6 +50,7 @@ class TreeArrays(_c.Structure):
         ("slot", _vp),
         ("keys", _vp),
         ("node_stats", _vp),
+        ("node_count", _vp),
     ]
 
 
@@ -66,13 +67,31 @@ class WalkTree(_c.Structure):
     ]
 
 
+class BuildCounts(_c.Structure):
+    _fields_ = [
+        ("nnodes", _i32),
+        ("max_level", _i32),
+        ("has_collisions", _i32),
+        ("has_coincident", _i32),
+        ("nonfinite", _i32),
+        ("overflow", _i32),
+        ("pad", _i32 * 2),
+        ("center", _dbl * 3),
+        ("size", _dbl),
+    ]
+
+
+PTG_OVERFLOW_NODES, PTG_OVERFLOW_LEVELS, PTG_OVERFLOW_COLLISIONS = 1, 2, 4
+PTG_DEFERRED_COLLISION_CAP = 8192
+
+
 class Forest(_c.Structure):
     _fields_ = [("nsys", _i64), ("offsets", _vp), ("roots", _vp), ("cubes", _vp)]
 
 
 class Criterion(_c.Structure):
     _fields_ = [("criterion", _i32), ("group_expansion", _i32), ("eps", _dbl), ("expansion_eps", _dbl),
-                ("eps_scale", _c.c_void_p)]
+                ("eps_scale", _c.c_void_p), ("variant", _i32), ("expansion_rel", _c.c_float)]
 
 
 PTG_CRIT_THETA, PTG_CRIT_QERR, PTG_CRIT_RADIUS = 0, 1, 2
@@ -97,6 +116,9 @@ SIGNATURES = {
     "ptg_build_tree": (_c.c_int, [_vp, _vp, _vp, _c.POINTER(BuildInfo), _F, _c.c_int, _c.c_int, _vp, _c.c_size_t,
                                   _vp, _c.c_size_t, _c.POINTER(TreeArrays), _c.POINTER(WalkTree), _vp, _vp]),
     "ptg_morton_order": (_c.c_int, [_vp, _i64, _vp, _vp, _c.c_size_t, _vp]),
+    "ptg_build_deferred": (_c.c_int, [_vp, _vp, _vp, _i64, _F, _c.c_int, _c.c_int, _i64, _c.c_int, _vp, _c.c_size_t,
+                                      _vp, _c.c_size_t, _c.POINTER(TreeArrays), _c.POINTER(WalkTree), _vp, _vp,
+                                      _vp]),
     "ptg_pack_walk": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _F, _c.POINTER(WalkTree), _vp]),
     "ptg_walk_workspace_bytes": (_c.c_size_t, [_i64, _c.c_int]),
     "ptg_column_rays": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _c.POINTER(WalkTree), _vp, _i64, _vp, _i64,
@@ -104,6 +126,7 @@ SIGNATURES = {
     "ptg_column_binned": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _c.POINTER(WalkTree), _vp, _i64, _i64, _dbl,
                                      _vp, _vp, _vp, _c.c_size_t, _vp, _vp]),
     "ptg_column_workspace_bytes": (_c.c_size_t, [_i64]),
+    "ptg_random_rotations": (_c.c_int, [_i64, _i64, _vp, _i64, _vp, _vp]),
     "ptg_walk_shared_workspace_bytes": (_c.c_size_t, [_i64, _i64, _c.c_int]),
     "ptg_walk": (_c.c_int, [_c.POINTER(WalkTree), _F, _vp, _vp, _vp, _c.c_int, _i64, _c.c_int, _c.c_int, _c.c_int,
                             _vp, _vp,

@@ -605,7 +605,7 @@ __global__ __launch_bounds__(256) void k_open_radii(const ptg_tree_arrays t, int
         __syncthreads();
     }
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nnodes) return;
+    if (a >= ptg_node_count(nnodes, t.node_count)) return;  // a deferred tree may hold fewer nodes
     const int64_t e = n + a;
     const int64_t sl = t.slot[e];
     if (sl < 0 || sl >= nslots) return;

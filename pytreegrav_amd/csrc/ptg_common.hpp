@@ -124,3 +124,13 @@ __device__ __forceinline__ int ptg_common_levels(uint64_t a, uint64_t b)
     int lcp = __clzll((long long)x) - 1;  // bit 63 is zero in both keys
     return lcp / 3;
 }
+
+// node count of a tree: nnodes as passed or, for a tree from ptg_build_deferred
+// (ptg_tree_arrays.node_count set), the count its build wrote there (0 when the tree did not fit),
+// never more than the capacity nnodes
+__device__ __forceinline__ int64_t ptg_node_count(int64_t nnodes, const int32_t *count)
+{
+    if (!count) return nnodes;
+    const int64_t c = *count;
+    return c < nnodes ? c : nnodes;
+}

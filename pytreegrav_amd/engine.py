@@ -76,7 +76,7 @@ class DeviceTree:
     system gets its own cube, keys and tree, exactly as if it were built alone; particles stay
     system-major in Morton order and the root of system s sits at walk slot s."""
 
-    def __init__(self, pos, m, h, quadrupole=False, compute_moments=True, offsets=None, prepack=True):
+    def __init__(self, pos, m, h, quadrupole=False, compute_moments=True, offsets=None, prepack=True, capacity=None):
         L = _lib.load()
         dev = device()
         pos = to_dev(pos, (-1, 3))
@@ -86,6 +86,9 @@ class DeviceTree:
         s = stream()
         self.forest = None
         self.nsys = 1
+        self.deferred = capacity is not None
+        if self.deferred:
+            return self._build_deferred(L, dev, pos, m, h, n, quadrupole, compute_moments, offsets, prepack, capacity)
         if offsets is not None:
             if isinstance(offsets, torch.Tensor):
                 off = offsets.to(device=dev, dtype=torch.int64).contiguous()
@@ -154,6 +157,93 @@ class DeviceTree:
         # ws / nws go back to torch's stream-ordered caching allocator here; later allocations on
         # this stream are ordered after the build kernels that use them
 
+    def _build_deferred(self, L, dev, pos, m, h, n, quadrupole, compute_moments, offsets, prepack, capacity):
+        """ptg_build_deferred: no host synchronisation (capturable).  capacity = (node capacity, level
+        capacity); the node arrays and the walk copy are sized for n + node capacity, the build writes
+        the node count to self.node_count (device) and its counts to self.counts_dev.  A single tree is
+        built as a one-system forest, so its cube stays on the device.  self.nnodes is the capacity;
+        check() reads the counts and raises if the tree did not fit."""
+        nn_cap, level_cap = int(capacity[0]), int(capacity[1])
+        if offsets is None:
+            offsets = [0, n]
+        off = offsets.to(device=dev, dtype=torch.int64).contiguous() if isinstance(offsets, torch.Tensor) else \
+            torch.from_numpy(np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))).to(dev)
+        nsys = off.numel() - 1
+        self.nsys = nsys
+        self.sys_offsets = off
+        self.roots = torch.empty(nsys, dtype=torch.int32, device=dev)
+        self.cubes = torch.empty((nsys, 4), dtype=torch.float64, device=dev)
+        self.forest = _lib.Forest(nsys, ptr(off).value, ptr(self.roots).value, ptr(self.cubes).value)
+        ne = n + nn_cap
+        f64 = dict(dtype=torch.float64, device=dev)
+        i32 = dict(dtype=torch.int32, device=dev)
+        self.n, self.nnodes, self.ne = n, nn_cap, ne
+        self.level_cap = level_cap
+        self.quadrupole = bool(quadrupole)
+        self.coordinates = torch.empty((ne, 3), **f64)
+        self.masses = torch.empty(ne, **f64)
+        self.softenings = torch.empty(ne, **f64)
+        self.sizes = torch.empty(ne, **f64)
+        self.deltas = torch.empty(ne, **f64)
+        self.quadrupoles = torch.empty((ne, 9), **f64) if quadrupole else None
+        self.first_subnode = torch.empty(ne, **i32)
+        self.next_branch = torch.empty(ne, **i32)
+        self.order = torch.empty(n, dtype=torch.int64, device=dev)
+        self.nchild = torch.empty(ne, **i32)
+        self.child_base = torch.empty(ne, **i32)
+        self.slot = torch.empty(ne, **i32)
+        self.keys = torch.empty(n, dtype=torch.int64, device=dev)
+        stats = bool(quadrupole) and bool(compute_moments)
+        self.node_stats = torch.empty((nn_cap, 5), **f64) if stats else None
+        self.node_count = torch.zeros(1, **i32)
+        self.counts_dev = torch.zeros(_lib.ctypes.sizeof(_lib.BuildCounts), dtype=torch.uint8, device=dev)
+        self.arrays = TreeArrays(
+            ptr(self.coordinates).value, ptr(self.masses).value, ptr(self.softenings).value, ptr(self.sizes).value,
+            ptr(self.deltas).value, ptr(self.quadrupoles).value if quadrupole else None,
+            ptr(self.first_subnode).value, ptr(self.next_branch).value, ptr(self.order).value,
+            ptr(self.nchild).value, ptr(self.child_base).value, ptr(self.slot).value, ptr(self.keys).value,
+            ptr(self.node_stats).value if stats else None, ptr(self.node_count).value)
+        info = BuildInfo()
+        info.n, info.nnodes, info.max_level = n, nn_cap, level_cap
+        self.info = info
+        ws = workspace(L.ptg_build_workspace_bytes(n, nsys))
+        nws = workspace(L.ptg_node_workspace_bytes(n, nn_cap, level_cap))
+        # the event that lets the target groups start on a side stream once the particle slots are
+        # placed; none inside a graph capture (the captured step runs on one stream)
+        self.particles_ready = None
+        if not torch.cuda.is_current_stream_capturing():
+            self.particles_ready = torch.cuda.Event()
+            self.particles_ready.record()  # creates the event; the build records it again
+        self._packs = {}
+        pk = self._new_pack(PTG_F32, False) if prepack else None
+        check(L.ptg_build_deferred(ptr(pos), ptr(m), ptr(h), n, _lib.ctypes.byref(self.forest), int(quadrupole),
+                                   int(compute_moments), nn_cap, level_cap, ptr(ws), ws.numel(), ptr(nws), nws.numel(),
+                                   _lib.ctypes.byref(self.arrays), _lib.ctypes.byref(pk["wt"]) if pk else None,
+                                   ptr(self.counts_dev),
+                                   _lib._vp(self.particles_ready.cuda_event) if self.particles_ready else None,
+                                   stream()), "ptg_build_deferred")
+        if pk:
+            self._packs[(PTG_F32, False)] = pk
+
+    def counts(self):
+        """The counts of a deferred build (synchronises): a _lib.BuildCounts."""
+        host = self.counts_dev.cpu().numpy().tobytes()
+        return _lib.BuildCounts.from_buffer_copy(host)
+
+    def check(self):
+        """Raise if a deferred build did not fit its capacities or saw non-finite input."""
+        c = self.counts()
+        if c.nonfinite:
+            raise NonFiniteInput("non-finite position, mass or softening")
+        if c.overflow:
+            why = [w for b, w in ((_lib.PTG_OVERFLOW_NODES, f"{c.nnodes} nodes > capacity {self.nnodes}"),
+                                  (_lib.PTG_OVERFLOW_LEVELS, f"level {c.max_level} > capacity {self.level_cap}"),
+                                  (_lib.PTG_OVERFLOW_COLLISIONS,
+                                   f"more than {_lib.PTG_DEFERRED_COLLISION_CAP} particles share 63-bit keys"))
+                   if c.overflow & b]
+            raise _lib.PtgError("deferred tree build over capacity: " + "; ".join(why))
+        return c
+
     @property
     def forest_ref(self):
         """ctypes reference to the forest descriptor (None for a single tree)."""
@@ -161,10 +251,14 @@ class DeviceTree:
 
     @property
     def center(self):
+        if getattr(self, "deferred", False):
+            return np.array(self.counts().center[:])
         return np.array(self.info.center[:])
 
     @property
     def size(self):
+        if getattr(self, "deferred", False):
+            return float(self.counts().size)
         return float(self.info.size)
 
     @property
@@ -202,7 +296,9 @@ class DeviceTree:
         aux = torch.empty((ne, 4), dtype=dt, device=dev)
         kids = torch.empty((ne, 2), dtype=torch.int32, device=dev)
         quad = torch.empty((ne, 8), dtype=dt, device=dev) if quadrupole else None
-        origin = self.center if prec == PTG_F32 else np.zeros(3)
+        # a forest (every deferred tree is one) ignores the origin: FP32 slots are relative to each
+        # system's cube on the device
+        origin = self.center if prec == PTG_F32 and self.forest is None else np.zeros(3)
         wt = WalkTree(ptr(xyzm).value, ptr(aux).value, ptr(kids).value, ptr(quad).value if quad is not None else None,
                       ne, prec, int(bool(quadrupole)), (_lib._dbl * 3)(*origin))
         return {"wt": wt, "tensors": (xyzm, aux, kids, quad), "norms": True}
@@ -215,6 +311,8 @@ class DeviceTree:
         key = (prec, bool(quadrupole))
         if cache and key in self._packs and (norms is not True or self._packs[key]["norms"]):
             return self._packs[key]
+        if getattr(self, "deferred", False):
+            self.check()  # a deferred tree over its capacity has no nodes to copy (synchronises)
         norms = norms is not False or bool(quadrupole)
         p = self._new_pack(prec, quadrupole)
         wt = p["wt"]
@@ -228,6 +326,50 @@ class DeviceTree:
         if cache:
             self._packs[key] = p
         return p
+
+
+class ResidentBuilder:
+    """Tree builds of a particle set that moves from step to step (same count and systems), without
+    host synchronisation inside the build: the first call builds with exact counts (two host
+    reads), later calls build with ptg_build_deferred into arrays sized by the last known node count
+    and depth times `slack` (plus a few levels).  After the step that used a tree, verify(tree) reads
+    its counts (one host read): if the tree did not fit, it raises TreeOverflow after growing the
+    capacities, and the caller redoes the step; a tree that fit updates the capacities it is sized
+    from."""
+
+    class TreeOverflow(RuntimeError):
+        pass
+
+    def __init__(self, quadrupole=False, offsets=None, slack=1.1, extra_levels=4):
+        self.quadrupole, self.offsets = bool(quadrupole), offsets
+        self.slack, self.extra_levels = float(slack), int(extra_levels)
+        self.capacity = None
+
+    def _size(self, nnodes, max_level):
+        self.capacity = (int(nnodes * self.slack) + 1024, min(255, int(max_level) + self.extra_levels))
+
+    def __call__(self, pos, m, h):
+        if self.capacity is None:
+            t = DeviceTree(pos, m, h, quadrupole=self.quadrupole, offsets=self.offsets)
+            self._size(t.nnodes, t.info.max_level)
+            return t
+        return DeviceTree(pos, m, h, quadrupole=self.quadrupole, offsets=self.offsets, capacity=self.capacity)
+
+    def verify(self, tree):
+        if not getattr(tree, "deferred", False):
+            return tree
+        c = tree.counts()
+        if c.nonfinite:
+            raise NonFiniteInput("non-finite position, mass or softening")
+        if c.overflow:
+            # the counts report the tree's true node count and depth even when they did not fit
+            self._size(max(c.nnodes, self.capacity[0]), max(c.max_level, self.capacity[1]))
+            if c.overflow & _lib.PTG_OVERFLOW_COLLISIONS:
+                self.capacity = None  # beyond the deferred build's re-keying: build with exact counts
+            raise ResidentBuilder.TreeOverflow("tree over its capacities; capacities grown, redo the step")
+        if c.nnodes * self.slack > self.capacity[0] or c.max_level + self.extra_levels > self.capacity[1]:
+            self._size(c.nnodes, c.max_level)
+        return tree
 
 
 def morton_order(pos):
@@ -268,8 +410,8 @@ def target_groups(pos, perm, group_size, tree=None, mode="cells", keys=None):
 
 def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7, G=1.0, precision="float64",
          counters=None, grouping=None, groups=None, criterion="theta", eps=None, model=None, bf16=False,
-         eps_scale=None, sorted_targets=False, expansion=None, supergroups=False, _events_end=None, _check=True,
-         _ws_slot=0):
+         eps_scale=None, sorted_targets=False, expansion=None, supergroups=False, variant=0, expansion_rel=0.0,
+         _events_end=None, _check=True, _ws_slot=0):
     """Grouped walk of `tree` at targets pos[perm[t]]; returns an (nt, W) float64 device tensor in the
     original target order (FieldsTarget_grouped, grouped_treewalk.py:350-397).
 
@@ -288,7 +430,11 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     Taylor expansion about it instead of being evaluated per target (FP32, monopole interactions,
     potential and/or acceleration).  supergroups (with expansion): a shared pass per 8 consecutive
     groups folds what all of them accept into one expansion and hands each group the frontier it
-    walks from (the same accepted elements per group, less traversal)."""
+    walks from (the same accepted elements per group, less traversal).  expansion_rel: also fold an
+    element only when its truncation bound is below expansion_rel x its own field M / D^2 at the group
+    centre (a cap on R / D; 0 = none).  variant: the FP32 walk
+    kernel's occupancy / interaction-list variant (an index into WALK_VARIANTS; the autotuner picks
+    it with group_size="auto"); results depend on it at most at the FP32 rounding level."""
     L = _lib.load()
     pos = to_dev(pos, (-1, 3))
     nt = pos.shape[0]
@@ -302,10 +448,13 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
         if groups is not None:
             raise ValueError("group_size='auto' cannot be combined with precomputed groups (they were formed for "
                              "one group size; a walk never splits a supplied group)")
-        group_size = autotune_group_size(tree, pos, soft, perm, mask, quadrupole=quadrupole, theta=theta,
-                                         precision=precision, grouping=grouping, criterion=criterion, eps=eps,
-                                         model=model, bf16=bf16, expansion=expansion, eps_scale=eps_scale,
-                                         supergroups=supergroups)["choice"]
+        rec = autotune_group_size(tree, pos, soft, perm, mask, quadrupole=quadrupole, theta=theta,
+                                  precision=precision, grouping=grouping, criterion=criterion, eps=eps,
+                                  model=model, bf16=bf16, expansion=expansion, eps_scale=eps_scale,
+                                  supergroups=supergroups)
+        group_size, variant = rec["choice"], rec["variant"]
+    if not 0 <= int(variant) < len(WALK_VARIANTS):
+        raise ValueError(f"variant must index WALK_VARIANTS (0..{len(WALK_VARIANTS) - 1}), got {variant!r}")
     if grouping is None:
         grouping = "cells" if prec == PTG_F32 else "fixed"
     if grouping not in ("fixed", "cells"):
@@ -321,6 +470,7 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
                                keys=tree.keys if own else None)
     goff, gcnt = groups if groups is not None else (None, None)
     crit = _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
+    crit.variant = int(variant)
     if criterion == "error":
         if prec != PTG_F32:
             raise ValueError("criterion='error' is available for the float32 walk")
@@ -329,6 +479,7 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
         if eps is None or not eps > 0:
             raise ValueError("criterion='error' needs eps > 0")
         crit = _lib.Criterion(_lib.PTG_CRIT_QERR, 0, float(eps))
+        crit.variant = int(variant)
     elif criterion == "learned":
         if prec != PTG_F32:
             raise ValueError("criterion='learned' is available for the float32 walk")
@@ -339,6 +490,7 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
         if eps is None or not eps > 0:
             raise ValueError("criterion='learned' needs eps > 0")
         crit = _lib.Criterion(_lib.PTG_CRIT_RADIUS, 0, float(eps))
+        crit.variant = int(variant)
     elif criterion != "theta":
         raise ValueError(f"criterion must be 'theta', 'error' or 'learned', got {criterion!r}")
     if expansion is not None:
@@ -349,8 +501,11 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
             raise ValueError("expansion must be a positive tolerance")
         if eps_scale is not None and eps_scale.numel() < (tree.nsys if forest else 1):
             raise ValueError("eps_scale needs one value per system of the forest")
+        if not expansion_rel >= 0:
+            raise ValueError("expansion_rel must be >= 0")
         crit.group_expansion = 2 if supergroups else 1
         crit.expansion_eps = float(expansion)
+        crit.expansion_rel = float(expansion_rel)
         crit.eps_scale = eps_scale.data_ptr() if eps_scale is not None else None
     if criterion == "learned":
         p = tree.pack_learned(model, eps, quadrupole, bf16, scale=eps_scale)
@@ -403,7 +558,7 @@ def _side_stream():
 
 def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, group_size=64, criterion="theta",
                 alpha=None, model=None, bf16=False, counters=None, walk_events=None, expansion=None,
-                supergroups=False, stats=None):
+                supergroups=False, stats=None, expansion_rel=0.0):
     """Self-gravity FP32 fields of the particles a tree was built from, in their input order.
 
     The cell-aligned target groups are formed on a side stream while the main stream packs the
@@ -479,7 +634,7 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
     out = walk(tree, tree.coordinates[:n], tree.softenings[:n], perm, mask, quadrupole=quadrupole,
                group_size=group_size, theta=theta, G=G, precision="float32", counters=counters,
                groups=(offsets, count), criterion=criterion, eps=eps, model=model, bf16=bf16, eps_scale=scale,
-               sorted_targets=True, expansion=exp_eps, supergroups=supergroups,
+               sorted_targets=True, expansion=exp_eps, supergroups=supergroups, expansion_rel=expansion_rel,
                _events_end=walk_events[1] if walk_events is not None else None)
     if scale is not None:
         check_walk(1)  # the field-scale sample walk was not checked when it ran
@@ -505,6 +660,17 @@ def _ray_orders(tp, rays):
         basis = torch.tensor(np.stack([e1, e2, np.zeros(3)], 1), dtype=torch.float64, device=tp.device)
         orders.append(morton_order(tp @ basis))
     return torch.stack(orders).contiguous()
+
+
+def random_rotations(rays, nt, first_seed=0):
+    """(nt, nr, 3) float64 device tensor: the unit rays (nr, 3) rotated, for target k, by the
+    rotation of misc.random_rotation(first_seed + k) (ptg_random_rotations)."""
+    L = _lib.load()
+    r = to_dev(rays, (-1, 3))
+    out = torch.empty((nt, r.shape[0], 3), dtype=torch.float64, device=device())
+    check(L.ptg_random_rotations(int(first_seed), int(nt), ptr(r), r.shape[0], ptr(out), stream()),
+          "ptg_random_rotations")
+    return out
 
 
 def column_rays(tree, targets, rays, per_target=False, counters=None):
@@ -619,9 +785,12 @@ def field_scale_sample(tree, groups=64, group_size=64, cells=None):
 
 
 # ---------------------------------------------------------------------------------------------
-# group-size autotuner
+# autotuner: target-group size x walk kernel variant
 # ---------------------------------------------------------------------------------------------
 TUNE_SIZES = (32, 64, 128, 256)
+# FP32 walk kernel variants (ptg_criterion.variant): waves per SIMD the one-target-per-lane kernel is
+# compiled for, and the interaction-list length at which it evaluates the lists (LDS per wave)
+WALK_VARIANTS = ("5 waves/SIMD, lists of 64", "4 waves/SIMD, lists of 64", "6 waves/SIMD, lists of 32")
 _tuned = {}
 
 
@@ -629,38 +798,58 @@ def _bucket(n):
     return int(round(np.log2(max(n, 1)) * 2))  # half-octave buckets of the problem size
 
 
+def distribution_key(tree, pos):
+    """Coarse statistics of the sources and targets that the best group size and kernel variant
+    depend on: tree depth, nodes per particle (in eighths), and how far the targets spread beyond
+    the source cube (the log2 of their extent over its side, rounded)."""
+    depth = int(tree.info.max_level)
+    npp = int(round(8.0 * tree.nnodes / max(tree.n, 1)))
+    ext = (pos.max(0).values - pos.min(0).values).max().item()
+    spread = int(round(np.log2(max(ext, 1e-300) / max(tree.size, 1e-300)))) if tree.size > 0 else 0
+    return depth, npp, spread
+
+
 def tuning_log():
-    """Every autotuning decision taken in this process: {key: {"choice", "ms", "interactions"}}."""
+    """Every autotuning decision taken in this process: {key: {"choice", "variant", "ms", "interactions"}},
+    "ms" keyed by "group_size/variant"."""
     return dict(_tuned)
 
 
 def autotune_group_size(tree, pos, soft, perm, mask, quadrupole=False, theta=0.7, precision="float32", grouping=None,
                         criterion="theta", eps=None, model=None, bf16=False, sizes=TUNE_SIZES, reps=2, force=False,
                         expansion=None, eps_scale=None, supergroups=False):
-    """Pick the walk's target-group size by timing the walk (groups included) at each candidate.
+    """Pick the walk's target-group size and kernel variant by timing the walk (groups included).
 
-    The choice depends on the target count, the tree and the opening criterion, not on the values
-    the walk returns: every candidate evaluates the same sums to the same truncation order (a
-    bigger group opens nodes for its whole bounding box, so it trades more interactions per target
-    for fewer, longer traversals).  Decisions are cached per (tree size, target count, fields,
-    moment order, theta, precision, grouping, criterion, tolerance, expansion) and listed by
-    tuning_log()."""
+    Candidates: every group size in `sizes`; for the FP32 one-target-per-lane kernels (groups of
+    at most 64) also every WALK_VARIANTS entry (occupancy and interaction-list LDS).  The choice
+    depends on the target count, the tree and the opening criterion, not on the values the walk
+    returns: every candidate evaluates the same sums to the same truncation order (a bigger group
+    opens nodes for its whole bounding box, so it trades more interactions per target for fewer,
+    longer traversals; the variants give the same fields, bit for bit or to FP32 rounding).  Decisions are cached per (tree
+    size, target count, distribution statistics, fields, moment order, theta, precision, grouping,
+    criterion, tolerance, expansion) and listed by tuning_log()."""
     pos = to_dev(pos, (-1, 3))
     nt = pos.shape[0]
-    key = (_bucket(tree.n), _bucket(nt), int(mask), bool(quadrupole), float(theta), str(precision), grouping,
-           criterion, None if eps is None else float(eps), None if model is None else model.key(), bool(bf16),
-           None if expansion is None else float(expansion), bool(supergroups))
+    key = (_bucket(tree.n), _bucket(nt), distribution_key(tree, pos), int(mask), bool(quadrupole), float(theta),
+           str(precision), grouping, criterion, None if eps is None else float(eps),
+           None if model is None else model.key(), bool(bf16), None if expansion is None else float(expansion),
+           bool(supergroups))
     if key in _tuned and not force:
         return _tuned[key]
-    times, inter = {}, {}
+    fp32 = precision in ("float32", "f32", np.float32, torch.float32)
+    cands = []
     for gs in sizes:
+        for v in (range(len(WALK_VARIANTS)) if fp32 and int(gs) <= 64 else (0,)):
+            cands.append((int(gs), v))
+    times, inter = {}, {}
+    for gs, v in cands:
         ctr = torch.zeros(4, dtype=torch.int64, device=device())
-        kw = dict(quadrupole=quadrupole, group_size=int(gs), theta=theta, precision=precision, grouping=grouping,
+        kw = dict(quadrupole=quadrupole, group_size=gs, theta=theta, precision=precision, grouping=grouping,
                   criterion=criterion, eps=eps, model=model, bf16=bf16, expansion=expansion, eps_scale=eps_scale,
-                  supergroups=supergroups)
+                  supergroups=supergroups, variant=v)
         walk(tree, pos, soft, perm, mask, counters=ctr, **kw)  # warm-up (and interaction count)
         torch.cuda.synchronize()
-        inter[int(gs)] = int(ctr[0].item())
+        inter[f"{gs}/{v}"] = int(ctr[0].item())
         best = float("inf")
         for _ in range(reps):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -669,8 +858,11 @@ def autotune_group_size(tree, pos, soft, perm, mask, quadrupole=False, theta=0.7
             e1.record()
             torch.cuda.synchronize()
             best = min(best, e0.elapsed_time(e1))
-        times[int(gs)] = best
-    choice = min(times, key=times.get)
-    rec = {"choice": choice, "ms": times, "interactions": inter}
+        times[f"{gs}/{v}"] = best
+    best_key = min(times, key=times.get)
+    gs, v = (int(t) for t in best_key.split("/"))
+    rec = {"choice": gs, "variant": v, "variant_name": WALK_VARIANTS[v], "ms": times, "interactions": inter,
+           "distribution": {"max_level": key[2][0], "nodes_per_particle_x8": key[2][1],
+                            "target_spread_log2": key[2][2]}}
     _tuned[key] = rec
     return rec

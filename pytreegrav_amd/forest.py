@@ -60,15 +60,15 @@ class Forest:
         self.tuning = None
 
     # ---------------------------------------------------------------------------------------------
-    def _groups(self, gs, goff, gcnt, gws):
-        t = self.tree
+    def _groups(self, gs, goff, gcnt, gws, t=None):
+        t = t or self.tree
         mode = _lib.PTG_GROUPS_CELLS if self.prec == PTG_F32 else _lib.PTG_GROUPS_RUNS
         check(_lib.load().ptg_target_groups(ptr(self.pos), ptr(t.order), ptr(t.keys), self.n, t.forest_ref, gs, mode,
                                             ptr(goff), ptr(gcnt), ptr(gws), gws.numel(), stream()),
               "ptg_target_groups")
 
-    def _walk(self, p, mask, theta, G, out, ws, gs, goff, gcnt, crit, counters=None):
-        t = self.tree
+    def _walk(self, p, mask, theta, G, out, ws, gs, goff, gcnt, crit, counters=None, t=None):
+        t = t or self.tree
         # targets read from the tree's Morton-ordered coordinates and softenings (sorted order)
         check(_lib.load().ptg_walk(_lib.ctypes.byref(p["wt"]), t.forest_ref, ptr(t.coordinates), ptr(t.softenings),
                                    ptr(t.order), 1, self.n, int(mask), int(self.quadrupole), gs, ptr(goff),
@@ -89,19 +89,22 @@ class Forest:
         gcnt = torch.empty(1, dtype=torch.int32, device=dev)
         return out, ws, goff, gcnt, gws
 
-    def system_field_scales(self, goff, gcnt, samples=8, gs=64):
+    def system_field_scales(self, goff, gcnt, samples=8, gs=64, t=None, check_walk=True):
         """RMS |acceleration| (G=1) of each system, a device (nsys,) float64 array: a theta=1 FP32
         walk of `samples` of each system's cell-aligned groups (each padded to gs targets by
-        repeating its last target), then one RMS per system (ptg_rms_rows)."""
-        t = self.tree
+        repeating its last target), then one RMS per system (ptg_rms_rows).  t: the tree (default the
+        forest's own); check_walk=False skips the overflow check, which synchronises (a captured step
+        checks after the replay)."""
+        t = t or self.tree
         dev = device()
-        G = int(gcnt.item())
-        starts = goff[:G].to(torch.int64)
+        # everything stays on the device (the count of groups too), so a captured step can run this:
+        # group starts past the count read as n, after every system start
         soff = t.sys_offsets
-        gsys = torch.searchsorted(soff, starts, right=True) - 1
-        sys_ids = torch.arange(self.nsys, device=dev, dtype=torch.int64)
-        gfirst = torch.searchsorted(gsys, sys_ids)
-        gnum = torch.bincount(gsys, minlength=self.nsys)
+        idx = torch.arange(self.n + 1, device=dev, dtype=torch.int64)
+        starts = torch.where(idx < gcnt.to(torch.int64), goff.to(torch.int64), self.n)
+        gfirst = torch.searchsorted(starts, soff)  # first group of each system, then the group count
+        gnum = gfirst[1:] - gfirst[:-1]
+        gfirst = gfirst[:-1]
         k = torch.arange(samples, device=dev, dtype=torch.int64)
         g = gfirst[:, None] + (k[None, :] * gnum[:, None]) // samples  # (nsys, samples)
         g = g.reshape(-1)
@@ -126,19 +129,23 @@ class Forest:
         check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), _lib.ctypes.byref(fs), ptr(pos), ptr(h), ptr(perm), 1, nt,
                          PTG_ACC, 0, gs, ptr(goff_s), ptr(gcnt_s), 1.0, _lib.ctypes.byref(crit), 1.0, 0, ptr(out),
                          None, ptr(ws), ws.numel(), stream()), "ptg_walk")
-        check(L.ptg_walk_check(ptr(ws), stream()), "ptg_walk")
+        if check_walk:
+            check(L.ptg_walk_check(ptr(ws), stream()), "ptg_walk")
+        else:
+            self._sample_ws = ws  # checked after the captured step's replay
         rms = torch.empty(self.nsys, dtype=torch.float64, device=dev)
         check(L.ptg_rms_rows(ptr(out), nt, 3, 0, 3, self.nsys, ptr(rms), stream()), "ptg_rms_rows")
         return rms
 
     def _prepare(self, gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask=PTG_ACC, expansion=None,
-                 supergroups=False):
+                 supergroups=False, t=None, check_walk=True):
         """Groups, and for the learned criterion the opening radii; returns (pack, criterion, theta).
         expansion: group-expansion tolerance in units of each system's RMS field (None: off);
         supergroups: with it, the shared pass per 8 groups of a system."""
         if supergroups and expansion is None:
             raise ValueError("supergroups needs the group expansion (expansion=...)")
-        self._groups(gs, goff, gcnt, gws)
+        t = t or self.tree
+        self._groups(gs, goff, gcnt, gws, t=t)
         if expansion is not None:
             if self.prec != PTG_F32 or self.quadrupole or (mask & PTG_TIDAL):
                 raise ValueError("expansion runs on the float32 walk with monopole interactions, for the potential "
@@ -147,8 +154,8 @@ class Forest:
                 raise ValueError("expansion must be a positive tolerance (relative to each system's RMS field)")
         if self.criterion != "learned":
             crit = _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
-            self._expand(crit, expansion, None, goff, gcnt, supergroups)
-            return self.tree.pack(self.prec, self.quadrupole), crit, (0.7 if theta is None else theta)
+            self._expand(crit, expansion, None, goff, gcnt, supergroups, t=t, check_walk=check_walk)
+            return t.pack(self.prec, self.quadrupole), crit, (0.7 if theta is None else theta)
         if model is None:
             from .learned import OpeningMLP
 
@@ -159,19 +166,20 @@ class Forest:
         else:
             if alpha is None or not alpha > 0:
                 raise ValueError("criterion='learned' needs alpha > 0 (or per-system eps)")
-            scale, base = self.system_field_scales(goff, gcnt), float(alpha)
-        p = self.tree.pack_learned(model, base, self.quadrupole, bf16, scale=scale)
+            scale, base = self.system_field_scales(goff, gcnt, t=t, check_walk=check_walk), float(alpha)
+        p = t.pack_learned(model, base, self.quadrupole, bf16, scale=scale)
         self._scale = scale  # keeps the scale alive while the pack is cached under its address
         crit = _lib.Criterion(_lib.PTG_CRIT_RADIUS, 0, 0.0)
-        self._expand(crit, expansion, scale if eps is None else None, goff, gcnt, supergroups)
+        self._expand(crit, expansion, scale if eps is None else None, goff, gcnt, supergroups, t=t,
+                     check_walk=check_walk)
         return p, crit, (2.0 if theta is None else theta)
 
-    def _expand(self, crit, expansion, rms, goff, gcnt, supergroups=False):
+    def _expand(self, crit, expansion, rms, goff, gcnt, supergroups=False, t=None, check_walk=True):
         """Switch the group expansion on in crit at expansion x each system's RMS field (device)."""
         if expansion is None:
             return
         if rms is None:
-            rms = self.system_field_scales(goff, gcnt)
+            rms = self.system_field_scales(goff, gcnt, t=t, check_walk=check_walk)
         self._exp_scale = rms  # the walk (and any captured graph) reads it through crit
         crit.group_expansion = 2 if supergroups else 1
         crit.expansion_eps = float(expansion)
@@ -246,6 +254,74 @@ class Forest:
         def replay():
             g.replay()
             check(_lib.load().ptg_walk_check(ptr(ws), stream()), "ptg_walk")
+            return out
+
+        return replay, out
+
+    def capture_step(self, potential=True, accel=True, theta=None, G=1.0, alpha=None, model=None, bf16=False,
+                     expansion=None, capacity=None):
+        """Capture one whole step -- tree build (ptg_build_deferred), target groups, for the learned
+        criterion the per-system field scales and opening radii, and the walk -- into one HIP graph.
+        Each replay recomputes every system's fields from the current contents of the input buffers
+        self.pos, self.m, self.h (write new positions into them in place; same particle count and
+        systems), with no host synchronisation inside the step.  capacity = (node capacity, level
+        capacity) of the build, by default 1.25x the node count and 4 levels more than the tree built
+        by the constructor.  Returns (replay, out): replay() runs the graph, then checks that the
+        tree fit its capacities and that no traversal stack overflowed (raising otherwise; capture
+        again with a larger capacity), and returns out, the (n, W) buffer every replay rewrites."""
+        if self.prec != PTG_F32:
+            raise ValueError("capture_step runs the float32 walk")
+        mask = fields_mask(potential, accel, False)
+        if self.auto and self.tuning is None:
+            self.autotune(potential, accel, False, theta, G, alpha=alpha, model=model, bf16=bf16,
+                          expansion=expansion)
+        gs = self.group_size
+        if self.criterion == "learned":
+            if alpha is None or not alpha > 0:
+                raise ValueError("capture_step with criterion='learned' needs alpha > 0")
+            if model is None:
+                from .learned import OpeningMLP
+
+                model = OpeningMLP.load()  # loaded before the capture (file and host-to-device copy)
+        if capacity is None:
+            capacity = (int(self.tree.nnodes * 1.25) + 64, min(255, int(self.tree.info.max_level) + 4))
+        out, ws, goff, gcnt, gws = self._buffers(mask, gs, False)
+        quad_build = self.quadrupole or self.criterion == "learned"
+        held = {}
+
+        def step():
+            t = DeviceTree(self.pos, self.m, self.h, quadrupole=quad_build, offsets=self.tree.sys_offsets,
+                           capacity=capacity)
+            p, crit, th = self._prepare(gs, theta, alpha, None, model, bf16, goff, gcnt, gws, mask, expansion, False,
+                                        t=t, check_walk=False)
+            self._walk(p, mask, th, G, out, ws, gs, goff, gcnt, crit, t=t)
+            # everything the graph reads through a captured pointer lives as long as the graph: the
+            # tree, packs and tolerances made inside the step, and the model's parameters (loaded
+            # outside it), whose memory would otherwise go back to the allocator for eager work
+            held.update(tree=t, pack=p, crit=crit, model=model, scale=getattr(self, "_scale", None),
+                        exp_scale=getattr(self, "_exp_scale", None), sample_ws=getattr(self, "_sample_ws", None))
+
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):  # warm-up on the capture stream: the same step, eagerly
+            step()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        held["tree"].check()
+        check(_lib.load().ptg_walk_check(ptr(ws), stream()), "ptg_walk")
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            step()
+        state = dict(held)  # the captured step's tree, packs and scales live as long as the graph
+        self._graphs[("step", mask, theta, G)] = (g, out, ws, goff, gcnt, gws, state)
+        L = _lib.load()
+
+        def replay():
+            g.replay()
+            state["tree"].check()  # synchronises: the tree fit its capacities, finite inputs
+            check(L.ptg_walk_check(ptr(ws), stream()), "ptg_walk")
+            if state["sample_ws"] is not None:
+                check(L.ptg_walk_check(ptr(state["sample_ws"]), stream()), "ptg_walk")
             return out
 
         return replay, out

@@ -44,7 +44,7 @@ def test_struct_layouts():
     from pytreegrav_amd import _lib
 
     assert ctypes.sizeof(_lib.BuildInfo) == 8 + 8 + 4 * 4 + 3 * 8 + 8
-    assert ctypes.sizeof(_lib.TreeArrays) == 14 * 8
+    assert ctypes.sizeof(_lib.TreeArrays) == 15 * 8
     assert ctypes.sizeof(_lib.WalkTree) == 4 * 8 + 8 + 4 + 4 + 3 * 8
 
 
@@ -54,7 +54,7 @@ def test_struct_field_offsets_match_the_header(tmp_path):
     from pytreegrav_amd import _lib
 
     mirror = {"ptg_build_info": _lib.BuildInfo, "ptg_tree_arrays": _lib.TreeArrays, "ptg_forest": _lib.Forest,
-              "ptg_walk_tree": _lib.WalkTree, "ptg_criterion": _lib.Criterion}
+              "ptg_walk_tree": _lib.WalkTree, "ptg_criterion": _lib.Criterion, "ptg_build_counts": _lib.BuildCounts}
     lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "ptg.h"', "int main(void) {"]
     expect = []
     for cname, cls in mirror.items():

@@ -193,3 +193,95 @@ def test_forest_autotune_and_learned_capture():
         replay()
         torch.cuda.synchronize()
         assert torch.equal(out[:, 0], ref["potential"]) and torch.equal(out[:, 1:4], ref["accel"])
+
+
+def _tree_rows(t, nn):
+    """The element rows of a tree's arrays (particles, then nn nodes) on the host."""
+    ne = t.n + nn
+    rows = {}
+    for name in ARRAYS + ("slot", "child_base"):
+        rows[name] = getattr(t, name)[:ne].cpu().numpy()
+    for name in ("sizes", "deltas"):  # particle rows are never written on the device (zero by definition)
+        rows[name] = rows[name][t.n:]
+    rows["order"] = t.order.cpu().numpy()
+    if t.quadrupoles is not None:
+        rows["quadrupoles"] = t.quadrupoles[t.n:ne].cpu().numpy()
+    return rows
+
+
+@pytest.mark.parametrize("quadrupole", [False, True])
+def test_deferred_build_equals_two_call_build(quadrupole):
+    """ptg_build_deferred (no host synchronisation, node arrays sized for a capacity) writes the same
+    tree as ptg_build_sort + ptg_build_tree, bit for bit, including the reference's collision and
+    coincident-point cases; over its capacity it reports the overflow and writes no nodes."""
+    systems = _systems()
+    x, m, h, off = _concat(systems)
+    print(f"[deferred] quad={quadrupole} n={len(x)}", flush=True)
+    e = engine.DeviceTree(x, m, h, quadrupole=quadrupole, offsets=off)
+    d = engine.DeviceTree(x, m, h, quadrupole=quadrupole, offsets=off,
+                          capacity=(e.nnodes + 1000, int(e.info.max_level) + 3))
+    c = d.check()
+    assert c.nnodes == e.nnodes and c.max_level == e.info.max_level and int(d.node_count.item()) == e.nnodes
+    assert bool(c.has_coincident) == bool(e.info.has_coincident)
+    re, rd = _tree_rows(e, e.nnodes), _tree_rows(d, e.nnodes)
+    for k in re:
+        assert np.array_equal(re[k], rd[k]), k
+    assert torch.equal(e.cubes, d.cubes) and torch.equal(e.roots, d.roots)
+    # the build's FP32 walk copies agree on every slot of the tree
+    pe, pd = e.pack(engine.PTG_F32, False), d.pack(engine.PTG_F32, False)
+    ne = e.n + e.nnodes
+    for a, b in zip(pe["tensors"][:3], pd["tensors"][:3]):
+        assert torch.equal(a[:ne], b[:ne])
+    # a single tree goes through the same path as a one-system forest
+    xs, ms, hs = systems[5]
+    e1 = engine.DeviceTree(xs, ms, hs, quadrupole=quadrupole)
+    d1 = engine.DeviceTree(xs, ms, hs, quadrupole=quadrupole, capacity=(e1.nnodes, int(e1.info.max_level)))
+    assert d1.check().nnodes == e1.nnodes
+    r1, r2 = _tree_rows(e1, e1.nnodes), _tree_rows(d1, e1.nnodes)
+    for k in r1:
+        assert np.array_equal(r1[k], r2[k]), k
+    # over capacity: reported, nothing written past the count (the count reads 0)
+    small = engine.DeviceTree(x, m, h, quadrupole=quadrupole, offsets=off, capacity=(e.nnodes // 2, 40))
+    assert int(small.node_count.item()) == 0
+    with pytest.raises(PtgError):
+        small.check()
+    low = engine.DeviceTree(x, m, h, quadrupole=quadrupole, offsets=off, capacity=(e.nnodes + 10, 3))
+    with pytest.raises(PtgError):
+        low.check()
+
+
+@pytest.mark.parametrize("criterion", ["theta", "learned"])
+def test_forest_capture_step_replays_the_whole_step(criterion):
+    """capture_step records build + groups (+ field scales and radii) + walk in one graph: each replay
+    equals an eager evaluation of the current positions bit for bit, also after the positions are
+    changed in place; a capacity too small is reported by the replay."""
+    systems = _systems()[:8]
+    x, m, h, off = _concat(systems)
+    kw = dict(alpha=1.6e-3, expansion=1e-3) if criterion == "learned" else {}
+    F = Forest(x, m, h, off, precision="float32", criterion=criterion)
+    replay, out = F.capture_step(potential=True, accel=True, **kw)
+    rng = np.random.default_rng(3)
+    for k in range(3):
+        print(f"[capture_step] {criterion} replay {k}", flush=True)
+        if k:  # new positions, written into the captured input buffer
+            x = x + 1e-3 * rng.normal(size=x.shape) * np.repeat([s[2][0] * 10 for s in systems], np.diff(off))[:, None]
+            F.pos.copy_(torch.from_numpy(x).to(F.pos.device))
+        out.zero_()
+        replay()
+        torch.cuda.synchronize()
+        E = Forest(x, m, h, off, precision="float32", criterion=criterion)
+        r = E.evaluate(potential=True, accel=True, **kw)
+        assert torch.equal(out[:, 0], r["potential"]) and torch.equal(out[:, 1:4], r["accel"]), k
+    # positions that make the captured capacities too small: every system collapsed onto one point,
+    # so all of its particles share a 63-bit key (more than the deferred build re-keys); the replay
+    # reports it, and the walk it ran saw massless roots only
+    F2 = Forest(x, m, h, off, precision="float32", criterion=criterion)
+    r2, out2 = F2.capture_step(potential=True, accel=True, **kw)
+    same = np.repeat(x[off[:-1]], np.diff(off), axis=0)
+    F2.pos.copy_(torch.from_numpy(same).to(F2.pos.device))
+    with pytest.raises(PtgError):
+        r2()
+    F2.pos.copy_(torch.from_numpy(x).to(F2.pos.device))  # and it recovers with fitting positions
+    r2()
+    torch.cuda.synchronize()
+    assert torch.equal(out2[:, 1:4], r["accel"])
