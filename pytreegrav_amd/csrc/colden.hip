@@ -668,6 +668,90 @@ extern "C" ptg_status ptg_column_rays(const ptg_tree_arrays *tree, int64_t n, in
     return PTG_OK;
 }
 
+/* Randomized ray grids (treewalk.py:1061-1078 with misc.py:6-54): target k sees the grid rotated by
+ * the rotation drawn from a Mersenne-Twister stream seeded with k.  Four doubles are drawn, i.e. the
+ * first eight 32-bit outputs of MT19937 after its integer seeding, so a thread needs only state
+ * words 0..8 and 397..404 of the seeded table: the first twist combines word i, i+1 and i+397.
+ * Each double is the 53-bit (a >> 5, b >> 6) combination of two outputs, as numpy's legacy
+ * generator and numba's draw it; uniform(lo, hi) = lo + (hi - lo) * u. */
+#define MT_M 397
+#define MT_NEED 9 // words 0..8 of the table, and 397..404
+
+__device__ __forceinline__ uint32_t mt_temper(uint32_t y)
+{
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+__global__ void __launch_bounds__(256) k_random_rotations(int64_t first_seed, int64_t nt, const double *__restrict__ rays,
+                                                          int nrays, double *__restrict__ out)
+{
+#pragma clang fp contract(off) // the host's rounding: no fused multiply-adds
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nt) return;
+    uint32_t lo[MT_NEED], hi[MT_NEED - 1];
+    uint32_t s = (uint32_t)(first_seed + k);
+    // integer seeding: word i = 1812433253 * (word(i-1) ^ word(i-1) >> 30) + i
+    for (int i = 0; i < MT_M + MT_NEED - 1; ++i) {
+        if (i < MT_NEED) lo[i] = s;
+        if (i >= MT_M) hi[i - MT_M] = s;
+        s = 1812433253u * (s ^ (s >> 30)) + (uint32_t)(i + 1);
+    }
+    double u[4];
+    for (int d = 0; d < 4; ++d) {
+        uint32_t w[2];
+        for (int q = 0; q < 2; ++q) {
+            const int i = 2 * d + q;
+            const uint32_t y = (lo[i] & 0x80000000u) | (lo[i + 1] & 0x7fffffffu);
+            w[q] = mt_temper(hi[i] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u));
+        }
+        u[d] = ((double)(w[0] >> 5) * 67108864.0 + (double)(w[1] >> 6)) / 9007199254740992.0;
+    }
+    double R[3][3];
+    for (int row = 0; row < 2; ++row) {
+        const double c = -1.0 + 2.0 * u[2 * row];
+        const double sn = sqrt(fmax(1.0 - c * c, 0.0));
+        const double phi = 2.0 * M_PI * (0.0 + 1.0 * u[2 * row + 1]);
+        R[row][0] = sn * cos(phi);
+        R[row][1] = sn * sin(phi);
+        R[row][2] = c;
+    }
+    double dot = 0.0;
+    for (int c = 0; c < 3; ++c) dot += R[0][c] * R[1][c];
+    for (int c = 0; c < 3; ++c) R[1][c] -= dot * R[0][c];
+    double nrm = 0.0;
+    for (int c = 0; c < 3; ++c) nrm += R[1][c] * R[1][c];
+    nrm = sqrt(nrm);
+    for (int c = 0; c < 3; ++c) R[1][c] /= nrm;
+    for (int c = 0; c < 3; ++c) {
+        const int a = (c + 1) % 3, b = (c + 2) % 3;
+        R[2][c] = R[0][a] * R[1][b] - R[1][a] * R[0][b];
+    }
+    double *o = out + k * nrays * 3;
+    for (int j = 0; j < nrays; ++j)
+        for (int c = 0; c < 3; ++c)
+            o[j * 3 + c] = rays[j * 3] * R[0][c] + rays[j * 3 + 1] * R[1][c] + rays[j * 3 + 2] * R[2][c];
+}
+
+extern "C" ptg_status ptg_random_rotations(int64_t first_seed, int64_t nt, const double *rays, int64_t nrays,
+                                           double *out, void *stream)
+{
+    if (nt < 0 || nrays < 1 || nrays > (1 << 20) || !rays || (nt > 0 && !out) || first_seed < 0 ||
+        first_seed + nt > (1ll << 32)) {
+        ptg_set_error("ptg_random_rotations: needs 0 <= first_seed, first_seed + nt <= 2^32, 1 <= nrays <= 2^20, "
+                      "rays and out");
+        return PTG_ERR_ARG;
+    }
+    if (nt == 0) return PTG_OK;
+    k_random_rotations<<<(unsigned)ptg_cdiv(nt, 256), 256, 0, (hipStream_t)stream>>>(first_seed, nt, rays, (int)nrays,
+                                                                                     out);
+    PTG_LAUNCH_CHECK();
+    return PTG_OK;
+}
+
 extern "C" ptg_status ptg_column_binned(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
                                         const ptg_walk_tree *wt, const double *targets, int64_t nt,
                                         int64_t group_size, double theta, double *out, int32_t *err, void *ws,

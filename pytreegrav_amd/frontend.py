@@ -24,6 +24,12 @@ from .engine import PTG_ACC, PTG_POT, PTG_TIDAL
 FP32_DEFAULT_GROUP = 64
 
 
+# Accel / Potential with method="adaptive" in float32 walk the tree above this many particles and sum
+# directly below it: on one MI355X a whole float32 call (tree build + walk vs the LDS-tiled direct sum,
+# Plummer spheres, h = 0.01) takes 0.83 vs 0.54 ms at N = 2000 and 0.91 vs 0.97 ms at N = 4000
+FP32_TREE_MIN_N = 3500
+
+
 class _DefaultTheta(float):
     """The reference's default opening angle, 0.7 (frontend.py:149, :425).  It is an ordinary float
     everywhere; only its identity tells a theta the caller left at the default from an explicit 0.7,
@@ -332,7 +338,12 @@ def _self_fields(pos, m, softening, mask, G, theta, tree, return_tree, parallel,
         softening = m * 0 if isinstance(m, torch.Tensor) else np.zeros_like(m)
     softening = _atleast_1d(_f64(softening))
     if method == "adaptive":
-        if tree is not None or len(pos) > (4000 if parallel else 1000):
+        # the reference crosses over where its 32-thread CPU curves do (frontend.py:210-220); float64
+        # keeps those thresholds, since the choice changes the answer and float64 reproduces the
+        # reference, while float32 uses the crossover measured for the GPU walk against the GPU
+        # direct sum (FP32_TREE_MIN_N)
+        n_min = FP32_TREE_MIN_N if precision == "float32" else (4000 if parallel else 1000)
+        if tree is not None or len(pos) > n_min:
             method = "tree"
         else:
             method = "bruteforce"
@@ -584,8 +595,6 @@ def ColumnDensity(pos, m, radii, rays=None, randomize_rays=False, healpix=False,
     Morton targets per traversal when there are at least COLUMN_GROUP_MIN_TARGETS of them).
     ``parallel`` is accepted and ignored; ``device`` is validated as the reference does, and both
     values run the FP64 walks on the GPU."""
-    from .misc import random_rotation
-
     if np.any(np.asarray(radii.cpu() if isinstance(radii, torch.Tensor) else radii) <= 0):
         warnings.warn(
             "Some particle radii are <= 0; these particles are point-like and contribute no column "
@@ -634,7 +643,7 @@ def ColumnDensity(pos, m, radii, rays=None, randomize_rays=False, healpix=False,
         columns = engine.column_binned(tree.dev, pos_sorted, gs, theta)
     elif randomize_rays:
         # target k (in Morton order) sees the grid rotated by random_rotation(k) (treewalk.py:1061-1078)
-        rot = np.stack([rays @ random_rotation(k) for k in range(nt)])
+        rot = engine.random_rotations(rays, nt)
         columns = engine.column_rays(tree.dev, pos_sorted, rot, per_target=True)
     else:
         columns = engine.column_rays(tree.dev, pos_sorted, rays)

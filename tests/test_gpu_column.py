@@ -217,3 +217,16 @@ def test_wavefront_ray_walk_equals_stackless_walk():
         if order is not None:
             assert np.array_equal(outs[0], outs[1])  # the packet composition does not change a column
             assert np.array_equal(np.sort(order.cpu().numpy(), 1), np.tile(np.arange(nt), (nr, 1)))
+
+
+def test_device_ray_rotations_equal_host_rotations():
+    """The per-target grids of randomize_rays, drawn on the device, equal misc.random_rotation's
+    (numpy's seeded Mersenne Twister) to the last few bits (device cos/sin may differ by an ulp)."""
+    rays = np.random.default_rng(9).normal(size=(7, 3))
+    rays /= np.linalg.norm(rays, axis=1, keepdims=True)
+    for first, nt in ((0, 3000), (2**32 - 64, 64), (123456789, 100)):
+        got = engine.random_rotations(rays, nt, first_seed=first).cpu().numpy()
+        want = np.stack([rays @ random_rotation(first + k) for k in range(nt)])
+        assert got.shape == (nt, 7, 3) and np.abs(got - want).max() < 1e-14, first
+    with pytest.raises(RuntimeError):
+        engine.random_rotations(rays, 65, first_seed=2**32 - 64)
