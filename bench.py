@@ -64,10 +64,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("-N", type=int, default=None, help="particles (default 1e7; 1e6 for --workload column)")
-    ap.add_argument("--workload", choices=("plummer", "batched", "column"), default="plummer",
+    ap.add_argument("--workload", choices=("plummer", "batched", "column", "targets"), default="plummer",
                     help="plummer: one N-particle Plummer sphere (the headline); batched: 512 independent "
                          "Plummer clusters (N log-uniform in [1e3, 1e5], scale radii 0.1-10) in one forest; "
-                         "column: ColumnDensity of an N=1e6 Gaussian cloud (angle-binned, 6-ray walk beside it)")
+                         "column: ColumnDensity of an N=1e6 Gaussian cloud (angle-binned, 6-ray walk beside it); "
+                         "targets: AccelTarget+PotentialTarget of 1e6 uniform targets from N=1e7 Plummer sources")
     ap.add_argument("--criterion", choices=("error", "theta", "learned"), default="learned",
                     help="opening criterion of the timed walk: 'error' bounds each accepted node's dropped "
                          "quadrupole term by alpha x (RMS field); 'learned' accepts a node beyond the opening "
@@ -82,9 +83,16 @@ def parse():
                          "from a group's centre are folded into a third-order expansion about it (default 4e-3 "
                          "with the learned criterion, off otherwise)")
     ap.add_argument("--no-expansion", action="store_true", help="walk without the group expansion")
+    ap.add_argument("--expansion-rel", type=float, default=None,
+                    help="fold an element only if its truncation bound is also below this fraction of its own "
+                         "field at the group centre (a cap on R/D; 0 = none)")
     ap.add_argument("--bf16", action="store_true", help="learned criterion: bfloat16 MLP inference for the radii")
     ap.add_argument("--supergroups", action="store_true", help="with the expansion: the shared pass per 8 groups")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain theta=0.7 comparison line")
+    ap.add_argument("--build", choices=("eager", "resident"), default="eager",
+                    help="eager: every step builds with exact node counts (host reads inside the build); "
+                         "resident: steps build without host synchronisation into arrays sized by the "
+                         "previous build's node count, verified after the step (ResidentBuilder)")
     ap.add_argument("--group-size", type=int, default=64)
     ap.add_argument("--softening", type=float, default=0.01)
     ap.add_argument("--seed", type=int, default=0)
@@ -300,6 +308,19 @@ def run_batched(args):
     torch.cuda.synchronize()
     graph_ms = e0.elapsed_time(e1) / args.steps
     graph_equal = bool(torch.equal(out[:, 0], r["potential"]) and torch.equal(out[:, 1:4], r["accel"]))
+    # the whole step (deferred tree build, groups, field scales and radii, walk) as one graph; each
+    # replay runs the graph and then reads the build's counts (one synchronisation per step)
+    step_kw = dict(kw)
+    step_kw.pop("supergroups", None)
+    replay_step, out_step = F.capture_step(potential=True, accel=True, **step_kw)
+    replay_step()
+    torch.cuda.synchronize()
+    step_equal = bool(torch.equal(out_step[:, 0], r["potential"]) and torch.equal(out_step[:, 1:4], r["accel"]))
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        replay_step()
+    torch.cuda.synchronize()
+    step_graph_ms = (time.perf_counter() - t0) / args.steps * 1e3
     # accuracy: 16 systems, up to 1000 targets each, FP64 direct sum within the system
     rng = np.random.default_rng(7)
     errs = []
@@ -331,6 +352,8 @@ def run_batched(args):
                    "group_size": gs, "grouping": "cell-aligned per system",
                    "softening": "0.01 x scale radius", "scale_radius": "log-uniform 0.1-10"},
         "breakdown": {"walk_stage_graph_ms": graph_ms, "graph_replay_equals_eager": graph_equal,
+                      "full_step_graph_ms": step_graph_ms, "full_step_graph_equals_eager": step_equal,
+                      "full_step_graph_particles_per_s": N / (step_graph_ms * 1e-3),
                       "interactions_per_particle": interactions / N,
                       "folded_elements_per_particle": folds / N},
         "roofline": {"kernel": "k_walk (FP32 grouped walk)", "bound": "valu",
@@ -340,6 +363,101 @@ def run_batched(args):
                      "traffic": None, "convention": "walk stage (groups + walk) per replay"},
         "accuracy": {"accel_rms_per_system_max": max(errs), "accel_rms_per_system_median": float(np.median(errs)),
                      "systems_checked": 16},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_targets(args):
+    """External targets: Plummer N=1e7 sources (truncated at 100 a, h=0.01) and 1e6 targets uniform in
+    the cube [-2, 2]^3 (AccelTarget / PotentialTarget, frontend.py:296-418, 563-685).  A step is one
+    whole call: source tree build, target Morton order, cell-aligned target groups and one fused
+    potential+acceleration FP32 walk (targets in input order on the device).  The group size and walk
+    kernel variant are autotuned once before the timed steps (group_size="auto")."""
+    import torch
+
+    from pytreegrav_amd import ConstructTree, engine
+    from pytreegrav_amd.sampling import plummer_truncated
+
+    dev = engine.device()
+    N = args.N or 10_000_000
+    NT = 1_000_000
+    x, m = plummer_truncated(N, seed=args.seed, rmax=100.0)
+    h = np.full(N, args.softening)
+    xt = np.random.default_rng(args.seed + 1).uniform(-2.0, 2.0, size=(NT, 3))
+    pos, mass, soft, tgt = (torch.from_numpy(a).to(dev) for a in (x, m, h, xt))
+    tsoft = torch.zeros(NT, dtype=torch.float64, device=dev)
+    mask = engine.PTG_POT | engine.PTG_ACC
+    theta = args.theta if args.theta is not None else 0.7
+    tree = ConstructTree(pos, mass, soft)
+    engine._tuned.clear()
+    order = engine.morton_order(tgt)
+    tune = engine.autotune_group_size(tree.dev, tgt, tsoft, order, mask, theta=theta, precision="float32",
+                                      grouping="cells")
+    gs, variant = tune["choice"], tune.get("variant", 0)
+    ctr = torch.zeros(4, dtype=torch.int64, device=dev)
+    walk_ev = []
+
+    def step(record):
+        t = ConstructTree(pos, mass, soft)
+        perm = engine.morton_order(tgt)
+        ev = None
+        if record:
+            ctr.zero_()
+            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            walk_ev.append(ev)
+        groups = engine.target_groups(tgt, perm, gs, mode="cells")
+        if ev:
+            ev[0].record()
+        out = engine.walk(t.dev, tgt, tsoft, perm, mask, group_size=gs, theta=theta, precision="float32",
+                          groups=groups, counters=ctr if record else None, variant=variant,
+                          _events_end=ev[1] if ev else None)
+        return out
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step(True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    walk_ms = float(np.mean([a.elapsed_time(b) for a, b in walk_ev]))
+    interactions = int(ctr[0].item())
+    sel = np.sort(np.random.default_rng(12345).choice(NT, size=min(args.oracle_targets, NT), replace=False))
+    exact = engine.direct(tgt[torch.from_numpy(sel).to(dev)], tsoft[:len(sel)], pos, mass, soft, mask,
+                          precision="float64").cpu().numpy()
+    res = out.cpu().numpy()[sel]
+    da = np.sqrt(np.sum((res[:, 1:4] - exact[:, 1:4]) ** 2, 1))
+    scale = np.sqrt(np.mean(np.sum(exact[:, 1:4] ** 2, 1)))
+    dp = res[:, 0] - exact[:, 0]
+    achieved = FLOPS_PER_INTERACTION * interactions / (walk_ms * 1e-3) / 1e12
+    line = {
+        "metric": "AccelTarget+PotentialTarget targets/sec, 1e6 uniform targets in [-2,2]^3 from N=1e7 Plummer sources",
+        "value": NT / dt,
+        "unit": "targets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plummer1e7_sources_uniform1e6_targets", "N_sources": N, "N_targets": NT,
+                   "theta": theta, "criterion": "theta", "group_size": "auto", "grouping": "cell-aligned",
+                   "walk_precision": "fp32 node-relative, fp32 accumulation", "build_precision": "fp64 keys and moments"},
+        "autotune": {"choice_group_size": gs, "choice_variant": engine.WALK_VARIANTS[variant], "ms": tune["ms"],
+                     "interactions": tune["interactions"]},
+        "breakdown": {"walk_ms": walk_ms, "interactions_per_target": interactions / NT},
+        "roofline": {"kernel": "k_walk (FP32 grouped walk)", "bound": "valu", "achieved": achieved,
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "convention": "20 flops per target-element interaction (monopole pot+acc)"},
+        "accuracy": {"oracle": f"FP64 direct sum on the GPU, {len(sel)} random targets x {N} sources",
+                     "accel_rms": float(np.sqrt(np.mean(da ** 2)) / scale), "accel_max": float(da.max() / scale),
+                     "pot_rms": float(np.std(dp) / np.std(exact[:, 0])),
+                     "pot_bias": float(np.mean(dp) / np.std(exact[:, 0]))},
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
@@ -472,6 +590,8 @@ def main():
         return run_batched(args)
     if args.workload == "column":
         return run_column(args)
+    if args.workload == "targets":
+        return run_targets(args)
     dev = engine.device()
     N = args.N or 10_000_000
     x, m = plummer_truncated(N, seed=args.seed, rmax=100.0)
@@ -503,20 +623,33 @@ def main():
         build_quads = quad_walk or criterion in ("error", "learned")
         walk_times = []
         stats = {}
+        # resident: each step's tree is built without host synchronisation into arrays sized by the
+        # previous step's node count (ptg_build_deferred), and verified after the step
+        builder = engine.ResidentBuilder(quadrupole=build_quads) if args.build == "resident" else None
+        if builder is not None:
+            builder.verify(builder(pos, mass, soft))  # the first build sizes the capacities
+
+        def make_tree():
+            return builder(pos, mass, soft) if builder is not None else \
+                ConstructTree(pos, mass, soft, quadrupole=build_quads).dev
 
         def step(record):
-            tree = ConstructTree(pos, mass, soft, quadrupole=build_quads)
+            tree = make_tree()
             ev = None
             if record:
                 ctr.zero_()
                 ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                 walk_times.append(ev)
-            return engine.self_fields(tree.dev, pos, soft, mask, theta=theta, quadrupole=quad_walk,
+            out = engine.self_fields(tree, pos, soft, mask, theta=theta, quadrupole=quad_walk,
                                       group_size=args.group_size, criterion=criterion,
                                       alpha=alpha if criterion != "theta" else None, model=model,
                                       counters=ctr if record else None, walk_events=ev, expansion=expansion,
                                       bf16=args.bf16, supergroups=args.supergroups and expansion is not None,
-                                      stats=stats if record else None)
+                                      stats=stats if record else None,
+                                      expansion_rel=args.expansion_rel if expansion is not None else 0.0)
+            if builder is not None:
+                builder.verify(tree)  # never over capacity here: the positions do not change
+            return out
 
         for _ in range(args.warmup):
             step(False)
@@ -534,11 +667,10 @@ def main():
         torch.cuda.synchronize()
         tb0 = time.perf_counter()
         for _ in range(args.steps):
-            ConstructTree(pos, mass, soft, quadrupole=build_quads)
+            make_tree()
         torch.cuda.synchronize()
         r["build_ms"] = (time.perf_counter() - tb0) / args.steps * 1e3
-        r["build_phases"] = build_phases(lambda: ConstructTree(pos, mass, soft, quadrupole=build_quads).dev, N,
-                                         build_quads, args.steps)
+        r["build_phases"] = build_phases(make_tree, N, build_quads, args.steps)
         r["accuracy"] = accuracy(out) if exact is not None else None
         return r
 
@@ -567,6 +699,8 @@ def main():
         args.expansion = None
     elif args.expansion is None and args.criterion == "learned" and not args.quadrupole:
         args.expansion = 4e-3
+    if args.expansion_rel is None:
+        args.expansion_rel = 0.0
     quad = bool(args.quadrupole)
     theta = args.theta if args.theta is not None else (0.7 if args.criterion == "theta" else 2.0)
     main = run_config(args.criterion, theta, args.alpha, quad, args.expansion)
@@ -612,6 +746,7 @@ def main():
             "criterion": args.criterion,
             "theta": theta,
             "group_expansion": args.expansion,
+            "expansion_rel": args.expansion_rel if args.expansion is not None else None,
             "supergroups": bool(args.supergroups and args.expansion is not None),
             "radius_inference": ("bf16" if args.bf16 else "fp32") if args.criterion == "learned" else None,
             "alpha": args.alpha if args.criterion in ("error", "learned") else None,
@@ -620,6 +755,7 @@ def main():
             "grouping": "cell-aligned",
             "walk_precision": "fp32 node-relative, fp32 accumulation",
             "build_precision": "fp64 keys and moments",
+            "build": args.build,
             "moments": ("quadrupole" if quad else "monopole") + " interactions"
                        + ("; fp64 quadrupoles built for the criterion" if args.criterion != "theta" else ""),
             "parallelism": "single GPU",
