@@ -262,18 +262,13 @@ ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes,
  * geometric test at `theta` (the guard) and r_min exceeds the opening radius stored in aux[3] of
  * its slot by ptg_open_radii (the learned criterion); eps is unused (it is baked into the radii). */
 #define PTG_CRIT_RADIUS 2
-/* group_expansion != 0 (no reference counterpart; FP32 walk, monopole interactions, potential and/or
+/* group_expansion = 1 (0 = off; no reference counterpart; FP32 walk, monopole interactions, potential and/or
  * acceleration): an accepted element of mass M at distance D from the centre of a group's bounding
  * box (R = its half diagonal, x = R / D < 1) whose truncation bound (M / D^2) x^3 (4 - 3x) / (1 - x)^2
  * is below expansion_eps is not evaluated per target but folded into a third-order Taylor expansion
  * of its field about the group centre, evaluated once per target after the traversal.
  * expansion_eps is an absolute acceleration (G = 1), times eps_scale[s] (device, nullable; s = 0 for
- * one tree, the group's system for a forest) when given.  group_expansion == 2 adds a shared pass
- * (needs group offsets and ptg_walk_shared_workspace_bytes more workspace): for each supergroup (8
- * consecutive groups of one system, counted from its first group) one wavefront folds the elements every group of it accepts and whose expansion
- * about the supergroup's centre is within the tolerance, opens the nodes every group opens, and lists
- * the rest as a frontier from which each of its groups then walks; the per-group accepted-element
- * sets are those of the one-level walk. */
+ * one tree, the group's system for a forest) when given. */
 /* expansion_rel (group expansion; 0 = off): an element is folded only if its truncation bound is
  * also below expansion_rel x (M / D^2), its own field at the group centre, i.e. x = R / D is capped
  * by x^3 (4 - 3x) / (1 - x)^2 < expansion_rel whatever the absolute tolerance allows.
@@ -293,9 +288,6 @@ typedef struct {
 } ptg_criterion;
 
 size_t ptg_walk_workspace_bytes(int64_t nt, int group_size);
-/* Extra workspace (appended after ptg_walk_workspace_bytes) for group_expansion == 2; nsys = the
- * forest's system count (1 for one tree). */
-size_t ptg_walk_shared_workspace_bytes(int64_t nt, int64_t nsys, int group_size);
 /* targets_sorted != 0: pos and soft are already in sorted target order (pos[t], soft[t], e.g. the
  * tree's own Morton-ordered coordinates and softenings for self gravity); perm then only places
  * the outputs. */
@@ -316,9 +308,8 @@ ptg_status ptg_walk_check(void *ws, void *stream);
  * spill entries per wave for the 5120 waves of a 256-CU grid; measured walks need at most 1344).
  * Nothing is allocated and nothing synchronises.  err (device int32, required) receives the
  * walk's flags instead of ptg_walk_check: 0 fine, bit 0 a traversal stack exceeded the spill
- * capacity, bit 1 a supplied group held more than group_size targets.  Rejected with PTG_ERR_ARG:
- * group_expansion == 2 (its frontier pool grows with nt), and a device whose resident grid would
- * leave fewer than 2048 spill entries per wave.  Every call on one GPU shares that GPU's array:
+ * capacity, bit 1 a supplied group held more than group_size targets.  Rejected with PTG_ERR_ARG: a
+ * device whose resident grid would leave fewer than 2048 spill entries per wave.  Every call on one GPU shares that GPU's array:
  * calls that overlap on one device (different streams) race. */
 ptg_status ptg_walk_noalloc(const ptg_walk_tree *wt, const ptg_forest *forest, const double *pos, const double *soft,
                             const int64_t *perm, int targets_sorted, int64_t nt, int fields, int quadrupole,

@@ -127,7 +127,6 @@ SIGNATURES = {
                                      _vp, _vp, _vp, _c.c_size_t, _vp, _vp]),
     "ptg_column_workspace_bytes": (_c.c_size_t, [_i64]),
     "ptg_random_rotations": (_c.c_int, [_i64, _i64, _vp, _i64, _vp, _vp]),
-    "ptg_walk_shared_workspace_bytes": (_c.c_size_t, [_i64, _i64, _c.c_int]),
     "ptg_walk": (_c.c_int, [_c.POINTER(WalkTree), _F, _vp, _vp, _vp, _c.c_int, _i64, _c.c_int, _c.c_int, _c.c_int,
                             _vp, _vp,
                             _dbl, _c.POINTER(Criterion), _dbl, _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),

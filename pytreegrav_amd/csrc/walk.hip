@@ -11,14 +11,7 @@
 // within the tolerance is not listed; the lane that accepted it adds its derivative tensors to the
 // lane's local-expansion coefficients, the wave sums them once per group and every target
 // evaluates the expansion at its offset from the centre.
-//
-// Shared supergroup pass (group expansion, second level): supergroups are runs of SUPER
-// consecutive groups of one system (counted from the system's first group).  One wave walks the tree for S's bounding box first: elements every
-// group of S accepts and whose expansion about S's centre is within the tolerance are folded into
-// S's expansion, nodes every group of S opens are opened, and everything else (the elements whose
-// fate differs between S's groups) is written to S's frontier list.  Each group of S then walks from
-// S's frontier instead of the root, with its own criterion and expansion, and adds S's expansion
-// evaluated at its targets.  The accepted-element set per group is the one-level walk's.
+
 #include <algorithm>
 #include <mutex>
 
@@ -32,8 +25,6 @@
 // six-wave variant); a list holds LF - 1 + one step's 64 entries (ICAP)
 #define NXCD 8
 #define CTR_STRIDE 32  // int32 words between per-XCD counters (128 B)
-#define SUPER 8        // groups per supergroup of the shared pass
-#define SX_STRIDE 20   // floats per supergroup record: 16 expansion coefficients, centre, pad
 
 template <typename T> struct alignas(4 * sizeof(T)) V4 {
     T x, y, z, w;
@@ -51,7 +42,6 @@ template <typename T, bool QUAD, int LF> struct WaveLds {
 
 struct WalkCtl {
     int32_t ctr[NXCD * CTR_STRIDE];   // per-XCD group counters
-    int32_t ctr2[NXCD * CTR_STRIDE];  // per-XCD supergroup counters (shared pass)
     int32_t err;
     int32_t hwm;  // deepest global spill stack (entries) any wave reached in the last walk
     int32_t pad[30];
@@ -79,12 +69,6 @@ template <typename T> struct WalkArgs {
     T eps_exp;                 // group expansion: per-element tolerance of the expansion test
     T exp_rel;                 // group expansion: cap of the truncation bound relative to M / D^2 (0: none)
     const double *eps_scale;   // nullable: eps_exp * eps_scale[system] (one scale per system)
-    // shared supergroup pass (two_level != 0; needs goff)
-    int two_level;
-    float *sexp;          // (nsup, SX_STRIDE) supergroup expansions and centres
-    int2 *srec;           // frontier (start in spool, count) per supergroup record; count -1: none
-    int *spool;           // frontier pool, pool_cap entries per wave of the shared pass
-    int64_t pool_cap;
     double G;
     double ox, oy, oz;
     // forest: group g walks the tree of the system s holding its first target (root at slot s),
@@ -416,39 +400,6 @@ __device__ __forceinline__ bool fold_test(T M, T Dx, T Dy, T Dz, T rg, T eps, T 
     return x < T(1) && M * lhs < eps * D2 * rhs && (rel <= T(0) || lhs < rel * rhs);
 }
 
-// first index i in [lo, hi) with a[i] >= v (a ascending)
-__device__ __forceinline__ int64_t lower_index(const int32_t *a, int64_t lo, int64_t hi, int64_t v)
-{
-    while (lo < hi) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (a[mid] < v) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// The supergroup of group g: the groups of g's system are cut into runs of SUPER from the system's
-// first group.  Returns the run's first group (leader) and end, and its record index
-// leader / SUPER + system (increasing along the groups, so unique; at most
-// ceil(ngroups / SUPER) + nsys records).
-struct SuperOf {
-    int64_t leader, end, rec;
-};
-template <typename T>
-__device__ __forceinline__ SuperOf super_of(const WalkArgs<T> &A, int64_t g, int64_t ngroups, int64_t sys)
-{
-    int64_t gfirst = 0, gnext = ngroups;
-    if (A.nsys > 0) {  // groups never cross systems, so system starts are group starts
-        gfirst = lower_index(A.goff, 0, ngroups, A.soff[sys]);
-        if (sys + 1 < A.nsys) gnext = lower_index(A.goff, gfirst, ngroups, A.soff[sys + 1]);
-    }
-    SuperOf r;
-    r.leader = gfirst + ((g - gfirst) / SUPER) * SUPER;
-    r.end = std::min<int64_t>(r.leader + SUPER, gnext);
-    r.rec = r.leader / SUPER + sys;
-    return r;
-}
-
 __device__ __forceinline__ float wave_fsum(float v)
 {
 #pragma unroll
@@ -716,40 +667,20 @@ void k_walk(WalkArgs<T> A)
                 eps_x = A.eps_exp * (A.eps_scale ? (T)A.eps_scale[root] : T(1));
             }
 
-            // with the shared pass the group walks from its supergroup's frontier (unless that pass
-            // gave up on the supergroup) and adds the supergroup's expansion at the end
-            int2 rec = make_int2(0, -1);
-            int64_t srec_i = 0;
-            if constexpr (EXP)
-                if (A.two_level) {
-                    srec_i = super_of(A, g, ngroups, root).rec;
-                    rec = A.srec[srec_i];
-                    rec.x = __builtin_amdgcn_readfirstlane(rec.x);
-                    rec.y = __builtin_amdgcn_readfirstlane(rec.y);
-                }
-            const bool shared = rec.y >= 0;
-            const int *front = A.spool + (shared ? rec.x : 0);
-            int fsp = shared ? rec.y : 0;
-            int sp = shared ? 0 : 1, gsp = 0, nn = 0, np = 0;
+            int sp = 1, gsp = 0, nn = 0, np = 0;
             bool overflow = false;
-            if (!shared && lane == 0) L.stack[0] = root;  // the root of system s occupies walk slot s
+            if (lane == 0) L.stack[0] = root;  // the root of system s occupies walk slot s
             __builtin_amdgcn_wave_barrier();
 
             for (;;) {
                 // pop up to 64 stack entries, one per lane, refilling the LDS stack from the spill
-                // region, then from the supergroup's frontier, when it is empty
+                // region when it is empty
                 if (sp == 0) {
                     if (gsp > 0) {
                         const int k = std::min(gsp, SPILL);
                         for (int q = lane; q < k; q += 64) L.stack[q] = spill[gsp - k + q];
                         __builtin_amdgcn_wave_barrier();
                         gsp -= k;
-                        sp = k;
-                    } else if (fsp > 0) {  // read like a spill region
-                        const int k = std::min(fsp, SPILL);
-                        for (int q = lane; q < k; q += 64) L.stack[q] = front[fsp - k + q];
-                        __builtin_amdgcn_wave_barrier();
-                        fsp -= k;
                         sp = k;
                     } else {
                         break;
@@ -900,14 +831,6 @@ void k_walk(WalkArgs<T> A)
                 exp_wave_sum(E);
 #pragma unroll
                 for (int j = 0; j < TPL; j++) exp_eval<T, FIELDS>(acc[j], E, tx[j] - cx, ty[j] - cy, tz[j] - cz);
-                if (shared) {
-                    const float *c = A.sexp + srec_i * SX_STRIDE;
-                    const GroupExp<T> ES{c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7],
-                                         c[8], c[9], c[10], c[11], c[12], c[13], c[14], c[15]};
-                    const T sx = c[16], sy = c[17], sz = c[18];
-#pragma unroll
-                    for (int j = 0; j < TPL; j++) exp_eval<T, FIELDS>(acc[j], ES, tx[j] - sx, ty[j] - sy, tz[j] - sz);
-                }
             }
 
             // ---- write G * fields in the caller's order ----
@@ -946,217 +869,6 @@ void k_walk(WalkArgs<T> A)
     }
 }
 
-// Shared supergroup pass (see the top of this file): one wave per supergroup, FP32 pack.  An
-// element is folded when every group of the supergroup accepts it (the test at the supergroup's
-// box with its largest target softening implies each group's test) and its expansion about the
-// supergroup's centre is within the tolerance; a node is opened when every group opens it (the
-// farthest point of the box, with the smallest target softening, still fails the test); anything
-// else goes to the frontier.  A supergroup whose stack or frontier region overflows is marked
-// (count -1) and its groups walk from the root instead.
-__global__ __launch_bounds__(64 * WALK_WAVES) void k_walk_super(WalkArgs<float> A)
-{
-    using T = float;
-    __shared__ int stk[WALK_WAVES][SCAP];
-    __shared__ int fb[WALK_WAVES][128];
-    const int wave = threadIdx.x >> 6, lane = ptg_lane();
-    int *stack = stk[wave];
-    int *fbuf = fb[wave];
-    const int64_t gwave = (int64_t)blockIdx.x * WALK_WAVES + wave;
-    int *spill = A.spill + gwave * (int64_t)A.gcap;
-    int *pool = A.spool + gwave * A.pool_cap;
-    int64_t pused = 0, n_vis = 0, n_exp = 0;
-    const int xcd = blockIdx.x % NXCD;
-    const int64_t ngroups = (int64_t)__builtin_amdgcn_readfirstlane(*A.gcount);
-
-    // persistent waves take groups in chunks of SUPER from per-XCD ranges; each group that leads a
-    // supergroup is processed, the others are skipped
-    const int64_t nchunk = (ngroups + SUPER - 1) / SUPER;
-    for (int attempt = 0; attempt < NXCD; attempt++) {
-        const int xx = (xcd + attempt) % NXCD;
-        const int64_t c_lo = nchunk * xx / NXCD, c_hi = nchunk * (xx + 1) / NXCD;
-        for (;;) {
-            int ci = 0;
-            if (lane == 0) ci = atomicAdd(&A.ctl->ctr2[xx * CTR_STRIDE], 1);
-            ci = __builtin_amdgcn_readfirstlane(ci);
-            const int64_t chunk = c_lo + ci;
-            if (chunk >= c_hi) break;
-          for (int64_t g = chunk * SUPER; g < std::min<int64_t>(chunk * SUPER + SUPER, ngroups); g++) {
-            const int64_t a0g = A.goff[g];
-            int root = 0;
-            double ox = A.ox, oy = A.oy, oz = A.oz;
-            if (A.nsys > 0) {
-                int64_t lo = 0, hi = A.nsys;  // last system with soff[s] <= a0g
-                while (hi - lo > 1) {
-                    const int64_t mid = (lo + hi) >> 1;
-                    if (A.soff[mid] <= a0g) lo = mid;
-                    else hi = mid;
-                }
-                root = (int)lo;
-                ox = A.cubes[4 * lo];
-                oy = A.cubes[4 * lo + 1];
-                oz = A.cubes[4 * lo + 2];
-            }
-            const SuperOf so = super_of(A, g, ngroups, root);
-            if (so.leader != g) continue;
-            const int64_t S = so.rec;
-            const int64_t a0 = a0g, b0 = A.goff[so.end];
-            // the supergroup's box and softening range, from the same FP32 target coordinates the
-            // groups compute
-            T bmin0 = INFINITY, bmin1 = INFINITY, bmin2 = INFINITY, hmin = INFINITY;
-            T bmax0 = -INFINITY, bmax1 = -INFINITY, bmax2 = -INFINITY, hmax = -INFINITY;
-            for (int64_t q = lane; q < b0 - a0; q += 64) {
-                const int64_t src = A.sorted ? a0 + q : A.perm[a0 + q];
-                const T px = (T)(A.pos[3 * src] - ox), py = (T)(A.pos[3 * src + 1] - oy);
-                const T pz = (T)(A.pos[3 * src + 2] - oz), ph = (T)A.soft[src];
-                bmin0 = std::min(bmin0, px);
-                bmin1 = std::min(bmin1, py);
-                bmin2 = std::min(bmin2, pz);
-                bmax0 = std::max(bmax0, px);
-                bmax1 = std::max(bmax1, py);
-                bmax2 = std::max(bmax2, pz);
-                hmin = std::min(hmin, ph);
-                hmax = std::max(hmax, ph);
-            }
-            bmin0 = ptg_wave_min(bmin0);
-            bmin1 = ptg_wave_min(bmin1);
-            bmin2 = ptg_wave_min(bmin2);
-            bmax0 = ptg_wave_max(bmax0);
-            bmax1 = ptg_wave_max(bmax1);
-            bmax2 = ptg_wave_max(bmax2);
-            hmin = ptg_wave_min(hmin);
-            hmax = ptg_wave_max(hmax);
-            const T cx = T(0.5) * (bmin0 + bmax0), cy = T(0.5) * (bmin1 + bmax1), cz = T(0.5) * (bmin2 + bmax2);
-            const T hx = T(0.5) * (bmax0 - bmin0), hy = T(0.5) * (bmax1 - bmin1), hz = T(0.5) * (bmax2 - bmin2);
-            const T rg = sqrt(hx * hx + hy * hy + hz * hz) * T(1.000001);
-            const T eps_x = A.eps_exp * (A.eps_scale ? (T)A.eps_scale[root] : T(1));
-            GroupExp<T> E{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-
-            const int64_t fstart = pused;
-            int sp = 1, gsp = 0, nf = 0;
-            bool fail = b0 <= a0;
-            if (lane == 0) stack[0] = root;
-            __builtin_amdgcn_wave_barrier();
-            while (!fail) {
-                if (sp == 0) {
-                    if (gsp == 0) break;
-                    const int k = std::min(gsp, SPILL);
-                    for (int q = lane; q < k; q += 64) stack[q] = spill[gsp - k + q];
-                    __builtin_amdgcn_wave_barrier();
-                    gsp -= k;
-                    sp = k;
-                }
-                const int k = std::min(sp, 64);
-                const int base = sp - k;
-                int e = lane < k ? stack[base + lane] : 0;
-                const bool ve = lane < k && e >= 0 && e < A.nslots;
-                if (!ve) e = 0;
-                __builtin_amdgcn_wave_barrier();
-                sp = base;
-                n_vis += k;
-                const V4<T> X = A.xyzm[e];
-                const V4<T> X2 = A.aux[e];
-                const int2 K = A.kids[e];
-                const bool ispart = K.y == 0;
-                // nearest and farthest points of the box from the element
-                const T dxm = box_gap(X.x, bmin0, bmax0), dym = box_gap(X.y, bmin1, bmax1), dzm = box_gap(X.z, bmin2, bmax2);
-                const T r2m = dxm * dxm + dym * dym + dzm * dzm;
-                const T fx = std::max(std::fabs(X.x - bmin0), std::fabs(X.x - bmax0));
-                const T fy = std::max(std::fabs(X.y - bmin1), std::fabs(X.y - bmax1));
-                const T fz = std::max(std::fabs(X.z - bmin2), std::fabs(X.z - bmax2));
-                const T rfar2 = fx * fx + fy * fy + fz * fz;
-                const T c1 = X2.y * A.inv_theta + X2.z;
-                const T hh_hi = X2.x > hmax ? X2.x : hmax, hh_lo = X2.x > hmin ? X2.x : hmin;
-                const T c2_hi = hh_hi + X2.y * T(0.6) + X2.z, c2_lo = hh_lo + X2.y * T(0.6) + X2.z;
-                T thr_hi = c1 > c2_hi ? c1 : c2_hi, thr_lo = c1 > c2_lo ? c1 : c2_lo;
-                if (A.crit == 2) {
-                    thr_hi = thr_hi > X2.w ? thr_hi : X2.w;
-                    thr_lo = thr_lo > X2.w ? thr_lo : X2.w;
-                }
-                bool acc = ve & !ispart & (r2m > thr_hi * thr_hi);
-                bool open_all = ve & !ispart & (rfar2 <= thr_lo * thr_lo);
-                if (A.crit == 1) {
-                    acc = acc & (X2.w < A.eps * r2m * r2m);
-                    open_all = open_all | (ve & !ispart & (X2.w >= A.eps * rfar2 * rfar2));
-                }
-                acc = acc | (ve & ispart & (r2m > hh_hi * hh_hi));
-                bool fold = false;
-                if (acc) {
-                    const T Dx = X.x - cx, Dy = X.y - cy, Dz = X.z - cz;
-                    T rinv;
-                    if (fold_test(X.w, Dx, Dy, Dz, rg, eps_x, A.exp_rel, rinv)) {
-                        exp_add(E, X.w, Dx, Dy, Dz, rinv);
-                        fold = true;
-                        n_exp++;
-                    }
-                }
-                const bool fr = ve & !fold & !open_all;
-                const uint64_t bf = ptg_ballot(fr);
-                if (fr) fbuf[nf + lanes_below(bf)] = e;
-                nf += (int)__popcll(bf);
-                __builtin_amdgcn_wave_barrier();
-                if (nf >= 64) {
-                    if (pused + 64 > A.pool_cap) {
-                        fail = true;
-                        break;
-                    }
-                    pool[pused + lane] = fbuf[lane];
-                    __builtin_amdgcn_wave_barrier();
-                    if (lane < nf - 64) fbuf[lane] = fbuf[64 + lane];  // reads [64, nf), writes [0, nf-64)
-                    __builtin_amdgcn_wave_barrier();
-                    nf -= 64;
-                    pused += 64;
-                }
-                const int cnt = open_all ? std::min(std::max(K.y, 0), 8) : 0;
-                int tot;
-                const int below = small_excl_scan(cnt, tot);
-                if (sp + tot > SCAP) {
-                    const int ks = spill_count(sp, tot);
-                    if (gsp + ks > A.gcap) {
-                        fail = true;
-                        break;
-                    }
-                    spill_bottom(stack, spill + gsp, sp, ks, lane);
-                    gsp += ks;
-                    sp -= ks;
-                }
-                int *dst = &stack[sp + below];
-#pragma unroll
-                for (int c = 0; c < 8; c++)
-                    if (c < cnt) dst[c] = K.x + c;
-                sp += tot;
-                __builtin_amdgcn_wave_barrier();
-            }
-            if (!fail && nf > 0) {
-                if (pused + nf > A.pool_cap) {
-                    fail = true;
-                } else {
-                    if (lane < nf) pool[pused + lane] = fbuf[lane];
-                    pused += nf;
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-            exp_wave_sum(E);
-            if (lane == 0) {
-                A.srec[S] = fail ? make_int2(0, -1) : make_int2((int)(gwave * A.pool_cap + fstart), (int)(pused - fstart));
-                float *o = A.sexp + S * SX_STRIDE;
-                o[0] = E.a0; o[1] = E.gx; o[2] = E.gy; o[3] = E.gz;
-                o[4] = E.xx; o[5] = E.yy; o[6] = E.xy; o[7] = E.xz; o[8] = E.yz;
-                o[9] = E.xxx; o[10] = E.xxy; o[11] = E.xxz; o[12] = E.xyy; o[13] = E.xyz; o[14] = E.yyy;
-                o[15] = E.yyz; o[16] = cx; o[17] = cy; o[18] = cz; o[19] = 0.f;
-            }
-            if (fail) pused = fstart;  // the supergroup's partial frontier is dropped
-          }
-        }
-    }
-    if (A.counters) {
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) n_exp += __shfl_xor(n_exp, d, 64);
-        if (lane == 0) {
-            atomicAdd((unsigned long long *)&A.counters[1], (unsigned long long)n_vis);
-            atomicAdd((unsigned long long *)&A.counters[3], (unsigned long long)n_exp);
-        }
-    }
-}
 
 __global__ void k_walk_init(WalkCtl *ctl)
 {
@@ -1196,31 +908,6 @@ extern "C" size_t ptg_walk_workspace_bytes(int64_t nt, int group_size)
     (void)group_size;
     size_t waves = (size_t)walk_grid_blocks() * WALK_WAVES;
     return ptg_align256(sizeof(WalkCtl)) + ptg_align256(waves * GCAP * sizeof(int));
-}
-
-// geometry of the shared supergroup pass for nt targets: supergroup records (at most
-// ceil(groups / SUPER) + nsys, and a group holds at least one target), blocks of the pass,
-// frontier entries per wave
-struct SharedGeom {
-    int64_t nsup, blocks, pool_cap;
-};
-static SharedGeom shared_geom(int64_t nt, int64_t nsys)
-{
-    SharedGeom g;
-    g.nsup = ptg_cdiv(nt, SUPER) + std::max<int64_t>(nsys, 1) + 1;
-    g.blocks = std::max<int64_t>(1, std::min<int64_t>(walk_grid_blocks(), ptg_cdiv(g.nsup, WALK_WAVES)));
-    const int64_t waves = g.blocks * WALK_WAVES;
-    g.pool_cap = std::max<int64_t>(8192, ptg_cdiv(8 * nt, waves));
-    g.pool_cap = std::min<int64_t>(g.pool_cap, (int64_t)(INT32_MAX - 64) / waves);  // int32 frontier offsets
-    return g;
-}
-
-extern "C" size_t ptg_walk_shared_workspace_bytes(int64_t nt, int64_t nsys, int group_size)
-{
-    (void)group_size;
-    const SharedGeom g = shared_geom(nt, nsys);
-    return ptg_align256((size_t)g.nsup * sizeof(int2)) + ptg_align256((size_t)g.nsup * SX_STRIDE * sizeof(float)) +
-           ptg_align256((size_t)g.blocks * WALK_WAVES * g.pool_cap * sizeof(int));
 }
 
 // ---- static scratch of ptg_walk_noalloc ----
@@ -1417,13 +1104,8 @@ static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, c
     }
     const double eps_exp = expand ? criterion->expansion_eps : 0.0;
     const double *eps_scale = expand ? criterion->eps_scale : nullptr;
-    const bool two_level = expand && criterion->group_expansion == 2;
-    if (expand && criterion->group_expansion != 1 && criterion->group_expansion != 2) {
-        ptg_set_error("ptg_walk: group_expansion must be 0, 1 (per group) or 2 (with the shared supergroup pass)");
-        return PTG_ERR_ARG;
-    }
-    if (two_level && !group_offsets) {
-        ptg_set_error("ptg_walk: the shared supergroup pass needs group offsets (ptg_target_groups)");
+    if (expand && criterion->group_expansion != 1) {
+        ptg_set_error("ptg_walk: group_expansion must be 0 (off) or 1");
         return PTG_ERR_ARG;
     }
     const int variant = criterion ? criterion->variant : 0;
@@ -1434,11 +1116,6 @@ static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, c
     int64_t blocks = walk_grid_blocks(variant);
     int gcap = GCAP;
     if (noalloc) {
-        if (two_level) {
-            ptg_set_error("ptg_walk_noalloc: the shared supergroup pass (group_expansion 2) needs a frontier pool "
-                          "that grows with the target count; use ptg_walk with a workspace");
-            return PTG_ERR_ARG;
-        }
         if (!err_out) {
             ptg_set_error("ptg_walk_noalloc: err (a device int32 that receives the overflow flag) is required");
             return PTG_ERR_ARG;
@@ -1456,27 +1133,13 @@ static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, c
     }
     if (nt == 0) return PTG_OK;
     const size_t base_bytes = noalloc ? ws_bytes : ptg_walk_workspace_bytes(nt, group_size);
-    const int64_t nsys_ws = forest ? forest->nsys : 1;
-    const size_t need = base_bytes + (two_level ? ptg_walk_shared_workspace_bytes(nt, nsys_ws, group_size) : 0);
+    const size_t need = base_bytes;
     if (ws_bytes < need || !ws) {
         ptg_set_error("ptg_walk: workspace too small (%zu < %zu)", ws_bytes, need);
         return PTG_ERR_WORKSPACE;
     }
     WalkCtl *ctl = (WalkCtl *)ws;
     int *spill = (int *)((char *)ws + ptg_align256(sizeof(WalkCtl)));
-    SharedGeom sg{0, 0, 0};
-    int2 *srec = nullptr;
-    float *sexp = nullptr;
-    int *spool = nullptr;
-    if (two_level) {
-        sg = shared_geom(nt, nsys_ws);
-        char *p = (char *)ws + base_bytes;
-        srec = (int2 *)p;
-        p += ptg_align256((size_t)sg.nsup * sizeof(int2));
-        sexp = (float *)p;
-        p += ptg_align256((size_t)sg.nsup * SX_STRIDE * sizeof(float));
-        spool = (int *)p;
-    }
     const int64_t fn = forest ? forest->nsys : 0;
     const int64_t *fo = forest ? forest->offsets : nullptr;
     const double *fc = forest ? forest->cubes : nullptr;
@@ -1487,22 +1150,18 @@ static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, c
         WalkArgs<float> a{(const V4<float> *)wt->xyzm, (const V4<float> *)wt->aux, (const int2 *)wt->kids,
                           (const float *)wt->quad, pos, soft, perm, targets_sorted != 0, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
                           group_size, (float)theta, (float)(1.0 / theta), crit, (float)eps, (float)eps_exp,
-                          expand ? criterion->expansion_rel : 0.0f, eps_scale,
-                          two_level ? 1 : 0, sexp, srec, spool, sg.pool_cap, G, wt->origin[0], wt->origin[1], wt->origin[2],
+                          expand ? criterion->expansion_rel : 0.0f, eps_scale, G, wt->origin[0], wt->origin[1],
+                          wt->origin[2],
                           fn, fo, fc, out, counters, ctl, spill, gcap};
         if (exact) {
             ptg_set_error("ptg_walk: the exact branch set is evaluated in FP64; use prec=PTG_F64");
             return PTG_ERR_ARG;
         }
-        if (two_level) {
-            k_walk_super<<<(unsigned)sg.blocks, 64 * WALK_WAVES, 0, s>>>(a);
-            PTG_LAUNCH_CHECK();
-        }
         st = launch_walk<float, false>(a, fields, quadrupole != 0, expand, variant, blocks, s);
     } else if (wt->prec == PTG_F64) {
         WalkArgs<double> a{(const V4<double> *)wt->xyzm, (const V4<double> *)wt->aux, (const int2 *)wt->kids,
                            (const double *)wt->quad, pos, soft, perm, targets_sorted != 0, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
-                           group_size, theta, 1.0 / theta, crit, eps, 0.0, 0.0, nullptr, 0, nullptr, nullptr, nullptr, 0, G,
+                           group_size, theta, 1.0 / theta, crit, eps, 0.0, 0.0, nullptr, G,
                            wt->origin[0], wt->origin[1], wt->origin[2],
                            fn, fo, fc, out, counters, ctl, spill, gcap};
         st = launch_walk<double, true>(a, fields, quadrupole != 0, false, 0, blocks, s);

@@ -410,7 +410,7 @@ def target_groups(pos, perm, group_size, tree=None, mode="cells", keys=None):
 
 def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7, G=1.0, precision="float64",
          counters=None, grouping=None, groups=None, criterion="theta", eps=None, model=None, bf16=False,
-         eps_scale=None, sorted_targets=False, expansion=None, supergroups=False, variant=0, expansion_rel=0.0,
+         eps_scale=None, sorted_targets=False, expansion=None, variant=0, expansion_rel=0.0,
          _events_end=None, _check=True, _ws_slot=0):
     """Grouped walk of `tree` at targets pos[perm[t]]; returns an (nt, W) float64 device tensor in the
     original target order (FieldsTarget_grouped, grouped_treewalk.py:350-397).
@@ -428,9 +428,7 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     expansion: tolerance (absolute acceleration, G=1, times eps_scale when given) of the group
     expansion: accepted elements far enough from a group's centre are folded into a third-order
     Taylor expansion about it instead of being evaluated per target (FP32, monopole interactions,
-    potential and/or acceleration).  supergroups (with expansion): a shared pass per 8 consecutive
-    groups folds what all of them accept into one expansion and hands each group the frontier it
-    walks from (the same accepted elements per group, less traversal).  expansion_rel: also fold an
+    potential and/or acceleration).  expansion_rel: also fold an
     element only when its truncation bound is below expansion_rel x its own field M / D^2 at the group
     centre (a cap on R / D; 0 = none).  variant: the FP32 walk
     kernel's occupancy / interaction-list variant (an index into WALK_VARIANTS; the autotuner picks
@@ -451,7 +449,7 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
         rec = autotune_group_size(tree, pos, soft, perm, mask, quadrupole=quadrupole, theta=theta,
                                   precision=precision, grouping=grouping, criterion=criterion, eps=eps,
                                   model=model, bf16=bf16, expansion=expansion, eps_scale=eps_scale,
-                                  supergroups=supergroups)
+                                  expansion_rel=expansion_rel)
         group_size, variant = rec["choice"], rec["variant"]
     if not 0 <= int(variant) < len(WALK_VARIANTS):
         raise ValueError(f"variant must index WALK_VARIANTS (0..{len(WALK_VARIANTS) - 1}), got {variant!r}")
@@ -462,9 +460,7 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     forest = tree.nsys > 1 or tree.forest is not None
     if tree.nsys > 1 and (nt != tree.n or perm.data_ptr() != tree.order.data_ptr()):
         raise ValueError("a forest is walked at its own particles (perm = tree.order)")
-    if supergroups and expansion is None:
-        raise ValueError("supergroups needs the group expansion (expansion=...)")
-    if groups is None and (grouping == "cells" or tree.nsys > 1 or supergroups):
+    if groups is None and (grouping == "cells" or tree.nsys > 1):
         own = nt == tree.n and perm.data_ptr() == tree.order.data_ptr()
         groups = target_groups(pos, perm, group_size, tree=tree if forest else None, mode=grouping,
                                keys=tree.keys if own else None)
@@ -503,7 +499,7 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
             raise ValueError("eps_scale needs one value per system of the forest")
         if not expansion_rel >= 0:
             raise ValueError("expansion_rel must be >= 0")
-        crit.group_expansion = 2 if supergroups else 1
+        crit.group_expansion = 1
         crit.expansion_eps = float(expansion)
         crit.expansion_rel = float(expansion_rel)
         crit.eps_scale = eps_scale.data_ptr() if eps_scale is not None else None
@@ -512,8 +508,6 @@ def walk(tree, pos, soft, perm, mask, quadrupole=False, group_size=8, theta=0.7,
     else:
         p = tree.pack(prec, quadrupole, norms=True if criterion == "error" else None)
     wsb = L.ptg_walk_workspace_bytes(nt, int(group_size))
-    if supergroups:
-        wsb += L.ptg_walk_shared_workspace_bytes(nt, int(tree.nsys if forest else 1), int(group_size))
     ws = _walk_workspace(wsb, _ws_slot)
     check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), tree.forest_ref if forest else None, ptr(pos), ptr(soft), ptr(perm),
                      int(bool(sorted_targets)), nt, int(mask), int(bool(quadrupole)),
@@ -558,7 +552,7 @@ def _side_stream():
 
 def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, group_size=64, criterion="theta",
                 alpha=None, model=None, bf16=False, counters=None, walk_events=None, expansion=None,
-                supergroups=False, stats=None, expansion_rel=0.0):
+                stats=None, expansion_rel=0.0, variant=0):
     """Self-gravity FP32 fields of the particles a tree was built from, in their input order.
 
     The cell-aligned target groups are formed on a side stream while the main stream packs the
@@ -568,7 +562,7 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
     the other two).  expansion: group-expansion tolerance in units of the RMS field (see walk);
     None walks without it.  walk_events: optional (start, end) torch.cuda.Event pair recorded around
     the walk launch alone.  stats: optional dict that receives "groups", the device int32 count of
-    the target groups walked."""
+    the target groups walked.  variant: the FP32 walk kernel variant (WALK_VARIANTS)."""
     L = _lib.load()
     pos = to_dev(pos, (-1, 3))
     n = pos.shape[0]
@@ -634,7 +628,7 @@ def self_fields(tree, pos, soft, mask, theta=None, G=1.0, quadrupole=False, grou
     out = walk(tree, tree.coordinates[:n], tree.softenings[:n], perm, mask, quadrupole=quadrupole,
                group_size=group_size, theta=theta, G=G, precision="float32", counters=counters,
                groups=(offsets, count), criterion=criterion, eps=eps, model=model, bf16=bf16, eps_scale=scale,
-               sorted_targets=True, expansion=exp_eps, supergroups=supergroups, expansion_rel=expansion_rel,
+               sorted_targets=True, expansion=exp_eps, expansion_rel=expansion_rel, variant=variant,
                _events_end=walk_events[1] if walk_events is not None else None)
     if scale is not None:
         check_walk(1)  # the field-scale sample walk was not checked when it ran
@@ -817,7 +811,7 @@ def tuning_log():
 
 def autotune_group_size(tree, pos, soft, perm, mask, quadrupole=False, theta=0.7, precision="float32", grouping=None,
                         criterion="theta", eps=None, model=None, bf16=False, sizes=TUNE_SIZES, reps=2, force=False,
-                        expansion=None, eps_scale=None, supergroups=False):
+                        expansion=None, eps_scale=None, expansion_rel=0.0):
     """Pick the walk's target-group size and kernel variant by timing the walk (groups included).
 
     Candidates: every group size in `sizes`; for the FP32 one-target-per-lane kernels (groups of
@@ -827,13 +821,13 @@ def autotune_group_size(tree, pos, soft, perm, mask, quadrupole=False, theta=0.7
     opens nodes for its whole bounding box, so it trades more interactions per target for fewer,
     longer traversals; the variants give the same fields, bit for bit or to FP32 rounding).  Decisions are cached per (tree
     size, target count, distribution statistics, fields, moment order, theta, precision, grouping,
-    criterion, tolerance, expansion) and listed by tuning_log()."""
+    criterion, tolerance, expansion and its relative cap) and listed by tuning_log()."""
     pos = to_dev(pos, (-1, 3))
     nt = pos.shape[0]
     key = (_bucket(tree.n), _bucket(nt), distribution_key(tree, pos), int(mask), bool(quadrupole), float(theta),
            str(precision), grouping, criterion, None if eps is None else float(eps),
            None if model is None else model.key(), bool(bf16), None if expansion is None else float(expansion),
-           bool(supergroups))
+           float(expansion_rel))
     if key in _tuned and not force:
         return _tuned[key]
     fp32 = precision in ("float32", "f32", np.float32, torch.float32)
@@ -846,7 +840,7 @@ def autotune_group_size(tree, pos, soft, perm, mask, quadrupole=False, theta=0.7
         ctr = torch.zeros(4, dtype=torch.int64, device=device())
         kw = dict(quadrupole=quadrupole, group_size=gs, theta=theta, precision=precision, grouping=grouping,
                   criterion=criterion, eps=eps, model=model, bf16=bf16, expansion=expansion, eps_scale=eps_scale,
-                  supergroups=supergroups, variant=v)
+                  expansion_rel=expansion_rel, variant=v)
         walk(tree, pos, soft, perm, mask, counters=ctr, **kw)  # warm-up (and interaction count)
         torch.cuda.synchronize()
         inter[f"{gs}/{v}"] = int(ctr[0].item())

@@ -76,14 +76,11 @@ class Forest:
                                    int(self.prec == PTG_F64), ptr(out), ptr(counters), ptr(ws), ws.numel(),
                                    stream()), "ptg_walk")
 
-    def _buffers(self, mask, gs, supergroups=False):
+    def _buffers(self, mask, gs):
         L = _lib.load()
         dev = device()
         gws = workspace(L.ptg_groups_workspace_bytes(self.n, self.nsys))
-        wsb = L.ptg_walk_workspace_bytes(self.n, gs)
-        if supergroups:
-            wsb += L.ptg_walk_shared_workspace_bytes(self.n, self.nsys, gs)
-        ws = workspace(wsb)
+        ws = workspace(L.ptg_walk_workspace_bytes(self.n, gs))
         out = torch.empty((self.n, field_width(mask)), dtype=torch.float64, device=dev)
         goff = torch.empty(self.n + 1, dtype=torch.int32, device=dev)
         gcnt = torch.empty(1, dtype=torch.int32, device=dev)
@@ -138,12 +135,10 @@ class Forest:
         return rms
 
     def _prepare(self, gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask=PTG_ACC, expansion=None,
-                 supergroups=False, t=None, check_walk=True):
+                 expansion_rel=0.0, t=None, check_walk=True):
         """Groups, and for the learned criterion the opening radii; returns (pack, criterion, theta).
         expansion: group-expansion tolerance in units of each system's RMS field (None: off);
-        supergroups: with it, the shared pass per 8 groups of a system."""
-        if supergroups and expansion is None:
-            raise ValueError("supergroups needs the group expansion (expansion=...)")
+        expansion_rel: its cap relative to each folded element's own field (0: none)."""
         t = t or self.tree
         self._groups(gs, goff, gcnt, gws, t=t)
         if expansion is not None:
@@ -152,9 +147,11 @@ class Forest:
                                  "and/or the acceleration")
             if not expansion > 0:
                 raise ValueError("expansion must be a positive tolerance (relative to each system's RMS field)")
+            if not expansion_rel >= 0:
+                raise ValueError("expansion_rel must be >= 0 (0: no cap)")
         if self.criterion != "learned":
             crit = _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
-            self._expand(crit, expansion, None, goff, gcnt, supergroups, t=t, check_walk=check_walk)
+            self._expand(crit, expansion, None, goff, gcnt, expansion_rel, t=t, check_walk=check_walk)
             return t.pack(self.prec, self.quadrupole), crit, (0.7 if theta is None else theta)
         if model is None:
             from .learned import OpeningMLP
@@ -170,35 +167,38 @@ class Forest:
         p = t.pack_learned(model, base, self.quadrupole, bf16, scale=scale)
         self._scale = scale  # keeps the scale alive while the pack is cached under its address
         crit = _lib.Criterion(_lib.PTG_CRIT_RADIUS, 0, 0.0)
-        self._expand(crit, expansion, scale if eps is None else None, goff, gcnt, supergroups, t=t,
+        self._expand(crit, expansion, scale if eps is None else None, goff, gcnt, expansion_rel, t=t,
                      check_walk=check_walk)
         return p, crit, (2.0 if theta is None else theta)
 
-    def _expand(self, crit, expansion, rms, goff, gcnt, supergroups=False, t=None, check_walk=True):
+    def _expand(self, crit, expansion, rms, goff, gcnt, expansion_rel=0.0, t=None, check_walk=True):
         """Switch the group expansion on in crit at expansion x each system's RMS field (device)."""
         if expansion is None:
             return
         if rms is None:
             rms = self.system_field_scales(goff, gcnt, t=t, check_walk=check_walk)
         self._exp_scale = rms  # the walk (and any captured graph) reads it through crit
-        crit.group_expansion = 2 if supergroups else 1
+        crit.group_expansion = 1
         crit.expansion_eps = float(expansion)
+        crit.expansion_rel = float(expansion_rel)
         crit.eps_scale = rms.data_ptr()
 
     def evaluate(self, potential=True, accel=True, tidal=False, theta=None, G=1.0, counters=None, alpha=None,
-                 eps=None, model=None, bf16=False, expansion=None, supergroups=False):
+                 eps=None, model=None, bf16=False, expansion=None, expansion_rel=0.0):
         """Fields of every particle from its own system, in input order: dict of device views
         ("potential" (n,), "accel" (n,3), "tidal" (n,3,3)).  theta defaults to 0.7 (geometric
         criterion) or 2.0 (the guard of the learned criterion, whose per-system tolerance is
         alpha x the system's RMS field, or the given per-system eps).  expansion: group-expansion
-        tolerance relative to each system's RMS field (float32 monopole walks; None: off)."""
+        tolerance relative to each system's RMS field (float32 monopole walks; None: off);
+        expansion_rel: its cap relative to each folded element's own field (0: none)."""
         if self.auto and self.tuning is None:
             self.autotune(potential, accel, tidal, theta, G, alpha=alpha, eps=eps, model=model, bf16=bf16,
-                          expansion=expansion, supergroups=supergroups)
+                          expansion=expansion, expansion_rel=expansion_rel)
         gs = self.group_size
         mask = fields_mask(potential, accel, tidal)
-        out, ws, goff, gcnt, gws = self._buffers(mask, gs, supergroups)
-        p, crit, th = self._prepare(gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask, expansion, supergroups)
+        out, ws, goff, gcnt, gws = self._buffers(mask, gs)
+        p, crit, th = self._prepare(gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask, expansion,
+                                    expansion_rel)
         self._walk(p, mask, th, G, out, ws, gs, goff, gcnt, crit, counters)
         check(_lib.load().ptg_walk_check(ptr(ws), stream()), "ptg_walk")
         return unpack_fields(out, mask)
@@ -209,10 +209,10 @@ class Forest:
         mask = fields_mask(potential, accel, tidal)
         times = {}
         for gs in sizes:
-            out, ws, goff, gcnt, gws = self._buffers(mask, gs, kw.get("supergroups", False))
+            out, ws, goff, gcnt, gws = self._buffers(mask, gs)
             p, crit, th = self._prepare(gs, theta, kw.get("alpha"), kw.get("eps"), kw.get("model"),
                                         kw.get("bf16", False), goff, gcnt, gws, mask, kw.get("expansion"),
-                                        kw.get("supergroups", False))
+                                        kw.get("expansion_rel", 0.0))
             best = float("inf")
             for _ in range(2):
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -229,15 +229,16 @@ class Forest:
         return self.tuning
 
     def capture(self, potential=True, accel=True, tidal=False, theta=None, G=1.0, alpha=None, eps=None, model=None,
-                bf16=False, expansion=None, supergroups=False):
+                bf16=False, expansion=None, expansion_rel=0.0):
         """Capture the walk stage (groups + walk over all systems) into a HIP graph.  The pack (and,
         for the learned criterion, the opening radii) are made once before the capture.  Returns
         (replay, out): replay() re-runs the stage and checks the traversal overflow flag; out is
         the (n, W) result buffer every replay rewrites."""
         gs = self.group_size
         mask = fields_mask(potential, accel, tidal)
-        out, ws, goff, gcnt, gws = self._buffers(mask, gs, supergroups)
-        p, crit, th = self._prepare(gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask, expansion, supergroups)
+        out, ws, goff, gcnt, gws = self._buffers(mask, gs)
+        p, crit, th = self._prepare(gs, theta, alpha, eps, model, bf16, goff, gcnt, gws, mask, expansion,
+                                    expansion_rel)
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):  # warm-up on the capture stream
@@ -259,7 +260,7 @@ class Forest:
         return replay, out
 
     def capture_step(self, potential=True, accel=True, theta=None, G=1.0, alpha=None, model=None, bf16=False,
-                     expansion=None, capacity=None):
+                     expansion=None, capacity=None, expansion_rel=0.0):
         """Capture one whole step -- tree build (ptg_build_deferred), target groups, for the learned
         criterion the per-system field scales and opening radii, and the walk -- into one HIP graph.
         Each replay recomputes every system's fields from the current contents of the input buffers
@@ -274,7 +275,7 @@ class Forest:
         mask = fields_mask(potential, accel, False)
         if self.auto and self.tuning is None:
             self.autotune(potential, accel, False, theta, G, alpha=alpha, model=model, bf16=bf16,
-                          expansion=expansion)
+                          expansion=expansion, expansion_rel=expansion_rel)
         gs = self.group_size
         if self.criterion == "learned":
             if alpha is None or not alpha > 0:
@@ -285,15 +286,15 @@ class Forest:
                 model = OpeningMLP.load()  # loaded before the capture (file and host-to-device copy)
         if capacity is None:
             capacity = (int(self.tree.nnodes * 1.25) + 64, min(255, int(self.tree.info.max_level) + 4))
-        out, ws, goff, gcnt, gws = self._buffers(mask, gs, False)
+        out, ws, goff, gcnt, gws = self._buffers(mask, gs)
         quad_build = self.quadrupole or self.criterion == "learned"
         held = {}
 
         def step():
             t = DeviceTree(self.pos, self.m, self.h, quadrupole=quad_build, offsets=self.tree.sys_offsets,
                            capacity=capacity)
-            p, crit, th = self._prepare(gs, theta, alpha, None, model, bf16, goff, gcnt, gws, mask, expansion, False,
-                                        t=t, check_walk=False)
+            p, crit, th = self._prepare(gs, theta, alpha, None, model, bf16, goff, gcnt, gws, mask, expansion,
+                                        expansion_rel, t=t, check_walk=False)
             self._walk(p, mask, th, G, out, ws, gs, goff, gcnt, crit, t=t)
             # everything the graph reads through a captured pointer lives as long as the graph: the
             # tree, packs and tolerances made inside the step, and the model's parameters (loaded

@@ -25,9 +25,10 @@ FP32_DEFAULT_GROUP = 64
 
 
 # Accel / Potential with method="adaptive" in float32 walk the tree above this many particles and sum
-# directly below it: on one MI355X a whole float32 call (tree build + walk vs the LDS-tiled direct sum,
-# Plummer spheres, h = 0.01) takes 0.83 vs 0.54 ms at N = 2000 and 0.91 vs 0.97 ms at N = 4000
-FP32_TREE_MIN_N = 3500
+# directly below it: on one MI355X a whole float32 Accel call from numpy inputs (tree build + walk vs
+# the direct sum, Plummer spheres, h = 0.01) takes 1.01 vs 0.62 ms at N = 16000 and 1.10 vs 1.48 ms
+# at N = 32000 (0.82 vs 0.42 and 0.84 vs 1.25 ms from device tensors)
+FP32_TREE_MIN_N = 24000
 
 
 class _DefaultTheta(float):
@@ -306,7 +307,7 @@ def TidalTensorTarget_grouped(pos, soft, tree, group_size=8, G=1.0, theta=0.7, q
 # shared drivers
 # ---------------------------------------------------------------------------------------------
 def _self_fields(pos, m, softening, mask, G, theta, tree, return_tree, parallel, method, quadrupole, group_size,
-                 precision, adaptive_n, criterion="theta", alpha=None, expansion=None):
+                 precision, adaptive_n, criterion="theta", alpha=None, expansion=None, expansion_rel=3.0):
     """Potential/Accel/TidalTensor (frontend.py:145-293, 421-560, 688-789).
 
     criterion "error"/"learned" (float32 tree walks only, no reference counterpart): accept a node
@@ -314,7 +315,8 @@ def _self_fields(pos, m, softening, mask, G, theta, tree, return_tree, parallel,
     theta acts as the reference's geometric guard (GUARD_THETA unless the caller gave one).  expansion (float32
     monopole walks of the potential and/or acceleration, no reference counterpart): fold accepted
     elements far enough from a target group into a third-order expansion about the group's centre
-    when its truncation bound is below expansion x the RMS field."""
+    when its truncation bound is below expansion x the RMS field and below expansion_rel x the
+    element's own field there (0: no such cap)."""
     valueTestMethod(method)
     if expansion is not None:
         if precision != "float32" or quadrupole or (mask & PTG_TIDAL):
@@ -322,6 +324,8 @@ def _self_fields(pos, m, softening, mask, G, theta, tree, return_tree, parallel,
                              "and/or the acceleration")
         if not expansion > 0:
             raise ValueError("expansion must be a positive tolerance (relative to the RMS field)")
+        if not expansion_rel >= 0:
+            raise ValueError("expansion_rel must be >= 0 (0: no cap)")
         method = "tree"
     if criterion not in ("theta", "error", "learned"):
         raise ValueError(f"criterion must be 'theta', 'error' or 'learned', got {criterion!r}")
@@ -356,22 +360,24 @@ def _self_fields(pos, m, softening, mask, G, theta, tree, return_tree, parallel,
             tree = ConstructTree(pos, m, softening, quadrupole=True)  # |Q| drives the criterion
             out = engine.self_fields(tree.dev, pos, softening, mask, theta=theta, G=G, quadrupole=quadrupole,
                                      group_size=64 if group_size == "auto" else group_size, criterion=criterion,
-                                     alpha=alpha, expansion=expansion)
+                                     alpha=alpha, expansion=expansion, expansion_rel=expansion_rel)
         else:
             checkTreeQuadrupoles(tree, True)
             fs = engine.field_scale(tree.dev)
             out = engine.walk(tree.dev, pos, softening, _morton_order(pos), mask, quadrupole=quadrupole,
                               group_size=group_size, theta=theta, G=G, precision=precision, criterion=criterion,
                               eps=alpha * fs, model=_learned_model() if criterion == "learned" else None,
-                              expansion=None if expansion is None else expansion * fs)
+                              expansion=None if expansion is None else expansion * fs, expansion_rel=expansion_rel)
     elif expansion is not None:
         if tree is None:
             tree = ConstructTree(pos, m, softening)
             out = engine.self_fields(tree.dev, pos, softening, mask, theta=theta, G=G,
-                                     group_size=64 if group_size == "auto" else group_size, expansion=expansion)
+                                     group_size=64 if group_size == "auto" else group_size, expansion=expansion,
+                                     expansion_rel=expansion_rel)
         else:
             out = engine.walk(tree.dev, pos, softening, _morton_order(pos), mask, group_size=group_size, theta=theta,
-                              G=G, precision=precision, expansion=expansion * engine.field_scale(tree.dev))
+                              G=G, precision=precision, expansion=expansion * engine.field_scale(tree.dev),
+                              expansion_rel=expansion_rel)
     else:
         if tree is None:
             tree = ConstructTree(pos, m, softening, quadrupole=quadrupole)
@@ -441,37 +447,41 @@ def _learned_model():
 # ---------------------------------------------------------------------------------------------
 def Potential(pos, m, softening=None, G=1.0, theta=THETA_DEFAULT, tree=None, return_tree=False, parallel=False,
               method="adaptive", quadrupole=False, group_size=None, device="cpu", precision=None, criterion="theta",
-              alpha=None, expansion=None):
+              alpha=None, expansion=None, expansion_rel=3.0):
     """Gravitational potential at the particle positions, shape (N,) (frontend.py:145-293).
 
-    criterion/alpha/expansion (float32 only): see Accel."""
+    criterion/alpha/expansion/expansion_rel (float32 only): see Accel."""
     valueTestMethod(method)
     _check_device(device)
     prec = _precision(precision, device)
     if criterion != "theta" and theta is THETA_DEFAULT:
         theta = GUARD_THETA
     out, tree, on_dev = _self_fields(pos, m, softening, PTG_POT, G, theta, tree, return_tree, parallel, method,
-                                     quadrupole, _group(group_size, prec), prec, None, criterion, alpha, expansion)
+                                     quadrupole, _group(group_size, prec), prec, None, criterion, alpha, expansion,
+                                     expansion_rel)
     return _ret(_finish(out[:, 0], on_dev), tree, return_tree)
 
 
 def Accel(pos, m, softening=None, G=1.0, theta=THETA_DEFAULT, tree=None, return_tree=False, parallel=False,
           method="adaptive", quadrupole=False, group_size=None, device="cpu", precision=None, criterion="theta",
-          alpha=None, expansion=None):
+          alpha=None, expansion=None, expansion_rel=3.0):
     """Gravitational acceleration at the particle positions, shape (N,3) (frontend.py:421-560).
 
     criterion="learned" / "error" with alpha (float32 only): the learned or analytic per-node error
     bound on top of the geometric criterion, which then acts as a guard at theta (GUARD_THETA = 1.0
     unless the caller passes a theta).
     expansion (float32, monopole): fold far accepted elements into a third-order expansion about
-    each target group's centre when its truncation bound is below expansion x the RMS field."""
+    each target group's centre when its truncation bound is below expansion x the RMS field and
+    below expansion_rel x the element's own field there (default 3; 0 = no such cap: faster, with
+    a larger worst-case error)."""
     valueTestMethod(method)
     _check_device(device)
     prec = _precision(precision, device)
     if criterion != "theta" and theta is THETA_DEFAULT:
         theta = GUARD_THETA
     out, tree, on_dev = _self_fields(pos, m, softening, PTG_ACC, G, theta, tree, return_tree, parallel, method,
-                                     quadrupole, _group(group_size, prec), prec, None, criterion, alpha, expansion)
+                                     quadrupole, _group(group_size, prec), prec, None, criterion, alpha, expansion,
+                                     expansion_rel)
     return _ret(_finish(out, on_dev), tree, return_tree)
 
 

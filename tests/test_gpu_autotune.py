@@ -56,7 +56,7 @@ def test_autotune_choice_recorded_and_results_invariant():
 
 def test_frontend_auto_group_size_runs_the_choice():
     x, m, h = case("uniform_1e4")
-    a = Accel(x, m, h, theta=0.7, device="cuda", group_size="auto")
+    a = Accel(x, m, h, theta=0.7, device="cuda", group_size="auto", method="tree")
     key_rec = [r for r in engine.tuning_log().values()]
     assert key_rec, "autotuning left no record"
     t = engine.DeviceTree(x, m, h)

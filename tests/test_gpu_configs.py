@@ -357,16 +357,21 @@ def test_c5_learned_vs_theta07_1e7():
     model = OpeningMLP.load()
     res = {}
     for name, kw in (("theta0.7", dict(criterion="theta", theta=0.7)),
-                     ("learned", dict(criterion="learned", alpha=5e-4, model=model, expansion=4e-3))):
-        tree = ConstructTree(xd, md, hd, quadrupole=name == "learned")
+                     ("learned", dict(criterion="learned", alpha=5e-4, model=model, expansion=4e-3, expansion_rel=3.0)),
+                     ("learned_uncapped", dict(criterion="learned", alpha=5e-4, model=model, expansion=4e-3))):
+        tree = ConstructTree(xd, md, hd, quadrupole=name != "theta0.7")
         ctr = torch.zeros(4, dtype=torch.int64, device=dev)
         out = engine.self_fields(tree.dev, xd, hd, mask, group_size=64, counters=ctr, **kw).cpu().numpy()
         ea, ema = _acc_err(out[sel, 1:4], d[:, 1:4])
         ep, epm = _pot_err(out[sel, 0], d[:, 0])
         res[name] = dict(ea=ea, ema=ema, ep=ep, epm=epm, inter=int(ctr[0]), visits=int(ctr[1]))
         print(f"[C5] {name}: {res[name]}", flush=True)
-    L, P = res["learned"], res["theta0.7"]
-    assert L["ea"] <= 1.8e-3 and L["ea"] <= 1.05 * P["ea"]
-    assert L["ep"] <= 1.05 * P["ep"]
+    L, P, U = res["learned"], res["theta0.7"], res["learned_uncapped"]
+    # the benchmark configuration: acceleration RMS and max and potential RMS errors at most the
+    # theta=0.7 walk's
+    assert L["ea"] <= 1.8e-3 and L["ea"] <= P["ea"] and L["ema"] <= P["ema"] and L["ep"] <= P["ep"]
     # the learned walk evaluates far fewer per-target interactions and visits fewer elements
-    assert L["inter"] < 0.2 * P["inter"] and L["visits"] < P["visits"]
+    assert L["inter"] < 0.3 * P["inter"] and L["visits"] < P["visits"]
+    # without the relative cap the expansion folds more (fewer interactions) at about the same RMS
+    # error, but its worst error is larger
+    assert U["inter"] < L["inter"] and U["ea"] <= 1.05 * P["ea"] and U["ema"] > L["ema"]

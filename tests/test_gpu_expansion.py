@@ -201,56 +201,17 @@ def test_frontend_expansion():
     assert q1 < max(1.3 * q0, q0 + 3e-4), (q0, q1)
     with pytest.raises(ValueError):
         Accel(x, m, h, device="cuda", precision="float64", expansion=3e-4)
-
-
-@pytest.mark.parametrize("criterion", ["theta", "learned"])
-def test_supergroup_pass_accuracy_and_work(criterion):
-    """The shared supergroup pass keeps each group's accepted elements (only where they are folded
-    changes): the error against FP64 stays that of the one-level expansion, the per-target work
-    does not grow and the total traversal shrinks."""
-    N = 200_000
-    x, m = plummer_truncated(N, seed=6)
-    h = np.full(N, 0.01)
-    print(f"[expansion] supergroups plummer N={N} criterion={criterion}", flush=True)
-    t = ConstructTree(x, m, h, quadrupole=True)
-    rng = np.random.default_rng(1)
-    sel = np.sort(rng.choice(N, 4000, replace=False))
-    d = engine.direct(x[sel], h[sel], x, m, h, PTG_ACC).cpu().numpy()
-    kw = dict(theta=0.7 if criterion == "theta" else 2.0, criterion=criterion,
-              alpha=5e-4 if criterion != "theta" else None, group_size=64, expansion=4e-3)
-    c1 = torch.zeros(4, dtype=torch.int64, device=engine.device())
-    c2 = torch.zeros(4, dtype=torch.int64, device=engine.device())
-    a1 = engine.self_fields(t.dev, x, h, PTG_POT | PTG_ACC, counters=c1, **kw).cpu().numpy()
-    a2 = engine.self_fields(t.dev, x, h, PTG_POT | PTG_ACC, counters=c2, supergroups=True, **kw).cpu().numpy()
-    e1, e2 = _rms(a1[sel, 1:4], d), _rms(a2[sel, 1:4], d)
-    c1, c2 = c1.cpu().numpy(), c2.cpu().numpy()
-    print(f"  rms {e1:.3e} -> {e2:.3e}; listed/particle {c1[0] / N:.0f} -> {c2[0] / N:.0f}; "
-          f"visits/particle {c1[1] / N:.1f} -> {c2[1] / N:.1f}; folds {c1[3]} -> {c2[3]}", flush=True)
-    assert e2 < max(1.2 * e1, e1 + 3e-4), (e1, e2)
-    assert c2[0] <= c1[0] * 1.01 and c2[1] < c1[1] and c2[3] < c1[3]
-
-
-def test_forest_supergroups_equal_single_walks():
-    from pytreegrav_amd import Forest
-
-    (x, m, h, off), systems = _forest_systems()
-    F = Forest(x, m, h, off, precision="float32")
-    r = F.evaluate(potential=True, accel=True, expansion=3e-4, supergroups=True)
-    fa, fp = r["accel"].cpu().numpy(), r["potential"].cpu().numpy()
-    scale = F._exp_scale
-    for s, (xs, ms, hs) in enumerate(systems):
-        u = engine.DeviceTree(xs, ms, hs)
-        o = engine.walk(u, xs, hs, u.order, PTG_POT | PTG_ACC, group_size=64, theta=0.7, precision="float32",
-                        expansion=3e-4, eps_scale=scale[s:s + 1].contiguous(), supergroups=True).cpu().numpy()
-        assert np.array_equal(fp[off[s]:off[s + 1]], o[:, 0]), s
-        assert np.array_equal(fa[off[s]:off[s + 1]], o[:, 1:4]), s
-    replay, out = F.capture(potential=True, accel=True, expansion=3e-4, supergroups=True)
-    ref = torch.cat([r["potential"][:, None], r["accel"]], 1).clone()
-    for _ in range(3):
-        out.zero_()
-        replay()
-        torch.cuda.synchronize()
-        assert torch.equal(out, ref)
+    with pytest.raises(ValueError):
+        Accel(x, m, h, device="cuda", criterion="learned", alpha=1.2e-3, expansion=1e-3, expansion_rel=-1.0)
+    # expansion_rel=0 (no cap) folds at least as much as the default cap
+    c_cap = torch.zeros(4, dtype=torch.int64, device=engine.device())
+    c_free = torch.zeros(4, dtype=torch.int64, device=engine.device())
+    t = engine.DeviceTree(x, m, h)
+    engine.walk(t, x, h, t.order, PTG_ACC, group_size=64, theta=0.7, precision="float32", counters=c_cap,
+                expansion=1e-3, expansion_rel=3.0)
+    engine.walk(t, x, h, t.order, PTG_ACC, group_size=64, theta=0.7, precision="float32", counters=c_free,
+                expansion=1e-3)
+    assert 0 < int(c_cap[3]) <= int(c_free[3])
 
 
 def test_autotune_with_expansion():

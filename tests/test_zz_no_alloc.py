@@ -81,7 +81,7 @@ def test_noalloc_walk_equals_workspace_walk(prec):
             assert torch.equal(out, ref), call
 
 
-def test_noalloc_walk_rejects_the_shared_pass_and_needs_err():
+def test_noalloc_walk_rejects_bad_criteria_and_needs_err():
     from pytreegrav_amd import ConstructTree
 
     x, m = plummer_truncated(20_000, seed=92)
@@ -98,7 +98,7 @@ def test_noalloc_walk_rejects_the_shared_pass_and_needs_err():
     crit = Criterion(_lib.PTG_CRIT_THETA, 2, 0.0, 1e-4, None)
     st = f(_c.byref(p["wt"]), None, ptr(xd), ptr(hd), ptr(tree.order), 0, len(x), 3, 0, 64, ptr(groups[0]),
            ptr(groups[1]), 0.7, _c.byref(crit), 1.0, 0, ptr(out), None, ptr(err), engine.stream())
-    assert st != 0 and b"supergroup" in _lib.load().ptg_last_error()
+    assert st != 0 and b"group_expansion" in _lib.load().ptg_last_error()
     crit = Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
     st = f(_c.byref(p["wt"]), None, ptr(xd), ptr(hd), ptr(tree.order), 0, len(x), 3, 0, 64, ptr(groups[0]),
            ptr(groups[1]), 0.7, _c.byref(crit), 1.0, 0, ptr(out), None, None, engine.stream())
