@@ -81,8 +81,9 @@ template <typename T> __device__ __forceinline__ T block_excl_scan(T v, T *total
 }
 
 template <typename Load>
-__global__ __launch_bounds__(SC_THREADS) void k_scan_reduce(Load ld, int64_t n, int32_t *bsum)
+__global__ __launch_bounds__(SC_THREADS) void k_scan_reduce(Load ld, int64_t n, int32_t *bsum, const int32_t *gate)
 {
+    if (gate && *gate == 0) return;
     __shared__ int32_t sh[8];
     int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_ITEMS;
     int32_t s = 0;
@@ -95,8 +96,10 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_reduce(Load ld, int64_t n, 
 }
 
 // single block: exclusive scan of nb block sums in place; grand total to *total
-__global__ __launch_bounds__(SC_THREADS) void k_scan_top(int32_t *bsum, int64_t nb, int32_t *total)
+__global__ __launch_bounds__(SC_THREADS) void k_scan_top(int32_t *bsum, int64_t nb, int32_t *total,
+                                                      const int32_t *gate)
 {
+    if (gate && *gate == 0) return;
     __shared__ int32_t sh[8];
     int32_t carry = 0;
     for (int64_t base = 0; base < nb; base += SC_THREADS) {
@@ -111,8 +114,10 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_top(int32_t *bsum, int64_t 
 }
 
 template <typename Load>
-__global__ __launch_bounds__(SC_THREADS) void k_scan_down(Load ld, int64_t n, const int32_t *bsum, int32_t *out)
+__global__ __launch_bounds__(SC_THREADS) void k_scan_down(Load ld, int64_t n, const int32_t *bsum, int32_t *out,
+                                                       const int32_t *gate)
 {
+    if (gate && *gate == 0) return;
     __shared__ int32_t sh[8];
     int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_ITEMS;
     int32_t v[SC_ITEMS];
@@ -133,13 +138,15 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Load ld, int64_t n, co
 
 static int64_t scan_blocks(int64_t n) { return std::max<int64_t>(1, ptg_cdiv(n, SC_TILE)); }
 
+// gate (nullable, device): when it reads 0 the scan does nothing (out and *total keep their contents)
 template <typename Load>
-static ptg_status excl_scan(Load ld, int64_t n, int32_t *out, int32_t *bsum, int32_t *total, hipStream_t s)
+static ptg_status excl_scan(Load ld, int64_t n, int32_t *out, int32_t *bsum, int32_t *total, hipStream_t s,
+                            const int32_t *gate = nullptr)
 {
     int64_t nb = scan_blocks(n);
-    k_scan_reduce<<<nb, SC_THREADS, 0, s>>>(ld, n, bsum);
-    k_scan_top<<<1, SC_THREADS, 0, s>>>(bsum, nb, total);
-    k_scan_down<<<nb, SC_THREADS, 0, s>>>(ld, n, bsum, out);
+    k_scan_reduce<<<nb, SC_THREADS, 0, s>>>(ld, n, bsum, gate);
+    k_scan_top<<<1, SC_THREADS, 0, s>>>(bsum, nb, total, gate);
+    k_scan_down<<<nb, SC_THREADS, 0, s>>>(ld, n, bsum, out, gate);
     PTG_LAUNCH_CHECK();
     return PTG_OK;
 }
@@ -152,9 +159,10 @@ struct LoadArr {
 // ============================================================================================
 // LSD radix sort of (u64 key, u32 value) pairs, 8-bit digits.  Each 256-thread block owns a tile of
 // RS_TILE consecutive elements.  Per pass: a tile histogram, the offset of every (digit, tile) (one
-// block per digit scans that digit's tile counts, then one block turns the digit totals into digit
-// starts), and a scatter that ranks the tile's keys stably (ballot-based digit matching inside each
-// wave, per-digit prefix over the waves), reorders the tile in LDS and writes each digit's run
+// block per digit scans that digit's tile counts and writes the digit's total; each scatter block
+// turns the 256 totals into digit starts itself), and a scatter that ranks the tile's keys stably
+// (ballot-based digit matching inside each wave, per-digit prefix over the waves), reorders the
+// tile in LDS and writes each digit's run
 // contiguously, so global writes come in runs rather than one scattered element at a time.  (A
 // single-pass decoupled look-back variant measured slower here, 116 against about 100 us per pass
 // at 1e7 keys: the inclusive prefixes propagate slowly through the many resident tiles, and every
@@ -185,13 +193,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *keys, in
 
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *kin, const uint32_t *vin, int64_t n,
                                                            int shift, int64_t ntiles, const int32_t *offs,
-                                                           const int32_t *dstart, uint64_t *kout, uint32_t *vout)
+                                                           const int32_t *dtot, uint64_t *kout, uint32_t *vout)
 {
     __shared__ uint64_t sk[RS_TILE];
     __shared__ uint32_t sv[RS_TILE];
     __shared__ uint32_t run[RS_WAVES][RS_BINS];
     __shared__ int64_t gbase[RS_BINS];
-    __shared__ int32_t sh[8];
+    __shared__ int32_t sh[8], sh2[8];
     const int wave = threadIdx.x >> 6, lane = ptg_lane();
     const uint64_t lt = ptg_lanemask_lt();
     const int64_t tile0 = (int64_t)blockIdx.x * RS_TILE;
@@ -232,9 +240,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t *kin, 
             c[w] = run[w][d];
             tot += c[w];
         }
-        int32_t all;
+        int32_t all, keys_all;
         const int32_t start = block_excl_scan<int32_t>((int32_t)tot, &all, sh);
-        gbase[d] = (int64_t)dstart[d] + offs[(int64_t)d * ntiles + blockIdx.x] - start;
+        // the digit's start in the output: exclusive scan of the digit totals (every block redoes
+        // this 256-entry scan instead of a separate launch)
+        const int32_t dstart = block_excl_scan<int32_t>(dtot[d], &keys_all, sh2);
+        gbase[d] = (int64_t)dstart + offs[(int64_t)d * ntiles + blockIdx.x] - start;
         uint32_t o = (uint32_t)start;
 #pragma unroll
         for (int w = 0; w < RS_WAVES; w++) {
@@ -265,7 +276,7 @@ static int64_t rs_nwt(int64_t n) { return std::max<int64_t>(1, ptg_cdiv(n, RS_TI
 
 struct RsWS {
     int32_t *hist;   // (RS_BINS, ntiles) tile counts, scanned in place per digit
-    int32_t *dtot;   // (RS_BINS) digit totals, then digit starts
+    int32_t *dtot;   // (RS_BINS) digit totals
     int64_t cap;     // elements the buffers were carved for
 };
 
@@ -295,16 +306,6 @@ __global__ __launch_bounds__(256) void k_rs_tile_scan(int32_t *hist, int64_t nti
     if (threadIdx.x == 0) dtot[blockIdx.x] = tot;
 }
 
-// digit starts: exclusive scan of the RS_BINS digit totals (one block)
-__global__ __launch_bounds__(RS_BINS) void k_rs_digit_starts(int32_t *dtot)
-{
-    __shared__ int32_t sh[8];
-    int32_t tot;
-    const int32_t v = dtot[threadIdx.x];
-    const int32_t ex = block_excl_scan<int32_t>(v, &tot, sh);
-    dtot[threadIdx.x] = ex;
-}
-
 // sorts (k0,v0) by the low `nbits` bits; the result ends in (k0,v0) or (k1,v1): *in_first tells which
 static ptg_status radix_sort(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t *v1, int64_t n, int nbits,
                              const RsWS &w, bool *in_first, hipStream_t s)
@@ -320,7 +321,6 @@ static ptg_status radix_sort(uint64_t *k0, uint32_t *v0, uint64_t *k1, uint32_t 
         uint32_t *vi = first ? v0 : v1, *vo = first ? v1 : v0;
         k_rs_hist<<<ntiles, RS_THREADS, 0, s>>>(ki, n, shift, ntiles, w.hist);
         k_rs_tile_scan<<<RS_BINS, 256, 0, s>>>(w.hist, ntiles, w.dtot);
-        k_rs_digit_starts<<<1, RS_BINS, 0, s>>>(w.dtot);
         k_rs_scatter<<<ntiles, RS_THREADS, 0, s>>>(ki, vi, n, shift, ntiles, w.hist, w.dtot, ko, vo);
         PTG_LAUNCH_CHECK();
         first = !first;
@@ -692,10 +692,10 @@ __global__ __launch_bounds__(256) void k_levels(const uint64_t *keys, const uint
 
 // slots whose 63-bit key is shared with a neighbour of the same system
 __global__ __launch_bounds__(256) void k_collision_flags(const uint64_t *keys, const uint32_t *sys, int64_t n,
-                                                         int32_t *flag)
+                                                         int32_t *flag, const int32_t *gate)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (gate && *gate == 0)) return;
     bool f = (i > 0 && keys[i] == keys[i - 1] && (!sys || sys[i] == sys[i - 1])) ||
              (i < n - 1 && keys[i] == keys[i + 1] && (!sys || sys[i] == sys[i + 1]));
     flag[i] = f ? 1 : 0;
@@ -728,10 +728,10 @@ __global__ __launch_bounds__(256) void k_collision_gather(const double *pos, con
                                                           const int32_t *subpos, int64_t n, const Geom *geoms,
                                                           const uint32_t *sys, uint64_t *sk0, uint64_t *sk1,
                                                           uint64_t *sk2, uint64_t *sk3, uint32_t *sslot,
-                                                          uint32_t *sval, uint32_t *perm)
+                                                          uint32_t *sval, uint32_t *perm, const int32_t *gate)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !flag[i]) return;
+    if (i >= n || (gate && *gate == 0) || !flag[i]) return;
     int64_t j = subpos[i];
     uint32_t o = order[i];
     double p[3] = {pos[3 * (int64_t)o], pos[3 * (int64_t)o + 1], pos[3 * (int64_t)o + 2]};
@@ -824,7 +824,7 @@ __global__ __launch_bounds__(256) void k_levels_ext(const uint64_t *keys, const 
                                                     const int64_t *soff, int64_t n, int32_t *lev, Counters *c)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1 || keys[i] != keys[i + 1]) return;
+    if (i >= n - 1 || c->ncollide == 0 || keys[i] != keys[i + 1]) return;
     if (sys && sys[i] != sys[i + 1]) return;
     int L;
     if (ext[i] != ext[i + 1]) L = 21 + ptg_common_levels(ext[i], ext[i + 1]);
@@ -1060,11 +1060,11 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
         if (hc.nonfinite) return nonfinite();
         if (hc.ncollide > 0) {
             // collision runs: compute extended keys and re-sort each run by them (stable)
-            k_collision_flags<<<grid1(n), 256, 0, s>>>(w.kA, w.sys, n, w.flag);
+            k_collision_flags<<<grid1(n), 256, 0, s>>>(w.kA, w.sys, n, w.flag, nullptr);
             st = excl_scan(LoadArr{w.flag}, n, w.subpos, w.bsum, &w.ctr->nsub, s);
             if (st) return st;
             k_collision_gather<<<grid1(n), 256, 0, s>>>(pos, w.kA, w.vA, w.flag, w.subpos, n, w.geoms, w.sys,
-                                                        w.sk0, w.sk1, w.sk2, w.sk3, w.sslot, w.sval, w.perm);
+                                                        w.sk0, w.sk1, w.sk2, w.sk3, w.sslot, w.sval, w.perm, nullptr);
             PTG_LAUNCH_CHECK();
             PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
             PTG_HIP_CHECK(hipStreamSynchronize(s));
@@ -1341,6 +1341,18 @@ __global__ void k_single_root(const Geom *g, const ptg_tree_arrays t)
 // links and moments
 // ============================================================================================
 // a child's moments as its parent reads them: mass, sum of m*x (nodes), softening, centre of mass
+// last index s in [0, cnt) with a[s] <= v (a ascending, a[0] <= v)
+template <typename I> __device__ __forceinline__ int64_t upper_index(const I *a, int64_t cnt, int64_t v)
+{
+    int64_t lo = 0, hi = cnt;
+    while (hi - lo > 1) {
+        int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)a[mid] <= v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 struct NodeWS {
     double *acc;        // (nnodes, 3) sum of m*x of each node, by node offset (element - n)
     int32_t *err;       // [0] corrupt-chain flag
@@ -2027,13 +2039,15 @@ extern "C" ptg_status ptg_build_deferred(const double *pos, const double *m, con
         k_forest_cubes<<<grid1(nsys), 256, 0, s>>>(w.geoms, nsys, forest->cubes);
     }
     k_levels<<<grid1(n - 1), 256, 0, s>>>(w.kA, w.sys, n, w.lev, w.ctr);
-    // collision runs: always re-keyed here (a run of no particles costs a few small launches)
-    k_collision_flags<<<grid1(n), 256, 0, s>>>(w.kA, w.sys, n, w.flag);
-    st = excl_scan(LoadArr{w.flag}, n, w.subpos, w.bsum, &w.ctr->nsub, s);
+    // collision runs: re-keyed here; the stage's kernels read k_levels' count of colliding pairs and
+    // return at once when it is 0 (nsub then stays 0 from the zeroed counters)
+    const int32_t *gate = &w.ctr->ncollide;
+    k_collision_flags<<<grid1(n), 256, 0, s>>>(w.kA, w.sys, n, w.flag, gate);
+    st = excl_scan(LoadArr{w.flag}, n, w.subpos, w.bsum, &w.ctr->nsub, s, gate);
     if (st) return st;
     k_budget_sub<<<1, 64, 0, s>>>(w.ctr, PTG_DEFERRED_COLLISION_CAP);
     k_collision_gather<<<grid1(n), 256, 0, s>>>(pos, w.kA, w.vA, w.flag, w.subpos, n, w.geoms, w.sys, w.sk0, w.sk1,
-                                                w.sk2, w.sk3, w.sslot, w.sval, w.perm);
+                                                w.sk2, w.sk3, w.sslot, w.sval, w.perm, gate);
     const int64_t mcap = std::min<int64_t>(n, PTG_DEFERRED_COLLISION_CAP);
     k_rank_sort4<<<grid1(mcap), 256, 0, s>>>(w.sk0, w.sk1, w.sk2, w.sk3, w.sslot, w.sys, mcap, w.perm,
                                              &w.ctr->eff_sub);
@@ -2110,18 +2124,6 @@ __device__ void pack_slot(int64_t e, int64_t slot, int64_t n, const ptg_tree_arr
         q[6] = (T)0;
         q[7] = (T)0;
     }
-}
-
-// last index s in [0, cnt) with a[s] <= v (a ascending, a[0] <= v)
-template <typename I> __device__ __forceinline__ int64_t upper_index(const I *a, int64_t cnt, int64_t v)
-{
-    int64_t lo = 0, hi = cnt;
-    while (hi - lo > 1) {
-        int64_t mid = (lo + hi) >> 1;
-        if ((int64_t)a[mid] <= v) lo = mid;
-        else hi = mid;
-    }
-    return lo;
 }
 
 // forest view for the pack: system s owns slots [soff[s], soff[s+1]) and nodes [roots[s], roots[s+1]);

@@ -88,8 +88,8 @@ def test_large_build_invariants(n):
 
 @pytest.mark.parametrize("quadrupole", [False, True])
 def test_build_fills_the_walk_copy_like_ptg_pack_walk(quadrupole):
-    """The FP32 walk copy the build writes in slot order equals ptg_pack_walk's, bit for bit, for one
-    tree and for a forest."""
+    """The FP32 walk copy the build writes (from its moment pass) equals ptg_pack_walk's, bit for bit,
+    for one tree, a forest and a capacity-sized sync-free build."""
     from pytreegrav_amd import engine
     from pytreegrav_amd._lib import PTG_F32
 
@@ -107,6 +107,14 @@ def test_build_fills_the_walk_copy_like_ptg_pack_walk(quadrupole):
             want = np.sqrt((q * q).sum(1))
             got = built[1][slot, 3].astype(np.float64)
             assert np.allclose(got, want, rtol=1e-6, atol=1e-30)
+    # the sync-free build (capacity-sized arrays) writes the same copy over its slots
+    e = engine.DeviceTree(x, m, h, quadrupole=quadrupole)
+    d = engine.DeviceTree(x, m, h, quadrupole=quadrupole, capacity=(e.nnodes + 500, int(e.info.max_level) + 3))
+    ne = d.n + d.check().nnodes
+    built = [a[:ne].cpu().numpy() for a in d._packs[(PTG_F32, False)]["tensors"][:3]]
+    fresh = [a[:ne].cpu().numpy() for a in d.pack(PTG_F32, False, cache=False, norms=True)["tensors"][:3]]
+    for b, f in zip(built, fresh):
+        assert np.array_equal(b, f)
 
 
 @pytest.mark.parametrize("quadrupole", [False, True])
