@@ -672,33 +672,59 @@ __global__ void k_zero_counters(Counters *c)
 
 // lev[i] = whole octree levels shared by Morton slots i and i+1; -1 where slots i and i+1 belong to
 // different systems of a forest (the pair then starts no node and the next pair starts a root)
+// one block's sum and max of per-thread values, then one atomic of each per block
+__device__ __forceinline__ void block_count_max(int cnt, int vmax, int32_t *sum_to, int32_t *max_to)
+{
+    __shared__ int ws[4], wm[4];
+    cnt = ptg_wave_sum(cnt);
+    vmax = ptg_wave_max(vmax);
+    if (ptg_lane() == 0) {
+        ws[threadIdx.x >> 6] = cnt;
+        wm[threadIdx.x >> 6] = vmax;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int s = ws[0] + ws[1] + ws[2] + ws[3];
+        const int m = max(max(wm[0], wm[1]), max(wm[2], wm[3]));
+        if (sum_to && s) atomicAdd(sum_to, s);
+        if (max_to && m > 0) atomicMax(max_to, m);
+    }
+}
+
+#define LEV_BLOCKS 2048  // grid of the per-pair level passes (grid-stride; one atomic per block)
+static unsigned lev_grid(int64_t n)
+{
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(LEV_BLOCKS, ptg_cdiv(n - 1, 256)));
+}
+
+// shared levels of every adjacent pair, the count of colliding pairs and the deepest level
 __global__ __launch_bounds__(256) void k_levels(const uint64_t *keys, const uint32_t *sys, int64_t n, int32_t *lev,
                                                 Counters *c)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool coll = false;
-    if (i < n - 1) {
+    int cnt = 0, vmax = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n - 1; i += (int64_t)gridDim.x * blockDim.x) {
         if (sys && sys[i] != sys[i + 1]) {
             lev[i] = -1;
         } else {
-            int L = ptg_common_levels(keys[i], keys[i + 1]);
+            const int L = ptg_common_levels(keys[i], keys[i + 1]);
             lev[i] = L;
-            coll = (keys[i] == keys[i + 1]);
+            cnt += keys[i] == keys[i + 1];
+            vmax = max(vmax, L);
         }
     }
-    uint64_t b = ptg_ballot(coll);
-    if (b && ptg_lane() == 0) atomicAdd(&c->ncollide, (int)__popcll(b));
+    block_count_max(cnt, vmax, &c->ncollide, &c->max_level);
 }
 
 // slots whose 63-bit key is shared with a neighbour of the same system
 __global__ __launch_bounds__(256) void k_collision_flags(const uint64_t *keys, const uint32_t *sys, int64_t n,
                                                          int32_t *flag, const int32_t *gate)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || (gate && *gate == 0)) return;
-    bool f = (i > 0 && keys[i] == keys[i - 1] && (!sys || sys[i] == sys[i - 1])) ||
-             (i < n - 1 && keys[i] == keys[i + 1] && (!sys || sys[i] == sys[i + 1]));
-    flag[i] = f ? 1 : 0;
+    if (gate && *gate == 0) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        bool f = (i > 0 && keys[i] == keys[i - 1] && (!sys || sys[i] == sys[i - 1])) ||
+                 (i < n - 1 && keys[i] == keys[i + 1] && (!sys || sys[i] == sys[i + 1]));
+        flag[i] = f ? 1 : 0;
+    }
 }
 
 // extended keys for one particle: re-keying relative to its level-21, -42, -63 cells
@@ -730,20 +756,22 @@ __global__ __launch_bounds__(256) void k_collision_gather(const double *pos, con
                                                           uint64_t *sk2, uint64_t *sk3, uint32_t *sslot,
                                                           uint32_t *sval, uint32_t *perm, const int32_t *gate)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || (gate && *gate == 0) || !flag[i]) return;
-    int64_t j = subpos[i];
-    uint32_t o = order[i];
-    double p[3] = {pos[3 * (int64_t)o], pos[3 * (int64_t)o + 1], pos[3 * (int64_t)o + 2]};
-    uint64_t k3[3];
-    extended_keys(p, keys[i], geoms + (sys ? sys[i] : 0), k3);
-    sk0[j] = keys[i];
-    sk1[j] = k3[0];
-    sk2[j] = k3[1];
-    sk3[j] = k3[2];
-    sslot[j] = (uint32_t)i;
-    sval[j] = o;
-    perm[j] = (uint32_t)j;
+    if (gate && *gate == 0) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (!flag[i]) continue;
+        int64_t j = subpos[i];
+        uint32_t o = order[i];
+        double p[3] = {pos[3 * (int64_t)o], pos[3 * (int64_t)o + 1], pos[3 * (int64_t)o + 2]};
+        uint64_t k3[3];
+        extended_keys(p, keys[i], geoms + (sys ? sys[i] : 0), k3);
+        sk0[j] = keys[i];
+        sk1[j] = k3[0];
+        sk2[j] = k3[1];
+        sk3[j] = k3[2];
+        sslot[j] = (uint32_t)i;
+        sval[j] = o;
+        perm[j] = (uint32_t)j;
+    }
 }
 
 // small collision subsets: stable sort by (system, K0, K1, K2, K3) through rank counting, rank of j =
@@ -820,12 +848,9 @@ __device__ __forceinline__ bool ext_less(const uint64_t *keys, const uint64_t *e
 
 // levels of pairs inside collision runs: re-keyed levels 22..84, then bucket chains of 7
 // (BuildBucket, octree.py:1128-1175: 7 particles and one deeper node per chain level)
-__global__ __launch_bounds__(256) void k_levels_ext(const uint64_t *keys, const uint64_t *ext, const uint32_t *sys,
-                                                    const int64_t *soff, int64_t n, int32_t *lev, Counters *c)
+__device__ __forceinline__ int level_ext(const uint64_t *keys, const uint64_t *ext, const uint32_t *sys,
+                                         const int64_t *soff, int64_t n, int64_t i, Counters *c)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1 || c->ncollide == 0 || keys[i] != keys[i + 1]) return;
-    if (sys && sys[i] != sys[i + 1]) return;
     int L;
     if (ext[i] != ext[i + 1]) L = 21 + ptg_common_levels(ext[i], ext[i + 1]);
     else if (ext[n + i] != ext[n + i + 1]) L = 42 + ptg_common_levels(ext[n + i], ext[n + i + 1]);
@@ -841,7 +866,22 @@ __global__ __launch_bounds__(256) void k_levels_ext(const uint64_t *keys, const 
         L = 84 + (int)((i - lo) / 7);
         c->coincident = 1;
     }
-    lev[i] = L;
+    return L;
+}
+
+// the levels of pairs inside collision runs (grid-stride; the deepest level joins the counters' max)
+__global__ __launch_bounds__(256) void k_levels_ext(const uint64_t *keys, const uint64_t *ext, const uint32_t *sys,
+                                                    const int64_t *soff, int64_t n, int32_t *lev, Counters *c)
+{
+    if (c->ncollide == 0) return;
+    int vmax = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n - 1; i += (int64_t)gridDim.x * blockDim.x) {
+        if (keys[i] != keys[i + 1] || (sys && sys[i] != sys[i + 1])) continue;
+        const int L = level_ext(keys, ext, sys, soff, n, i, c);
+        lev[i] = L;
+        vmax = max(vmax, L);
+    }
+    block_count_max(0, vmax, nullptr, &c->max_level);
 }
 
 struct LoadNodeCount {
@@ -853,22 +893,6 @@ struct LoadNodeCount {
         return d > 0 ? d : 0;
     }
 };
-
-__global__ __launch_bounds__(256) void k_max_level(const int32_t *lev, int64_t m, Counters *c)
-{
-    __shared__ int wmax[4];
-    int v = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
-        v = max(v, lev[i]);
-    v = ptg_wave_max(v);
-    if (ptg_lane() == 0) wmax[threadIdx.x >> 6] = v;
-    __syncthreads();
-    // one atomic per block
-    if (threadIdx.x == 0) {
-        v = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
-        if (v > 0) atomicMax(&c->max_level, v);
-    }
-}
 
 __global__ void k_fill_i32(int32_t *a, int64_t n, int32_t v)
 {
@@ -1053,7 +1077,7 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
         PTG_LAUNCH_CHECK();
     }
     if (n >= 2) {
-        k_levels<<<grid1(n - 1), 256, 0, s>>>(w.kA, w.sys, n, w.lev, w.ctr);
+        k_levels<<<lev_grid(n), 256, 0, s>>>(w.kA, w.sys, n, w.lev, w.ctr);
         PTG_LAUNCH_CHECK();
         PTG_HIP_CHECK(hipMemcpyAsync(&hc, w.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s));
         PTG_HIP_CHECK(hipStreamSynchronize(s));
@@ -1094,13 +1118,11 @@ extern "C" ptg_status ptg_build_sort(const double *pos, const double *m, const d
             PTG_HIP_CHECK(hipMemsetAsync(w.ext, 0, 3 * n * sizeof(uint64_t), s));
             k_collision_apply<<<grid1(m), 256, 0, s>>>(w.perm, m, w.sslot, w.sval, w.sk1, w.sk2, w.sk3, w.vA, w.ext,
                                                        n, nullptr);
-            k_levels_ext<<<grid1(n - 1), 256, 0, s>>>(w.kA, w.ext, w.sys, w.soff, n, w.lev, w.ctr);
+            k_levels_ext<<<lev_grid(n), 256, 0, s>>>(w.kA, w.ext, w.sys, w.soff, n, w.lev, w.ctr);
             PTG_LAUNCH_CHECK();
         }
         st = excl_scan(LoadNodeCount{w.lev}, n - 1, w.off, w.bsum, &w.ctr->nnodes, s);
         if (st) return st;
-        k_max_level<<<(unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, ptg_cdiv(n - 1, 256 * 8))), 256, 0, s>>>(
-            w.lev, n - 1, w.ctr);
         PTG_LAUNCH_CHECK();
     }
     phase_mark(PTG_PHASE_COUNT_NODES, s);
@@ -1219,12 +1241,15 @@ __device__ __forceinline__ int32_t bfs_elem(int32_t h, int64_t n, const uint32_t
 }
 
 // pass 2, one thread per node in node order (coalesced writes): geometry by the reference's centre
-// recurrence from the root down the opening pair's key digits, and FirstSubnode.  order[q] is the
+// recurrence from the root down the opening pair's key digits, FirstSubnode, and NextBranch (the
+// element after the last slot of the node's cell, found by galloping then bisecting on "same level-L
+// cell as slot i"; the long searches of the large cells run side by side).  order[q] is the
 // preorder offset of node offset q.
-__global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *lev, int64_t n, int64_t nnodes,
-                                                    const Geom *geoms, const uint32_t *order, const int32_t *nlev,
-                                                    const int32_t *npair, const uint32_t *pi, double *coord,
-                                                    double *sizes, int32_t *first, const int32_t *count)
+__global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *lev, const int32_t *off, int64_t n,
+                                                    int64_t nnodes, const Geom *geoms, const uint32_t *order,
+                                                    const int32_t *nlev, const int32_t *npair, const uint32_t *pi,
+                                                    double *coord, double *sizes, int32_t *first, int32_t *next,
+                                                    const int32_t *count)
 {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= ptg_node_count(nnodes, count)) return;
@@ -1233,6 +1258,7 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
     const int L = nlev[a];
     if (i < 0 || i >= n - 1 || L < 0) {  // every node is opened by one pair: never taken
         first[idx] = -1;
+        next[idx] = -1;
         return;
     }
     const Geom *g = geoms + (kv.sys ? kv.sys[i] : 0);
@@ -1249,26 +1275,6 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
     coord[3 * idx + 2] = c[2];
     sizes[idx] = s;
     first[idx] = (L + 1 <= lev[i]) ? (int32_t)(n + pi[a + 1]) : (int32_t)i;
-}
-
-// NextBranch of node offset q (preorder offset a, opened by pair i at level L): the element after the
-// last slot of its cell, found by galloping then bisecting on "same level-L cell as slot i"; one
-// thread per node, so the long searches of the large cells run side by side
-__global__ __launch_bounds__(256) void k_node_ends(KeyView kv, const int32_t *lev, const int32_t *off, int64_t n,
-                                                   int64_t nnodes, const uint32_t *order, const int32_t *nlev,
-                                                   const int32_t *npair, const uint32_t *pi, int32_t *next,
-                                                   const int32_t *count)
-{
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= ptg_node_count(nnodes, count)) return;
-    const int64_t a = order[q];
-    const int64_t i = npair[a];
-    const int L = nlev[a];
-    const int64_t idx = n + q;
-    if (i < 0 || i >= n - 1 || L < 0) {  // every node is opened by one pair: never taken
-        next[idx] = -1;
-        return;
-    }
     int64_t lo = i + 1;  // slot i+1 shares the pair's level >= L with slot i
     int64_t step = 1;
     while (lo + step < n && kv.same_cell(i, lo + step, L)) {
@@ -1779,11 +1785,9 @@ static ptg_status tree_phase(const double *pos, const double *m, const double *h
     if (n == 1) {
         k_single_root<<<1, 1, 0, s>>>(w.geoms, t);
     } else {
-        k_emit_nodes<<<grid1(nnodes), 256, 0, s>>>(kv, w.lev, n, nnodes, w.geoms, in_first ? nw.idA : nw.idB,
+        k_emit_nodes<<<grid1(nnodes), 256, 0, s>>>(kv, w.lev, w.off, n, nnodes, w.geoms, in_first ? nw.idA : nw.idB,
                                                    nw.nlev, nw.npair, nw.pi, t.coordinates, t.sizes, t.first_subnode,
-                                                   cnt);
-        k_node_ends<<<grid1(nnodes), 256, 0, s>>>(kv, w.lev, w.off, n, nnodes, in_first ? nw.idA : nw.idB, nw.nlev,
-                                                  nw.npair, nw.pi, t.next_branch, cnt);
+                                                   t.next_branch, cnt);
     }
     k_particle_next<<<grid1(n), 256, 0, s>>>(w.lev, w.off, w.sys, n, nw.pi, t.next_branch, cnt);
     if (F) k_forest_roots<<<grid1(nsys), 256, 0, s>>>(w.soff, nsys, w.off, n, nw.pi, forest->roots, cnt);
@@ -2038,15 +2042,15 @@ extern "C" ptg_status ptg_build_deferred(const double *pos, const double *m, con
         if (F) k_check_finite<<<(unsigned)std::min<int64_t>(1024, ptg_cdiv(n, 256)), 256, 0, s>>>(pos, m, h, n, w.ctr);
         k_forest_cubes<<<grid1(nsys), 256, 0, s>>>(w.geoms, nsys, forest->cubes);
     }
-    k_levels<<<grid1(n - 1), 256, 0, s>>>(w.kA, w.sys, n, w.lev, w.ctr);
+    k_levels<<<lev_grid(n), 256, 0, s>>>(w.kA, w.sys, n, w.lev, w.ctr);
     // collision runs: re-keyed here; the stage's kernels read k_levels' count of colliding pairs and
     // return at once when it is 0 (nsub then stays 0 from the zeroed counters)
     const int32_t *gate = &w.ctr->ncollide;
-    k_collision_flags<<<grid1(n), 256, 0, s>>>(w.kA, w.sys, n, w.flag, gate);
+    k_collision_flags<<<lev_grid(n + 1), 256, 0, s>>>(w.kA, w.sys, n, w.flag, gate);
     st = excl_scan(LoadArr{w.flag}, n, w.subpos, w.bsum, &w.ctr->nsub, s, gate);
     if (st) return st;
     k_budget_sub<<<1, 64, 0, s>>>(w.ctr, PTG_DEFERRED_COLLISION_CAP);
-    k_collision_gather<<<grid1(n), 256, 0, s>>>(pos, w.kA, w.vA, w.flag, w.subpos, n, w.geoms, w.sys, w.sk0, w.sk1,
+    k_collision_gather<<<lev_grid(n + 1), 256, 0, s>>>(pos, w.kA, w.vA, w.flag, w.subpos, n, w.geoms, w.sys, w.sk0, w.sk1,
                                                 w.sk2, w.sk3, w.sslot, w.sval, w.perm, gate);
     const int64_t mcap = std::min<int64_t>(n, PTG_DEFERRED_COLLISION_CAP);
     k_rank_sort4<<<grid1(mcap), 256, 0, s>>>(w.sk0, w.sk1, w.sk2, w.sk3, w.sslot, w.sys, mcap, w.perm,
@@ -2054,11 +2058,9 @@ extern "C" ptg_status ptg_build_deferred(const double *pos, const double *m, con
     // (the extended keys are read only for colliding slots, all of which this writes)
     k_collision_apply<<<grid1(mcap), 256, 0, s>>>(w.perm, mcap, w.sslot, w.sval, w.sk1, w.sk2, w.sk3, w.vA, w.ext, n,
                                                   &w.ctr->eff_sub);
-    k_levels_ext<<<grid1(n - 1), 256, 0, s>>>(w.kA, w.ext, w.sys, w.soff, n, w.lev, w.ctr);
+    k_levels_ext<<<lev_grid(n), 256, 0, s>>>(w.kA, w.ext, w.sys, w.soff, n, w.lev, w.ctr);
     st = excl_scan(LoadNodeCount{w.lev}, n - 1, w.off, w.bsum, &w.ctr->nnodes, s);
     if (st) return st;
-    k_max_level<<<(unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, ptg_cdiv(n - 1, 256 * 8))), 256, 0, s>>>(
-        w.lev, n - 1, w.ctr);
     k_budget_nodes<<<1, 64, 0, s>>>(w.ctr, nnodes_cap, level_cap, n, t.node_count);
     PTG_LAUNCH_CHECK();
     phase_mark(PTG_PHASE_COUNT_NODES, s);
