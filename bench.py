@@ -58,7 +58,7 @@ REF_PUBLISHED = 1.0 / (1.005e-6 + 1.124e-6)
 # HBM traffic per walk launch is measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes, which
 # cannot run inside this process: the bench line leaves roofline.traffic null and names the profile
 # summary that holds the calibrated figure for the default configuration.
-WALK_TRAFFIC_PROFILE = "profiles/k_walk_expansion_plummer1e7_hbm_traffic.md"
+WALK_TRAFFIC_PROFILE = "profiles/k_walk_capped_plummer1e7_hbm_traffic.md"
 
 
 def parse():
