@@ -104,6 +104,7 @@ def parse():
     ap.add_argument("--oracle-targets", type=int, default=10_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="skip the whole-step HIP graph line")
     ap.add_argument("--cpu-sample-frac", type=float, default=0.125)
     ap.add_argument("--quadrupole", action="store_true", help="walk with quadrupole moments")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
@@ -676,7 +677,35 @@ def main():
         r["build_ms"] = (time.perf_counter() - tb0) / args.steps * 1e3
         r["build_phases"] = build_phases(make_tree, N, build_quads, args.steps)
         r["accuracy"] = accuracy(out) if exact is not None else None
+        r["out"] = out
         return r
+
+    def run_graph_step(theta, eager_out):
+        """The same evaluation as one HIP graph per step: Forest.capture_step on the single system
+        (sync-free build sized by the eager tree, cell groups, field scale from the same 64 sample
+        groups, opening radii, walk), replayed on the resident inputs; each replay reads the build's
+        counts once afterwards (one host synchronisation per step)."""
+        from pytreegrav_amd import Forest
+
+        F = Forest(pos, mass, soft, [0, N], precision="float32", group_size=args.group_size,
+                   criterion=args.criterion, field_samples=64)
+        kw = dict(theta=theta, expansion=args.expansion,
+                  expansion_rel=args.expansion_rel if args.expansion is not None else 0.0)
+        if args.criterion == "learned":
+            kw.update(alpha=args.alpha, bf16=args.bf16)
+        replay, gout = F.capture_step(potential=True, accel=True, **kw)
+        for _ in range(max(1, args.warmup)):
+            replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            replay()
+        torch.cuda.synchronize()
+        gdt = (time.perf_counter() - t0) / args.steps
+        return {"ms_per_step": gdt * 1e3, "particles_per_s": N / gdt,
+                "equals_eager": bool(torch.equal(gout, eager_out)),
+                "what": "build (sync-free, node capacity 1.25x the eager tree's), groups, field scale, radii "
+                        "and walk of the same configuration captured in one HIP graph and replayed"}
 
     def accuracy(out):
         res = out.index_select(0, torch.from_numpy(sel).to(dev)).cpu().numpy()
@@ -719,6 +748,9 @@ def main():
     quad = bool(args.quadrupole)
     theta = args.theta if args.theta is not None else (0.7 if args.criterion == "theta" else 2.0)
     main = run_config(args.criterion, theta, args.alpha, quad, args.expansion)
+    graph = None
+    if not args.no_graph and args.criterion in ("learned", "theta") and not quad:
+        graph = run_graph_step(theta, main["out"])
     plain = None
     if not args.no_plain and not (args.criterion == "theta" and theta == 0.7 and not quad):
         p = run_config("theta", 0.7, 0.0, False)
@@ -788,6 +820,7 @@ def main():
             "target_groups": main["groups"],
             "folded_elements_per_group": (folds / main["groups"]) if args.expansion is not None else None,
             "visits_per_target": visits / N,
+            "full_step_graph": graph,
         },
         "roofline": {
             "kernel": "k_walk (FP32 grouped walk)",

@@ -31,10 +31,12 @@ class Forest:
     precision "float32" (default): FP32 walk with cell-aligned groups of up to group_size targets;
     "float64": the reference's FP64 branch set with runs of group_size targets per system.
     group_size "auto": chosen by timing the walk stage at 32/64/128/256 (autotune()).
-    criterion "learned" (FP32): the tree is built with quadrupoles for the opening-radius features."""
+    criterion "learned" (FP32): the tree is built with quadrupoles for the opening-radius features.
+    field_samples: cell groups per system walked to estimate its RMS field (the scale of the
+    learned criterion's and the expansion's tolerances)."""
 
     def __init__(self, pos, m, h, offsets, quadrupole=False, precision="float32", group_size=None,
-                 criterion="theta"):
+                 criterion="theta", field_samples=8):
         n = len(pos)
         off = np.asarray(offsets.cpu() if isinstance(offsets, torch.Tensor) else offsets, dtype=np.int64)
         if off.ndim != 1 or off.size < 2 or off[0] != 0 or off[-1] != n or np.any(np.diff(off) < 2):
@@ -58,6 +60,7 @@ class Forest:
         self.tree = DeviceTree(self.pos, self.m, self.h, quadrupole=quadrupole or criterion == "learned", offsets=off)
         self._graphs = {}
         self.tuning = None
+        self.field_samples = int(field_samples)  # sample groups per system of the RMS-field estimate
 
     # ---------------------------------------------------------------------------------------------
     def _groups(self, gs, goff, gcnt, gws, t=None):
@@ -86,13 +89,14 @@ class Forest:
         gcnt = torch.empty(1, dtype=torch.int32, device=dev)
         return out, ws, goff, gcnt, gws
 
-    def system_field_scales(self, goff, gcnt, samples=8, gs=64, t=None, check_walk=True):
+    def system_field_scales(self, goff, gcnt, samples=None, gs=64, t=None, check_walk=True):
         """RMS |acceleration| (G=1) of each system, a device (nsys,) float64 array: a theta=1 FP32
         walk of `samples` of each system's cell-aligned groups (each padded to gs targets by
         repeating its last target), then one RMS per system (ptg_rms_rows).  t: the tree (default the
         forest's own); check_walk=False skips the overflow check, which synchronises (a captured step
         checks after the replay)."""
         t = t or self.tree
+        samples = self.field_samples if samples is None else int(samples)
         dev = device()
         # everything stays on the device (the count of groups too), so a captured step can run this:
         # group starts past the count read as n, after every system start

@@ -285,3 +285,26 @@ def test_forest_capture_step_replays_the_whole_step(criterion):
     r2()
     torch.cuda.synchronize()
     assert torch.equal(out2[:, 1:4], r["accel"])
+
+
+def test_single_system_captured_step_equals_self_fields():
+    """A one-system forest's captured step (sync-free build, cell groups, field scale from the same
+    64 sample groups, radii, walk) reproduces engine.self_fields bit for bit: the graph form of the
+    self-gravity evaluation that the benchmark times beside the eager one."""
+    from pytreegrav_amd.sampling import plummer_truncated
+
+    n = 200_000
+    x, m = plummer_truncated(n, seed=17)
+    h = np.full(n, 0.01)
+    dev = engine.device()
+    xd, md, hd = (torch.from_numpy(a).to(dev) for a in (x, m, h))
+    kw = dict(alpha=5e-4, expansion=4e-3, expansion_rel=3.0)
+    t = engine.DeviceTree(xd, md, hd, quadrupole=True)
+    ref = engine.self_fields(t, xd, hd, PTG_POT | PTG_ACC, theta=2.0, group_size=64, criterion="learned", **kw)
+    F = Forest(xd, md, hd, [0, n], precision="float32", criterion="learned", field_samples=64)
+    replay, out = F.capture_step(potential=True, accel=True, theta=2.0, **kw)
+    for _ in range(2):
+        out.zero_()
+        replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref)
