@@ -122,6 +122,28 @@ def test_binned_far_cluster_lands_in_one_bin():
     assert got[2] == pytest.approx(64 / r2 / (4 * np.pi / 6), rel=1e-6)
 
 
+def test_binned_overlap_blend_is_pinned_by_the_uniform_sphere():
+    """The binned estimator's treatment of a target inside a particle's sphere, fixed by geometry
+    rather than by restating its formula: at the sphere's centre every direction sees the column
+    rho * h = 3 m / (4 pi h^2) (so every sky bin gets exactly that); and at the surface the inside
+    form meets the outside point-mass form m / r^2 / (4 pi / 6) in the bin facing the sphere and 0 in
+    the others, so the estimate is continuous across r = h."""
+    m1, h1 = 2.0, 0.5
+    x = np.array([[0.0, 0.0, 0.0], [40.0, 0.0, 0.0]])  # the second particle: a far point mass in bin 0
+    m = np.array([m1, 1e-30])
+    h = np.array([h1, 1e-3])
+    t = oracle.build_tree(x, m, h)
+    centre = oracle.column_binned(t, [[0.0, 0.0, 0.0]], 1, 1e-9)[0]
+    assert np.allclose(centre, 3 * m1 / (4 * np.pi * h1 ** 2), rtol=1e-12, atol=0)
+    # just inside and just outside the surface, below the sphere: the sphere lies in bin 4 (+z)
+    size = 4 * np.pi / 6
+    for eps in (1e-9, -1e-9):
+        z = -h1 * (1 + eps)
+        col = oracle.column_binned(t, [[0.0, 0.0, z]], 1, 1e-9)[0]
+        assert col[4] == pytest.approx(m1 / z ** 2 / size, rel=1e-6)
+        assert np.abs(np.delete(col, 4)).max() < 1e-6 * col[4]
+
+
 def test_random_rotation_is_a_reproducible_rotation():
     for seed in (0, 1, 17, 123456):
         R = random_rotation(seed)

@@ -308,3 +308,35 @@ def test_single_system_captured_step_equals_self_fields():
         replay()
         torch.cuda.synchronize()
         assert torch.equal(out, ref)
+
+
+def test_resident_builder_sizes_grows_and_matches_eager_builds():
+    """engine.ResidentBuilder: the first call builds with exact counts, later calls build sync-free
+    into arrays sized from it and equal the eager build; a step whose tree outgrows the capacities
+    is reported by verify() (TreeOverflow, capacities grown) and the rebuilt tree fits."""
+    from pytreegrav_amd.sampling import plummer_truncated
+
+    n = 50_000
+    x, m = plummer_truncated(n, seed=23)
+    h = np.full(n, 0.01)
+    dev = engine.device()
+    xd, md, hd = (torch.from_numpy(a).to(dev) for a in (x, m, h))
+    b = engine.ResidentBuilder(quadrupole=True, slack=1.0, extra_levels=0)
+    t0 = b.verify(b(xd, md, hd))
+    cap0 = b.capacity
+    assert not getattr(t0, "deferred", False) and cap0[0] >= t0.nnodes
+    t1 = b.verify(b(xd, md, hd))
+    assert t1.deferred and t1.counts().nnodes == t0.nnodes
+    ne = t0.n + t0.nnodes
+    assert torch.equal(t1.coordinates[:ne], t0.coordinates[:ne]) and torch.equal(t1.masses[:ne], t0.masses[:ne])
+    # a more clustered set needs more nodes and levels than the capacities allow
+    xc = x * np.where(np.arange(n) % 2 == 0, 1e-3, 1.0)[:, None]
+    xcd = torch.from_numpy(xc).to(dev)
+    tc = b(xcd, md, hd)
+    with pytest.raises(engine.ResidentBuilder.TreeOverflow):
+        b.verify(tc)
+    assert b.capacity is None or b.capacity[0] > cap0[0] or b.capacity[1] > cap0[1]
+    t2 = b.verify(b(xcd, md, hd))
+    e = engine.DeviceTree(xcd, md, hd, quadrupole=True)
+    nn = t2.counts().nnodes if getattr(t2, "deferred", False) else t2.nnodes
+    assert nn == e.nnodes
