@@ -13,9 +13,8 @@
  *     as torch tensors).  Scratch memory is a caller-provided workspace whose size is returned by
  *     the matching *_workspace_bytes function.
  *   - Work is enqueued asynchronously on the given HIP stream (NULL = default stream).  Host
- *     synchronisations: ptg_build_sort (finiteness and forest checks, collision count, node count),
- *     ptg_build_tree (the per-level node counts that size the moment launches, when moments are
- *     computed) and ptg_walk_check (reads the overflow flag).  Every other call only enqueues work.
+ *     synchronisations: ptg_build_sort (finiteness and forest checks, collision count, node count)
+ *     and ptg_walk_check (reads the overflow flag).  Every other call only enqueues work.
  *   - Element numbering follows the reference Octree (octree.py:8-28): particles occupy
  *     [0, n) in Morton order, the root is element n, internal nodes follow.  Links are int32.
  */

@@ -1649,23 +1649,6 @@ __global__ __launch_bounds__(MOM_THREADS) void k_moments_level(int64_t n, int64_
         lane_moments<QUAD>(n + (byid ? (int64_t)byid[qi] : qi), n, n + nnodes, t, w);
 }
 
-// levels Lhi down to Llo (each with few nodes) in one block, a barrier between levels: all waves
-// of a workgroup share one CU's L1, so the barrier orders one level's writes before the next
-// level's reads
-#define MS_THREADS 1024
-template <bool QUAD>
-__global__ __launch_bounds__(MS_THREADS) void k_moments_small(int64_t n, int64_t nnodes, int Lhi, int Llo,
-                                                              const int32_t *lvstart, const uint32_t *byid,
-                                                              const ptg_tree_arrays t, NodeWS w)
-{
-    for (int L = Lhi; L >= Llo; L--) {
-        const int64_t lo = lvstart[L], hi = lvstart[L + 1];
-        for (int64_t qi = lo + threadIdx.x; qi < hi; qi += MS_THREADS)
-            lane_moments<QUAD>(n + (byid ? (int64_t)byid[qi] : qi), n, n + nnodes, t, w);
-        __syncthreads();
-    }
-}
-
 // compute_moments=False: the split has still folded particle children into their parent's mass
 // and softening (octree.py:956-966); nothing else is rolled up.
 __global__ __launch_bounds__(256) void k_fold_particles(int64_t n, int64_t nnodes, const ptg_tree_arrays t)
@@ -1747,7 +1730,7 @@ __global__ void k_empty_roots(const ptg_walk_tree wt, int64_t nsys, const int32_
 static ptg_status tree_phase(const double *pos, const double *m, const double *h, int64_t n, int64_t nnodes,
                              int max_level, bool has_collisions, const ptg_forest *forest, int64_t nsys,
                              int quadrupole, int compute_moments, BuildWS &w, NodeWS &nw, const ptg_tree_arrays &t,
-                             const ptg_walk_tree *pack, void *particles_ready, bool deferred, hipStream_t s)
+                             const ptg_walk_tree *pack, void *particles_ready, hipStream_t s)
 {
     (void)pos;
     (void)m;
@@ -1830,41 +1813,14 @@ static ptg_status tree_phase(const double *pos, const double *m, const double *h
     k_slots<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, nsys, F ? forest->roots : nullptr, t, nw);
     PTG_LAUNCH_CHECK();
     phase_mark(PTG_PHASE_LINKS, s);
-    if (compute_moments && deferred) {
-        // levels are not known on the host: one launch per possible level, each reading its node
-        // range from the device
+    if (compute_moments) {
+        // one launch per level, deepest first, each reading its node range from the device (no host
+        // round trip for the level sizes; small levels cost a few microseconds of dispatch each)
         for (int L = max_level; L >= 0; L--) {
             if (quadrupole)
                 k_moments_level<true><<<MOM_DEFERRED_BLOCKS, MOM_THREADS, 0, s>>>(n, nnodes, L, nw.lvstart, byid, t, nw);
             else
                 k_moments_level<false><<<MOM_DEFERRED_BLOCKS, MOM_THREADS, 0, s>>>(n, nnodes, L, nw.lvstart, byid, t, nw);
-        }
-    } else if (compute_moments) {
-        // per-level node counts decide the launches: runs of small levels go to one block, the
-        // others get a grid each
-        std::vector<int32_t> ls(max_level + 2);
-        PTG_HIP_CHECK(hipMemcpyAsync(ls.data(), nw.lvstart, (max_level + 2) * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        PTG_HIP_CHECK(hipStreamSynchronize(s));
-        const int64_t small = MS_THREADS;
-        int L = max_level;
-        while (L >= 0) {
-            if (ls[L + 1] - ls[L] <= small) {
-                int Llo = L;
-                while (Llo - 1 >= 0 && ls[Llo] - ls[Llo - 1] <= small) Llo--;
-                if (quadrupole)
-                    k_moments_small<true><<<1, MS_THREADS, 0, s>>>(n, nnodes, L, Llo, nw.lvstart, byid, t, nw);
-                else
-                    k_moments_small<false><<<1, MS_THREADS, 0, s>>>(n, nnodes, L, Llo, nw.lvstart, byid, t, nw);
-                L = Llo - 1;
-            } else {
-                const int64_t cnt = ls[L + 1] - ls[L];
-                const unsigned blocks = (unsigned)std::min<int64_t>(8192, ptg_cdiv(cnt, MOM_THREADS));
-                if (quadrupole)
-                    k_moments_level<true><<<blocks, MOM_THREADS, 0, s>>>(n, nnodes, L, nw.lvstart, byid, t, nw);
-                else
-                    k_moments_level<false><<<blocks, MOM_THREADS, 0, s>>>(n, nnodes, L, nw.lvstart, byid, t, nw);
-                L--;
-            }
         }
     } else {
         k_fold_particles<<<grid1(nnodes), 256, 0, s>>>(n, nnodes, t);
@@ -1933,7 +1889,7 @@ extern "C" ptg_status ptg_build_tree(const double *pos, const double *m, const d
         PTG_LAUNCH_CHECK();
     }
     return tree_phase(pos, m, h, n, nnodes, max_level, info->has_collisions != 0, forest, nsys, quadrupole,
-                      compute_moments, w, nw, t, pack, particles_ready, false, s);
+                      compute_moments, w, nw, t, pack, particles_ready, s);
 }
 
 // ============================================================================================
@@ -2066,7 +2022,7 @@ extern "C" ptg_status ptg_build_deferred(const double *pos, const double *m, con
     phase_mark(PTG_PHASE_COUNT_NODES, s);
     // ---- the second half, over the capacities ----
     st = tree_phase(pos, m, h, n, nnodes_cap, level_cap, true, forest, nsys, quadrupole, compute_moments, w, nw, t,
-                    pack, particles_ready, true, s);
+                    pack, particles_ready, s);
     if (st) return st;
     k_counts_out<<<1, 64, 0, s>>>(w.ctr, w.geoms, n, counts);
     PTG_LAUNCH_CHECK();
