@@ -227,6 +227,18 @@ ptg_status ptg_target_groups(const double *pos, const int64_t *perm, const uint6
                              const ptg_forest *forest, int group_size, int mode, int32_t *offsets, int32_t *ngroups,
                              void *ws, size_t ws_bytes, void *stream);
 
+/* Field-scale sample targets (no reference counterpart: the sample walk that sets the RMS-field
+ * unit of the error-bounded and learned criteria).  From the groups (offsets, *ngroups) of
+ * ptg_target_groups over a tree's own n particles, takes `samples` groups of every system, evenly
+ * spaced over the system's groups (the system's first group + k * its group count / samples), pads
+ * each to group_size targets by repeating its last target, and writes their sorted coordinates and
+ * softenings (read from the tree's Morton-ordered coordinates / softenings) to row
+ * (s * samples + k) * group_size + j of pos (rows x 3) and soft (rows), rows = nsys * samples *
+ * group_size.  forest: NULL for a single tree.  Asynchronous; no host synchronisation. */
+ptg_status ptg_field_sample(const int32_t *offsets, const int32_t *ngroups, const ptg_forest *forest, int64_t n,
+                            int samples, int group_size, const double *coordinates, const double *softenings,
+                            double *pos, double *soft, void *stream);
+
 
 ptg_status ptg_pack_walk(const ptg_tree_arrays *tree, int64_t n, int64_t nnodes, const ptg_forest *forest,
                          const ptg_walk_tree *wt, void *stream);

@@ -119,6 +119,7 @@ SIGNATURES = {
     "ptg_build_deferred": (_c.c_int, [_vp, _vp, _vp, _i64, _F, _c.c_int, _c.c_int, _i64, _c.c_int, _vp, _c.c_size_t,
                                       _vp, _c.c_size_t, _c.POINTER(TreeArrays), _c.POINTER(WalkTree), _vp, _vp,
                                       _vp]),
+    "ptg_field_sample": (_c.c_int, [_vp, _vp, _F, _i64, _c.c_int, _c.c_int, _vp, _vp, _vp, _vp, _vp]),
     "ptg_pack_walk": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _F, _c.POINTER(WalkTree), _vp]),
     "ptg_walk_workspace_bytes": (_c.c_size_t, [_i64, _c.c_int]),
     "ptg_column_rays": (_c.c_int, [_c.POINTER(TreeArrays), _i64, _i64, _c.POINTER(WalkTree), _vp, _i64, _vp, _i64,

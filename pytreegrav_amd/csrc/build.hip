@@ -1815,7 +1815,9 @@ static ptg_status tree_phase(const double *pos, const double *m, const double *h
     phase_mark(PTG_PHASE_LINKS, s);
     if (compute_moments) {
         // one launch per level, deepest first, each reading its node range from the device (no host
-        // round trip for the level sizes; small levels cost a few microseconds of dispatch each)
+        // round trip for the level sizes; small levels cost a few microseconds of dispatch and load
+        // latency each: running the levels nearest the root in one workgroup with barriers measured
+        // no faster, as each level's chain of dependent loads remains)
         for (int L = max_level; L >= 0; L--) {
             if (quadrupole)
                 k_moments_level<true><<<MOM_DEFERRED_BLOCKS, MOM_THREADS, 0, s>>>(n, nnodes, L, nw.lvstart, byid, t, nw);
@@ -2191,131 +2193,233 @@ __global__ __launch_bounds__(256) void k_sorted_keys(const double *pos, const in
     keys[t] = morton_key(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], g->cmin, g->inv);
 }
 
-// number of consecutive positions on one side of t (direction dir), inside [lo, hi), whose key
-// shares the top 3L bits with keys[t], capped at cap
-__device__ int64_t same_cell_extent(const uint64_t *keys, int64_t lo_b, int64_t hi_b, int64_t t, int L, int dir,
-                                    int64_t cap)
+// One workgroup per tile of GT_TILE consecutive sorted targets.  A capped cell starts at t when t
+// starts a system or the smallest cell holding t-1 and t (the cell of the L levels their keys share)
+// has more than cap = 4 x group_size targets.  With keys sorted within each system that cell is a
+// contiguous run containing t-1 and t, so it holds more than cap targets exactly when one of the
+// cap + 1-target windows [j, j + cap] with t - cap <= j <= t - 1 lies inside it, i.e. when the
+// levels keys[j] and keys[j + cap] share (the window's own smallest cell; the window must not leave
+// the system) are at least L.  Each tile therefore sets, for every level l = 0..21, one bit per
+// window whose shared levels are >= l, and a cut flag is the OR of at most cap / 64 + 2 LDS words
+// of level L's mask: no search.  Between two cuts there are at most cap targets (a longer stretch
+// would contain a window inside the cell of its least-sharing adjacent pair), so the capped cell
+// holding a target of the tile is found from the cut flags within margin >= cap of the tile.
+#define GT_TILE 1024
+#define GT_WORDS (GT_TILE / 64)
+#define GT_MAX_MARGIN 1024  // group_size <= 256
+#define GT_LEVELS 22
+
+static size_t group_tiles_lds(int64_t margin, int64_t cap)
 {
-    auto same = [&](int64_t j) {
-        if (L == 0) return true;
-        if (L >= 21) return keys[j] == keys[t];
-        return (keys[j] >> (63 - 3 * L)) == (keys[t] >> (63 - 3 * L));
-    };
-    int64_t good = 0, step = 1;
-    while (good < cap) {
-        int64_t probe = std::min(good + step, cap);
-        int64_t j = t + dir * probe;
-        if (j < lo_b || j >= hi_b || !same(j)) {
-            // bisect between good (matching) and probe (not matching or outside)
-            int64_t lo = good, hi = probe;
-            while (hi - lo > 1) {
-                int64_t mid = (lo + hi) >> 1;
-                int64_t jm = t + dir * mid;
-                if (jm >= lo_b && jm < hi_b && same(jm)) lo = mid;
-                else hi = mid;
+    const int64_t nkeys = GT_TILE + 3 * margin + cap, nwin = GT_TILE + 3 * margin;
+    return (size_t)(nkeys + GT_LEVELS * (nwin / 64) + ptg_cdiv(nkeys, 64)) * sizeof(uint64_t);
+}
+
+// OR of bits [a, b] (a <= b) of the bitmask m
+__device__ __forceinline__ uint64_t bits_any(const uint64_t *m, int64_t a, int64_t b)
+{
+    const int64_t wa = a >> 6, wb = b >> 6;
+    uint64_t acc = 0;
+    for (int64_t w = wa; w <= wb; ++w) {
+        uint64_t x = m[w];
+        if (w == wa) x &= ~0ull << (a & 63);
+        if (w == wb) x &= ~0ull >> (63 - (b & 63));
+        acc |= x;
+    }
+    return acc;
+}
+
+// gmask[tile]: one bit per target of the tile, set where a group starts; tcount[tile]: their number.
+// cells == 0 is the fixed grouping (runs of gs restarted at every system start).  Dynamic LDS in
+// cells mode: group_tiles_lds(margin, cap) bytes, holding the keys of positions
+// [T0 - 2 margin, T0 + GT_TILE + margin + cap), the level masks of the windows starting in
+// [T0 - 2 margin, T0 + GT_TILE + margin) and the system-start bits of the key positions.  Four
+// wavefronts per tile, so that several tiles share a CU and hide each other's load latency.
+#define GT_THREADS 256
+__global__ __launch_bounds__(GT_THREADS) void k_group_tiles(const uint64_t *keys, int64_t n, const int64_t *soff,
+                                                            int64_t nsys, int gs, int cells, int64_t margin,
+                                                            uint64_t *gmask, int32_t *tcount)
+{
+    extern __shared__ uint64_t sk[];
+    __shared__ uint64_t cutw[(GT_TILE + 2 * GT_MAX_MARGIN) / 64];
+    __shared__ int32_t wcnt[GT_WORDS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t T0 = (int64_t)blockIdx.x * GT_TILE;
+    const int64_t cap = 4 * (int64_t)gs;
+    const int64_t K0 = T0 - 2 * margin;  // first key position held, also the first window
+    if (cells) {
+        const int64_t nwin = GT_TILE + 3 * margin;  // windows j in [K0, K0 + nwin), a multiple of 64
+        const int64_t nkeys = nwin + cap;           // keys of positions [K0, K0 + nkeys)
+        const int64_t nw = nwin >> 6;
+        uint64_t *lvm = sk + nkeys;                 // level l's mask: lvm[l * nw + word]
+        uint64_t *sst = lvm + GT_LEVELS * nw;       // system starts among the key positions
+        // positions outside [0, n) load a clamped copy that is never read
+#pragma unroll 4
+        for (int64_t i = threadIdx.x; i < nkeys; i += GT_THREADS)
+            sk[i] = keys[std::min<int64_t>(std::max<int64_t>(K0 + i, 0), n - 1)];
+        if (nsys > 1) {
+            for (int64_t base = 64 * wave; base < nkeys; base += GT_THREADS) {
+                const int64_t j = K0 + base + lane;
+                const bool st = j >= 0 && j < n && soff[upper_index(soff, nsys, j)] == j;
+                const uint64_t b = __ballot(st);
+                if (lane == 0) sst[base >> 6] = b;
             }
-            return lo;
         }
-        good = probe;
-        step <<= 1;
+        __syncthreads();
+        // the levels each window [j, j + cap] shares, -1 where it leaves the targets or its system;
+        // level l's bit is set where that is >= l: below the wave's smallest level every valid window
+        // qualifies, above its largest none does, so only the levels in between take a ballot
+        for (int64_t base = 64 * wave; base < nwin; base += GT_THREADS) {
+            const int64_t j = K0 + base + lane;
+            int lc = -1;
+            if (j >= 0 && j + cap < n && (nsys == 1 || !bits_any(sst, j + 1 - K0, j + cap - K0)))
+                lc = ptg_common_levels(sk[j - K0], sk[j + cap - K0]);
+            int lo = lc < 0 ? GT_LEVELS : lc, hi = lc;
+            for (int d = 32; d >= 1; d >>= 1) {
+                lo = min(lo, __shfl_xor(lo, d));
+                hi = max(hi, __shfl_xor(hi, d));
+            }
+            const uint64_t valid = __ballot(lc >= 0);
+            uint64_t mine = lane <= lo ? valid : 0;
+            for (int l = lo + 1; l <= hi; ++l) {
+                const uint64_t b = __ballot(lc >= l);
+                if (lane == l) mine = b;
+            }
+            if (lane < GT_LEVELS) lvm[lane * nw + (base >> 6)] = mine;
+        }
+        __syncthreads();
+        // cut flags of positions [T0 - margin, T0 + GT_TILE + margin), one ballot word per 64; a
+        // position at or past n counts as a cut (the end of the last cell), one before 0 never does
+        // (position 0 always starts a cell)
+        const int64_t E = GT_TILE + 2 * margin;
+        for (int64_t base = 64 * wave; base < E; base += GT_THREADS) {
+            const int64_t p = T0 - margin + base + lane;
+            bool c = p >= n;
+            if (p >= 0 && p < n) {
+                const int64_t i = p - K0;
+                if (p == 0 || (nsys > 1 && ((sst[i >> 6] >> (i & 63)) & 1))) {
+                    c = true;
+                } else {
+                    const int L = ptg_common_levels(sk[i - 1], sk[i]);
+                    c = bits_any(lvm + L * nw, i - cap, i - 1) != 0;
+                }
+            }
+            const uint64_t b = __ballot(c);
+            if (lane == 0) cutw[base >> 6] = b;
+        }
+        __syncthreads();
     }
-    return cap;
-}
-
-// pflag[t] = 1 where a new capped cell starts (t == 0 or the start of a system, or the smallest cell
-// holding t-1 and t has more than cap targets)
-__global__ __launch_bounds__(256) void k_cell_cuts(const uint64_t *keys, int64_t n, const int64_t *soff, int64_t nsys,
-                                                   int64_t cap, int32_t *pflag)
-{
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int64_t lo_b = 0, hi_b = n;
-    if (nsys > 1) {
-        const int64_t sy = upper_index(soff, nsys, t);
-        lo_b = soff[sy];
-        hi_b = soff[sy + 1];
-    }
-    if (t == lo_b) {
-        pflag[t] = 1;
-        return;
-    }
-    int L = ptg_common_levels(keys[t - 1], keys[t]);
-    // quick rejection (most targets sit in small cells): with h = cap / 2, if neither position
-    // t - 1 - h nor t + h is in the cell, each side extends less than h and the cell holds at most
-    // cap targets
-    {
-        const int64_t h = cap / 2;
-        const uint64_t kt = keys[t];
-        auto in_cell = [&](int64_t j) {
-            if (j < lo_b || j >= hi_b) return false;
-            if (L == 0) return true;
-            if (L >= 21) return keys[j] == kt;
-            return (keys[j] >> (63 - 3 * L)) == (kt >> (63 - 3 * L));
-        };
-        if (!in_cell(t - 1 - h) && !in_cell(t + h)) {
-            pflag[t] = 0;
-            return;
+    if (cells) {
+        // group starts from every capped cell [ps, pe) starting in [T0 - margin, T0 + GT_TILE) (one
+        // starting earlier ends before the tile, as cells hold at most cap <= margin targets): its
+        // chunks of cs = ceil(len / ceil(len / gs)) targets start at ps, ps + cs, ...
+        __shared__ uint32_t gm[GT_TILE / 32];
+        for (int w = threadIdx.x; w < GT_TILE / 32; w += GT_THREADS) gm[w] = 0;
+        __syncthreads();
+        const int nwc = (int)((GT_TILE + 2 * margin) >> 6);
+        for (int wi = threadIdx.x; wi < (int)((margin + GT_TILE) >> 6); wi += GT_THREADS) {
+            uint64_t m = cutw[wi];
+            while (m) {
+                const int bit = __ffsll((long long)m) - 1;
+                m &= m - 1;
+                const int64_t ps = T0 - margin + 64 * wi + bit;
+                if (ps >= n) break;
+                // the next cut, within cap of ps, so inside the margin
+                const int64_t i1 = ps - (T0 - margin) + 1;
+                int w2 = (int)(i1 >> 6);
+                uint64_t m2 = cutw[w2] & (~0ull << (i1 & 63));
+                while (!m2 && w2 + 1 < nwc) m2 = cutw[++w2];
+                if (!m2) break;  // not for keys sorted within each system
+                const int64_t pe = T0 - margin + 64 * w2 + __ffsll((long long)m2) - 1;
+                // len <= cap <= 1024: 32-bit arithmetic
+                const uint32_t len = (uint32_t)(pe - ps), ugs = (uint32_t)gs;
+                const uint32_t nch = (len + ugs - 1) / ugs;
+                const uint32_t cs = (len + nch - 1) / nch;
+                for (int64_t q = ps; q < pe; q += cs)
+                    if (q >= T0 && q < T0 + GT_TILE) atomicOr(&gm[(q - T0) >> 5], 1u << ((q - T0) & 31));
+            }
+        }
+        __syncthreads();
+        for (int w = threadIdx.x; w < GT_WORDS; w += GT_THREADS) {
+            const uint64_t b = (uint64_t)gm[2 * w] | ((uint64_t)gm[2 * w + 1] << 32);
+            gmask[(int64_t)blockIdx.x * GT_WORDS + w] = b;
+            wcnt[w] = __popcll(b);
+        }
+    } else {
+        for (int k = 0; k < GT_TILE / GT_THREADS; ++k) {
+            const int64_t t = T0 + k * GT_THREADS + threadIdx.x;
+            bool g = false;
+            if (t < n) {
+                const int64_t start = nsys > 1 ? soff[upper_index(soff, nsys, t)] : 0;
+                g = (t - start) % gs == 0;
+            }
+            const uint64_t b = __ballot(g);
+            const int word = k * (GT_THREADS / 64) + wave;
+            if (lane == 0) {
+                gmask[(int64_t)blockIdx.x * GT_WORDS + word] = b;
+                wcnt[word] = __popcll(b);
+            }
         }
     }
-    int64_t left = same_cell_extent(keys, lo_b, hi_b, t - 1, L, -1, cap);
-    int64_t right = left >= cap ? 0 : same_cell_extent(keys, lo_b, hi_b, t, L, +1, cap);
-    pflag[t] = (left + right + 2 > cap) ? 1 : 0;
-}
-
-__global__ __launch_bounds__(256) void k_scatter_starts(const int32_t *flag, const int32_t *ex, int64_t n,
-                                                        int32_t *starts, int32_t *count)
-{
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    if (flag[t]) starts[ex[t]] = (int32_t)t;
-    if (t == n - 1) {
-        int32_t total = ex[t] + flag[t];
-        starts[total] = (int32_t)n;
-        if (count) *count = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t c = 0;
+        for (int w = 0; w < GT_WORDS; ++w) c += wcnt[w];
+        tcount[blockIdx.x] = c;
     }
 }
 
-__global__ __launch_bounds__(256) void k_chunk_flags(const int32_t *pflag, const int32_t *pex, const int32_t *pstart,
-                                                     int64_t n, int gs, int32_t *gflag)
+// offsets[tbase[tile] + rank] = position of each group start, one lane per 64-target word (a wave
+// covers four tiles); the last word closes the list
+__global__ __launch_bounds__(256) void k_group_write(const uint64_t *gmask, const int32_t *tcount,
+                                                     const int32_t *tbase, int64_t n, int64_t ntiles,
+                                                     int32_t *offsets, int32_t *ngroups)
 {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int32_t c = pex[t] + pflag[t] - 1;
-    int64_t ps = pstart[c], pe = pstart[c + 1];
-    int64_t len = pe - ps;
-    int64_t nch = (len + gs - 1) / gs;
-    int64_t cs = (len + nch - 1) / nch;
-    gflag[t] = ((t - ps) % cs == 0) ? 1 : 0;
-}
-
-// fixed runs of gs consecutive targets, restarted at every system start (the reference's grouping,
-// grouped_treewalk.py:179-182, applied to each system of a forest)
-__global__ __launch_bounds__(256) void k_run_flags(int64_t n, const int64_t *soff, int64_t nsys, int gs, int32_t *gflag)
-{
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int64_t start = nsys > 1 ? soff[upper_index(soff, nsys, t)] : 0;
-    gflag[t] = ((t - start) % gs == 0) ? 1 : 0;
+    const int64_t word = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nwords = ntiles * GT_WORDS;
+    const int sub = threadIdx.x & (GT_WORDS - 1);
+    uint64_t w = word < nwords ? gmask[word] : 0;
+    // exclusive prefix of the popcounts within each tile's GT_WORDS lanes
+    int c = __popcll(w), incl = c;
+    for (int d = 1; d < GT_WORDS; d <<= 1) {
+        const int v = __shfl_up(incl, d, GT_WORDS);
+        if (sub >= d) incl += v;
+    }
+    if (word >= nwords) return;
+    const int64_t tile = word / GT_WORDS;
+    int32_t r = tbase[tile] + incl - c;
+    const int64_t p0 = word * 64;
+    while (w) {
+        offsets[r++] = (int32_t)(p0 + __ffsll((long long)w) - 1);
+        w &= w - 1;
+    }
+    if (word == nwords - 1) {
+        const int32_t total = tbase[tile] + tcount[tile];
+        offsets[total] = (int32_t)n;
+        *ngroups = total;
+    }
 }
 
 struct GroupWS {
     double *part;
     Geom *geom;
     uint64_t *keys;
-    int32_t *pflag, *pex, *pstart, *gflag, *gex, *bsum;
+    uint64_t *gmask;
+    int32_t *tcount, *tbase, *bsum;
 };
+
+static int64_t group_tiles(int64_t n) { return std::max<int64_t>(1, ptg_cdiv(n, GT_TILE)); }
 
 static bool carve_groups(PtgArena &A, int64_t n, int64_t nsys, GroupWS &w)
 {
+    const int64_t nt = group_tiles(n);
     w.part = A.take<double>(6 * BB_BLOCKS);
     w.geom = A.take<Geom>(nsys);
     w.keys = A.take<uint64_t>(n);
-    w.pflag = A.take<int32_t>(n);
-    w.pex = A.take<int32_t>(n);
-    w.pstart = A.take<int32_t>(n + 1);
-    w.gflag = A.take<int32_t>(n);
-    w.gex = A.take<int32_t>(n);
-    w.bsum = A.take<int32_t>(scan_blocks(n) + 64);
+    w.gmask = A.take<uint64_t>(nt * GT_WORDS);
+    w.tcount = A.take<int32_t>(nt);
+    w.tbase = A.take<int32_t>(nt);
+    w.bsum = A.take<int32_t>(scan_blocks(nt) + 64);
     return A.ok();
 }
 
@@ -2355,16 +2459,8 @@ extern "C" ptg_status ptg_target_groups(const double *pos, const int64_t *perm, 
         ptg_set_error("ptg_target_groups: mode must be PTG_GROUPS_RUNS or PTG_GROUPS_CELLS");
         return PTG_ERR_ARG;
     }
-    if (mode == PTG_GROUPS_RUNS) {
-        k_run_flags<<<grid1(n), 256, 0, s>>>(n, soff, nsys, group_size, w.gflag);
-        PTG_LAUNCH_CHECK();
-        st = excl_scan(LoadArr{w.gflag}, n, w.gex, w.bsum, nullptr, s);
-        if (st) return st;
-        k_scatter_starts<<<grid1(n), 256, 0, s>>>(w.gflag, w.gex, n, offsets, ngroups);
-        PTG_LAUNCH_CHECK();
-        return PTG_OK;
-    }
-    if (!keys) {
+    const bool cells = mode == PTG_GROUPS_CELLS;
+    if (cells && !keys) {
         if (nsys > 1) {
             // targets are the forest's own particles: system s owns input indices and sorted
             // positions [offsets[s], offsets[s+1])
@@ -2377,16 +2473,77 @@ extern "C" ptg_status ptg_target_groups(const double *pos, const int64_t *perm, 
         k_sorted_keys<<<grid1(n), 256, 0, s>>>(pos, perm, n, w.geom, soff, nsys, w.keys);
         keys = w.keys;
     }
-    k_cell_cuts<<<grid1(n), 256, 0, s>>>(keys, n, soff, nsys, 4 * (int64_t)group_size, w.pflag);
+    const int64_t cap = 4 * (int64_t)group_size;
+    const int64_t margin = cells ? ptg_cdiv(cap, 64) * 64 : 0;  // <= GT_MAX_MARGIN for group_size <= 256
+    const size_t lds = cells ? group_tiles_lds(margin, cap) : 0;
+    const int64_t nt = group_tiles(n);
+    k_group_tiles<<<(unsigned)nt, GT_THREADS, lds, s>>>(keys, n, soff, nsys, group_size, cells ? 1 : 0, margin, w.gmask,
+                                                     w.tcount);
     PTG_LAUNCH_CHECK();
-    st = excl_scan(LoadArr{w.pflag}, n, w.pex, w.bsum, nullptr, s);
+    st = excl_scan(LoadArr{w.tcount}, nt, w.tbase, w.bsum, nullptr, s);
     if (st) return st;
-    k_scatter_starts<<<grid1(n), 256, 0, s>>>(w.pflag, w.pex, n, w.pstart, nullptr);
-    k_chunk_flags<<<grid1(n), 256, 0, s>>>(w.pflag, w.pex, w.pstart, n, group_size, w.gflag);
+    k_group_write<<<(unsigned)ptg_cdiv(nt * GT_WORDS, 256), 256, 0, s>>>(w.gmask, w.tcount, w.tbase, n, nt, offsets,
+                                                                       ngroups);
     PTG_LAUNCH_CHECK();
-    st = excl_scan(LoadArr{w.gflag}, n, w.gex, w.bsum, nullptr, s);
+    return PTG_OK;
+}
+
+// ============================================================================================
+// field-scale sample targets: `samples` of each system's target groups, evenly spaced over the
+// system's groups (group gfirst + k * gnum / samples for k < samples), each padded to gs targets by
+// repeating its last target; row (s * samples + k) * gs + j of pos / soft.  The groups of system s
+// are those whose start lies in [soff[s], soff[s+1]): a binary search over the group starts.
+// ============================================================================================
+__device__ __forceinline__ int64_t first_group_at(const int32_t *offsets, int64_t ng, int64_t v)
+{
+    int64_t lo = 0, hi = ng;  // first group whose start is >= v (ng if none)
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (offsets[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_field_sample(const int32_t *offsets, const int32_t *ngroups,
+                                                      const int64_t *soff, int64_t n, int64_t nsys, int samples,
+                                                      int gs, const double *coords, const double *soft, double *pos,
+                                                      double *h)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nsys * samples * gs) return;
+    const int64_t j = r % gs, sk = r / gs, k = sk % samples, s = sk / samples;
+    const int64_t ng = *ngroups;
+    int64_t g0 = 0, g1 = ng;
+    if (nsys > 1) {
+        g0 = first_group_at(offsets, ng, soff[s]);
+        g1 = s + 1 < nsys ? first_group_at(offsets, ng, soff[s + 1]) : ng;
+    }
+    const int64_t g = g0 + (k * (g1 - g0)) / samples;
+    const int64_t a = offsets[g], b = offsets[g + 1];
+    const int64_t i = a + std::min<int64_t>(j, b - a - 1);
+    pos[3 * r] = coords[3 * i];
+    pos[3 * r + 1] = coords[3 * i + 1];
+    pos[3 * r + 2] = coords[3 * i + 2];
+    h[r] = soft[i];
+}
+
+extern "C" ptg_status ptg_field_sample(const int32_t *offsets, const int32_t *ngroups, const ptg_forest *forest,
+                                       int64_t n, int samples, int group_size, const double *coordinates,
+                                       const double *softenings, double *pos, double *soft, void *stream)
+{
+    if (n < 1 || !offsets || !ngroups || samples < 1 || group_size < 1 || !coordinates || !softenings || !pos ||
+        !soft) {
+        ptg_set_error("ptg_field_sample: bad arguments");
+        return PTG_ERR_ARG;
+    }
+    int64_t nsys;
+    ptg_status st = forest_check(forest, n, "ptg_field_sample", &nsys);
     if (st) return st;
-    k_scatter_starts<<<grid1(n), 256, 0, s>>>(w.gflag, w.gex, n, offsets, ngroups);
+    const int64_t rows = nsys * samples * (int64_t)group_size;
+    k_field_sample<<<grid1(rows), 256, 0, (hipStream_t)stream>>>(offsets, ngroups, nsys > 1 ? forest->offsets : nullptr,
+                                                                 n, nsys, samples, group_size, coordinates, softenings,
+                                                                 pos, soft);
     PTG_LAUNCH_CHECK();
     return PTG_OK;
 }

@@ -766,10 +766,13 @@ def field_scale_sample(tree, groups=64, group_size=64, cells=None):
     gs = min(group_size, n)
     j = torch.arange(gs, device=device(), dtype=torch.int64)[None, :]
     if cells is not None:
+        # groups k * count / groups of the cell-aligned groups, each padded to gs targets
         off, cnt = cells
-        g = (torch.arange(k, device=device(), dtype=torch.int64) * cnt.to(torch.int64)) // k
-        a, b = off.index_select(0, g).to(torch.int64), off.index_select(0, g + 1).to(torch.int64)
-        idx = (a[:, None] + torch.minimum(j, (b - a - 1)[:, None])).reshape(-1)
+        pos = torch.empty((k * gs, 3), dtype=torch.float64, device=device())
+        h = torch.empty(k * gs, dtype=torch.float64, device=device())
+        check(_lib.load().ptg_field_sample(ptr(off), ptr(cnt), None, n, k, gs, ptr(tree.coordinates),
+                                           ptr(tree.softenings), ptr(pos), ptr(h), stream()), "ptg_field_sample")
+        return pos, h, torch.arange(k * gs, device=device()), gs
     else:
         starts = (torch.arange(k, device=device(), dtype=torch.int64) * (n // k))
         idx = (starts[:, None] + j).reshape(-1)

@@ -61,6 +61,7 @@ class Forest:
         self._graphs = {}
         self.tuning = None
         self.field_samples = int(field_samples)  # sample groups per system of the RMS-field estimate
+        self._sample_consts = {}
 
     # ---------------------------------------------------------------------------------------------
     def _groups(self, gs, goff, gcnt, gws, t=None):
@@ -98,35 +99,30 @@ class Forest:
         t = t or self.tree
         samples = self.field_samples if samples is None else int(samples)
         dev = device()
-        # everything stays on the device (the count of groups too), so a captured step can run this:
-        # group starts past the count read as n, after every system start
-        soff = t.sys_offsets
-        idx = torch.arange(self.n + 1, device=dev, dtype=torch.int64)
-        starts = torch.where(idx < gcnt.to(torch.int64), goff.to(torch.int64), self.n)
-        gfirst = torch.searchsorted(starts, soff)  # first group of each system, then the group count
-        gnum = gfirst[1:] - gfirst[:-1]
-        gfirst = gfirst[:-1]
-        k = torch.arange(samples, device=dev, dtype=torch.int64)
-        g = gfirst[:, None] + (k[None, :] * gnum[:, None]) // samples  # (nsys, samples)
-        g = g.reshape(-1)
-        a = goff.index_select(0, g).to(torch.int64)
-        b = goff.index_select(0, g + 1).to(torch.int64)
-        j = torch.arange(gs, device=dev, dtype=torch.int64)[None, :]
-        idx = (a[:, None] + torch.minimum(j, (b - a - 1)[:, None])).reshape(-1)
-        pos = t.coordinates.index_select(0, idx)
-        h = t.softenings.index_select(0, idx)
-        nt = idx.numel()
-        # the samples form a forest of their own: system s owns sample targets [s, s+1) * samples * gs
-        soff_s = torch.arange(self.nsys + 1, device=dev, dtype=torch.int64) * (samples * gs)
+        # everything stays on the device (the count of groups too), so a captured step can run this;
+        # the sample forest's offsets, groups and order are constants, made once per (samples, gs)
+        key = (samples, gs)
+        if key not in self._sample_consts:
+            # the samples form a forest of their own: system s owns sample targets [s, s+1) * samples * gs
+            nt = self.nsys * samples * gs
+            self._sample_consts[key] = (
+                torch.arange(self.nsys + 1, device=dev, dtype=torch.int64) * (samples * gs),
+                (torch.arange(self.nsys * samples + 1, device=dev, dtype=torch.int64) * gs).to(torch.int32),
+                torch.full((1,), self.nsys * samples, dtype=torch.int32, device=dev),
+                torch.arange(nt, device=dev, dtype=torch.int64))
+        soff_s, goff_s, gcnt_s, perm = self._sample_consts[key]
+        nt = perm.numel()
+        pos = torch.empty((nt, 3), dtype=torch.float64, device=dev)
+        h = torch.empty(nt, dtype=torch.float64, device=dev)
+        check(_lib.load().ptg_field_sample(ptr(goff), ptr(gcnt), t.forest_ref, self.n, samples, gs,
+                                           ptr(t.coordinates), ptr(t.softenings), ptr(pos), ptr(h), stream()),
+              "ptg_field_sample")
         fs = _lib.Forest(self.nsys, ptr(soff_s).value, ptr(t.roots).value, ptr(t.cubes).value)
-        goff_s = (torch.arange(self.nsys * samples + 1, device=dev, dtype=torch.int64) * gs).to(torch.int32)
-        gcnt_s = torch.full((1,), self.nsys * samples, dtype=torch.int32, device=dev)
         out = torch.empty((nt, 3), dtype=torch.float64, device=dev)
         L = _lib.load()
         ws = workspace(L.ptg_walk_workspace_bytes(nt, gs))
         p = t.pack(PTG_F32, False, norms=False if self.criterion == "learned" else None)
         crit = _lib.Criterion(_lib.PTG_CRIT_THETA, 0, 0.0)
-        perm = torch.arange(nt, device=dev, dtype=torch.int64)
         check(L.ptg_walk(_lib.ctypes.byref(p["wt"]), _lib.ctypes.byref(fs), ptr(pos), ptr(h), ptr(perm), 1, nt,
                          PTG_ACC, 0, gs, ptr(goff_s), ptr(gcnt_s), 1.0, _lib.ctypes.byref(crit), 1.0, 0, ptr(out),
                          None, ptr(ws), ws.numel(), stream()), "ptg_walk")
