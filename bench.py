@@ -17,7 +17,16 @@ below expansion_rel x the element's own field there (a cap on the group radius o
 which keeps the worst-case error down).  The defaults (alpha 5e-4, expansion 4e-3, expansion_rel 3)
 give acceleration RMS and max errors and a potential RMS error at or below the plain theta=0.7
 walk's, which the same run also times ("plain_theta_0.7") on the same targets.  Inputs (positions, masses, softenings) are resident in HBM
-before the timed region.  Particles: Plummer sphere, M=1, a=1, truncated at 100 scale radii,
+before the timed region.
+
+Execution (--mode): the headline ("graph") captures the whole step once with Forest.capture_step --
+a sync-free build into arrays sized from an eager build of the same particles, the groups, the field
+scale, the radii and the walk -- and times replays of that one HIP graph; every replay recomputes
+everything from the positions in HBM and reads the build's counts back once (overflow check).  Its
+output is checked bit-equal to the eager step's and its errors are scored on it.  The eager step
+(--mode eager: launches from the host, host reads inside the build) is timed in the same run, and
+the phase breakdown (build alone, walk by HIP events) comes from it.  The plain theta=0.7 step is
+timed the same way as the headline.  Particles: Plummer sphere, M=1, a=1, truncated at 100 scale radii,
 h=0.01, seed 0.
 
 Accuracy gates, against FP64 direct summation on 1e4 random targets with the error definitions of
@@ -96,6 +105,10 @@ def parse():
                     help="eager: every step builds with exact node counts (host reads inside the build); "
                          "resident: steps build without host synchronisation into arrays sized by the "
                          "previous build's node count, verified after the step (ResidentBuilder)")
+    ap.add_argument("--mode", choices=("graph", "eager"), default="graph",
+                    help="graph (headline): the whole step -- sync-free build, groups, field scale, radii, walk -- "
+                         "captured once in one HIP graph and replayed per step; eager: the same step as a stream "
+                         "of launches from the host (its phases are timed either way)")
     ap.add_argument("--group-size", type=int, default=64)
     ap.add_argument("--variant", type=int, default=0,
                     help="FP32 walk kernel variant (engine.WALK_VARIANTS: waves per SIMD, interaction-list length)")
@@ -680,19 +693,19 @@ def main():
         r["out"] = out
         return r
 
-    def run_graph_step(theta, eager_out):
+    def run_graph_step(criterion, theta, alpha, expansion, eager_out):
         """The same evaluation as one HIP graph per step: Forest.capture_step on the single system
         (sync-free build sized by the eager tree, cell groups, field scale from the same 64 sample
         groups, opening radii, walk), replayed on the resident inputs; each replay reads the build's
-        counts once afterwards (one host synchronisation per step)."""
+        counts once afterwards (one host synchronisation per step).  Returns (summary, output)."""
         from pytreegrav_amd import Forest
 
         F = Forest(pos, mass, soft, [0, N], precision="float32", group_size=args.group_size,
-                   criterion=args.criterion, field_samples=64)
-        kw = dict(theta=theta, expansion=args.expansion,
-                  expansion_rel=args.expansion_rel if args.expansion is not None else 0.0)
-        if args.criterion == "learned":
-            kw.update(alpha=args.alpha, bf16=args.bf16)
+                   criterion=criterion, field_samples=64)
+        kw = dict(theta=theta, expansion=expansion,
+                  expansion_rel=args.expansion_rel if expansion is not None else 0.0)
+        if criterion == "learned":
+            kw.update(alpha=alpha, bf16=args.bf16)
         replay, gout = F.capture_step(potential=True, accel=True, **kw)
         for _ in range(max(1, args.warmup)):
             replay()
@@ -702,10 +715,10 @@ def main():
             replay()
         torch.cuda.synchronize()
         gdt = (time.perf_counter() - t0) / args.steps
-        return {"ms_per_step": gdt * 1e3, "particles_per_s": N / gdt,
+        return {"dt": gdt, "ms_per_step": gdt * 1e3, "particles_per_s": N / gdt,
                 "equals_eager": bool(torch.equal(gout, eager_out)),
                 "what": "build (sync-free, node capacity 1.25x the eager tree's), groups, field scale, radii "
-                        "and walk of the same configuration captured in one HIP graph and replayed"}
+                        "and walk of the same configuration captured in one HIP graph and replayed"}, gout
 
     def accuracy(out):
         res = out.index_select(0, torch.from_numpy(sel).to(dev)).cpu().numpy()
@@ -747,21 +760,35 @@ def main():
         args.expansion_rel = 3.0 if args.expansion is not None and args.criterion == "learned" else 0.0
     quad = bool(args.quadrupole)
     theta = args.theta if args.theta is not None else (0.7 if args.criterion == "theta" else 2.0)
+    graph_mode = args.mode == "graph"
+    if graph_mode and (args.criterion not in ("learned", "theta") or quad):
+        raise SystemExit("--mode graph captures the monopole theta or learned step; use --mode eager")
     main = run_config(args.criterion, theta, args.alpha, quad, args.expansion)
     graph = None
-    if not args.no_graph and args.criterion in ("learned", "theta") and not quad:
-        graph = run_graph_step(theta, main["out"])
+    if graph_mode or (not args.no_graph and args.criterion in ("learned", "theta") and not quad):
+        graph, gout = run_graph_step(args.criterion, theta, args.alpha, args.expansion, main["out"])
+        if graph_mode:  # the headline's errors are those of the output the timed replays wrote
+            main["accuracy"] = accuracy(gout) if exact is not None else None
+        del gout
     plain = None
     if not args.no_plain and not (args.criterion == "theta" and theta == 0.7 and not quad):
         p = run_config("theta", 0.7, 0.0, False)
-        plain = {"criterion": "theta", "theta": 0.7, "moments": "monopole", "ms_per_step": p["dt"] * 1e3,
-                 "particles_per_s": N / p["dt"], "walk_ms": p["walk_ms"], "tree_build_ms": p["build_ms"],
-                 "interactions_per_particle": p["interactions"] / N, "accuracy": p["accuracy"],
+        p_dt, p_acc, p_eq = p["dt"], p["accuracy"], None
+        if graph_mode:
+            pg, pout = run_graph_step("theta", 0.7, 0.0, None, p["out"])
+            p_dt, p_eq = pg["dt"], pg["equals_eager"]
+            p_acc = accuracy(pout) if exact is not None else None
+            del pout
+        plain = {"criterion": "theta", "theta": 0.7, "moments": "monopole", "ms_per_step": p_dt * 1e3,
+                 "particles_per_s": N / p_dt, "eager_ms_per_step": p["dt"] * 1e3, "graph_equals_eager": p_eq,
+                 "walk_ms": p["walk_ms"], "tree_build_ms": p["build_ms"],
+                 "interactions_per_particle": p["interactions"] / N, "accuracy": p_acc,
                  "walk_speedup_of_headline": p["walk_ms"] / main["walk_ms"],
-                 "step_speedup_of_headline": p["dt"] / main["dt"]}
-        if main["accuracy"] is not None and p["accuracy"] is not None:
-            main["accuracy"]["gate_vs_plain_theta_0.7"] = gate_vs_plain(main["accuracy"], p["accuracy"])
-    dt, walk_ms, build_ms = main["dt"], main["walk_ms"], main["build_ms"]
+                 "step_speedup_of_headline": p_dt / (graph["dt"] if graph_mode else main["dt"])}
+        if main["accuracy"] is not None and p_acc is not None:
+            main["accuracy"]["gate_vs_plain_theta_0.7"] = gate_vs_plain(main["accuracy"], p_acc)
+    dt = graph["dt"] if graph_mode else main["dt"]
+    walk_ms, build_ms = main["walk_ms"], main["build_ms"]
     interactions, visits, acc = main["interactions"], main["visits"], main["accuracy"]
     pinter, folds = main["particle_interactions"], main["folds"]
 
@@ -804,15 +831,21 @@ def main():
             "grouping": "cell-aligned",
             "walk_precision": "fp32 node-relative, fp32 accumulation",
             "build_precision": "fp64 keys and moments",
-            "build": args.build,
+            "execution": ("one HIP graph per step (Forest.capture_step, captured once; each replay rebuilds the tree "
+                          "from the resident positions and reads its counts back once)") if graph_mode else
+                         "eager launches from the host, one stream plus a side stream for the groups",
+            "build": "sync-free (ptg_build_deferred, node capacity 1.25x)" if graph_mode else args.build,
             "moments": ("quadrupole" if quad else "monopole") + " interactions"
                        + ("; fp64 quadrupoles built for the criterion" if args.criterion != "theta" else ""),
             "parallelism": "single GPU",
         },
         "breakdown": {
+            "phases_from": "the eager steps of the same configuration: build timed alone, walk by HIP events "
+                           "around its launch, the rest by difference",
+            "eager_step": {"ms_per_step": main["dt"] * 1e3, "particles_per_s": N / main["dt"], "build": args.build},
             "tree_build_ms": build_ms,
             "tree_build_phases": main["build_phases"],
-            "pack_groups_fieldscale_radii_ms": dt * 1e3 - build_ms - walk_ms,
+            "pack_groups_fieldscale_radii_ms": main["dt"] * 1e3 - build_ms - walk_ms,
             "walk_ms": walk_ms,
             "walk_only_particles_per_s": N / (walk_ms * 1e-3),
             "interactions_per_particle": interactions / N,
