@@ -765,22 +765,37 @@ def main():
         raise SystemExit("--mode graph captures the monopole theta or learned step; use --mode eager")
     main = run_config(args.criterion, theta, args.alpha, quad, args.expansion)
     graph = None
+    graph_error = None
     if graph_mode or (not args.no_graph and args.criterion in ("learned", "theta") and not quad):
-        graph, gout = run_graph_step(args.criterion, theta, args.alpha, args.expansion, main["out"])
-        if graph_mode:  # the headline's errors are those of the output the timed replays wrote
-            main["accuracy"] = accuracy(gout) if exact is not None else None
-        del gout
+        try:
+            graph, gout = run_graph_step(args.criterion, theta, args.alpha, args.expansion, main["out"])
+        except Exception as e:  # the line still reports the eager step, and says why
+            graph_error = f"{type(e).__name__}: {e}"
+            print(f"bench.py: the captured step failed ({graph_error}); reporting the eager step",
+                  file=sys.stderr, flush=True)
+            graph_mode = False
+        else:
+            if graph_mode:  # the headline's errors are those of the output the timed replays wrote
+                main["accuracy"] = accuracy(gout) if exact is not None else None
+            del gout
     plain = None
     if not args.no_plain and not (args.criterion == "theta" and theta == 0.7 and not quad):
         p = run_config("theta", 0.7, 0.0, False)
-        p_dt, p_acc, p_eq = p["dt"], p["accuracy"], None
+        p_dt, p_acc, p_eq, p_err = p["dt"], p["accuracy"], None, None
         if graph_mode:
-            pg, pout = run_graph_step("theta", 0.7, 0.0, None, p["out"])
-            p_dt, p_eq = pg["dt"], pg["equals_eager"]
-            p_acc = accuracy(pout) if exact is not None else None
-            del pout
+            try:
+                pg, pout = run_graph_step("theta", 0.7, 0.0, None, p["out"])
+            except Exception as e:  # timed eagerly instead, and the line says so
+                p_err = f"{type(e).__name__}: {e}"
+                print(f"bench.py: the captured theta=0.7 step failed ({p_err}); timing it eagerly",
+                      file=sys.stderr, flush=True)
+            else:
+                p_dt, p_eq = pg["dt"], pg["equals_eager"]
+                p_acc = accuracy(pout) if exact is not None else None
+                del pout
         plain = {"criterion": "theta", "theta": 0.7, "moments": "monopole", "ms_per_step": p_dt * 1e3,
                  "particles_per_s": N / p_dt, "eager_ms_per_step": p["dt"] * 1e3, "graph_equals_eager": p_eq,
+                 "graph_error": p_err,
                  "walk_ms": p["walk_ms"], "tree_build_ms": p["build_ms"],
                  "interactions_per_particle": p["interactions"] / N, "accuracy": p_acc,
                  "walk_speedup_of_headline": p["walk_ms"] / main["walk_ms"],
@@ -833,7 +848,8 @@ def main():
             "build_precision": "fp64 keys and moments",
             "execution": ("one HIP graph per step (Forest.capture_step, captured once; each replay rebuilds the tree "
                           "from the resident positions and reads its counts back once)") if graph_mode else
-                         "eager launches from the host, one stream plus a side stream for the groups",
+                         "eager launches from the host, one stream plus a side stream for the groups"
+                         + (f" (the captured step failed: {graph_error})" if graph_error else ""),
             "build": "sync-free (ptg_build_deferred, node capacity 1.25x)" if graph_mode else args.build,
             "moments": ("quadrupole" if quad else "monopole") + " interactions"
                        + ("; fp64 quadrupoles built for the criterion" if args.criterion != "theta" else ""),
