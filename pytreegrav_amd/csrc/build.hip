@@ -53,9 +53,14 @@ extern "C" int ptg_version(void) { return 10000; }
 // ============================================================================================
 // generic exclusive scan of int32 values produced by a loader functor
 // ============================================================================================
+// exclusive scans of n int32 values given by a load functor: per-tile sums, one workgroup scanning
+// the tile sums, and a pass that rescans each tile from its offset.  Tiles of SC_TILE values are
+// read striped (value base + q * SC_THREADS + thread, coalesced per load); the rescan moves them
+// through LDS to contiguous runs of SC_ITEMS per thread and back.
 #define SC_THREADS 256
-#define SC_ITEMS 8
+#define SC_ITEMS 16
 #define SC_TILE (SC_THREADS * SC_ITEMS)
+#define SC_PAD(j) ((j) + ((j) >> 4))  // one pad word per 16: a thread's run of 16 starts on its own bank
 
 template <typename T> __device__ __forceinline__ T block_excl_scan(T v, T *total, T *sh)
 {
@@ -85,32 +90,37 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_reduce(Load ld, int64_t n, 
 {
     if (gate && *gate == 0) return;
     __shared__ int32_t sh[8];
-    int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_ITEMS;
+    const int64_t base = (int64_t)blockIdx.x * SC_TILE + threadIdx.x;
     int32_t s = 0;
 #pragma unroll
-    for (int q = 0; q < SC_ITEMS; q++)
-        if (base + q < n) s += ld(base + q);
+    for (int q = 0; q < SC_ITEMS; q++) {
+        const int64_t i = base + (int64_t)q * SC_THREADS;
+        if (i < n) s += ld(i);
+    }
     int32_t tot;
     block_excl_scan<int32_t>(s, &tot, sh);
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// single block: exclusive scan of nb block sums in place; grand total to *total
+// single workgroup: exclusive scan of nb tile sums in place (each thread a contiguous slice);
+// grand total to *total
 __global__ __launch_bounds__(SC_THREADS) void k_scan_top(int32_t *bsum, int64_t nb, int32_t *total,
                                                       const int32_t *gate)
 {
     if (gate && *gate == 0) return;
     __shared__ int32_t sh[8];
-    int32_t carry = 0;
-    for (int64_t base = 0; base < nb; base += SC_THREADS) {
-        int64_t i = base + threadIdx.x;
-        int32_t v = i < nb ? bsum[i] : 0;
-        int32_t tot;
-        int32_t ex = block_excl_scan<int32_t>(v, &tot, sh);
-        if (i < nb) bsum[i] = carry + ex;
-        carry += tot;
+    const int64_t per = (nb + SC_THREADS - 1) / SC_THREADS;
+    const int64_t a = (int64_t)threadIdx.x * per, b = std::min<int64_t>(a + per, nb);
+    int32_t s = 0;
+    for (int64_t i = a; i < b; i++) s += bsum[i];
+    int32_t tot;
+    int32_t run = block_excl_scan<int32_t>(s, &tot, sh);
+    for (int64_t i = a; i < b; i++) {
+        const int32_t v = bsum[i];
+        bsum[i] = run;
+        run += v;
     }
-    if (threadIdx.x == 0 && total) *total = carry;
+    if (threadIdx.x == 0 && total) *total = tot;
 }
 
 template <typename Load>
@@ -119,20 +129,34 @@ __global__ __launch_bounds__(SC_THREADS) void k_scan_down(Load ld, int64_t n, co
 {
     if (gate && *gate == 0) return;
     __shared__ int32_t sh[8];
-    int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_ITEMS;
+    __shared__ int32_t tile[SC_PAD(SC_TILE)];
+    const int64_t base = (int64_t)blockIdx.x * SC_TILE;
+#pragma unroll
+    for (int q = 0; q < SC_ITEMS; q++) {
+        const int j = q * SC_THREADS + threadIdx.x;
+        tile[SC_PAD(j)] = base + j < n ? ld(base + j) : 0;
+    }
+    __syncthreads();
+    const int j0 = threadIdx.x * SC_ITEMS;
     int32_t v[SC_ITEMS];
     int32_t s = 0;
 #pragma unroll
-    for (int q = 0; q < SC_ITEMS; q++) {
-        v[q] = base + q < n ? ld(base + q) : 0;
-        s += v[q];
+    for (int k = 0; k < SC_ITEMS; k++) {
+        v[k] = tile[SC_PAD(j0 + k)];
+        s += v[k];
     }
     int32_t tot;
     int32_t ex = block_excl_scan<int32_t>(s, &tot, sh) + bsum[blockIdx.x];
 #pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        tile[SC_PAD(j0 + k)] = ex;
+        ex += v[k];
+    }
+    __syncthreads();
+#pragma unroll
     for (int q = 0; q < SC_ITEMS; q++) {
-        if (base + q < n) out[base + q] = ex;
-        ex += v[q];
+        const int j = q * SC_THREADS + threadIdx.x;
+        if (base + j < n) out[base + j] = tile[SC_PAD(j)];
     }
 }
 
