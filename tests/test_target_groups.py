@@ -228,3 +228,58 @@ def test_gpu_forest_groups_equal_restatement(gs):
     assert np.array_equal(_gpu_groups(x, F.tree.order, gs, "cells", keys=F.tree.keys, tree=F.tree), ref)
     assert np.array_equal(_gpu_groups(x, F.tree.order, gs, "cells", tree=F.tree), ref)
     assert np.array_equal(_gpu_groups(x, F.tree.order, gs, "fixed", tree=F.tree), fixed_groups(off, gs))
+
+
+def field_sample_rows(offsets, ngroups, soff, samples, gs):
+    """Sorted positions of the field-scale sample targets: per system, groups first + k * count //
+    samples of its groups, each padded to gs targets by repeating its last target."""
+    starts = np.asarray(offsets[:ngroups], np.int64)
+    rows = []
+    for s in range(len(soff) - 1):
+        g0 = int(np.searchsorted(starts, soff[s], "left"))
+        g1 = int(np.searchsorted(starts, soff[s + 1], "left")) if s + 1 < len(soff) - 1 else ngroups
+        for k in range(samples):
+            g = g0 + (k * (g1 - g0)) // samples
+            a, b = int(offsets[g]), int(offsets[g + 1])
+            rows.extend(a + min(j, b - a - 1) for j in range(gs))
+    return np.array(rows, np.int64)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("samples,gs", [(64, 64), (7, 16)])
+def test_gpu_field_sample_equals_restatement(samples, gs):
+    """ptg_field_sample gathers the sample targets the restatement picks, bit for bit, for one tree
+    and for a forest (per-system groups)."""
+    import torch
+
+    from pytreegrav_amd import ConstructTree, Forest, _lib, engine
+    from pytreegrav_amd._lib import check, ptr
+
+    x, m, h = case("plummer_1e5_h")
+    t = ConstructTree(x, m, h).dev
+    n = len(x)
+    off, cnt = engine.target_groups(x, t.order, gs, keys=t.keys)
+    G = int(cnt.item())
+    pos, hh, _, g = engine.field_scale_sample(t, samples, gs, (off, cnt))
+    idx = field_sample_rows(off.cpu().numpy(), G, np.array([0, n]), samples, gs)
+    coords = t.coordinates[:n].cpu().numpy()
+    assert g == gs and np.array_equal(pos.cpu().numpy(), coords[idx])
+    assert np.array_equal(hh.cpu().numpy(), t.softenings[:n].cpu().numpy()[idx])
+
+    rng = np.random.default_rng(4)
+    parts = [x[:3000] * 0.5, x[3000:3500] + 4.0, x[3500:9000] * 2.0 - 7.0, case("dups")[0]]
+    xf = np.vstack(parts)
+    soff = np.concatenate([[0], np.cumsum([len(p) for p in parts])])
+    nf = len(xf)
+    F = Forest(xf, np.full(nf, 1.0 / nf), rng.random(nf) * 0.01, soff, precision="float32")
+    goff, gcnt = engine.target_groups(xf, F.tree.order, gs, tree=F.tree, keys=F.tree.keys)
+    Gf = int(gcnt.item())
+    rows = len(parts) * samples * gs
+    pf = torch.empty((rows, 3), dtype=torch.float64, device=engine.device())
+    hf = torch.empty(rows, dtype=torch.float64, device=engine.device())
+    check(_lib.load().ptg_field_sample(ptr(goff), ptr(gcnt), F.tree.forest_ref, nf, samples, gs,
+                                       ptr(F.tree.coordinates), ptr(F.tree.softenings), ptr(pf), ptr(hf),
+                                       engine.stream()), "ptg_field_sample")
+    idx = field_sample_rows(goff.cpu().numpy(), Gf, soff, samples, gs)
+    assert np.array_equal(pf.cpu().numpy(), F.tree.coordinates[:nf].cpu().numpy()[idx])
+    assert np.array_equal(hf.cpu().numpy(), F.tree.softenings[:nf].cpu().numpy()[idx])
