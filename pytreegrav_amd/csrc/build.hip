@@ -2282,11 +2282,25 @@ __global__ __launch_bounds__(GT_THREADS) void k_group_tiles(const uint64_t *keys
         for (int64_t i = threadIdx.x; i < nkeys; i += GT_THREADS)
             sk[i] = keys[std::min<int64_t>(std::max<int64_t>(K0 + i, 0), n - 1)];
         if (nsys > 1) {
-            for (int64_t base = 64 * wave; base < nkeys; base += GT_THREADS) {
-                const int64_t j = K0 + base + lane;
-                const bool st = j >= 0 && j < n && soff[upper_index(soff, nsys, j)] == j;
-                const uint64_t b = __ballot(st);
-                if (lane == 0) sst[base >> 6] = b;
+            // the system starts among the key positions: the first at or after K0 by one binary
+            // search, then the following ones (few per window) one per thread
+            __shared__ int64_t s0;
+            const int64_t nsw = (nkeys + 63) >> 6;
+            for (int64_t q = threadIdx.x; q < nsw; q += GT_THREADS) sst[q] = 0;
+            if (threadIdx.x == 0) {
+                int64_t lo = 0, hi = nsys;  // first system whose start is >= max(K0, 0)
+                const int64_t v = K0 > 0 ? K0 : 0;
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (soff[mid] < v) lo = mid + 1;
+                    else hi = mid;
+                }
+                s0 = lo;
+            }
+            __syncthreads();
+            for (int64_t sy = s0 + threadIdx.x; sy < nsys && soff[sy] < K0 + nkeys; sy += GT_THREADS) {
+                const int64_t i = soff[sy] - K0;
+                atomicOr((unsigned long long *)&sst[i >> 6], 1ull << (i & 63));
             }
         }
         __syncthreads();
