@@ -650,8 +650,11 @@ def main():
             return builder(pos, mass, soft) if builder is not None else \
                 ConstructTree(pos, mass, soft, quadrupole=build_quads).dev
 
+        last = {}
+
         def step(record):
             tree = make_tree()
+            last["tree"] = tree
             ev = None
             if record:
                 ctr.zero_()
@@ -678,7 +681,23 @@ def main():
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / args.steps
         walk_ms = float(np.mean([a.elapsed_time(b) for a, b in walk_times]))
-        r = {"dt": dt, "walk_ms": walk_ms, "interactions": int(ctr[0].item()), "visits": int(ctr[1].item()),
+        # walk-only with the tree resident: the same evaluation repeated on the last step's tree, only
+        # its walk launch timed (the step's own walk follows a fresh build and is a little slower)
+        res_ev = []
+        for _ in range(args.steps):
+            ev = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+            res_ev.append(ev)
+            engine.self_fields(last["tree"], pos, soft, mask, theta=theta, quadrupole=quad_walk,
+                               group_size=args.group_size, criterion=criterion,
+                               alpha=alpha if criterion != "theta" else None, model=model, walk_events=ev,
+                               expansion=expansion, bf16=args.bf16,
+                               expansion_rel=args.expansion_rel if expansion is not None else 0.0,
+                               variant=args.variant)
+        torch.cuda.synchronize()
+        walk_resident_ms = float(np.median([a.elapsed_time(b) for a, b in res_ev]))
+        last.clear()
+        r = {"dt": dt, "walk_ms": walk_ms, "walk_resident_ms": walk_resident_ms,
+             "interactions": int(ctr[0].item()), "visits": int(ctr[1].item()),
              "particle_interactions": int(ctr[2].item()), "folds": int(ctr[3].item()),
              "groups": int(stats["groups"].item())}
         # tree build alone (same inputs), for the breakdown
@@ -864,6 +883,8 @@ def main():
             "pack_groups_fieldscale_radii_ms": main["dt"] * 1e3 - build_ms - walk_ms,
             "walk_ms": walk_ms,
             "walk_only_particles_per_s": N / (walk_ms * 1e-3),
+            "walk_resident_ms": main["walk_resident_ms"],
+            "walk_resident_particles_per_s": N / (main["walk_resident_ms"] * 1e-3),
             "interactions_per_particle": interactions / N,
             "near_field_particle_interactions_per_particle": pinter / N,
             "target_groups": main["groups"],
