@@ -156,6 +156,92 @@ __device__ __forceinline__ float mlp_forward_f32(const float *__restrict__ P, co
     return y;
 }
 
+// The same FP32 forward for the 64 nodes of a wavefront (f: this lane's node's features), layers 1
+// and 2 as 16x16x4 FP32 matrix products on the matrix cores: rows are the wave's nodes (four row
+// tiles), columns the hidden units (two column tiles).  In a 16x16x4 product lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16] and receives D[4 (l / 16) + q][l % 16] in its q-th
+// register.  Layer 2 hands lane group g = l / 16 the inputs 8g .. 8g+7 over its eight steps (the sum
+// over the 32 inputs is the same), so each lane reads its A values as two contiguous runs.  The
+// activations go through LDS (xs: 12 floats per node, hs: MLP_HS per node); the activation and
+// layer 3 are those of mlp_forward_f32, in its order.  xs may alias hs.  Every lane of the wave must
+// call it.
+#define MLP_HS 33
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ float mlp_forward_f32_wave(const float *__restrict__ P, const float *f, float *xs, float *hs)
+{
+    const int lane = ptg_lane(), g = lane >> 4, c16 = lane & 15;
+#pragma unroll
+    for (int k = 0; k < 12; k++) xs[lane * 12 + k] = k < MLP_IN ? f[k] : 0.f;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    floatx4 acc[4][2];
+    // layer 1: z = b1 + x W1 over the 10 inputs (padded to 12, three steps of 4)
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const float b = P[OFF_B1 + 16 * c + c16];
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[r][c] = floatx4{b, b, b, b};
+    }
+#pragma unroll
+    for (int st = 0; st < 3; st++) {
+        const int k = 4 * st + g;
+        const float w0 = k < MLP_IN ? P[OFF_W1 + k * MLP_H + c16] : 0.f;
+        const float w1 = k < MLP_IN ? P[OFF_W1 + k * MLP_H + 16 + c16] : 0.f;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const float a = xs[(16 * r + c16) * 12 + k];
+            acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, w0, acc[r][0], 0, 0, 0);
+            acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, w1, acc[r][1], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) hs[(16 * r + 4 * g + q) * MLP_HS + 16 * c + c16] = act(acc[r][c][q]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // layer 2: z = b2 + h W2 over the 32 hidden inputs
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        const float b = P[OFF_B2 + 16 * c + c16];
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[r][c] = floatx4{b, b, b, b};
+    }
+#pragma unroll
+    for (int st = 0; st < 8; st++) {
+        const int k = 8 * g + st;
+        const float w0 = P[OFF_W2 + k * MLP_H + c16], w1 = P[OFF_W2 + k * MLP_H + 16 + c16];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const float a = hs[(16 * r + c16) * MLP_HS + k];
+            acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, w0, acc[r][0], 0, 0, 0);
+            acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, w1, acc[r][1], 0, 0, 0);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // every lane has read layer 1's activations
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) hs[(16 * r + 4 * g + q) * MLP_HS + 16 * c + c16] = act(acc[r][c][q]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // layer 3 for this lane's node, in mlp_forward_f32's order
+    float y = P[OFF_B3];
+#pragma unroll
+    for (int j = 0; j < MLP_H; j++) y = fmaf(hs[lane * MLP_HS + j], P[OFF_W3 + j], y);
+    return y;
+}
+
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 // BF16 weights, transposed so that each output unit's inputs are pairs: W1T[h][in/2], W2T[h][h/2];
@@ -600,33 +686,51 @@ __global__ __launch_bounds__(256) void k_open_radii(const ptg_tree_arrays t, int
                                                     float rho_max)
 {
     __shared__ Bf16Params Q;
+    // per wavefront: the features (64 x 12) then, over them, the activations (64 x MLP_HS); a wave's
+    // layer-1 products have read every feature before the first activation is stored
+    __shared__ float hs[4][64 * MLP_HS];
     if (bf16) {
         load_params_bf16(P, Q);
         __syncthreads();
     }
     const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= ptg_node_count(nnodes, t.node_count)) return;  // a deferred tree may hold fewer nodes
+    // every lane reaches the FP32 forward (a wave-wide product); lanes without a node carry zeros
+    bool valid = a < ptg_node_count(nnodes, t.node_count);  // a deferred tree may hold fewer nodes
     const int64_t e = n + a;
-    const int64_t sl = t.slot[e];
-    if (sl < 0 || sl >= nslots) return;
-    if (scale) {  // one scale, or one per system of a forest
-        int64_t sy = 0;
-        if (fo.nsys > 1) {
-            int64_t lo = 0, hi = fo.nsys;
-            while (hi - lo > 1) {
-                const int64_t mid = (lo + hi) >> 1;
-                if (fo.roots[mid] <= e) lo = mid;
-                else hi = mid;
-            }
-            sy = lo;
-        }
-        eps *= scale[sy];
-    }
+    const int64_t sl = valid ? t.slot[e] : -1;
+    valid = valid && sl >= 0 && sl < nslots;
     float f[MLP_IN];
+    bool has = false;
+    if (valid) {
+        if (scale) {  // one scale, or one per system of a forest
+            int64_t sy = 0;
+            if (fo.nsys > 1) {
+                int64_t lo = 0, hi = fo.nsys;
+                while (hi - lo > 1) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (fo.roots[mid] <= e) lo = mid;
+                    else hi = mid;
+                }
+                sy = lo;
+            }
+            eps *= scale[sy];
+        }
+        has = t.node_stats ? node_features_stats(t, n, e, eps, f) : node_features(t, n + nnodes, e, eps, f);
+    } else {
+#pragma unroll
+        for (int k = 0; k < MLP_IN; k++) f[k] = 0.f;
+    }
+    float yhat;
+    if (bf16) {
+        yhat = mlp_forward_bf16(Q, f);
+    } else {
+        const int w = threadIdx.x >> 6;
+        yhat = mlp_forward_f32_wave(P, f, hs[w], hs[w]);
+    }
+    if (!valid) return;
     float radius = 0.f;
-    if (t.node_stats ? node_features_stats(t, n, e, eps, f) : node_features(t, n + nnodes, e, eps, f)) {
-        const float yhat = (bf16 ? mlp_forward_bf16(Q, f) : mlp_forward_f32(P, f)) + margin;
-        const float rho = fminf(exp10f(yhat), rho_max);
+    if (has) {
+        const float rho = fminf(exp10f(yhat + margin), rho_max);
         radius = (float)(t.sizes[e] * (double)rho);
     }
     aux[4 * sl + 3] = radius;

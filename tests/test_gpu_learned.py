@@ -320,3 +320,37 @@ def test_learned_criterion_outside_the_training_distributions(name):
     # (the lattice is hard for any monopole walk: theta=0.7 itself reaches 5.2e-3 there)
     assert e_l < max(4e-3, 1.5 * e_g)
     assert mx_l < max(5e-2, 2 * mx_g)
+
+
+def test_radius_pass_matches_the_forward_on_the_same_features():
+    """The FP32 radius pass evaluates the network on the matrix cores, 64 nodes per wavefront: its
+    radii must equal size * min(10^(MLP(features) + margin), 1e4) with the per-lane FP32 forward
+    (ptg_mlp_forward, checked against numpy above) on the features ptg_node_features gives, to FP32
+    summation-order differences; across a single tree and a ragged node count."""
+    for name in ("plummer_3e4", "clustered"):
+        x, m, h = case(name)
+        t = engine.DeviceTree(x, m, np.full(len(m), 0.01), quadrupole=True)
+        model = learned.OpeningMLP.load()
+        eps = 1e-3
+        p = t.pack(engine.PTG_F32, False, cache=False, norms=False)
+        saved = t.arrays
+        t.arrays = engine.TreeArrays.from_buffer_copy(saved)
+        t.arrays.node_stats = None  # the link-walk features, as ptg_node_features computes them
+        try:
+            learned.open_radii(t, p, model, eps)
+        finally:
+            t.arrays = saved
+        got = p["tensors"][1][:, 3].cpu().numpy()
+        nodes = torch.arange(t.n, t.n + t.nnodes, dtype=torch.int32, device="cuda")
+        feats = learned.node_features(t, nodes, eps)
+        y = model.forward(feats).cpu().numpy().astype(np.float64)
+        sizes = t.sizes[t.n:t.n + t.nnodes].cpu().numpy()
+        masses = t.masses[t.n:t.n + t.nnodes].cpu().numpy()
+        ref = sizes * np.minimum(10.0 ** (y + model.margin), 1e4)
+        ref[~((sizes > 0) & (masses > 0))] = 0.0
+        slots = t.slot[t.n:t.n + t.nnodes].cpu().numpy()
+        g = got[slots]
+        print(f"[learned] {name}: {t.nnodes} nodes, max rel diff "
+              f"{np.max(np.abs(g - ref) / np.maximum(np.abs(ref), 1e-30)):.3g}", flush=True)
+        assert np.all((g == 0) == (ref == 0))
+        assert np.max(np.abs(g - ref) / np.maximum(np.abs(ref), 1e-30)) < 1e-4
