@@ -309,8 +309,22 @@ ptg_status ptg_walk(const ptg_walk_tree *wt, const ptg_forest *forest, const dou
                     int exact, double *out, int64_t *counters, void *ws, size_t ws_bytes, void *stream);
 /* Reads the overflow flag of the last walk on this workspace (synchronises the stream):
  * PTG_ERR_OVERFLOW if a traversal stack exceeded its spill capacity, PTG_ERR_ARG if a supplied
- * group held more than group_size targets. */
+ * group held more than group_size targets or a subtree walk's start slot was out of range. */
 ptg_status ptg_walk_check(void *ws, void *stream);
+
+/* Subtree walks: replaces the start node `no` of treewalk.PotentialWalk / AccelWalk /
+ * PotentialWalk_quad / AccelWalk_quad (treewalk.py:78-277; the walk stops at subtree_end, :28-34).
+ * Every target pos[perm[t]] (softening soft[perm[t]]) walks only the subtree of one element, read
+ * on the device from *start_slot: that element's walk slot (the tree's slot map at the element, a
+ * particle index or a node index).  A particle start contributes that particle alone; a node start
+ * is tested and opened as in the whole-tree walk, so the root's slot gives ptg_walk's result and the
+ * subtrees of a node's children sum to the node's own.  One tree (no forest), the FP64 walk tree with
+ * the reference's branches, one target per group; the other arguments and the output as ptg_walk's
+ * (ws sized by ptg_walk_workspace_bytes(nt, 1)).  A start slot outside the walk tree walks nothing
+ * and is reported by ptg_walk_check as PTG_ERR_ARG. */
+ptg_status ptg_walk_subtree(const ptg_walk_tree *wt, const int32_t *start_slot, const double *pos, const double *soft,
+                            const int64_t *perm, int64_t nt, int fields, int quadrupole, double theta, double G,
+                            double *out, int64_t *counters, void *ws, size_t ws_bytes, void *stream);
 
 /* ptg_walk without a workspace, for callers that cannot allocate (code under HIP graph capture,
  * serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  The same

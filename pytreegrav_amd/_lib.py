@@ -132,6 +132,8 @@ SIGNATURES = {
                             _vp, _vp,
                             _dbl, _c.POINTER(Criterion), _dbl, _c.c_int, _vp, _vp, _vp, _c.c_size_t, _vp]),
     "ptg_walk_check": (_c.c_int, [_vp, _vp]),
+    "ptg_walk_subtree": (_c.c_int, [_c.POINTER(WalkTree), _vp, _vp, _vp, _vp, _i64, _c.c_int, _c.c_int, _dbl, _dbl,
+                                    _vp, _vp, _vp, _c.c_size_t, _vp]),
     "ptg_walk_noalloc": (_c.c_int, [_c.POINTER(WalkTree), _F, _vp, _vp, _vp, _c.c_int, _i64, _c.c_int, _c.c_int,
                                     _c.c_int, _vp, _vp, _dbl, _c.POINTER(Criterion), _dbl, _c.c_int, _vp, _vp, _vp,
                                     _vp]),

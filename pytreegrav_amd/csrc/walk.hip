@@ -81,6 +81,7 @@ template <typename T> struct WalkArgs {
     WalkCtl *ctl;
     int *spill;
     int gcap;  // global spill entries per wave (GCAP, or what the static scratch of ptg_walk_noalloc gives)
+    const int32_t *start = nullptr;  // subtree walks: device slot of the element every walk starts at
 };
 
 // ---- softening kernels (kernel.py:4-75), generic precision ----
@@ -667,9 +668,17 @@ void k_walk(WalkArgs<T> A)
                 eps_x = A.eps_exp * (A.eps_scale ? (T)A.eps_scale[root] : T(1));
             }
 
+            // the root of system s occupies walk slot s; a subtree walk starts at its element's slot
+            // instead (wave-uniform; an out-of-range slot is reported, and dropped when popped like
+            // any corrupt link, so it walks nothing)
+            int first = root;
+            if (A.start) {
+                first = __builtin_amdgcn_readfirstlane(*A.start);
+                if ((first < 0 || first >= A.nslots) && lane == 0) atomicOr(&A.ctl->err, 4);
+            }
             int sp = 1, gsp = 0, nn = 0, np = 0;
             bool overflow = false;
-            if (lane == 0) L.stack[0] = root;  // the root of system s occupies walk slot s
+            if (lane == 0) L.stack[0] = first;
             __builtin_amdgcn_wave_barrier();
 
             for (;;) {
@@ -1036,7 +1045,8 @@ static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, c
                             const int64_t *perm, int targets_sorted, int64_t nt, int fields, int quadrupole,
                             int group_size, const int32_t *group_offsets, const int32_t *group_count, double theta,
                             const ptg_criterion *criterion, double G, int exact, double *out, int64_t *counters,
-                            void *ws, size_t ws_bytes, bool noalloc, int32_t *err_out, void *stream)
+                            void *ws, size_t ws_bytes, bool noalloc, int32_t *err_out, void *stream,
+                            const int32_t *start_slot = nullptr)
 {
     hipStream_t s = (hipStream_t)stream;
     if (!wt || nt < 0 || (nt > 0 && (!pos || !soft || !perm || !out))) {
@@ -1152,7 +1162,7 @@ static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, c
                           group_size, (float)theta, (float)(1.0 / theta), crit, (float)eps, (float)eps_exp,
                           expand ? criterion->expansion_rel : 0.0f, eps_scale, G, wt->origin[0], wt->origin[1],
                           wt->origin[2],
-                          fn, fo, fc, out, counters, ctl, spill, gcap};
+                          fn, fo, fc, out, counters, ctl, spill, gcap, start_slot};
         if (exact) {
             ptg_set_error("ptg_walk: the exact branch set is evaluated in FP64; use prec=PTG_F64");
             return PTG_ERR_ARG;
@@ -1163,7 +1173,7 @@ static ptg_status walk_impl(const ptg_walk_tree *wt, const ptg_forest *forest, c
                            (const double *)wt->quad, pos, soft, perm, targets_sorted != 0, wt->nslots, nt, ptg_cdiv(nt, group_size), group_offsets, group_count,
                            group_size, theta, 1.0 / theta, crit, eps, 0.0, 0.0, nullptr, G,
                            wt->origin[0], wt->origin[1], wt->origin[2],
-                           fn, fo, fc, out, counters, ctl, spill, gcap};
+                           fn, fo, fc, out, counters, ctl, spill, gcap, start_slot};
         st = launch_walk<double, true>(a, fields, quadrupole != 0, false, 0, blocks, s);
     } else {
         ptg_set_error("ptg_walk: prec must be PTG_F32 or PTG_F64");
@@ -1197,6 +1207,25 @@ extern "C" ptg_status ptg_walk_noalloc(const ptg_walk_tree *wt, const ptg_forest
                      group_count, theta, criterion, G, exact, out, counters, nullptr, 0, true, err, stream);
 }
 
+// every target walks the subtree of one element (treewalk.py:78-277 with no >= 0, stopping at
+// subtree_end :28-34): the FP64 walk with the reference's branches, one target per group
+extern "C" ptg_status ptg_walk_subtree(const ptg_walk_tree *wt, const int32_t *start_slot, const double *pos,
+                                       const double *soft, const int64_t *perm, int64_t nt, int fields,
+                                       int quadrupole, double theta, double G, double *out, int64_t *counters,
+                                       void *ws, size_t ws_bytes, void *stream)
+{
+    if (!wt || !start_slot) {
+        ptg_set_error("ptg_walk_subtree: bad arguments (walk tree and start slot are required)");
+        return PTG_ERR_ARG;
+    }
+    if (wt->prec != PTG_F64) {
+        ptg_set_error("ptg_walk_subtree: the per-target walks run on the FP64 walk tree (prec=PTG_F64)");
+        return PTG_ERR_ARG;
+    }
+    return walk_impl(wt, nullptr, pos, soft, perm, 0, nt, fields, quadrupole, 1, nullptr, nullptr, theta, nullptr, G,
+                     1, out, counters, ws, ws_bytes, false, nullptr, stream, start_slot);
+}
+
 // overflow flag of the last walk on this workspace (0 = fine); synchronises the stream
 extern "C" ptg_status ptg_walk_check(void *ws, void *stream)
 {
@@ -1207,6 +1236,10 @@ extern "C" ptg_status ptg_walk_check(void *ws, void *stream)
     if (err & 2) {
         ptg_set_error("ptg_walk: a supplied target group holds more than group_size targets (its rows past "
                       "group_size were not evaluated)");
+        return PTG_ERR_ARG;
+    }
+    if (err & 4) {
+        ptg_set_error("ptg_walk_subtree: the start element's slot is outside the walk tree (nothing was walked)");
         return PTG_ERR_ARG;
     }
     if (err) {

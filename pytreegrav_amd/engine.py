@@ -540,6 +540,37 @@ def check_walk(slot=0):
         check(_lib.load().ptg_walk_check(ptr(ws), stream()), "ptg_walk")
 
 
+def walk_subtree(tree, pos, soft, no, mask, quadrupole=False, theta=0.7, G=1.0, counters=None):
+    """Per-target FP64 walks of the subtree of element `no` (a particle index or a node index
+    NumParticles + k; -1 = the root) at every target pos[t] (ptg_walk_subtree; treewalk.py:78-277
+    with the stop at subtree_end, :28-34); (nt, W) float64 device tensor in target order, scaled by G."""
+    if tree.nsys > 1 or tree.forest is not None:
+        raise ValueError("subtree walks run on a single tree, not a forest")
+    no = int(no)
+    if no < 0:
+        no = tree.n  # the root, as the reference defaults
+    if no >= tree.ne:
+        raise ValueError(f"no={no} is not an element of the tree (it has {tree.ne})")
+    if quadrupole and not tree.quadrupole:
+        raise ValueError("a quadrupole walk needs a tree built with quadrupole=True")
+    L = _lib.load()
+    pos = to_dev(pos, (-1, 3))
+    nt = pos.shape[0]
+    soft = to_dev(soft, (nt,))
+    out = torch.empty((nt, field_width(mask)), dtype=torch.float64, device=device())
+    if nt == 0:
+        return out
+    perm = morton_order(pos) if nt > 1 else torch.zeros(1, dtype=torch.int64, device=device())
+    p = tree.pack(PTG_F64, quadrupole)
+    start = tree.slot[no:no + 1]  # the element's walk slot, read by the kernel (no host round trip)
+    ws = _walk_workspace(L.ptg_walk_workspace_bytes(nt, 1))
+    check(L.ptg_walk_subtree(_lib.ctypes.byref(p["wt"]), ptr(start), ptr(pos), ptr(soft), ptr(perm), nt, int(mask),
+                             int(bool(quadrupole)), float(theta), float(G), ptr(out), ptr(counters), ptr(ws),
+                             ws.numel(), stream()), "ptg_walk_subtree")
+    check(L.ptg_walk_check(ptr(ws), stream()), "ptg_walk_subtree")
+    return out
+
+
 _side_streams = {}
 
 
