@@ -668,13 +668,15 @@ void k_walk(WalkArgs<T> A)
                 eps_x = A.eps_exp * (A.eps_scale ? (T)A.eps_scale[root] : T(1));
             }
 
-            // the root of system s occupies walk slot s; a subtree walk starts at its element's slot
-            // instead (wave-uniform; an out-of-range slot is reported, and dropped when popped like
-            // any corrupt link, so it walks nothing)
+            // the root of system s occupies walk slot s; a subtree walk (the FP64 reference-branch
+            // walk only) starts at its element's slot instead (wave-uniform; an out-of-range slot is
+            // reported, and dropped when popped like any corrupt link, so it walks nothing)
             int first = root;
-            if (A.start) {
-                first = __builtin_amdgcn_readfirstlane(*A.start);
-                if ((first < 0 || first >= A.nslots) && lane == 0) atomicOr(&A.ctl->err, 4);
+            if constexpr (EXACT) {
+                if (A.start) {
+                    first = __builtin_amdgcn_readfirstlane(*A.start);
+                    if ((first < 0 || first >= A.nslots) && lane == 0) atomicOr(&A.ctl->err, 4);
+                }
             }
             int sp = 1, gsp = 0, nn = 0, np = 0;
             bool overflow = false;
