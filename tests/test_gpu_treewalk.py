@@ -224,3 +224,24 @@ def test_bruteforce_forms_match_oracle(name):
     if name != "tidal":
         sym = {"potential": p.Potential_bruteforce_symmetric, "accel": p.Accel_bruteforce_symmetric}[name]
         assert np.array_equal(sym(x, m, h, G=0.5), got_s)
+
+
+def test_bruteforce_invariants_of_the_reference_tests():
+    """tests/test_bruteforce_symmetric.py:56-76, 169-172: momentum conservation (the per-target GPU
+    sums cancel to rounding, not exactly as the reference's symmetrised sum does, hence 1e-12),
+    coincident unsoftened particles give finite zeros, and G scales every field linearly."""
+    import pytreegrav_amd as p
+
+    rng = np.random.default_rng(7)
+    x = rng.normal(size=(500, 3))
+    m = rng.uniform(0.5, 1.5, 500)
+    h = rng.uniform(0.05, 0.3, 500)
+    a = p.Accel_bruteforce_symmetric(x, m, h)
+    net = np.abs((m[:, None] * a).sum(axis=0)).max()
+    assert net < 1e-12 * np.abs(m[:, None] * a).sum()
+    z = np.zeros((4, 3))
+    phi0, a0 = p.Potential_bruteforce(z, np.ones(4), np.zeros(4)), p.Accel_bruteforce(z, np.ones(4), np.zeros(4))
+    assert np.all(np.isfinite(phi0)) and np.all(np.isfinite(a0))
+    assert not np.any(phi0) and not np.any(a0)
+    for f in (p.Potential_bruteforce, p.Accel_bruteforce, p.TidalTensor_bruteforce):
+        assert rel(f(x[:200], m[:200], h[:200], 3.0), 3.0 * f(x[:200], m[:200], h[:200])) < 1e-14
