@@ -29,6 +29,12 @@ from .frontend import (  # noqa: F401
 )
 from .forest import BatchedAccelPotential, Forest  # noqa: F401
 from .treewalk import (  # noqa: F401
+    ColumnDensity_grouped,
+    ColumnDensity_grouped_parallel,
+    ColumnDensity_tree,
+    ColumnDensity_tree_parallel,
+    ColumnDensityBinned_grouped,
+    ColumnDensityBinned_grouped_parallel,
     AccelTarget_tree,
     AccelTarget_tree_parallel,
     AccelWalk,

@@ -1,11 +1,13 @@
-"""The reference's per-target tree walks (treewalk.py:22-341) on the GPU.
+"""The reference's per-target tree walks (treewalk.py:22-341) and column-density drivers
+(:1031-1298) on the GPU.
 
 ``PotentialWalk``/``AccelWalk`` and their ``_quad`` forms walk one position through the tree, over
 the whole tree or, with ``no``, over the subtree of one element only (the reference stops at
 ``subtree_end``, treewalk.py:28-34); ``PotentialTarget_tree``/``AccelTarget_tree`` walk many
 targets, one walk per target.  All of them run the FP64 walk of libptg_hip.so with the reference's
 branches and one target per group (ptg_walk_subtree), so each result is the reference's per-target
-sum: the same accepted elements, the same self skip at r = 0, the same softening branches.
+sum: the same accepted elements, the same self skip at r = 0, the same softening branches.  The
+column-density drivers call the ray-traced and angle-binned walks of colden.hip.
 """
 
 import numpy as np
@@ -85,3 +87,61 @@ def AccelTarget_tree(pos_target, softening_target, tree, G=1.0, theta=0.7, quadr
 # the reference compiles a thread-parallel copy of each; every GPU walk is parallel
 PotentialTarget_tree_parallel = PotentialTarget_tree
 AccelTarget_tree_parallel = AccelTarget_tree
+
+
+# ---- column density drivers (treewalk.py:1031-1298); the per-ray physics is in colden.hip ----
+
+def _targets(pos_target):
+    on_dev = _is_dev(pos_target)
+    return engine.to_dev(_atleast_2d(_f64(pos_target)), (-1, 3)), on_dev
+
+
+def _ray_array(rays):
+    rays = rays.cpu().numpy() if isinstance(rays, torch.Tensor) else np.asarray(rays, dtype=np.float64)
+    if rays.ndim != 2 or rays.shape[1] != 3:
+        raise ValueError("rays must be an (N_rays, 3) array of directions")
+    return rays
+
+
+def ColumnDensity_tree(pos_target, tree, rays=None, randomize_rays=False, theta=0.7):
+    """Column density from each target to infinity through the tree's particles (uniform spheres
+    of radius = softening), (N, N_rays), or (N, 6) sky-bin averages when rays is None
+    (treewalk.py:1031-1080).  randomize_rays rotates target i's ray grid by random_rotation(i), i its
+    index in pos_target.  Every ray is traced exactly (no opening-angle truncation, as the
+    reference's ray walks); theta applies to the binned estimate, one target per traversal."""
+    pos, on_dev = _targets(pos_target)
+    if rays is None:
+        out = engine.column_binned(tree.dev, pos, 1, theta)
+    else:
+        rays = _ray_array(rays)
+        if randomize_rays:
+            out = engine.column_rays(tree.dev, pos, engine.random_rotations(rays, pos.shape[0]), per_target=True)
+        else:
+            out = engine.column_rays(tree.dev, pos, rays)
+    return _finish(out, on_dev)
+
+
+def ColumnDensity_grouped(pos_target, rays, tree, group_size=16):
+    """Ray-traced column density, (N, N_rays) (treewalk.py:1087-1191).  The reference's grouped
+    traversal is bit-identical to the per-target one at every group size, and so is this: the GPU
+    walk forms its own wavefront groups (targets adjacent across each ray), so group_size is only
+    checked."""
+    if int(group_size) < 1:
+        raise ValueError("group_size must be at least 1")
+    pos, on_dev = _targets(pos_target)
+    return _finish(engine.column_rays(tree.dev, pos, _ray_array(rays)), on_dev)
+
+
+def ColumnDensityBinned_grouped(pos_target, tree, theta=0.5, group_size=16):
+    """Angle-binned column density, (N, 6), one traversal per group of group_size consecutive
+    targets (Morton-ordered targets make compact groups; group_size=1 is the per-target walk)
+    (treewalk.py:1197-1294)."""
+    if int(group_size) < 1:
+        raise ValueError("group_size must be at least 1")
+    pos, on_dev = _targets(pos_target)
+    return _finish(engine.column_binned(tree.dev, pos, int(group_size), theta), on_dev)
+
+
+ColumnDensity_tree_parallel = ColumnDensity_tree
+ColumnDensity_grouped_parallel = ColumnDensity_grouped
+ColumnDensityBinned_grouped_parallel = ColumnDensityBinned_grouped
