@@ -1270,7 +1270,8 @@ __device__ __forceinline__ int32_t bfs_elem(int32_t h, int64_t n, const uint32_t
 // cell as slot i"; the long searches of the large cells run side by side).  order[q] is the
 // preorder offset of node offset q.
 __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *lev, const int32_t *off, int64_t n,
-                                                    int64_t nnodes, const Geom *geoms, const uint32_t *order,
+                                                    int64_t nnodes, const Geom *geoms, const uint64_t *skey,
+                                                    const uint32_t *order,
                                                     const int32_t *nlev, const int32_t *npair, const uint32_t *pi,
                                                     double *coord, double *sizes, int32_t *first, int32_t *next,
                                                     const int32_t *count)
@@ -1299,13 +1300,26 @@ __global__ __launch_bounds__(256) void k_emit_nodes(KeyView kv, const int32_t *l
     coord[3 * idx + 2] = c[2];
     sizes[idx] = s;
     first[idx] = (L + 1 <= lev[i]) ? (int32_t)(n + pi[a + 1]) : (int32_t)i;
+    // the next node of the same level and system (sorted key equal) starts the next level-L cell in
+    // Morton order, so this cell ends before its first slot; where the cells are dense it ends just
+    // there, found with one probe instead of a gallop
+    const int64_t nn = ptg_node_count(nnodes, count);
+    int64_t bound = n;  // first slot known to lie outside the cell (or n)
+    if (q + 1 < nn && skey[q + 1] == skey[q]) {
+        const int64_t i2 = npair[order[q + 1]];
+        if (i2 >= i + 2 && i2 < n) bound = i2;
+    }
     int64_t lo = i + 1;  // slot i+1 shares the pair's level >= L with slot i
     int64_t step = 1;
-    while (lo + step < n && kv.same_cell(i, lo + step, L)) {
-        lo += step;
-        step <<= 1;
+    if (bound < n && kv.same_cell(i, bound - 1, L)) {
+        lo = bound - 1;
+    } else {
+        while (lo + step < bound && kv.same_cell(i, lo + step, L)) {
+            lo += step;
+            step <<= 1;
+        }
     }
-    int64_t hi = std::min<int64_t>(lo + step, n);  // first candidate known not to match (or n)
+    int64_t hi = std::min<int64_t>(lo + step, bound);  // first candidate known not to match (or n)
     while (hi - lo > 1) {
         int64_t mid = (lo + hi) >> 1;
         if (kv.same_cell(i, mid, L)) lo = mid;
@@ -1792,7 +1806,7 @@ static ptg_status tree_phase(const double *pos, const double *m, const double *h
     if (n == 1) {
         k_single_root<<<1, 1, 0, s>>>(w.geoms, t);
     } else {
-        k_emit_nodes<<<grid1(nnodes), 256, 0, s>>>(kv, w.lev, w.off, n, nnodes, w.geoms, in_first ? nw.idA : nw.idB,
+        k_emit_nodes<<<grid1(nnodes), 256, 0, s>>>(kv, w.lev, w.off, n, nnodes, w.geoms, skey, in_first ? nw.idA : nw.idB,
                                                    nw.nlev, nw.npair, nw.pi, t.coordinates, t.sizes, t.first_subnode,
                                                    t.next_branch, cnt);
     }
